@@ -1,0 +1,239 @@
+/*
+ * libcusna.h -- C ABI of the MI355X SAOM period simulator (libcusna_amd.so).
+ *
+ * This header declares the entry points of the drop-in library. The structs
+ * and the v1 functions keep the layout and meaning of the reference ABI
+ * (cran/cusna src/libcusna.h) so that the R glue
+ * (src/cusna_saom_glue.cpp:57-189, src/cusna_glue.cpp:55-89) can be relinked
+ * against this library unchanged except for the symbol it calls. Every
+ * declaration cites the reference declaration it replaces.
+ *
+ * Conventions (reference libcusna.h:28-35):
+ *   - extern "C", fixed-width integers, plain-old-data structs;
+ *   - the caller owns every buffer; NULL skips an optional output;
+ *   - matrices are row-major; a packed bit row is W = ceil(n/32) uint32 words
+ *     and bit (j & 31) of word (j >> 5) is the tie to actor j.
+ *
+ * All compute runs on the GPU (gfx950). There is no CPU fallback: without a
+ * usable device the compute entry points return CUSNA_ERR_NO_CUDA.
+ */
+#ifndef CUSNA_AMD_LIBCUSNA_H
+#define CUSNA_AMD_LIBCUSNA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CUSNA_API __attribute__((visibility("default")))
+
+/* Capacity limits (reference libcusna.h:61-65). */
+#define CUSNA_MAXEFF 24
+#define CUSNA_MAXBEFF 6
+#define CUSNA_MAXRATE 4
+#define CUSNA_MAXNET 4
+#define CUSNA_MAX_N 8192
+
+/* ABI version (reference libcusna.h:69). */
+#define CUSNA_ABI_VERSION 1
+
+/* Status codes (reference libcusna.h:72-80). */
+typedef enum cusna_status {
+  CUSNA_OK = 0,
+  CUSNA_ERR_NULL_ARG = 1,    /* a required pointer is NULL                   */
+  CUSNA_ERR_SHAPE = 2,       /* n/W/k/M out of range, or struct version      */
+  CUSNA_ERR_NO_CUDA = 3,     /* no usable GPU (name kept from the ABI)       */
+  CUSNA_ERR_CUDA = 4,        /* a HIP runtime call failed                    */
+  CUSNA_ERR_UNSUPPORTED = 5, /* option not implemented                       */
+  CUSNA_ERR_INTERNAL = 6
+} cusna_status;
+
+/*
+ * One SAOM period for M chains (reference libcusna.h:90-180). Field order and
+ * types are the reference's, byte for byte; the comments give the meaning
+ * this implementation honours.
+ */
+typedef struct cusna_saom_problem {
+  int32_t struct_version; /* CUSNA_ABI_VERSION                                */
+
+  int32_t n;       /* actors, 1..CUSNA_MAX_N                                  */
+  int32_t W;       /* (n + 31) / 32                                           */
+  int32_t k;       /* network effects, 1..CUSNA_MAXEFF                        */
+  int32_t M;       /* chains                                                  */
+  int32_t threads; /* 0 = let the library choose the launch shape             */
+
+  const uint32_t *x0_out;  /* data start network, packed out-rows (n*W)       */
+  const uint32_t *x0_in;   /* packed in-rows; accepted, not needed            */
+  const uint32_t *sim_out; /* [opt] simulation start network; NULL = x0_out   */
+  const uint32_t *sim_in;  /* [opt] accepted, not needed                      */
+
+  const float *lam;   /* [M] basic rate per chain                             */
+  const float *theta; /* [M*k] evaluation parameters per chain                */
+
+  const int32_t *type_ids;  /* [k] effect ids (0 density ... 32 interaction) */
+  const int32_t *etypes;    /* [k] 0 eval, 1 creation, 2 endowment; NULL = 0 */
+  const float *wtab;        /* [k*(n+1)] gwesp/gwdsp weight tables           */
+  const float *p1;          /* [k] effect parameter 1                        */
+  const float *p2;          /* [k] effect parameter 2                        */
+  const float *vmat;        /* [k*n] covariate values                        */
+  const uint8_t *vmiss;     /* [k*n] covariate missing flags (statistics)    */
+  const int32_t *dyc_idx;   /* [k] index into dyc or -1; NULL = all -1       */
+  const float *dyc;         /* [opt] nd*n*n dyadic covariates; NULL = zeros  */
+  const int32_t *int0;      /* [k] interaction component 0 or -1             */
+  const int32_t *int1;      /* [k] interaction component 1 or -1             */
+  const int32_t *int2;      /* [k] interaction component 2 or -1             */
+  const int32_t *dyn_flags; /* [k] covariate is the co-evolving behaviour    */
+
+  const uint32_t *kdist;     /* [opt] dyads in the period distance (n*W)     */
+  const uint32_t *keep_out;  /* [opt] statistic network (A & keep) | force   */
+  const uint32_t *keep_in;   /* accepted, not needed                         */
+  const uint32_t *force_out; /* [opt]                                        */
+  const uint32_t *force_in;  /* accepted, not needed                         */
+  const uint32_t *perm;      /* [opt] toggleable dyads (n*W)                 */
+
+  const int32_t *od0; /* [n] start out-degrees (degree effects, p1 <= 0)      */
+  const int32_t *id0; /* [n] start in-degrees                                 */
+
+  int32_t maxdegree;   /* > 0 forbids new ties of egos at the bound           */
+  int32_t need_mask;   /* accepted, not needed                                */
+  int32_t conditional; /* 1 = stop at target_dist                             */
+  int32_t target_dist;
+  int32_t symflag; /* symmetric network: both cells flip                      */
+  uint64_t seed;   /* Philox key                                              */
+
+  int32_t has_comp;       /* restore inactive actors' ties at period end      */
+  int32_t nact;           /* active actors (with act)                         */
+  const int32_t *act;     /* [opt] active actor ids, first nact used          */
+  const uint32_t *inactw; /* [opt] W words, inactive-actor bit mask           */
+
+  int32_t kb;            /* behaviour effects, 0 = network only               */
+  const float *lamb;     /* [M] behaviour rate per chain                      */
+  const float *thetab;   /* [M*kb] behaviour parameters per chain             */
+  const int32_t *bty;    /* [kb] behaviour effect ids (0 linear ... 4 totSim) */
+  const int32_t *z0;     /* [n] start behaviour                               */
+  const float *vb;       /* [n] centred start behaviour                       */
+  const int32_t *bkdist; /* [n] actor counted in the behaviour distance       */
+  int32_t zmin, zmax;
+  float zbar, brange, bsimmean;
+
+  int32_t nrate;        /* non-basic rate effects (not supported: must be 0)  */
+  const int32_t *rkind;
+  const float *ralpha;
+  const float *ru;
+
+  /* float64 descriptors; when given they replace the float32 ones above. */
+  const double *vmat64;
+  const double *wtab64;
+  const double *vb64;
+  const double *p1_64;
+  const double *p2_64;
+} cusna_saom_problem;
+
+/* Outputs (reference libcusna.h:185-208). */
+typedef struct cusna_saom_result {
+  int32_t struct_version;
+
+  double *stats;      /* [M*k] statistics of the masked end network          */
+  double *ntim;       /* [M] simulated time                                  */
+  float *scores;      /* [M*(k+1)] column 0 rate score, then the k effects   */
+  int32_t *dist;      /* [M] signed simulated distance                       */
+  int32_t *dist_stat; /* [M] |x_end xor x0| over kdist                       */
+  int32_t *steps;     /* [M] ministeps                                       */
+
+  int32_t *bdist;   /* [opt] [M]                                             */
+  float *bscores;   /* [opt] [M*(kb+1)]                                      */
+  int32_t *z_end;   /* [opt] [M*n]                                           */
+
+  float *rscores;      /* [opt] [M*nrate]                                    */
+  int32_t *actor_dist; /* [opt] [M*n] per-ego distance over kdist            */
+
+  int32_t want_end_nets;
+  uint8_t *a_ends; /* [opt] [M*n*n] dense end networks, [chain][i][j]        */
+} cusna_saom_result;
+
+/*
+ * M chains of one period on the GPU. Replaces cusna_saom_simulate_period
+ * (reference libcusna.h:212-213). Synchronous: results are in the caller's
+ * host buffers on return. Unlike the reference CPU mirror
+ * (libcusna_saom_cpu.cpp:334-336, 369) every chain c uses its own theta row
+ * and lam[c], as the ABI comment at libcusna.h:107-109 specifies.
+ */
+CUSNA_API cusna_status cusna_saom_simulate_period(const cusna_saom_problem *problem,
+                                                  cusna_saom_result *result);
+
+/* Pack dense n*n uint8 into out/in bit rows (reference libcusna.h:299-301). */
+CUSNA_API cusna_status cusna_pack_bits(const uint8_t *dense, int32_t n, uint32_t *out_rows,
+                                       uint32_t *in_rows, int32_t *W_out);
+
+/* One Philox4x32-10 block, evaluated on the GPU (reference libcusna.h:304-305). */
+CUSNA_API cusna_status cusna_philox4x32(const uint32_t counter[4], const uint32_t key[2],
+                                        uint32_t out[4]);
+
+/*
+ * Statistics of ONE dense network (reference libcusna.h:313-318), computed on
+ * the GPU by the same kernel that scores simulated end networks.
+ */
+CUSNA_API cusna_status cusna_network_stats(const uint8_t *a, int32_t n, int32_t k,
+                                           const int32_t *type_ids, const double *p1,
+                                           const double *p2, const double *vmat,
+                                           const uint8_t *vmiss, const int32_t *dyn_flags,
+                                           const double *vb, const int32_t *od0,
+                                           const int32_t *id0, const double *wtab,
+                                           int32_t symflag, double *out);
+
+/* Behaviour statistics over M behaviour vectors (reference libcusna.h:359-362). */
+CUSNA_API cusna_status cusna_behavior_stats(const int32_t *z_batch, int32_t n, int32_t M,
+                                            int32_t kb, const int32_t *bty,
+                                            const uint8_t *xnet, const int32_t *bkdist,
+                                            double zbar, double brange, double bsimmean,
+                                            double *out);
+
+/* Introspection (reference libcusna.h:527-530). cusna_has_cuda keeps its ABI
+ * name and reports whether a HIP device is usable. */
+CUSNA_API int32_t cusna_abi_version(void);
+CUSNA_API const char *cusna_version_string(void);
+CUSNA_API int32_t cusna_has_cuda(void);
+CUSNA_API const char *cusna_last_error(void);
+
+/* ------------------------------------------------------------------------
+ * Extensions above v1. None of them changes a v1 struct.
+ * ------------------------------------------------------------------------ */
+
+/*
+ * Replay variant of cusna_saom_simulate_period, for verification. No
+ * counterpart in the reference: it exists so that trajectories can be checked
+ * bit for bit against the CPU path independently of how the device and the
+ * host C library round log(). Instead of evaluating log() on its Philox
+ * uniforms, chain c at ministep s reads
+ *   log_u0[(c*smax + s)]           = log(u0)        (waiting time), and
+ *   loglog_u[(c*smax + s)*n + j]   = log(-log(u_j)) (Gumbel noise of j),
+ * as recorded by a host. A chain that reaches smax ministeps stops there.
+ */
+CUSNA_API cusna_status cusna_saom_simulate_period_replay(const cusna_saom_problem *problem,
+                                                         cusna_saom_result *result,
+                                                         const double *log_u0,
+                                                         const double *loglog_u, int32_t smax);
+
+/*
+ * Statistics of M packed networks in one launch. No counterpart in the
+ * reference (which scores one dense network per call, libcusna.h:313-318):
+ * rows[M*n*W] are packed out-rows; keep/force ([opt], n*W packed) give the
+ * statistic network (A & keep) | force as in libcusna_saom_cpu.cpp:415-417.
+ * Descriptor meaning is that of cusna_network_stats. out is [M*k].
+ */
+CUSNA_API cusna_status cusna_network_stats_packed(const uint32_t *rows, int32_t n, int32_t M,
+                                                  int32_t k, const int32_t *type_ids,
+                                                  const double *p1, const double *p2,
+                                                  const double *vmat, const uint8_t *vmiss,
+                                                  const int32_t *dyn_flags, const double *vb,
+                                                  const int32_t *od0, const int32_t *id0,
+                                                  const double *wtab, int32_t symflag,
+                                                  const uint32_t *keep, const uint32_t *force,
+                                                  double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CUSNA_AMD_LIBCUSNA_H */
