@@ -1,0 +1,72 @@
+"""Build of the gfx950 library ``cusna_amd/_lib/libcusna_amd.so``.
+
+Every HIP source is compiled by hipcc for gfx950 into an object file (in
+parallel), then linked into one shared library exporting the C ABI of
+include/libcusna.h. Device code is compiled with -ffp-contract=off so that no
+multiply-add is fused: the change statistics and linear predictors must round
+as the reference's separate multiply and add do.
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+OUT_DIR = os.path.join(HERE, "_lib")
+LIB = os.path.join(OUT_DIR, "libcusna_amd.so")
+ARCH = "gfx950"
+
+HIPCC_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-fvisibility=hidden", "-Wno-unused-result",
+]
+
+# (source, extra defines, object name)
+UNITS = [("capi.hip", [], "capi.o"), ("saom_stats.hip", [], "saom_stats.o"),
+         ("saom_launch.hip", [], "saom_launch.o")] + [
+    ("saom_period.hip", [f"-DSAOM_WMAX={w}"], f"saom_period_w{w}.o") for w in (1, 2, 4, 8, 16)]
+
+
+def hipcc() -> str:
+    for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if c and os.path.exists(c):
+            return c
+    raise FileNotFoundError("hipcc not found")
+
+
+def _compile(src: str, defs: list, obj: str, verbose: bool) -> None:
+    cmd = [hipcc(), *HIPCC_FLAGS, *defs, "-c", os.path.join(CSRC, src), "-o", obj]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed on {src}:\n{r.stderr[-4000:]}")
+
+
+def build(verbose: bool = False, jobs: int | None = None) -> str:
+    """Compile every HIP unit for gfx950 and link the shared library."""
+    os.makedirs(OUT_DIR, exist_ok=True)
+    objdir = os.path.join(OUT_DIR, "obj")
+    os.makedirs(objdir, exist_ok=True)
+    jobs = jobs or min(len(UNITS), max(1, (os.cpu_count() or 4)))
+    with cf.ThreadPoolExecutor(jobs) as ex:
+        futs = [ex.submit(_compile, s, d, os.path.join(objdir, o), verbose) for s, d, o in UNITS]
+        for f in futs:
+            f.result()
+    tmp = LIB + ".tmp"
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp,
+           *[os.path.join(objdir, o) for _, _, o in UNITS]]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

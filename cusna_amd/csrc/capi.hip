@@ -1,0 +1,843 @@
+// capi.hip -- the libcusna C ABI on top of the gfx950 kernels.
+//
+// Host side of the drop-in library: argument validation with the reference's
+// status codes (cran/cusna src/libcusna_saom_cpu.cpp:317-320), conversion of
+// the ABI's u32 bit rows and float32/float64 descriptors to the kernels'
+// u64 rows and float64 descriptors (the promotions of :334-353), device
+// buffers, launches on a per-thread HIP stream, and copies back. Every call is
+// synchronous: outputs are in the caller's buffers on return.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/libcusna.h"
+#include "saom_kernels.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+void set_error(const std::string& s) { g_last_error = s; }
+
+// One HIP stream per host thread: concurrent callers never share a stream.
+struct ThreadStream {
+  hipStream_t s = nullptr;
+  int dev = -1;
+  ~ThreadStream() {
+    if (s) (void)hipStreamDestroy(s);
+  }
+};
+thread_local ThreadStream g_stream;
+
+int device_count() {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+hipStream_t thread_stream() {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  if (!g_stream.s || g_stream.dev != dev) {
+    if (g_stream.s) (void)hipStreamDestroy(g_stream.s);
+    g_stream.s = nullptr;
+    if (hipStreamCreateWithFlags(&g_stream.s, hipStreamNonBlocking) != hipSuccess) return nullptr;
+    g_stream.dev = dev;
+  }
+  return g_stream.s;
+}
+
+// Device allocation owned by one call: one hipMalloc, carved in 256-byte
+// aligned pieces, freed on scope exit.
+class DeviceBlock {
+ public:
+  size_t add(size_t bytes) {
+    const size_t off = total_;
+    total_ += (bytes + 255) & ~(size_t)255;
+    return off;
+  }
+  hipError_t alloc() {
+    if (total_ == 0) total_ = 256;
+    return hipMalloc(&base_, total_);
+  }
+  template <class T>
+  T* at(size_t off) const {
+    return reinterpret_cast<T*>(static_cast<char*>(base_) + off);
+  }
+  ~DeviceBlock() {
+    if (base_) (void)hipFree(base_);
+  }
+
+ private:
+  void* base_ = nullptr;
+  size_t total_ = 0;
+};
+
+#define HIP_TRY(expr)                                                                    \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess) {                                                              \
+      set_error(std::string(#expr) + ": " + hipGetErrorString(e_));                      \
+      return CUSNA_ERR_CUDA;                                                             \
+    }                                                                                    \
+  } while (0)
+
+// ABI bit rows (n*W u32) -> kernel rows (n*W64 u64); NULL -> off-diagonal
+// (the reference's default masks, libcusna_saom_cpu.cpp:326-331) or zero.
+std::vector<uint64_t> rows64(const uint32_t* rows, int n, int W, int W64, int dflt) {
+  std::vector<uint64_t> out((size_t)n * W64, 0ull);
+  for (int i = 0; i < n; ++i) {
+    uint64_t* o = &out[(size_t)i * W64];
+    if (rows) {
+      for (int w = 0; w < W; ++w) o[w >> 1] |= (uint64_t)rows[(size_t)i * W + w] << (32 * (w & 1));
+      // bits past n are not ties
+      if (n & 63) o[W64 - 1] &= (1ull << (n & 63)) - 1;
+    } else if (dflt == 1) {
+      for (int j = 0; j < n; ++j)
+        if (j != i) o[j >> 6] |= 1ull << (j & 63);
+    }
+  }
+  return out;
+}
+
+std::vector<uint64_t> dense_rows64(const uint8_t* a, int n, int W64) {
+  std::vector<uint64_t> out((size_t)n * W64, 0ull);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j)
+      if (a[(size_t)i * n + j]) out[(size_t)i * W64 + (j >> 6)] |= 1ull << (j & 63);
+  return out;
+}
+
+int wmax_for(int W64) {
+  int w = 1;
+  while (w < W64) w <<= 1;
+  return w;
+}
+
+constexpr size_t kLdsLimit = 160 * 1024;
+
+// Effects whose change statistic needs a given per-alternative count.
+int effect_flags(const int32_t* t, int k, int kb, const int32_t* dyn) {
+  int f = 0;
+  for (int e = 0; e < k; ++e) {
+    switch (t[e]) {
+      case 2: case 24: f |= saom::F_TP; break;
+      case 5: case 29: f |= saom::F_TP | saom::F_TPALL; break;
+      case 6: f |= saom::F_TP; break;
+      case 19: f |= saom::F_RDEG; break;
+      case 30: f |= saom::F_SOALL; break;
+      default: break;
+    }
+    if (dyn && dyn[e]) f |= saom::F_Z;
+  }
+  if (kb > 0) f |= saom::F_Z;
+  return f;
+}
+
+int pick_nw(int wmax, int threads) {
+  if (threads > 0) {
+    int nw = threads / 64;
+    if (nw < 1) nw = 1;
+    int p = 1;
+    while (p * 2 <= nw) p *= 2;
+    nw = p;
+    if (nw > wmax) nw = wmax;
+    if (wmax == 16 && nw < 2) nw = 2;
+    return nw;
+  }
+  // default shape: about four alternatives per lane
+  int nw = wmax / 4;
+  if (nw < 1) nw = 1;
+  if (wmax == 16 && nw < 2) nw = 2;
+  return nw;
+}
+
+cusna_status validate(const cusna_saom_problem* pb, const cusna_saom_result* r) {
+  if (!pb || !r) return CUSNA_ERR_NULL_ARG;
+  if (pb->struct_version != CUSNA_ABI_VERSION || r->struct_version != CUSNA_ABI_VERSION)
+    return CUSNA_ERR_SHAPE;
+  const int n = pb->n, W = pb->W, k = pb->k, M = pb->M;
+  if (n < 1 || n > CUSNA_MAX_N || W != (n + 31) / 32 || k < 1 || M < 1) return CUSNA_ERR_SHAPE;
+  if (k > CUSNA_MAXEFF || pb->kb < 0 || pb->kb > CUSNA_MAXBEFF) return CUSNA_ERR_SHAPE;
+  if (!pb->x0_out || !pb->type_ids || !pb->theta) return CUSNA_ERR_NULL_ARG;
+  if (pb->nrate != 0) {
+    set_error("non-basic rate effects (nrate > 0) are not supported by the GPU path");
+    return CUSNA_ERR_UNSUPPORTED;
+  }
+  if (pb->kb > 0 && (!pb->lamb || !pb->thetab || !pb->bty || !pb->z0)) return CUSNA_ERR_NULL_ARG;
+  for (int e = 0; e < k; ++e) {
+    const int t = pb->type_ids[e];
+    if (t < 0 || t > 32) {
+      set_error("unknown effect type id");
+      return CUSNA_ERR_UNSUPPORTED;
+    }
+    if (t == 32) {
+      const int c0 = pb->int0 ? pb->int0[e] : -1, c1 = pb->int1 ? pb->int1[e] : -1;
+      const int c2 = pb->int2 ? pb->int2[e] : -1;
+      if (c0 < 0 || c0 >= e || c1 < 0 || c1 >= e || c2 >= e) return CUSNA_ERR_SHAPE;
+      if (pb->type_ids[c0] == 32 || pb->type_ids[c1] == 32 || (c2 >= 0 && pb->type_ids[c2] == 32))
+        return CUSNA_ERR_SHAPE;
+    }
+  }
+  if (pb->act && (pb->nact < 1 || pb->nact > n)) return CUSNA_ERR_SHAPE;
+  if (pb->act)
+    for (int q = 0; q < pb->nact; ++q)
+      if (pb->act[q] < 0 || pb->act[q] >= n) return CUSNA_ERR_SHAPE;
+  if (pb->kb > 0)
+    for (int e = 0; e < pb->kb; ++e)
+      if (pb->bty[e] < 0 || pb->bty[e] > 4) return CUSNA_ERR_UNSUPPORTED;
+  return CUSNA_OK;
+}
+
+}  // namespace
+
+// A period plan owns the device copy of one problem: chain-invariant inputs
+// are uploaded once, per-chain parameters can be replaced between runs, and
+// outputs stay in HBM until fetched. The one-shot ABI call is create, run,
+// fetch, destroy.
+struct cusna_period_plan {
+  int n = 0, k = 0, kb = 0, M = 0, W64 = 0, wmax = 0, nw = 0, nd = 0;
+  bool replay = false, want_dense = false, own_stream = false;
+  int symflag = 0;
+  bool has_vmiss = false, has_od0 = false, has_id0 = false;
+  saom::SimParams P{};
+  DeviceBlock D;
+  hipStream_t s = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  size_t o_keep = 0, o_force = 0, o_vb = 0, o_vmiss = 0, o_od0 = 0, o_id0 = 0, o_stats = 0;
+  size_t o_dense = 0, o_th = 0, o_lam = 0, o_thb = 0, o_lamb = 0;
+  bool stats_valid = false;
+  ~cusna_period_plan() {
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (own_stream && s) (void)hipStreamDestroy(s);
+  }
+};
+
+namespace {
+
+cusna_status plan_create(const cusna_saom_problem* pb, bool want_dense, const double* log_u0,
+                         const double* loglog_u, int32_t smax, hipStream_t stream,
+                         cusna_period_plan** out) {
+  cusna_saom_result dummy{};
+  dummy.struct_version = CUSNA_ABI_VERSION;
+  cusna_status st = validate(pb, &dummy);
+  if (st != CUSNA_OK) return st;
+  if (device_count() < 1) {
+    set_error("no HIP device");
+    return CUSNA_ERR_NO_CUDA;
+  }
+  const int n = pb->n, W = pb->W, k = pb->k, M = pb->M, kb = pb->kb;
+  const int W64 = (n + 63) / 64;
+  const int wmax = wmax_for(W64);
+  if (wmax > 16) {
+    set_error("n above 1024 is not supported by the LDS-resident kernel");
+    return CUSNA_ERR_UNSUPPORTED;
+  }
+  const bool replay = log_u0 != nullptr;
+  std::unique_ptr<cusna_period_plan> pl(new cusna_period_plan());
+  pl->n = n;
+  pl->k = k;
+  pl->kb = kb;
+  pl->M = M;
+  pl->W64 = W64;
+  pl->wmax = wmax;
+  pl->replay = replay;
+  pl->want_dense = want_dense;
+  pl->symflag = pb->symflag;
+  saom::SimParams& P = pl->P;
+  std::memset(&P, 0, sizeof(P));
+  P.n = n;
+  P.k = k;
+  P.kb = kb;
+  P.M = M;
+  P.W64 = W64;
+  P.RS = W64;
+  P.nact = pb->act ? pb->nact : n;
+  P.has_comp = pb->has_comp;
+  P.symflag = pb->symflag;
+  P.maxdeg = pb->maxdegree;
+  P.cond = pb->conditional;
+  P.target = pb->target_dist;
+  P.zmin = pb->zmin;
+  P.zmax = pb->zmax;
+  P.smax = replay ? smax : 0;
+  P.key0 = (uint32_t)(pb->seed & 0xFFFFFFFFu);
+  P.key1 = (uint32_t)(pb->seed >> 32);
+  P.zbar = pb->zbar;
+  P.brange = pb->brange;
+  P.bsim = pb->bsimmean;
+  P.flags = effect_flags(pb->type_ids, k, kb, pb->dyn_flags);
+  int nd = 0;
+  for (int e = 0; e < k; ++e) {
+    P.tid[e] = pb->type_ids[e];
+    P.ety[e] = pb->etypes ? pb->etypes[e] : 0;
+    P.dyn[e] = pb->dyn_flags ? pb->dyn_flags[e] : 0;
+    P.dci[e] = pb->dyc_idx ? pb->dyc_idx[e] : -1;
+    P.c0[e] = pb->int0 ? pb->int0[e] : -1;
+    P.c1[e] = pb->int1 ? pb->int1[e] : -1;
+    P.c2[e] = pb->int2 ? pb->int2[e] : -1;
+    P.p1[e] = pb->p1_64 ? pb->p1_64[e] : (pb->p1 ? (double)pb->p1[e] : 0.0);
+    P.p2[e] = pb->p2_64 ? pb->p2_64[e] : (pb->p2 ? (double)pb->p2[e] : 0.0);
+    if (P.dci[e] + 1 > nd) nd = P.dci[e] + 1;
+  }
+  for (int e = 0; e < kb; ++e) P.bty[e] = pb->bty[e];
+  pl->nw = pick_nw(wmax, pb->threads);
+  if (saom::sim_lds_bytes(P, pl->nw) > kLdsLimit) {
+    set_error("chain state exceeds the 160 KB LDS of a CU");
+    return CUSNA_ERR_UNSUPPORTED;
+  }
+  {
+    bool need_cols = false;
+    for (int e = 0; e < k; ++e)
+      if (P.tid[e] == 27 || P.tid[e] == 28 || P.tid[e] == 30) need_cols = true;
+    if (saom::net_stats_lds(n, W64, need_cols) > kLdsLimit) {
+      set_error("statistics of this network size need more than the 160 KB LDS of a CU");
+      return CUSNA_ERR_UNSUPPORTED;
+    }
+  }
+
+  // host-side staging
+  const std::vector<uint64_t> x0 = rows64(pb->x0_out, n, W, W64, 0);
+  const std::vector<uint64_t> xs = rows64(pb->sim_out ? pb->sim_out : pb->x0_out, n, W, W64, 0);
+  const std::vector<uint64_t> kd = rows64(pb->kdist, n, W, W64, 1);
+  const std::vector<uint64_t> pm = rows64(pb->perm, n, W, W64, 1);
+  const std::vector<uint64_t> keep = rows64(pb->keep_out, n, W, W64, 1);
+  const std::vector<uint64_t> force = rows64(pb->force_out, n, W, W64, 0);
+  std::vector<uint64_t> inact;
+  if (pb->has_comp && pb->inactw) {
+    inact.assign(W64, 0ull);
+    for (int w = 0; w < W; ++w) inact[w >> 1] |= (uint64_t)pb->inactw[w] << (32 * (w & 1));
+    if (n & 63) inact[W64 - 1] &= (1ull << (n & 63)) - 1;
+  }
+  std::vector<double> vmat((size_t)k * n), wtab((size_t)k * (n + 1)), vb(n);
+  for (size_t q = 0; q < vmat.size(); ++q)
+    vmat[q] = pb->vmat64 ? pb->vmat64[q] : (pb->vmat ? (double)pb->vmat[q] : 0.0);
+  for (size_t q = 0; q < wtab.size(); ++q)
+    wtab[q] = pb->wtab64 ? pb->wtab64[q] : (pb->wtab ? (double)pb->wtab[q] : 0.0);
+  for (int q = 0; q < n; ++q) vb[q] = pb->vb64 ? pb->vb64[q] : (pb->vb ? (double)pb->vb[q] : 0.0);
+  std::vector<double> dyc;
+  if (pb->dyc && nd > 0) {
+    dyc.resize((size_t)nd * n * n);
+    for (size_t q = 0; q < dyc.size(); ++q) dyc[q] = pb->dyc[q];
+  }
+  std::vector<int32_t> z0(n, 0), bkd(n, 1);
+  for (int q = 0; q < n; ++q) {
+    if (pb->z0) z0[q] = pb->z0[q];
+    if (pb->bkdist) bkd[q] = pb->bkdist[q];
+  }
+
+  // device buffers
+  DeviceBlock& D = pl->D;
+  const size_t rowsb = (size_t)n * W64 * 8;
+  const size_t o_x0 = D.add(rowsb), o_xs = D.add(rowsb), o_kd = D.add(rowsb), o_pm = D.add(rowsb);
+  pl->o_keep = D.add(rowsb);
+  pl->o_force = D.add(rowsb);
+  const size_t o_inact = D.add(W64 * 8);
+  pl->o_th = D.add((size_t)M * k * 8);
+  pl->o_lam = D.add((size_t)M * 8);
+  pl->o_thb = D.add((size_t)M * (kb > 0 ? kb : 1) * 8);
+  pl->o_lamb = D.add((size_t)M * 8);
+  const size_t o_vmat = D.add(vmat.size() * 8), o_wtab = D.add(wtab.size() * 8);
+  pl->o_vb = D.add((size_t)n * 8);
+  const size_t o_dyc = D.add(dyc.size() * 8);
+  const size_t o_z0 = D.add((size_t)n * 4), o_bkd = D.add((size_t)n * 4), o_act = D.add((size_t)n * 4);
+  pl->o_vmiss = D.add((size_t)k * n);
+  pl->o_od0 = D.add((size_t)n * 4);
+  pl->o_id0 = D.add((size_t)n * 4);
+  const size_t o_end = D.add((size_t)M * rowsb);
+  const size_t o_ntim = D.add((size_t)M * 8), o_sc = D.add((size_t)M * (k + 1) * 8);
+  const size_t o_bsc = D.add((size_t)M * (kb + 1) * 8);
+  const size_t o_dist = D.add((size_t)M * 4), o_dstat = D.add((size_t)M * 4);
+  const size_t o_steps = D.add((size_t)M * 4);
+  const size_t o_bdist = D.add((size_t)M * 4), o_zend = D.add((size_t)M * n * 4);
+  pl->o_stats = D.add((size_t)M * k * 8);
+  if (want_dense) pl->o_dense = D.add((size_t)M * n * n);
+  size_t o_lu0 = 0, o_llu = 0;
+  if (replay) {
+    o_lu0 = D.add((size_t)M * smax * 8);
+    o_llu = D.add((size_t)M * smax * n * 8);
+  }
+  HIP_TRY(D.alloc());
+  if (stream) {
+    pl->s = stream;
+  } else {
+    HIP_TRY(hipStreamCreateWithFlags(&pl->s, hipStreamNonBlocking));
+    pl->own_stream = true;
+  }
+  HIP_TRY(hipEventCreate(&pl->ev0));
+  HIP_TRY(hipEventCreate(&pl->ev1));
+  hipStream_t s = pl->s;
+  auto up = [&](size_t off, const void* src, size_t bytes) -> hipError_t {
+    if (!bytes) return hipSuccess;
+    return hipMemcpyAsync(D.at<char>(off), src, bytes, hipMemcpyHostToDevice, s);
+  };
+  HIP_TRY(up(o_x0, x0.data(), rowsb));
+  HIP_TRY(up(o_xs, xs.data(), rowsb));
+  HIP_TRY(up(o_kd, kd.data(), rowsb));
+  HIP_TRY(up(o_pm, pm.data(), rowsb));
+  HIP_TRY(up(pl->o_keep, keep.data(), rowsb));
+  HIP_TRY(up(pl->o_force, force.data(), rowsb));
+  if (!inact.empty()) HIP_TRY(up(o_inact, inact.data(), W64 * 8));
+  HIP_TRY(up(o_vmat, vmat.data(), vmat.size() * 8));
+  HIP_TRY(up(o_wtab, wtab.data(), wtab.size() * 8));
+  HIP_TRY(up(pl->o_vb, vb.data(), (size_t)n * 8));
+  if (!dyc.empty()) HIP_TRY(up(o_dyc, dyc.data(), dyc.size() * 8));
+  HIP_TRY(up(o_z0, z0.data(), (size_t)n * 4));
+  HIP_TRY(up(o_bkd, bkd.data(), (size_t)n * 4));
+  if (pb->act) HIP_TRY(up(o_act, pb->act, (size_t)P.nact * 4));
+  if (pb->vmiss) HIP_TRY(up(pl->o_vmiss, pb->vmiss, (size_t)k * n));
+  if (pb->od0) HIP_TRY(up(pl->o_od0, pb->od0, (size_t)n * 4));
+  if (pb->id0) HIP_TRY(up(pl->o_id0, pb->id0, (size_t)n * 4));
+  pl->has_vmiss = pb->vmiss != nullptr;
+  pl->has_od0 = pb->od0 != nullptr;
+  pl->has_id0 = pb->id0 != nullptr;
+  if (replay) {
+    HIP_TRY(up(o_lu0, log_u0, (size_t)M * smax * 8));
+    HIP_TRY(up(o_llu, loglog_u, (size_t)M * smax * n * 8));
+  }
+  P.x0 = D.at<uint64_t>(o_x0);
+  P.xs = D.at<uint64_t>(o_xs);
+  P.kd = D.at<uint64_t>(o_kd);
+  P.pm = D.at<uint64_t>(o_pm);
+  P.inactw = inact.empty() ? nullptr : D.at<uint64_t>(o_inact);
+  P.theta = D.at<double>(pl->o_th);
+  P.lam = D.at<double>(pl->o_lam);
+  P.thetab = D.at<double>(pl->o_thb);
+  P.lamb = D.at<double>(pl->o_lamb);
+  P.vmat = D.at<double>(o_vmat);
+  P.wtab = D.at<double>(o_wtab);
+  P.dyc = dyc.empty() ? nullptr : D.at<double>(o_dyc);
+  P.z0 = D.at<int32_t>(o_z0);
+  P.bkd = D.at<int32_t>(o_bkd);
+  P.act = pb->act ? D.at<int32_t>(o_act) : nullptr;
+  P.lu0 = replay ? D.at<double>(o_lu0) : nullptr;
+  P.llu = replay ? D.at<double>(o_llu) : nullptr;
+  P.end_rows = D.at<uint64_t>(o_end);
+  P.ntim = D.at<double>(o_ntim);
+  P.scores = D.at<double>(o_sc);
+  P.bscores = D.at<double>(o_bsc);
+  P.dist = D.at<int32_t>(o_dist);
+  P.dist_stat = D.at<int32_t>(o_dstat);
+  P.steps = D.at<int32_t>(o_steps);
+  P.bdist = D.at<int32_t>(o_bdist);
+  P.z_end = kb > 0 ? D.at<int32_t>(o_zend) : nullptr;
+  P.actor_dist = nullptr;  // written by the reference only when nrate > 0 (:406-409)
+  *out = pl.release();
+  return CUSNA_OK;
+}
+
+cusna_status plan_set_params(cusna_period_plan* pl, const float* theta, const float* lam,
+                             const float* thetab, const float* lamb) {
+  const int M = pl->M, k = pl->k, kb = pl->kb;
+  std::vector<double> th((size_t)M * k), lm(M, 1.0), thb((size_t)M * (kb > 0 ? kb : 1), 0.0),
+      lmb(M, 0.0);
+  for (size_t q = 0; q < th.size(); ++q) th[q] = theta[q];
+  if (lam)
+    for (int c = 0; c < M; ++c) lm[c] = lam[c];
+  if (kb > 0) {
+    for (size_t q = 0; q < (size_t)M * kb; ++q) thb[q] = thetab[q];
+    for (int c = 0; c < M; ++c) lmb[c] = lamb[c];
+  }
+  HIP_TRY(hipMemcpyAsync(pl->D.at<char>(pl->o_th), th.data(), th.size() * 8,
+                         hipMemcpyHostToDevice, pl->s));
+  HIP_TRY(hipMemcpyAsync(pl->D.at<char>(pl->o_lam), lm.data(), (size_t)M * 8,
+                         hipMemcpyHostToDevice, pl->s));
+  HIP_TRY(hipMemcpyAsync(pl->D.at<char>(pl->o_thb), thb.data(), thb.size() * 8,
+                         hipMemcpyHostToDevice, pl->s));
+  HIP_TRY(hipMemcpyAsync(pl->D.at<char>(pl->o_lamb), lmb.data(), (size_t)M * 8,
+                         hipMemcpyHostToDevice, pl->s));
+  // pageable host sources: make sure the copies have consumed them
+  HIP_TRY(hipStreamSynchronize(pl->s));
+  return CUSNA_OK;
+}
+
+cusna_status plan_run(cusna_period_plan* pl, uint64_t seed, bool want_stats) {
+  saom::SimParams& P = pl->P;
+  P.key0 = (uint32_t)(seed & 0xFFFFFFFFu);
+  P.key1 = (uint32_t)(seed >> 32);
+  HIP_TRY(hipEventRecord(pl->ev0, pl->s));
+  int le = saom::launch_period(P, saom::LaunchShape{pl->nw, pl->wmax}, pl->replay, pl->s);
+  if (le != 0) {
+    set_error(std::string("period kernel launch: ") + hipGetErrorString((hipError_t)le));
+    return CUSNA_ERR_CUDA;
+  }
+  HIP_TRY(hipEventRecord(pl->ev1, pl->s));
+  pl->stats_valid = false;
+  if (want_stats) {
+    saom::StatParams SP;
+    std::memset(&SP, 0, sizeof(SP));
+    SP.n = pl->n;
+    SP.k = pl->k;
+    SP.M = pl->M;
+    SP.W64 = pl->W64;
+    SP.symflag = pl->symflag;
+    for (int e = 0; e < pl->k; ++e) {
+      SP.tid[e] = P.tid[e];
+      SP.dyn[e] = P.dyn[e];
+      SP.p1[e] = P.p1[e];
+      SP.p2[e] = P.p2[e];
+    }
+    SP.rows = P.end_rows;
+    SP.keep = pl->D.at<uint64_t>(pl->o_keep);
+    SP.force = pl->D.at<uint64_t>(pl->o_force);
+    SP.vmat = P.vmat;
+    SP.vb = pl->D.at<double>(pl->o_vb);
+    SP.wtab = P.wtab;
+    SP.vmiss = pl->has_vmiss ? pl->D.at<uint8_t>(pl->o_vmiss) : nullptr;
+    // start degrees pass through as the reference passes them (:420)
+    SP.od0 = pl->has_od0 ? pl->D.at<int32_t>(pl->o_od0) : nullptr;
+    SP.id0 = pl->has_id0 ? pl->D.at<int32_t>(pl->o_id0) : nullptr;
+    SP.out = pl->D.at<double>(pl->o_stats);
+    le = saom::launch_net_stats(SP, pl->s);
+    if (le != 0) {
+      set_error(std::string("stats kernel launch: ") + hipGetErrorString((hipError_t)le));
+      return CUSNA_ERR_CUDA;
+    }
+    pl->stats_valid = true;
+  }
+  return CUSNA_OK;
+}
+
+cusna_status plan_fetch(cusna_period_plan* pl, cusna_saom_result* r) {
+  if (!r) return CUSNA_ERR_NULL_ARG;
+  if (r->struct_version != CUSNA_ABI_VERSION) return CUSNA_ERR_SHAPE;
+  const int n = pl->n, k = pl->k, kb = pl->kb, M = pl->M;
+  const saom::SimParams& P = pl->P;
+  hipStream_t s = pl->s;
+  const bool dense = r->want_end_nets && r->a_ends;
+  if (dense) {
+    if (!pl->want_dense) {
+      set_error("plan was created without end-network output");
+      return CUSNA_ERR_UNSUPPORTED;
+    }
+    if (saom::launch_expand_dense(P.end_rows, n, pl->W64, M, pl->D.at<uint8_t>(pl->o_dense), s))
+      return CUSNA_ERR_CUDA;
+  }
+  std::vector<double> sc((size_t)M * (k + 1)), bsc((size_t)M * (kb + 1));
+  auto down = [&](void* dst, const void* src, size_t bytes) -> hipError_t {
+    if (!dst || !bytes) return hipSuccess;
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s);
+  };
+  if (r->stats && pl->stats_valid)
+    HIP_TRY(down(r->stats, pl->D.at<char>(pl->o_stats), (size_t)M * k * 8));
+  HIP_TRY(down(r->ntim, P.ntim, (size_t)M * 8));
+  HIP_TRY(down(sc.data(), P.scores, sc.size() * 8));
+  HIP_TRY(down(r->dist, P.dist, (size_t)M * 4));
+  HIP_TRY(down(r->dist_stat, P.dist_stat, (size_t)M * 4));
+  HIP_TRY(down(r->steps, P.steps, (size_t)M * 4));
+  if (kb > 0) {
+    HIP_TRY(down(r->bdist, P.bdist, (size_t)M * 4));
+    HIP_TRY(down(bsc.data(), P.bscores, bsc.size() * 8));
+    HIP_TRY(down(r->z_end, P.z_end, (size_t)M * n * 4));
+  }
+  if (dense) HIP_TRY(down(r->a_ends, pl->D.at<char>(pl->o_dense), (size_t)M * n * n));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (r->scores)
+    for (size_t q = 0; q < sc.size(); ++q) r->scores[q] = (float)sc[q];
+  if (kb > 0 && r->bscores)
+    for (size_t q = 0; q < bsc.size(); ++q) r->bscores[q] = (float)bsc[q];
+  return CUSNA_OK;
+}
+
+cusna_status simulate(const cusna_saom_problem* pb, cusna_saom_result* r, const double* log_u0,
+                      const double* loglog_u, int32_t smax) {
+  cusna_status st = validate(pb, r);
+  if (st != CUSNA_OK) return st;
+  cusna_period_plan* raw = nullptr;
+  st = plan_create(pb, r->want_end_nets && r->a_ends, log_u0, loglog_u, smax, thread_stream(),
+                   &raw);
+  if (st != CUSNA_OK) return st;
+  std::unique_ptr<cusna_period_plan> pl(raw);
+  st = plan_set_params(pl.get(), pb->theta, pb->lam, pb->thetab, pb->lamb);
+  if (st != CUSNA_OK) return st;
+  st = plan_run(pl.get(), pb->seed, r->stats != nullptr);
+  if (st != CUSNA_OK) return st;
+  return plan_fetch(pl.get(), r);
+}
+
+cusna_status net_stats_packed(const uint64_t* rows64_host, int n, int M, int k,
+                              const int32_t* type_ids, const double* p1, const double* p2,
+                              const double* vmat, const uint8_t* vmiss, const int32_t* dyn_flags,
+                              const double* vb, const int32_t* od0, const int32_t* id0,
+                              const double* wtab, int symflag, const uint64_t* keep,
+                              const uint64_t* force, double* out) {
+  if (device_count() < 1) {
+    set_error("no HIP device");
+    return CUSNA_ERR_NO_CUDA;
+  }
+  if (k > CUSNA_MAXEFF) return CUSNA_ERR_SHAPE;
+  const int W64 = (n + 63) / 64;
+  bool need_cols = false;
+  for (int e = 0; e < k; ++e)
+    if (type_ids[e] == 27 || type_ids[e] == 28 || type_ids[e] == 30) need_cols = true;
+  if (saom::net_stats_lds(n, W64, need_cols) > kLdsLimit) {
+    set_error("network too large for the LDS-resident statistics kernel");
+    return CUSNA_ERR_UNSUPPORTED;
+  }
+  saom::StatParams SP;
+  std::memset(&SP, 0, sizeof(SP));
+  SP.n = n;
+  SP.k = k;
+  SP.M = M;
+  SP.W64 = W64;
+  SP.symflag = symflag;
+  for (int e = 0; e < k; ++e) {
+    SP.tid[e] = type_ids[e];
+    SP.dyn[e] = dyn_flags ? dyn_flags[e] : 0;
+    SP.p1[e] = p1 ? p1[e] : 0.0;
+    SP.p2[e] = p2 ? p2[e] : 0.0;
+  }
+  DeviceBlock D;
+  const size_t rowsb = (size_t)n * W64 * 8;
+  const size_t o_rows = D.add((size_t)M * rowsb);
+  const size_t o_keep = D.add(keep ? rowsb : 0), o_force = D.add(force ? rowsb : 0);
+  const size_t o_vmat = D.add((size_t)k * n * 8), o_vb = D.add((size_t)n * 8);
+  const size_t o_wtab = D.add((size_t)k * (n + 1) * 8), o_vmiss = D.add((size_t)k * n);
+  const size_t o_od0 = D.add((size_t)n * 4), o_id0 = D.add((size_t)n * 4);
+  const size_t o_out = D.add((size_t)M * k * 8);
+  HIP_TRY(D.alloc());
+  hipStream_t s = thread_stream();
+  std::vector<double> zk((size_t)k * (n + 1), 0.0);
+  HIP_TRY(hipMemcpyAsync(D.at<char>(o_rows), rows64_host, (size_t)M * rowsb,
+                         hipMemcpyHostToDevice, s));
+  if (keep) HIP_TRY(hipMemcpyAsync(D.at<char>(o_keep), keep, rowsb, hipMemcpyHostToDevice, s));
+  if (force) HIP_TRY(hipMemcpyAsync(D.at<char>(o_force), force, rowsb, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(D.at<char>(o_vmat), vmat ? vmat : zk.data(), (size_t)k * n * 8,
+                         hipMemcpyHostToDevice, s));
+  if (vb) HIP_TRY(hipMemcpyAsync(D.at<char>(o_vb), vb, (size_t)n * 8, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(D.at<char>(o_wtab), wtab ? wtab : zk.data(), (size_t)k * (n + 1) * 8,
+                         hipMemcpyHostToDevice, s));
+  if (vmiss) HIP_TRY(hipMemcpyAsync(D.at<char>(o_vmiss), vmiss, (size_t)k * n,
+                                    hipMemcpyHostToDevice, s));
+  if (od0) HIP_TRY(hipMemcpyAsync(D.at<char>(o_od0), od0, (size_t)n * 4, hipMemcpyHostToDevice, s));
+  if (id0) HIP_TRY(hipMemcpyAsync(D.at<char>(o_id0), id0, (size_t)n * 4, hipMemcpyHostToDevice, s));
+  SP.rows = D.at<uint64_t>(o_rows);
+  SP.keep = keep ? D.at<uint64_t>(o_keep) : nullptr;
+  SP.force = force ? D.at<uint64_t>(o_force) : nullptr;
+  SP.vmat = D.at<double>(o_vmat);
+  SP.vb = vb ? D.at<double>(o_vb) : nullptr;
+  SP.wtab = D.at<double>(o_wtab);
+  SP.vmiss = vmiss ? D.at<uint8_t>(o_vmiss) : nullptr;
+  SP.od0 = od0 ? D.at<int32_t>(o_od0) : nullptr;
+  SP.id0 = id0 ? D.at<int32_t>(o_id0) : nullptr;
+  SP.out = D.at<double>(o_out);
+  const int le = saom::launch_net_stats(SP, s);
+  if (le != 0) {
+    set_error(std::string("stats kernel launch: ") + hipGetErrorString((hipError_t)le));
+    return CUSNA_ERR_CUDA;
+  }
+  HIP_TRY(hipMemcpyAsync(out, D.at<char>(o_out), (size_t)M * k * 8, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return CUSNA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+CUSNA_API int32_t cusna_abi_version(void) { return CUSNA_ABI_VERSION; }
+
+CUSNA_API const char* cusna_version_string(void) {
+  return "cusna-amd 0.1 (gfx950 HIP kernels)";
+}
+
+CUSNA_API int32_t cusna_has_cuda(void) { return device_count() > 0 ? 1 : 0; }
+
+CUSNA_API const char* cusna_last_error(void) { return g_last_error.c_str(); }
+
+CUSNA_API cusna_status cusna_saom_simulate_period(const cusna_saom_problem* problem,
+                                                  cusna_saom_result* result) {
+  g_last_error.clear();
+  return simulate(problem, result, nullptr, nullptr, 0);
+}
+
+CUSNA_API cusna_status cusna_saom_simulate_period_replay(const cusna_saom_problem* problem,
+                                                         cusna_saom_result* result,
+                                                         const double* log_u0,
+                                                         const double* loglog_u, int32_t smax) {
+  g_last_error.clear();
+  if (!log_u0 || !loglog_u) return CUSNA_ERR_NULL_ARG;
+  if (smax < 1) return CUSNA_ERR_SHAPE;
+  return simulate(problem, result, log_u0, loglog_u, smax);
+}
+
+CUSNA_API cusna_status cusna_pack_bits(const uint8_t* dense, int32_t n, uint32_t* out_rows,
+                                       uint32_t* in_rows, int32_t* W_out) {
+  if (!dense || !out_rows || !in_rows) return CUSNA_ERR_NULL_ARG;
+  if (n < 1 || n > CUSNA_MAX_N) return CUSNA_ERR_SHAPE;
+  const int W = (n + 31) / 32;
+  std::memset(out_rows, 0, sizeof(uint32_t) * (size_t)n * W);
+  std::memset(in_rows, 0, sizeof(uint32_t) * (size_t)n * W);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j)
+      if (dense[(size_t)i * n + j]) {
+        out_rows[(size_t)i * W + (j >> 5)] |= 1u << (j & 31);
+        in_rows[(size_t)j * W + (i >> 5)] |= 1u << (i & 31);
+      }
+  if (W_out) *W_out = W;
+  return CUSNA_OK;
+}
+
+CUSNA_API cusna_status cusna_philox4x32(const uint32_t counter[4], const uint32_t key[2],
+                                        uint32_t out[4]) {
+  if (!counter || !key || !out) return CUSNA_ERR_NULL_ARG;
+  if (device_count() < 1) return CUSNA_ERR_NO_CUDA;
+  DeviceBlock D;
+  const size_t o_c = D.add(16), o_k = D.add(8), o_o = D.add(16);
+  HIP_TRY(D.alloc());
+  hipStream_t s = thread_stream();
+  HIP_TRY(hipMemcpyAsync(D.at<char>(o_c), counter, 16, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(D.at<char>(o_k), key, 8, hipMemcpyHostToDevice, s));
+  if (saom::launch_philox(D.at<uint32_t>(o_c), D.at<uint32_t>(o_k), D.at<uint32_t>(o_o), s))
+    return CUSNA_ERR_CUDA;
+  HIP_TRY(hipMemcpyAsync(out, D.at<char>(o_o), 16, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return CUSNA_OK;
+}
+
+CUSNA_API cusna_status cusna_network_stats(const uint8_t* a, int32_t n, int32_t k,
+                                           const int32_t* type_ids, const double* p1,
+                                           const double* p2, const double* vmat,
+                                           const uint8_t* vmiss, const int32_t* dyn_flags,
+                                           const double* vb, const int32_t* od0,
+                                           const int32_t* id0, const double* wtab,
+                                           int32_t symflag, double* out) {
+  g_last_error.clear();
+  // argument checks of the reference (:625), including its status code
+  if (!a || !type_ids || !out || n < 1 || n > CUSNA_MAX_N || k < 1) return CUSNA_ERR_NULL_ARG;
+  const int W64 = (n + 63) / 64;
+  const std::vector<uint64_t> rows = dense_rows64(a, n, W64);
+  return net_stats_packed(rows.data(), n, 1, k, type_ids, p1, p2, vmat, vmiss, dyn_flags, vb, od0,
+                          id0, wtab, symflag, nullptr, nullptr, out);
+}
+
+CUSNA_API cusna_status cusna_network_stats_packed(
+    const uint32_t* rows, int32_t n, int32_t M, int32_t k, const int32_t* type_ids,
+    const double* p1, const double* p2, const double* vmat, const uint8_t* vmiss,
+    const int32_t* dyn_flags, const double* vb, const int32_t* od0, const int32_t* id0,
+    const double* wtab, int32_t symflag, const uint32_t* keep, const uint32_t* force,
+    double* out) {
+  g_last_error.clear();
+  if (!rows || !type_ids || !out) return CUSNA_ERR_NULL_ARG;
+  if (n < 1 || n > CUSNA_MAX_N || k < 1 || M < 1) return CUSNA_ERR_SHAPE;
+  const int W = (n + 31) / 32, W64 = (n + 63) / 64;
+  std::vector<uint64_t> r64((size_t)M * n * W64);
+  for (int c = 0; c < M; ++c) {
+    const std::vector<uint64_t> one = rows64(rows + (size_t)c * n * W, n, W, W64, 0);
+    std::copy(one.begin(), one.end(), r64.begin() + (size_t)c * n * W64);
+  }
+  std::vector<uint64_t> k64, f64;
+  if (keep) k64 = rows64(keep, n, W, W64, 0);
+  if (force) f64 = rows64(force, n, W, W64, 0);
+  return net_stats_packed(r64.data(), n, M, k, type_ids, p1, p2, vmat, vmiss, dyn_flags, vb, od0,
+                          id0, wtab, symflag, keep ? k64.data() : nullptr,
+                          force ? f64.data() : nullptr, out);
+}
+
+CUSNA_API cusna_status cusna_behavior_stats(const int32_t* z_batch, int32_t n, int32_t M,
+                                            int32_t kb, const int32_t* bty,
+                                            const uint8_t* xnet, const int32_t* bkdist,
+                                            double zbar, double brange, double bsimmean,
+                                            double* out) {
+  g_last_error.clear();
+  // argument checks of the reference (:773-774), including its status code
+  if (!z_batch || !bty || !xnet || !out || n < 1 || n > CUSNA_MAX_N || M < 1 || kb < 1)
+    return CUSNA_ERR_NULL_ARG;
+  if (kb > CUSNA_MAXBEFF) return CUSNA_ERR_SHAPE;
+  if (device_count() < 1) {
+    set_error("no HIP device");
+    return CUSNA_ERR_NO_CUDA;
+  }
+  if ((size_t)n * 5 * 8 > kLdsLimit) return CUSNA_ERR_UNSUPPORTED;
+  const int W64 = (n + 63) / 64;
+  const std::vector<uint64_t> xr = dense_rows64(xnet, n, W64);
+  saom::BehStatParams BP;
+  std::memset(&BP, 0, sizeof(BP));
+  BP.n = n;
+  BP.M = M;
+  BP.kb = kb;
+  BP.W64 = W64;
+  for (int e = 0; e < kb; ++e) BP.bty[e] = bty[e];
+  BP.zbar = zbar;
+  BP.brange = brange;
+  BP.bsim = bsimmean;
+  DeviceBlock D;
+  const size_t o_z = D.add((size_t)M * n * 4), o_x = D.add(xr.size() * 8);
+  const size_t o_bk = D.add((size_t)n * 4), o_out = D.add((size_t)M * kb * 8);
+  HIP_TRY(D.alloc());
+  hipStream_t s = thread_stream();
+  HIP_TRY(hipMemcpyAsync(D.at<char>(o_z), z_batch, (size_t)M * n * 4, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(D.at<char>(o_x), xr.data(), xr.size() * 8, hipMemcpyHostToDevice, s));
+  if (bkdist)
+    HIP_TRY(hipMemcpyAsync(D.at<char>(o_bk), bkdist, (size_t)n * 4, hipMemcpyHostToDevice, s));
+  BP.z = D.at<int32_t>(o_z);
+  BP.xnet = D.at<uint64_t>(o_x);
+  BP.bkdist = bkdist ? D.at<int32_t>(o_bk) : nullptr;
+  BP.out = D.at<double>(o_out);
+  if (saom::launch_beh_stats(BP, s)) return CUSNA_ERR_CUDA;
+  HIP_TRY(hipMemcpyAsync(out, D.at<char>(o_out), (size_t)M * kb * 8, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return CUSNA_OK;
+}
+
+CUSNA_API cusna_status cusna_period_plan_create(const cusna_saom_problem* problem,
+                                                int32_t want_end_nets, cusna_period_plan** plan) {
+  g_last_error.clear();
+  if (!problem || !plan) return CUSNA_ERR_NULL_ARG;
+  cusna_status st = plan_create(problem, want_end_nets != 0, nullptr, nullptr, 0, nullptr, plan);
+  if (st != CUSNA_OK) return st;
+  st = plan_set_params(*plan, problem->theta, problem->lam, problem->thetab, problem->lamb);
+  if (st != CUSNA_OK) {
+    delete *plan;
+    *plan = nullptr;
+  }
+  return st;
+}
+
+CUSNA_API cusna_status cusna_period_plan_set_params(cusna_period_plan* plan, const float* theta,
+                                                    const float* lam, const float* thetab,
+                                                    const float* lamb) {
+  if (!plan || !theta) return CUSNA_ERR_NULL_ARG;
+  if (plan->kb > 0 && (!thetab || !lamb)) return CUSNA_ERR_NULL_ARG;
+  return plan_set_params(plan, theta, lam, thetab, lamb);
+}
+
+CUSNA_API cusna_status cusna_period_plan_run(cusna_period_plan* plan, uint64_t seed,
+                                             int32_t want_stats) {
+  if (!plan) return CUSNA_ERR_NULL_ARG;
+  return plan_run(plan, seed, want_stats != 0);
+}
+
+CUSNA_API cusna_status cusna_period_plan_sync(cusna_period_plan* plan) {
+  if (!plan) return CUSNA_ERR_NULL_ARG;
+  HIP_TRY(hipStreamSynchronize(plan->s));
+  return CUSNA_OK;
+}
+
+CUSNA_API cusna_status cusna_period_plan_fetch(cusna_period_plan* plan, cusna_saom_result* result) {
+  if (!plan) return CUSNA_ERR_NULL_ARG;
+  return plan_fetch(plan, result);
+}
+
+CUSNA_API double cusna_period_plan_kernel_ms(cusna_period_plan* plan) {
+  if (!plan) return -1.0;
+  float ms = -1.0f;
+  if (hipEventSynchronize(plan->ev1) != hipSuccess) return -1.0;
+  if (hipEventElapsedTime(&ms, plan->ev0, plan->ev1) != hipSuccess) return -1.0;
+  return (double)ms;
+}
+
+CUSNA_API int32_t cusna_period_plan_shape(const cusna_period_plan* plan) {
+  return plan ? 64 * plan->nw : 0;
+}
+
+CUSNA_API void cusna_period_plan_destroy(cusna_period_plan* plan) { delete plan; }
+
+}  // extern "C"

@@ -1,0 +1,91 @@
+// saom_kernels.h -- launch parameters of the gfx950 SAOM kernels (host + device).
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+#include "saom_common.h"
+
+namespace saom {
+
+// Feature bits: which per-alternative counts the effect list needs.
+enum : int {
+  F_TP = 1 << 0,     // two-paths i->h->j per alternative (tp)
+  F_TPALL = 1 << 1,  // tp of every actor in LDS (transTies, gwdspFF)
+  F_TPIN = 1 << 2,   // common in-neighbours of i and j (gwespBF)
+  F_RR = 1 << 3,     // |row_i & col_i & row_j| (transRecTrip)
+  F_TT = 1 << 4,     // transTies masked popcount
+  F_SOALL = 1 << 5,  // shared out-ties of every actor with i in LDS (gwdspFB)
+  F_Z = 1 << 6,      // behaviour values live in LDS
+  F_RDEG = 1 << 7,   // reciprocated degree of the ego (reciAct)
+};
+
+// One SAOM period for a batch of chains (see saom_period.hip). Rows are u64
+// bit rows: word w of actor h holds ties h->64w..64w+63 (bit j&63).
+struct SimParams {
+  int n, k, kb, M, W64, RS;
+  int nact, has_comp, symflag, maxdeg, cond, target;
+  int zmin, zmax;
+  int smax;  // replay: recorded ministeps per chain (0 = free running)
+  int flags;
+  uint32_t key0, key1;
+  double zbar, brange, bsim;
+  int tid[kMaxEff], ety[kMaxEff], dyn[kMaxEff], dci[kMaxEff];
+  int c0[kMaxEff], c1[kMaxEff], c2[kMaxEff];
+  double p1[kMaxEff], p2[kMaxEff];
+  int bty[kMaxBeff];
+  const uint64_t *x0, *xs, *kd, *pm;  // n*W64 words each
+  const uint64_t *inactw;             // W64 words or null
+  const double *theta, *lam, *thetab, *lamb;  // per chain: M*k, M, M*kb, M
+  const double *vmat, *wtab, *dyc;            // k*n, k*(n+1), nd*n*n (or null)
+  const int32_t *z0, *bkd, *act;              // n each (act: nact used)
+  const double *lu0, *llu;                    // replay inputs (or null)
+  uint64_t *end_rows;                         // M*n*W64 end networks
+  double *ntim, *scores, *bscores;            // M, M*(k+1), M*(kb+1)
+  int32_t *dist, *dist_stat, *steps, *bdist, *z_end, *actor_dist;
+};
+
+// Launch shape chosen for one call.
+struct LaunchShape {
+  int nw;    // waves per chain (workgroup = 64*nw threads)
+  int wmax;  // compile-time bound on W64 (1, 2, 4, 8, 16)
+};
+
+// LDS bytes one chain needs with the given shape.
+size_t sim_lds_bytes(const SimParams& P, int nw);
+
+// Returns 0 on success, a hipError_t value otherwise.
+int launch_period(const SimParams& P, LaunchShape s, bool replay, void* stream);
+
+// Statistics of M packed networks (rows u64, n*W64 per network).
+struct StatParams {
+  int n, k, M, W64, symflag;
+  int tid[kMaxEff], dyn[kMaxEff];
+  double p1[kMaxEff], p2[kMaxEff];
+  const uint64_t* rows;   // M*n*W64
+  const uint64_t *keep, *force;  // n*W64 or null
+  const double *vmat, *vb, *wtab;
+  const uint8_t* vmiss;   // k*n or null
+  const int32_t *od0, *id0;  // n or null
+  double* out;  // M*k
+};
+int launch_net_stats(const StatParams& P, void* stream);
+size_t net_stats_lds(int n, int W64, bool need_cols);
+
+struct BehStatParams {
+  int n, M, kb, W64;
+  int bty[kMaxBeff];
+  const int32_t* z;        // M*n
+  const uint64_t* xnet;    // n*W64
+  const int32_t* bkdist;   // n or null
+  double zbar, brange, bsim;
+  double* out;             // M*kb
+};
+int launch_beh_stats(const BehStatParams& P, void* stream);
+
+// Expand packed end rows (M*n*W64) to dense uint8 (M*n*n).
+int launch_expand_dense(const uint64_t* rows, int n, int W64, int M, uint8_t* dense,
+                        void* stream);
+
+int launch_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out, void* stream);
+
+}  // namespace saom

@@ -1,0 +1,917 @@
+// saom_period.hip -- the SAOM period simulator for gfx950 (MI355X).
+//
+// One workgroup of NW wave64s runs one chain of one period, i.e. the
+// reference's simulate_one (cran/cusna src/libcusna_saom_cpu.cpp:142-303).
+//
+// State in LDS: the network as u64 bit rows (n rows of RS words), in/out
+// degrees, behaviour values. Alternative j of a network ministep belongs to
+// lane (j & 63) of wave ((j >> 6) % NW) in round (j >> 6) / NW, so round r of
+// wave w covers u64 word NW*r + w of every bit row: a lane's own j is always
+// bit `lane` of the word the whole wave is looking at, and broadcasts of a
+// row word serve all 64 lanes at once.
+//
+// Per network ministep (all uniform control flow):
+//   1. Philox block (steps, chain, 0, 0): waiting time, net/behaviour, ego i.
+//   2. row_i from LDS (broadcast), col_i from a ballot of bit i of each row.
+//   3. Features per alternative j: tp_j = sum over out-neighbours h of i of
+//      bit j of row_h (two-paths i->h->j); so_j = |row_i & row_j|;
+//      cy_j = |row_j & col_i| (popcounts over W64 words).
+//   4. f_j = sum_e theta_e d_e(j), effects in list order, no fused multiply-add
+//      (same rounding as the reference's f += th[e] * d).
+//   5. g_j = f_j - log(-log u_j) with u_j from Philox (steps, chain, 1, j);
+//      block argmax with lowest-j ties (the reference's strict > scan from
+//      gbest = -1e30, jstar = i).
+//   6. Second pass: w_j = exp(f_j - lmax); S = sum w_j, T_e = sum w_j d_e(j);
+//      score_e += d_e(j*) - T_e / S.
+//   7. Toggle i->j* (and j*->i when symmetric), degrees, distance.
+// Behaviour ministeps follow libcusna_saom_cpu.cpp:202-227.
+#include <hip/hip_runtime.h>
+
+#include "saom_kernels.h"
+
+namespace saom {
+namespace {
+
+__device__ __forceinline__ int popc64(uint64_t x) { return __popcll(x); }
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ long long wave_sum_ll(long long v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// lexicographic max of (g, -j): larger g wins, lower j breaks ties.
+__device__ __forceinline__ void arg_combine(double& g, int& j, double g2, int j2) {
+  if (g2 > g || (g2 == g && j2 < j)) {
+    g = g2;
+    j = j2;
+  }
+}
+
+__device__ __forceinline__ void wave_argmax(double& g, int& j) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    const double g2 = __shfl_xor(g, o, 64);
+    const int j2 = __shfl_xor(j, o, 64);
+    arg_combine(g, j, g2, j2);
+  }
+}
+
+__device__ __forceinline__ uint32_t philox_word0(uint32_t c0, uint32_t c1, uint32_t c2,
+                                                 uint32_t c3, uint32_t k0, uint32_t k1) {
+  philox10(c0, c1, c2, c3, k0, k1);
+  return c0;
+}
+
+// Per-alternative features of one lane (R alternatives). Alternative r of
+// this lane is actor j = 64*(NW*r + wave) + lane; lanes past n read actor i
+// (any valid row) and are masked out.
+template <int R>
+struct Alt {
+  int j[R];         // alternative actor (clamped to i where j >= n)
+  uint32_t in;      // bit r: j < n
+  uint32_t ok;      // bit r: choosable (j == i always)
+  uint32_t xij;     // bit r: tie i->j
+  uint32_t xji;     // bit r: tie j->i
+  int so[R], cy[R], tp[R];
+  __device__ __forceinline__ int bij(int r) const { return (xij >> r) & 1; }
+  __device__ __forceinline__ int bji(int r) const { return (xji >> r) & 1; }
+  __device__ __forceinline__ double dl(int r) const { return 1.0 - 2.0 * (double)bij(r); }
+};
+
+// Ego-level state shared by all alternatives of the ministep.
+template <int WMAX>
+struct Ego {
+  int i;
+  int outdeg_i, indeg_i, rdeg_i;
+  double zvi;
+  uint64_t rowi[WMAX];  // out-ties of i
+  uint64_t coli[WMAX];  // in-ties of i
+};
+
+// LDS views of the chain state.
+struct ChainLds {
+  const uint64_t* rows;
+  int RS, W64, NW, wv, lane;
+  const int *indeg, *outdeg, *z, *tpall, *soall;
+  const uint64_t *tpm0, *tpm1;
+};
+
+// Change statistics of one non-interaction effect e for the lane's R
+// alternatives (libcusna_saom_cpu.cpp:49-101), creation/endowment masking
+// applied (:258-259). Covariate values: vmat row e, or the current behaviour
+// minus zbar for behaviour-dynamic effects (:233, 244, 252-253).
+template <int R, int WMAX>
+__device__ __forceinline__ void eff_base(const SimParams& P, int e, const Alt<R>& A,
+                                         const Ego<WMAX>& U, const ChainLds& S, double (&d)[R]) {
+  const uint64_t* rows = S.rows;
+  const int RS = S.RS;
+  const int* indeg = S.indeg;
+  const int* outdeg = S.outdeg;
+  const int* z = S.z;
+  const int* tpall = S.tpall;
+  const int* soall = S.soall;
+  const int t = P.tid[e];
+  const int n = P.n;
+  const bool dyn = P.dyn[e] != 0;
+  const double* vrow = P.vmat + (size_t)e * n;
+  const double* w = P.wtab + (size_t)e * (n + 1);
+  const double p1 = P.p1[e], p2 = P.p2[e];
+  auto VJ = [&](int r) -> double {
+    return dyn ? ((double)z[A.j[r]] - P.zbar) : vrow[A.j[r]];
+  };
+  const double vi = dyn ? U.zvi : vrow[U.i];
+  switch (t) {
+    case 0:
+#pragma unroll
+      for (int r = 0; r < R; ++r) d[r] = A.dl(r);
+      break;
+    case 1:
+#pragma unroll
+      for (int r = 0; r < R; ++r) d[r] = A.dl(r) * (double)A.bji(r);
+      break;
+    case 2:
+#pragma unroll
+      for (int r = 0; r < R; ++r) d[r] = A.dl(r) * (double)(A.tp[r] + A.so[r]);
+      break;
+    case 3:
+    case 23:
+#pragma unroll
+      for (int r = 0; r < R; ++r) d[r] = A.dl(r) * (double)A.cy[r];
+      break;
+    case 4:
+#pragma unroll
+      for (int r = 0; r < R; ++r) d[r] = A.dl(r) * (double)(indeg[A.j[r]] + (1 - A.bij(r)));
+      break;
+    case 5:  // transTies: |row_i & row_j & {h: tp_h == x_ij}| + [tp_j > 0]
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const uint64_t* rj = rows + (size_t)A.j[r] * RS;
+        const uint64_t* tm = A.bij(r) ? S.tpm1 : S.tpm0;
+        int c = 0;
+#pragma unroll
+        for (int wd = 0; wd < WMAX; ++wd)
+          if (wd < S.W64) c += popc64(U.rowi[wd] & rj[wd] & tm[wd]);
+        d[r] = A.dl(r) * (double)(c + (A.tp[r] > 0 ? 1 : 0));
+      }
+      break;
+    case 6:  // transRecTrip: x_ji tp_j + |row_i & col_i & row_j|
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const uint64_t* rj = rows + (size_t)A.j[r] * RS;
+        int c = 0;
+#pragma unroll
+        for (int wd = 0; wd < WMAX; ++wd)
+          if (wd < S.W64) c += popc64(U.rowi[wd] & U.coli[wd] & rj[wd]);
+        d[r] = A.dl(r) * (double)((A.bji(r) ? A.tp[r] : 0) + c);
+      }
+      break;
+    case 7:
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        d[r] = A.dl(r) * sqrt((double)(indeg[A.j[r]] + (1 - A.bij(r))));
+      break;
+    case 8:
+#pragma unroll
+      for (int r = 0; r < R; ++r) d[r] = A.dl(r) * (double)outdeg[A.j[r]];
+      break;
+    case 9:
+#pragma unroll
+      for (int r = 0; r < R; ++r) d[r] = A.dl(r) * sqrt((double)outdeg[A.j[r]]);
+      break;
+    case 10:
+#pragma unroll
+      for (int r = 0; r < R; ++r) d[r] = A.dl(r) * vi;
+      break;
+    case 11:
+#pragma unroll
+      for (int r = 0; r < R; ++r) d[r] = A.dl(r) * VJ(r);
+      break;
+    case 12:
+#pragma unroll
+      for (int r = 0; r < R; ++r) d[r] = A.dl(r) * (1.0 - fabs(vi - VJ(r)) * p1 - p2);
+      break;
+    case 13:
+#pragma unroll
+      for (int r = 0; r < R; ++r) d[r] = A.dl(r) * (fabs(vi - VJ(r)) < 1e-9 ? 1.0 : 0.0);
+      break;
+    case 14:
+#pragma unroll
+      for (int r = 0; r < R; ++r) d[r] = A.dl(r) * (double)U.indeg_i;
+      break;
+    case 15:
+#pragma unroll
+      for (int r = 0; r < R; ++r) d[r] = A.dl(r) * sqrt((double)U.indeg_i);
+      break;
+    case 16:
+#pragma unroll
+      for (int r = 0; r < R; ++r) d[r] = A.dl(r) * (2.0 * (double)U.outdeg_i + A.dl(r));
+      break;
+    case 17: {
+      const double d0 = (double)U.outdeg_i;
+      const double s0 = d0 * sqrt(d0);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const double d1 = d0 + A.dl(r);
+        d[r] = d1 * sqrt(d1) - s0;
+      }
+      break;
+    }
+    case 18: {
+      const double d0 = (double)U.outdeg_i;
+#pragma unroll
+      for (int r = 0; r < R; ++r) d[r] = fmin(d0 + A.dl(r), p1) - fmin(d0, p1);
+      break;
+    }
+    case 19:
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        d[r] = A.dl(r) *
+               ((double)U.rdeg_i + (double)A.bji(r) * ((double)U.outdeg_i + A.dl(r)));
+      break;
+    case 20:
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const double vj = VJ(r);
+        d[r] = vi > vj + 1e-9 ? A.dl(r) : (fabs(vi - vj) <= 1e-9 ? A.dl(r) * 0.5 : 0.0);
+      }
+      break;
+    case 21:
+#pragma unroll
+      for (int r = 0; r < R; ++r) d[r] = A.dl(r) * vi * VJ(r);
+      break;
+    case 22: {
+      const int dc = P.dci[e];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const double dyw =
+            (dc >= 0 && P.dyc) ? P.dyc[((size_t)dc * n + U.i) * n + A.j[r]] : 0.0;
+        d[r] = A.dl(r) * dyw;
+      }
+      break;
+    }
+    case 24:
+#pragma unroll
+      for (int r = 0; r < R; ++r) d[r] = A.dl(r) * w[A.tp[r]];
+      break;
+    case 25:
+#pragma unroll
+      for (int r = 0; r < R; ++r) d[r] = A.dl(r) * (vi + VJ(r));
+      break;
+    case 26:
+#pragma unroll
+      for (int r = 0; r < R; ++r) d[r] = A.dl(r) * w[A.so[r]];
+      break;
+    case 27: {  // gwespBF: common in-neighbours #{h in in(i): h -> j}
+      int c[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) c[r] = 0;
+#pragma unroll
+      for (int wd = 0; wd < WMAX; ++wd) {
+        uint64_t m = wd < S.W64 ? U.coli[wd] : 0ull;
+        while (m) {
+          const int h = wd * 64 + __ffsll((long long)m) - 1;
+          m &= m - 1;
+          const uint64_t* rh = rows + (size_t)h * RS;
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            const int word = S.NW * r + S.wv;
+            const uint64_t x = word < S.W64 ? rh[word] : 0ull;
+            c[r] += (int)((x >> S.lane) & 1);
+          }
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) d[r] = A.dl(r) * w[c[r]];
+      break;
+    }
+    case 28:
+#pragma unroll
+      for (int r = 0; r < R; ++r) d[r] = A.dl(r) * w[A.cy[r]];
+      break;
+    case 29:  // gwdspFF: out-neighbours h of j (h != i), in ascending order
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        double s = 0.0;
+        const uint64_t* rj = rows + (size_t)A.j[r] * RS;
+        for (int wd = 0; wd < P.W64; ++wd) {
+          uint64_t m = rj[wd];
+          while (m) {
+            const int h = wd * 64 + __ffsll((long long)m) - 1;
+            m &= m - 1;
+            if (h == U.i) continue;
+            const int p = tpall[h];
+            s += A.bij(r) ? (w[p] - w[p - 1]) : (w[p + 1] - w[p]);
+          }
+        }
+        d[r] = A.dl(r) * s;
+      }
+      break;
+    case 30:  // gwdspFB: in-neighbours h of j (h != i), in ascending order
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        double s = 0.0;
+        const int jw = A.j[r] >> 6, jb = A.j[r] & 63;
+        for (int h = 0; h < n; ++h) {
+          if (h == U.i || !((rows[(size_t)h * RS + jw] >> jb) & 1)) continue;
+          const int p = soall[h];
+          s += A.bij(r) ? (w[p] - w[p - 1]) : (w[p + 1] - w[p]);
+        }
+        d[r] = A.dl(r) * s;
+      }
+      break;
+    case 31:
+#pragma unroll
+      for (int r = 0; r < R; ++r) d[r] = A.dl(r) * (VJ(r) >= p1 - 1e-12 ? 1.0 : 0.0);
+      break;
+    default:
+#pragma unroll
+      for (int r = 0; r < R; ++r) d[r] = 0.0;
+      break;
+  }
+  const int et = P.ety[e];
+  if (et != 0) {
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      if ((et == 1 && A.bij(r) == 1) || (et == 2 && A.bij(r) == 0)) d[r] = 0.0;
+  }
+}
+
+// Full change statistic of effect e, including interactions (type 32):
+// d = (c2 < 0 ? delta : 1) * (d_c0 * d_c1) [* d_c2] (:246-251). Alternatives
+// j == i get 0 (:238).
+template <int R, int WMAX>
+__device__ __forceinline__ void eff_full(const SimParams& P, int e, const Alt<R>& A,
+                                         const Ego<WMAX>& U, const ChainLds& S, double (&d)[R]) {
+  // One eff_base call site: a plain effect runs the loop once; an interaction
+  // runs it over its two or three components.
+  const bool inter = P.tid[e] == 32;
+  const int c2 = P.c2[e];
+  double prod[R];
+  for (int q = 0; q < 3; ++q) {
+    const int ce = !inter ? (q == 0 ? e : -1) : (q == 0 ? P.c0[e] : (q == 1 ? P.c1[e] : c2));
+    if (ce < 0) break;
+    double tmp[R];
+    eff_base<R, WMAX>(P, ce, A, U, S, tmp);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (!inter) d[r] = tmp[r];
+      else if (q == 0) prod[r] = tmp[r];
+      else if (q == 1) d[r] = (c2 < 0 ? A.dl(r) : 1.0) * (prod[r] * tmp[r]);
+      else d[r] = d[r] * tmp[r];
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    if (A.j[r] == U.i || !((A.in >> r) & 1)) d[r] = 0.0;
+}
+
+// Behaviour change statistic (libcusna_saom_cpu.cpp:103-113).
+__device__ __forceinline__ double beh_d(int t, double dz, double zic, long long nb, double zsum,
+                                        long long nH, long long nE, long long nL, double brange,
+                                        double bsim) {
+  switch (t) {
+    case 0:
+      return dz;
+    case 1:
+      return (2.0 * zic + dz) * dz;
+    case 2:
+      return nb > 0 ? dz * zsum / (double)nb : 0.0;
+    case 3:
+    case 4: {
+      if (nb == 0) return 0.0;
+      const double tc = dz > 0 ? (double)(nH - nE - nL) : (double)(nL - nE - nH);
+      return t == 3 ? tc / (brange * (double)nb) : tc / brange - (double)nb * bsim;
+    }
+    default:
+      return 0.0;
+  }
+}
+
+struct LdsLayout {
+  size_t rows, indeg, outdeg, z, tpall, soall, coli, tpm0, tpm1, red, tsl, dstar, sc, bsc, cnt,
+      total;
+};
+
+__host__ __device__ inline size_t al8(size_t x) { return (x + 7) & ~(size_t)7; }
+
+__host__ __device__ inline LdsLayout lds_layout(int n, int k, int kb, int W64, int RS, int nw) {
+  LdsLayout L;
+  size_t o = 0;
+  L.rows = o; o += (size_t)n * RS * 8;
+  L.indeg = o; o += (size_t)n * 4;
+  L.outdeg = o; o += (size_t)n * 4;
+  L.z = o; o += (size_t)n * 4;
+  L.tpall = o; o += (size_t)n * 4;
+  L.soall = o; o += (size_t)n * 4;
+  o = al8(o);
+  L.coli = o; o += (size_t)W64 * 8;
+  L.tpm0 = o; o += (size_t)W64 * 8;
+  L.tpm1 = o; o += (size_t)W64 * 8;
+  L.red = o; o += (size_t)nw * 4 * 8;
+  L.tsl = o; o += (size_t)nw * (k + 1) * 8;
+  L.dstar = o; o += (size_t)k * 8;
+  L.sc = o; o += (size_t)k * 8;
+  L.bsc = o; o += (size_t)(kb > 0 ? kb : 1) * 8;
+  L.cnt = o; o += (size_t)nw * 8 * 8;
+  L.total = o;
+  return L;
+}
+
+template <int NW, int R, int WMAX, bool REPLAY>
+__global__ void __launch_bounds__(64 * NW) saom_period_kernel(const SimParams P) {
+  constexpr int NT = 64 * NW;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int n = P.n, k = P.k, kb = P.kb, W64 = P.W64, RS = P.RS;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int chain = blockIdx.x;
+  const LdsLayout L = lds_layout(n, k, kb, W64, RS, NW);
+  uint64_t* rows = (uint64_t*)(smem + L.rows);
+  int* indeg = (int*)(smem + L.indeg);
+  int* outdeg = (int*)(smem + L.outdeg);
+  int* z = (int*)(smem + L.z);
+  int* tpall = (int*)(smem + L.tpall);
+  int* soall = (int*)(smem + L.soall);
+  uint64_t* coli_s = (uint64_t*)(smem + L.coli);
+  uint64_t* tpm0 = (uint64_t*)(smem + L.tpm0);
+  uint64_t* tpm1 = (uint64_t*)(smem + L.tpm1);
+  double* red = (double*)(smem + L.red);
+  double* tsl = (double*)(smem + L.tsl);
+  double* dstar = (double*)(smem + L.dstar);
+  double* sc = (double*)(smem + L.sc);
+  double* bsc = (double*)(smem + L.bsc);
+  long long* cnt = (long long*)(smem + L.cnt);
+  const int flags = P.flags;
+  const ChainLds S{rows, RS, W64, NW, wv, lane, indeg, outdeg, z, tpall, soall, tpm0, tpm1};
+
+  // ---- chain set-up (libcusna_saom_cpu.cpp:146-160)
+  for (int q = tid; q < n * W64; q += NT) {
+    const int h = q / W64, wd = q - h * W64;
+    rows[(size_t)h * RS + wd] = P.xs[q];
+  }
+  for (int q = tid; q < n; q += NT) {
+    indeg[q] = 0;
+    z[q] = (flags & F_Z) ? P.z0[q] : 0;
+  }
+  for (int q = tid; q < k; q += NT) sc[q] = 0.0;
+  for (int q = tid; q < kb; q += NT) bsc[q] = 0.0;
+  __syncthreads();
+  for (int h = tid; h < n; h += NT) {
+    int od = 0;
+    for (int wd = 0; wd < W64; ++wd) {
+      uint64_t m = rows[(size_t)h * RS + wd];
+      od += popc64(m);
+      while (m) {
+        const int j = wd * 64 + __ffsll((long long)m) - 1;
+        m &= m - 1;
+        atomicAdd(&indeg[j], 1);
+      }
+    }
+    outdeg[h] = od;
+  }
+  __syncthreads();
+
+  const double lam = P.lam[chain];
+  const double lamb = kb > 0 ? P.lamb[chain] : 0.0;
+  const double* th = P.theta + (size_t)chain * k;
+  const double* thb = kb > 0 ? P.thetab + (size_t)chain * kb : nullptr;
+  const int nact = P.nact;
+  long long steps = 0, steps_beh = 0, dist = 0, bdist = 0;
+  long long cap = kMaxSteps;
+  if (REPLAY && P.smax < cap) cap = P.smax;
+  double t = 0.0;
+
+  for (;;) {
+    // 1. ministep draw (:167, 194-200)
+    uint32_t c0 = (uint32_t)steps, c1 = (uint32_t)chain, c2 = 0u, c3 = 0u;
+    philox10(c0, c1, c2, c3, P.key0, P.key1);
+    double lu0;
+    if (REPLAY) lu0 = P.lu0[(size_t)chain * P.smax + steps];
+    else lu0 = log(unit_open(c0));
+    t += -lu0 / ((double)nact * (lam + lamb));
+    if (!P.cond && t > 1.0) break;
+    const bool net = kb == 0 || unit_open(c1) < lam / (lam + lamb);
+    long long ix = (long long)(unit_open(c2) * nact);
+    if (ix >= nact) ix = nact - 1;
+    const int i = P.act ? P.act[ix] : (int)ix;
+    const uint64_t* rowi_p = rows + (size_t)i * RS;
+
+    if (!net) {
+      // ---- behaviour ministep (:202-227)
+      const int zi = z[i];
+      const double zic = (double)zi - P.zbar;
+      long long nb = 0, nH = 0, nE = 0, nL = 0;
+      double zs = 0.0;  // exact in any order: terms are multiples of ulp(float zbar)
+      for (int q = tid; q < n; q += NT) {
+        if ((rowi_p[q >> 6] >> (q & 63)) & 1) {
+          nb++;
+          const int zj = z[q];
+          zs += (double)zj - P.zbar;
+          if (zj > zi) nH++;
+          else if (zj == zi) nE++;
+          else nL++;
+        }
+      }
+      nb = wave_sum_ll(nb);
+      nH = wave_sum_ll(nH);
+      nE = wave_sum_ll(nE);
+      nL = wave_sum_ll(nL);
+      zs = wave_sum(zs);
+      if (NW > 1) {
+        if (lane == 0) {
+          cnt[wv * 8 + 0] = nb;
+          cnt[wv * 8 + 1] = nH;
+          cnt[wv * 8 + 2] = nE;
+          cnt[wv * 8 + 3] = nL;
+          red[wv * 4 + 0] = zs;
+        }
+        __syncthreads();
+        nb = nH = nE = nL = 0;
+        zs = 0.0;
+        for (int q = 0; q < NW; ++q) {
+          nb += cnt[q * 8 + 0];
+          nH += cnt[q * 8 + 1];
+          nE += cnt[q * 8 + 2];
+          nL += cnt[q * 8 + 3];
+          zs += red[q * 4 + 0];
+        }
+      }
+      const bool up = zi < P.zmax, dn = zi > P.zmin;
+      double fu = 0.0, fd = 0.0;
+      double cup[kMaxBeff], cdn[kMaxBeff];
+#pragma unroll
+      for (int e = 0; e < kMaxBeff; ++e) {
+        cup[e] = 0.0;
+        cdn[e] = 0.0;
+        if (e < kb) {
+          cup[e] = beh_d(P.bty[e], 1.0, zic, nb, zs, nH, nE, nL, P.brange, P.bsim);
+          cdn[e] = beh_d(P.bty[e], -1.0, zic, nb, zs, nH, nE, nL, P.brange, P.bsim);
+          const double pu_ = thb[e] * cup[e];
+          const double pd_ = thb[e] * cdn[e];
+          fu = fu + pu_;
+          fd = fd + pd_;
+        }
+      }
+      double mx = 0.0;
+      if (up && fu > mx) mx = fu;
+      if (dn && fd > mx) mx = fd;
+      double pd = dn ? exp(fd - mx) : 0.0, p0 = exp(-mx), pu = up ? exp(fu - mx) : 0.0;
+      const double ps = pd + p0 + pu;
+      pd /= ps;
+      p0 /= ps;
+      pu /= ps;
+      const double uc = unit_open(c3);
+      const int dz = uc < pd ? -1 : (uc < pd + p0 ? 0 : 1);
+      __syncthreads();
+      if (tid == 0) {
+#pragma unroll
+        for (int e = 0; e < kMaxBeff; ++e) {
+          if (e < kb) {
+            double s = dz == 1 ? cup[e] : (dz == -1 ? cdn[e] : 0.0);
+            if (up) s -= pu * cup[e];
+            if (dn) s -= pd * cdn[e];
+            bsc[e] += s;
+          }
+        }
+        if (dz != 0) z[i] = zi + dz;
+      }
+      if (dz != 0 && P.bkd[i]) {
+        const long long a = (long long)zi + dz - P.z0[i], b = (long long)zi - P.z0[i];
+        bdist += (a < 0 ? -a : a) - (b < 0 ? -b : b);
+      }
+      __syncthreads();
+      steps_beh++;
+      steps++;
+      if (P.cond && dist >= P.target) break;
+      if (steps >= cap) break;
+      continue;
+    }
+
+    // ---- network ministep (:229-288)
+    // 2. row i (broadcast) and col i (ballot of bit i over the rows)
+    Ego<WMAX> U;
+    U.i = i;
+#pragma unroll
+    for (int wd = 0; wd < WMAX; ++wd) U.rowi[wd] = wd < W64 ? rowi_p[wd] : 0ull;
+    Alt<R> A;
+    A.in = A.ok = A.xij = A.xji = 0u;
+    const int iw = i >> 6, ib = i & 63;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int word = NW * r + wv;
+      const int j = word * 64 + lane;
+      const bool in = j < n;
+      A.j[r] = in ? j : i;
+      const int xji = in ? (int)((rows[(size_t)A.j[r] * RS + iw] >> ib) & 1) : 0;
+      const uint64_t bal = __ballot(xji);
+      if (lane == 0 && word < W64) coli_s[word] = bal;
+      const uint64_t rw = word < W64 ? rowi_p[word] : 0ull;
+      const int xij = (int)((rw >> lane) & 1);
+      A.in |= (uint32_t)in << r;
+      A.xji |= (uint32_t)xji << r;
+      A.xij |= (uint32_t)xij << r;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int wd = 0; wd < WMAX; ++wd) U.coli[wd] = wd < W64 ? coli_s[wd] : 0ull;
+    U.outdeg_i = outdeg[i];
+    U.indeg_i = indeg[i];
+    U.zvi = (double)z[i] - P.zbar;
+    U.rdeg_i = 0;
+    if (flags & F_RDEG) {
+#pragma unroll
+      for (int wd = 0; wd < WMAX; ++wd) U.rdeg_i += popc64(U.rowi[wd] & U.coli[wd]);
+    }
+
+    // 3. features: tp_j (two-paths i->h->j), so_j = |row_i & row_j|,
+    //    cy_j = |row_j & col_i|
+#pragma unroll
+    for (int r = 0; r < R; ++r) A.tp[r] = 0;
+    if (flags & F_TP) {
+#pragma unroll
+      for (int wd = 0; wd < WMAX; ++wd) {
+        uint64_t m = U.rowi[wd];
+        while (m) {
+          const int h = wd * 64 + __ffsll((long long)m) - 1;
+          m &= m - 1;
+          const uint64_t* rh = rows + (size_t)h * RS;
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            const int word = NW * r + wv;
+            const uint64_t x = word < W64 ? rh[word] : 0ull;
+            A.tp[r] += (int)((x >> lane) & 1);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const uint64_t* rj = rows + (size_t)A.j[r] * RS;
+      int so = 0, cy = 0;
+#pragma unroll
+      for (int wd = 0; wd < WMAX; ++wd) {
+        if (wd < W64) {
+          const uint64_t x = rj[wd];
+          so += popc64(U.rowi[wd] & x);
+          cy += popc64(x & U.coli[wd]);
+        }
+      }
+      A.so[r] = so;
+      A.cy[r] = cy;
+    }
+    if (flags & (F_TPALL | F_SOALL)) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int word = NW * r + wv;
+        const bool in = (A.in >> r) & 1;
+        if (in) {
+          tpall[A.j[r]] = A.tp[r];
+          soall[A.j[r]] = A.so[r];
+        }
+        const uint64_t b0 = __ballot(in && A.tp[r] == 0);
+        const uint64_t b1 = __ballot(in && A.tp[r] == 1);
+        if (lane == 0 && word < W64) {
+          tpm0[word] = b0;
+          tpm1[word] = b1;
+        }
+      }
+      __syncthreads();
+    }
+    // choosable alternatives (:238-241)
+    const uint64_t* pmrow = P.pm + (size_t)i * W64;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int word = NW * r + wv;
+      bool ok;
+      if (!((A.in >> r) & 1)) ok = false;
+      else if (A.j[r] == i) ok = true;
+      else {
+        ok = (pmrow[word] >> lane) & 1;
+        if (ok && P.maxdeg > 0 && A.bij(r) == 0 && U.outdeg_i >= P.maxdeg) ok = false;
+      }
+      A.ok |= (uint32_t)ok << r;
+    }
+
+    // 4. f_j in effect order
+    double f[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) f[r] = 0.0;
+    for (int e = 0; e < k; ++e) {
+      double d[R];
+      eff_full<R, WMAX>(P, e, A, U, S, d);
+      const double te = th[e];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const double prod = te * d[r];
+        f[r] = f[r] + prod;
+      }
+    }
+    // 5. Gumbel-max choice (:265-268) and lmax
+    double gb = -1e30, lm = -1e30;
+    int jb = i;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (A.j[r] == i) f[r] = 0.0;
+      if (!((A.ok >> r) & 1)) continue;
+      if (f[r] > lm) lm = f[r];
+      double ll;
+      if (REPLAY) {
+        ll = P.llu[((size_t)chain * P.smax + steps) * n + A.j[r]];
+      } else {
+        const uint32_t u = philox_word0((uint32_t)steps, (uint32_t)chain, 1u, (uint32_t)A.j[r],
+                                        P.key0, P.key1);
+        ll = log(-log(unit_open(u)));
+      }
+      arg_combine(gb, jb, f[r] - ll, A.j[r]);
+    }
+    wave_argmax(gb, jb);
+    lm = wave_max(lm);
+    if (NW > 1) {
+      if (lane == 0) {
+        red[wv * 4 + 0] = gb;
+        red[wv * 4 + 1] = lm;
+        red[wv * 4 + 2] = (double)jb;
+      }
+      __syncthreads();
+      gb = red[0];
+      lm = red[1];
+      jb = (int)red[2];
+      for (int q = 1; q < NW; ++q) {
+        arg_combine(gb, jb, red[q * 4 + 0], (int)red[q * 4 + 2]);
+        lm = fmax(lm, red[q * 4 + 1]);
+      }
+    }
+    const int jstar = jb;
+
+    // 6. softmax-weighted expected change statistics (:271-274)
+    double wgt[R];
+    double Sw = 0.0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      wgt[r] = ((A.ok >> r) & 1) ? exp(f[r] - lm) : 0.0;
+      Sw += wgt[r];
+    }
+    Sw = wave_sum(Sw);
+    if (lane == 0) tsl[wv * (k + 1) + k] = Sw;
+    for (int e = 0; e < k; ++e) {
+      double d[R];
+      eff_full<R, WMAX>(P, e, A, U, S, d);
+      double T = 0.0;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        T += wgt[r] * d[r];
+        if (((A.in >> r) & 1) && A.j[r] == jstar) dstar[e] = d[r];
+      }
+      T = wave_sum(T);
+      if (lane == 0) tsl[wv * (k + 1) + e] = T;
+    }
+    // 7. state of the chosen dyad before the toggle (:277-281)
+    int aij = 0, x0ij = 0, kdij = 0;
+    if (jstar != i) {
+      const int jw = jstar >> 6, jbit = jstar & 63;
+      aij = (int)((rowi_p[jw] >> jbit) & 1);
+      x0ij = (int)((P.x0[(size_t)i * W64 + jw] >> jbit) & 1);
+      kdij = (int)((P.kd[(size_t)i * W64 + jw] >> jbit) & 1);
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double St = 0.0;
+      for (int q = 0; q < NW; ++q) St += tsl[q * (k + 1) + k];
+      for (int e = 0; e < k; ++e) {
+        double Tt = 0.0;
+        for (int q = 0; q < NW; ++q) Tt += tsl[q * (k + 1) + e];
+        sc[e] += dstar[e] - Tt / St;
+      }
+      if (jstar != i) {
+        const int dd = 1 - 2 * aij;
+        rows[(size_t)i * RS + (jstar >> 6)] ^= (1ull << (jstar & 63));
+        indeg[jstar] += dd;
+        outdeg[i] += dd;
+        if (P.symflag) {
+          rows[(size_t)jstar * RS + (i >> 6)] ^= (1ull << (i & 63));
+          indeg[i] += dd;
+          outdeg[jstar] += dd;
+        }
+      }
+    }
+    steps++;
+    if (jstar != i && kdij) {
+      const long long chg = P.symflag ? 2 : 1;
+      dist += (aij == x0ij) ? chg : -chg;
+    }
+    __syncthreads();
+    if (P.cond && dist >= P.target) break;
+    if (steps >= cap) break;
+  }
+
+  // ---- period end (:290-302)
+  __syncthreads();
+  if (P.has_comp && P.inactw) {
+    for (int h = tid; h < n; h += NT) {
+      const bool hin = (P.inactw[h >> 6] >> (h & 63)) & 1;
+      for (int wd = 0; wd < W64; ++wd) {
+        const uint64_t x0 = P.x0[(size_t)h * W64 + wd];
+        const uint64_t im = P.inactw[wd];
+        const uint64_t v = rows[(size_t)h * RS + wd];
+        rows[(size_t)h * RS + wd] = hin ? x0 : ((v & ~im) | (x0 & im));
+      }
+    }
+    __syncthreads();
+  }
+  long long dsum = 0;
+  uint64_t* out_rows = P.end_rows + (size_t)chain * n * W64;
+  for (int h = tid; h < n; h += NT) {
+    int c = 0;
+    for (int wd = 0; wd < W64; ++wd) {
+      const uint64_t v = rows[(size_t)h * RS + wd];
+      out_rows[(size_t)h * W64 + wd] = v;
+      c += popc64((v ^ P.x0[(size_t)h * W64 + wd]) & P.kd[(size_t)h * W64 + wd]);
+    }
+    if (P.actor_dist) P.actor_dist[(size_t)chain * n + h] = c;
+    dsum += c;
+    if (P.z_end) P.z_end[(size_t)chain * n + h] = z[h];
+  }
+  dsum = wave_sum_ll(dsum);
+  if (lane == 0) cnt[wv * 8 + 4] = dsum;
+  __syncthreads();
+  if (tid == 0) {
+    long long ds = 0;
+    for (int q = 0; q < NW; ++q) ds += cnt[q * 8 + 4];
+    P.dist[chain] = (int32_t)dist;
+    P.dist_stat[chain] = (int32_t)ds;
+    P.steps[chain] = (int32_t)steps;
+    P.ntim[chain] = t;
+    double* so = P.scores + (size_t)chain * (k + 1);
+    so[0] = (double)(steps - steps_beh) / lam - (double)nact;
+    for (int e = 0; e < k; ++e) so[1 + e] = sc[e];
+    if (kb > 0) {
+      P.bdist[chain] = (int32_t)bdist;
+      double* bo = P.bscores + (size_t)chain * (kb + 1);
+      bo[0] = (double)steps_beh / lamb - (double)nact;
+      for (int e = 0; e < kb; ++e) bo[1 + e] = bsc[e];
+    }
+  }
+}
+
+template <int NW, int WMAX>
+hipError_t launch_shape(const SimParams& P, bool replay, hipStream_t st) {
+  constexpr int R = WMAX / NW;
+  const size_t lds = lds_layout(P.n, P.k, P.kb, P.W64, P.RS, NW).total;
+  auto kern = replay ? saom_period_kernel<NW, R, WMAX, true>
+                     : saom_period_kernel<NW, R, WMAX, false>;
+  hipError_t e = hipFuncSetAttribute((const void*)kern,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(kern, dim3(P.M), dim3(64 * NW), lds, st, P);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+// This file is compiled once per word bound SAOM_WMAX (1, 2, 4, 8, 16) so the
+// instantiations build in parallel; launch_period (saom_launch.hip) dispatches.
+#ifndef SAOM_WMAX
+#error "compile with -DSAOM_WMAX=<1|2|4|8|16>"
+#endif
+#define SAOM_CAT2(a, b) a##b
+#define SAOM_CAT(a, b) SAOM_CAT2(a, b)
+
+int SAOM_CAT(launch_period_w, SAOM_WMAX)(const SimParams& P, int nw, bool replay, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  constexpr int WM = SAOM_WMAX;
+  hipError_t e = hipErrorInvalidValue;
+  if constexpr (WM <= 8) {
+    if (nw == 1) e = launch_shape<1, WM>(P, replay, st);
+  }
+  if constexpr (WM >= 2) {
+    if (nw == 2) e = launch_shape<2, WM>(P, replay, st);
+  }
+  if constexpr (WM >= 4) {
+    if (nw == 4) e = launch_shape<4, WM>(P, replay, st);
+  }
+  if constexpr (WM >= 8) {
+    if (nw == 8) e = launch_shape<8, WM>(P, replay, st);
+  }
+  if constexpr (WM >= 16) {
+    if (nw == 16) e = launch_shape<16, WM>(P, replay, st);
+  }
+  return (int)e;
+}
+
+size_t SAOM_CAT(sim_lds_bytes_w, SAOM_WMAX)(const SimParams& P, int nw) {
+  return lds_layout(P.n, P.k, P.kb, P.W64, P.RS, nw).total;
+}
+
+}  // namespace saom

@@ -1,0 +1,507 @@
+// saom_stats.hip -- statistics of packed networks and behaviour vectors (gfx950).
+//
+// net_stats_kernel restates cusna_network_stats (cran/cusna
+// src/libcusna_saom_cpu.cpp:606-689) on u64 bit rows, one workgroup per
+// network. Integer-valued statistics (density, recip, transTrip, cycle3,
+// transTies, transRecTrip, inPop, outAct, reciAct, transTriads) are exact
+// int64 counts from popcounts and out-neighbour walks, which equal the
+// reference's double sums of integers. Statistics that sum non-integers are
+// accumulated by one lane in the reference's order (actors ascending; ties
+// row-major), so they round exactly as the reference does.
+//
+// beh_stats_kernel restates cusna_behavior_stats (:767-817).
+#include <hip/hip_runtime.h>
+
+#include "saom_kernels.h"
+
+namespace saom {
+namespace {
+
+constexpr int kStatThreads = 256;
+
+__device__ __forceinline__ int bit_of(const uint64_t* row, int j) {
+  return (int)((row[j >> 6] >> (j & 63)) & 1);
+}
+
+// Degree statistic term of outPop/inAct families (:608-616).
+__device__ __forceinline__ double degree_term(int t, double pm, double cur, double start) {
+  if (t == 8 || t == 9) {
+    if (pm >= 1.0) return t == 9 ? sqrt(cur) : cur;
+    return pm == 0.0 ? start : cur + start;
+  }
+  if (pm >= 1.0) return t == 15 ? sqrt(cur) : cur;
+  const double s = pm == 0.0 ? start : cur + start;
+  return t == 15 ? sqrt(s) : s;
+}
+
+__device__ __forceinline__ long long block_sum_ll(long long v, long long* slots) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) slots[wv] = v;
+  __syncthreads();
+  long long s = 0;
+  for (int q = 0; q < (int)(blockDim.x >> 6); ++q) s += slots[q];
+  __syncthreads();
+  return s;
+}
+
+enum : int {
+  I_E = 0, I_RECIP, I_TT, I_C3, I_TTIES, I_TRT, I_INPOP, I_OUTACT, I_RECIACT, I_N
+};
+
+__global__ void __launch_bounds__(kStatThreads) net_stats_kernel(const StatParams P) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int n = P.n, W64 = P.W64, k = P.k;
+  const int tid = threadIdx.x, NT = blockDim.x;
+  const int net = blockIdx.x;
+  size_t o = 0;
+  uint64_t* rows = (uint64_t*)(smem + o);
+  o += (size_t)n * W64 * 8;
+  bool need_cols = false;
+  for (int e = 0; e < k; ++e)
+    if (P.tid[e] == 27 || P.tid[e] == 28 || P.tid[e] == 30) need_cols = true;
+  uint64_t* cols = (uint64_t*)(smem + o);
+  if (need_cols) o += (size_t)n * W64 * 8;
+  int* indeg = (int*)(smem + o);
+  o += (size_t)n * 4;
+  int* outdeg = (int*)(smem + o);
+  o += (size_t)n * 4;
+  o = (o + 7) & ~(size_t)7;
+  long long* slots = (long long*)(smem + o);
+
+  const uint64_t* src = P.rows + (size_t)net * n * W64;
+  for (int q = tid; q < n * W64; q += NT) {
+    uint64_t v = src[q];
+    if (P.keep) v &= P.keep[q];
+    if (P.force) v |= P.force[q];
+    rows[q] = v;
+    if (need_cols) cols[q] = 0ull;
+  }
+  for (int q = tid; q < n; q += NT) indeg[q] = 0;
+  __syncthreads();
+  for (int h = tid; h < n; h += NT) {
+    int od = 0;
+    for (int wd = 0; wd < W64; ++wd) {
+      uint64_t m = rows[(size_t)h * W64 + wd];
+      od += __popcll(m);
+      while (m) {
+        const int j = wd * 64 + __ffsll((long long)m) - 1;
+        m &= m - 1;
+        atomicAdd(&indeg[j], 1);
+        if (need_cols) atomicOr(&cols[(size_t)j * W64 + (h >> 6)], 1ull << (h & 63));
+      }
+    }
+    outdeg[h] = od;
+  }
+  __syncthreads();
+
+  // integer statistics, actors split over the workgroup
+  long long acc[I_N];
+  for (int q = 0; q < I_N; ++q) acc[q] = 0;
+  for (int i = tid; i < n; i += NT) {
+    const uint64_t* ri = rows + (size_t)i * W64;
+    const long long od = outdeg[i], idg = indeg[i];
+    acc[I_E] += od;
+    acc[I_INPOP] += idg * idg;
+    acc[I_OUTACT] += od * od;
+    long long rd = 0;
+    for (int wd = 0; wd < W64; ++wd) {
+      uint64_t m = ri[wd];
+      while (m) {
+        const int h = wd * 64 + __ffsll((long long)m) - 1;
+        m &= m - 1;
+        const uint64_t* rh = rows + (size_t)h * W64;
+        const int hi = bit_of(rh, i);  // a_hi
+        rd += hi;
+        int tt = 0, c3 = 0, trt = 0;
+        for (int w2 = 0; w2 < W64; ++w2) tt += __popcll(ri[w2] & rh[w2]);
+        // cycle3: j in out(h) with a_ji; transRecTrip: j in out(h) & out(i) with a_ji
+        for (int w2 = 0; w2 < W64; ++w2) {
+          uint64_t m2 = rh[w2];
+          while (m2) {
+            const int j = w2 * 64 + __ffsll((long long)m2) - 1;
+            m2 &= m2 - 1;
+            const int ji = bit_of(rows + (size_t)j * W64, i);
+            c3 += ji;
+            if (ji && bit_of(ri, j)) trt++;
+          }
+        }
+        acc[I_TT] += tt;
+        acc[I_C3] += c3;
+        acc[I_TRT] += trt;
+      }
+    }
+    acc[I_RECIP] += rd;
+    acc[I_RECIACT] += od * rd;
+    // transTies: j in out(i) reachable by a two-path from i
+    int tties = 0;
+    for (int wd = 0; wd < W64; ++wd) {
+      uint64_t reach = 0;
+      uint64_t m = ri[wd];
+      (void)m;
+      for (int w2 = 0; w2 < W64; ++w2) {
+        uint64_t mm = ri[w2];
+        while (mm) {
+          const int h = w2 * 64 + __ffsll((long long)mm) - 1;
+          mm &= mm - 1;
+          reach |= rows[(size_t)h * W64 + wd];
+        }
+      }
+      tties += __popcll(ri[wd] & reach);
+    }
+    acc[I_TTIES] += tties;
+  }
+  long long tot[I_N];
+  for (int q = 0; q < I_N; ++q) tot[q] = block_sum_ll(acc[q], slots);
+
+  // per effect; order-sensitive sums on one lane
+  if (tid == 0) {
+    double* out = P.out + (size_t)net * k;
+    for (int e = 0; e < k; ++e) {
+      const int t = P.tid[e];
+      const double pm = P.p1[e], q2 = P.p2[e];
+      const double* wr = P.wtab + (size_t)e * (n + 1);
+      const bool dyn = P.dyn[e] != 0;
+      const double* vrow = P.vmat + (size_t)e * n;
+      const uint8_t* mrow = P.vmiss ? P.vmiss + (size_t)e * n : nullptr;
+      auto V = [&](int i) -> double { return dyn ? (P.vb ? P.vb[i] : 0.0) : vrow[i]; };
+      auto MISS = [&](int i) -> bool { return mrow && mrow[i]; };
+      double s = 0.0;
+      switch (t) {
+        case 0:
+          s = (double)tot[I_E];
+          if (P.symflag) s *= 0.5;
+          break;
+        case 1:
+          s = (double)tot[I_RECIP];
+          break;
+        case 2:
+          s = (double)tot[I_TT];
+          break;
+        case 3:
+          s = (double)tot[I_C3] / 3.0;
+          break;
+        case 23:
+          s = (double)tot[I_TT] / 6.0;
+          break;
+        case 5:
+          s = (double)tot[I_TTIES];
+          break;
+        case 6:
+          s = (double)tot[I_TRT];
+          break;
+        case 4:
+          s = (double)tot[I_INPOP];
+          break;
+        case 16:
+          s = (double)tot[I_OUTACT];
+          break;
+        case 19:
+          s = (double)tot[I_RECIACT];
+          break;
+        case 7:
+          for (int j = 0; j < n; ++j) s += pow_three_halves_libm((double)indeg[j]);
+          break;
+        case 17:
+          for (int i = 0; i < n; ++i) s += pow_three_halves_libm((double)outdeg[i]);
+          break;
+        case 8:
+        case 9:
+          for (int i = 0; i < n; ++i)
+            s += (double)indeg[i] *
+                 degree_term(t, pm, (double)outdeg[i], P.od0 ? (double)P.od0[i] : 0.0);
+          break;
+        case 14:
+        case 15:
+          for (int i = 0; i < n; ++i)
+            s += (double)outdeg[i] *
+                 degree_term(t, pm, (double)indeg[i], P.id0 ? (double)P.id0[i] : 0.0);
+          break;
+        case 18:
+          for (int i = 0; i < n; ++i) s += fmin((double)outdeg[i], pm);
+          break;
+        case 10:
+          for (int i = 0; i < n; ++i)
+            if (!MISS(i)) s += (double)outdeg[i] * V(i);
+          break;
+        case 11:
+          for (int i = 0; i < n; ++i)
+            if (!MISS(i)) s += (double)indeg[i] * V(i);
+          break;
+        case 12: case 13: case 20: case 21: case 24: case 25: case 26: case 27: case 28:
+        case 29: case 30: case 31: {
+          // sums over ties (i, j) in row-major order
+          const bool all_pairs = (t == 29 || t == 30);
+          for (int i = 0; i < n; ++i) {
+            const uint64_t* ri = rows + (size_t)i * W64;
+            for (int wd = 0; wd < W64; ++wd) {
+              uint64_t m;
+              if (!all_pairs) {
+                m = ri[wd];
+              } else if (wr[0] != 0.0) {
+                m = ~0ull;  // every pair contributes
+                if (wd == W64 - 1 && (n & 63)) m &= (1ull << (n & 63)) - 1;
+              } else if (t == 29) {  // pairs with a two-path i->h->j
+                m = 0;
+                for (int w2 = 0; w2 < W64; ++w2) {
+                  uint64_t mm = ri[w2];
+                  while (mm) {
+                    const int h = w2 * 64 + __ffsll((long long)mm) - 1;
+                    mm &= mm - 1;
+                    m |= rows[(size_t)h * W64 + wd];
+                  }
+                }
+              } else {  // t == 30: pairs sharing an out-neighbour h
+                m = 0;
+                for (int w2 = 0; w2 < W64; ++w2) {
+                  uint64_t mm = ri[w2];
+                  while (mm) {
+                    const int h = w2 * 64 + __ffsll((long long)mm) - 1;
+                    mm &= mm - 1;
+                    m |= cols[(size_t)h * W64 + wd];
+                  }
+                }
+              }
+              while (m) {
+                const int j = wd * 64 + __ffsll((long long)m) - 1;
+                m &= m - 1;
+                const uint64_t* rj = rows + (size_t)j * W64;
+                switch (t) {
+                  case 12:
+                    if (!MISS(i) && !MISS(j)) s += 1.0 - fabs(V(i) - V(j)) * pm - q2;
+                    break;
+                  case 13:
+                    if (!MISS(i) && !MISS(j)) s += fabs(V(i) - V(j)) < 1e-9 ? 1.0 : 0.0;
+                    break;
+                  case 20: {
+                    double x;
+                    if (MISS(i) || MISS(j)) {
+                      x = 0.5;
+                    } else {
+                      const double dd = V(i) - V(j);
+                      x = (dd > 1e-9 ? 1.0 : 0.0) + 0.5 * (fabs(dd) <= 1e-9 ? 1.0 : 0.0);
+                    }
+                    s += x;
+                    break;
+                  }
+                  case 21:
+                    if (!MISS(i) && !MISS(j)) s += V(i) * V(j);
+                    break;
+                  case 25:
+                    if (!MISS(i) && !MISS(j)) s += V(i) + V(j);
+                    break;
+                  case 31:
+                    if (!MISS(j) && V(j) >= pm - 1e-12) s += 1.0;
+                    break;
+                  case 24:
+                  case 29: {
+                    if (t == 29 && i == j) break;
+                    int c = 0;  // two-paths i->h->j
+                    for (int w2 = 0; w2 < W64; ++w2) {
+                      uint64_t mm = ri[w2];
+                      while (mm) {
+                        const int h = w2 * 64 + __ffsll((long long)mm) - 1;
+                        mm &= mm - 1;
+                        c += bit_of(rows + (size_t)h * W64, j);
+                      }
+                    }
+                    s += wr[c];
+                    break;
+                  }
+                  case 26:
+                  case 30: {
+                    if (t == 30 && i == j) break;
+                    int c = 0;
+                    for (int w2 = 0; w2 < W64; ++w2) c += __popcll(ri[w2] & rj[w2]);
+                    s += wr[c];
+                    break;
+                  }
+                  case 27: {
+                    int c = 0;
+                    for (int w2 = 0; w2 < W64; ++w2)
+                      c += __popcll(cols[(size_t)i * W64 + w2] & cols[(size_t)j * W64 + w2]);
+                    s += wr[c];
+                    break;
+                  }
+                  case 28: {
+                    int c = 0;
+                    for (int w2 = 0; w2 < W64; ++w2)
+                      c += __popcll(cols[(size_t)i * W64 + w2] & rj[w2]);
+                    s += wr[c];
+                    break;
+                  }
+                  default:
+                    break;
+                }
+              }
+            }
+          }
+          break;
+        }
+        default:
+          break;  // X (22) and interactions (32) have no statistic (:684)
+      }
+      out[e] = s;
+    }
+  }
+}
+
+// Behaviour statistics (:767-817), one workgroup per behaviour vector.
+__global__ void __launch_bounds__(kStatThreads) beh_stats_kernel(const BehStatParams P) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int n = P.n, W64 = P.W64;
+  const int tid = threadIdx.x, NT = blockDim.x;
+  const int c = blockIdx.x;
+  double* u = (double*)smem;
+  double* term = u + n;
+  double* okd = term + n;
+  double* od = okd + n;
+  double* nb = od + n;
+  const int32_t* zc = P.z + (size_t)c * n;
+  for (int i = tid; i < n; i += NT) {
+    okd[i] = (P.bkdist && !P.bkdist[i]) ? 0.0 : 1.0;
+  }
+  __syncthreads();
+  for (int i = tid; i < n; i += NT) {
+    u[i] = okd[i] != 0.0 ? ((double)zc[i] - P.zbar) : 0.0;
+    double d = 0.0, b = 0.0;
+    const uint64_t* xr = P.xnet + (size_t)i * W64;
+    for (int wd = 0; wd < W64; ++wd) {
+      uint64_t m = xr[wd];
+      while (m) {
+        const int j = wd * 64 + __ffsll((long long)m) - 1;
+        m &= m - 1;
+        d += 1.0;
+        b += okd[j];
+      }
+    }
+    od[i] = d;
+    nb[i] = b;
+  }
+  __syncthreads();
+  for (int e = 0; e < P.kb; ++e) {
+    const int t = P.bty[e];
+    if (t == 2 || t == 3 || t == 4) {
+      for (int i = tid; i < n; i += NT) {
+        const uint64_t* xr = P.xnet + (size_t)i * W64;
+        double x = 0.0;
+        bool skip;
+        if (t == 2) {
+          skip = od[i] <= 0.0;
+          if (!skip) {
+            for (int wd = 0; wd < W64; ++wd) {
+              uint64_t m = xr[wd];
+              while (m) {
+                const int j = wd * 64 + __ffsll((long long)m) - 1;
+                m &= m - 1;
+                x += u[j];
+              }
+            }
+            x = u[i] * (x / od[i]);
+          }
+        } else {
+          skip = okd[i] == 0.0;
+          if (!skip) {
+            for (int wd = 0; wd < W64; ++wd) {
+              uint64_t m = xr[wd];
+              while (m) {
+                const int j = wd * 64 + __ffsll((long long)m) - 1;
+                m &= m - 1;
+                if (okd[j] != 0.0) x += 1.0 - fabs(u[i] - u[j]) / P.brange;
+              }
+            }
+            x -= nb[i] * P.bsim;
+            if (t == 3) x = nb[i] > 0.0 ? x / nb[i] : 0.0;
+          }
+        }
+        // A skipped actor adds nothing; adding +0.0 instead is bit-identical
+        // because the running sum starts at +0.0 and can never become -0.0.
+        term[i] = skip ? 0.0 : x;
+      }
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double s = 0.0;
+      if (t == 0) {
+        for (int i = 0; i < n; ++i) s += u[i];
+      } else if (t == 1) {
+        for (int i = 0; i < n; ++i) s += u[i] * u[i];
+      } else if (t == 2 || t == 3 || t == 4) {
+        for (int i = 0; i < n; ++i) s += term[i];
+      }
+      P.out[(size_t)c * P.kb + e] = s;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void expand_dense_kernel(const uint64_t* rows, int n, int W64, long long total,
+                                    uint8_t* dense) {
+  const long long nn = (long long)n * n;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (long long)gridDim.x * blockDim.x) {
+    const long long c = q / nn;
+    const long long rem = q - c * nn;
+    const int i = (int)(rem / n), j = (int)(rem - (long long)i * n);
+    dense[q] = (uint8_t)((rows[(c * n + i) * W64 + (j >> 6)] >> (j & 63)) & 1);
+  }
+}
+
+__global__ void philox_kernel(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    philox10(c0, c1, c2, c3, key[0], key[1]);
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+  }
+}
+
+}  // namespace
+
+int launch_net_stats(const StatParams& P, void* stream) {
+  bool need_cols = false;
+  for (int e = 0; e < P.k; ++e)
+    if (P.tid[e] == 27 || P.tid[e] == 28 || P.tid[e] == 30) need_cols = true;
+  const size_t lds = net_stats_lds(P.n, P.W64, need_cols);
+  hipError_t e = hipFuncSetAttribute((const void*)net_stats_kernel,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return (int)e;
+  hipLaunchKernelGGL(net_stats_kernel, dim3(P.M), dim3(kStatThreads), lds, (hipStream_t)stream,
+                     P);
+  return (int)hipGetLastError();
+}
+
+size_t net_stats_lds(int n, int W64, bool need_cols) {
+  return (size_t)n * W64 * 8 * (need_cols ? 2 : 1) + (size_t)n * 8 + 8 + 64 * 8;
+}
+
+int launch_beh_stats(const BehStatParams& P, void* stream) {
+  const size_t lds = (size_t)P.n * 5 * 8;
+  hipError_t e = hipFuncSetAttribute((const void*)beh_stats_kernel,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return (int)e;
+  hipLaunchKernelGGL(beh_stats_kernel, dim3(P.M), dim3(kStatThreads), lds, (hipStream_t)stream,
+                     P);
+  return (int)hipGetLastError();
+}
+
+int launch_expand_dense(const uint64_t* rows, int n, int W64, int M, uint8_t* dense,
+                        void* stream) {
+  const long long total = (long long)M * n * n;
+  long long blocks = (total + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(expand_dense_kernel, dim3((unsigned)blocks), dim3(256), 0,
+                     (hipStream_t)stream, rows, n, W64, total, dense);
+  return (int)hipGetLastError();
+}
+
+int launch_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out, void* stream) {
+  hipLaunchKernelGGL(philox_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, ctr, key, out);
+  return (int)hipGetLastError();
+}
+
+}  // namespace saom

@@ -1,0 +1,248 @@
+"""Loader and thin Python bindings of libcusna_amd.so (include/libcusna.h).
+
+The library is the product: every function here calls into it. There is no
+CPU fallback; if the library is missing, ``load()`` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .abi import CusnaError, SaomProblem, SaomResult, pack_rows, run_period
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "libcusna_amd.so")
+_vp = C.c_void_p
+
+# Every symbol include/libcusna.h declares.
+EXPORTS = (
+    "cusna_saom_simulate_period", "cusna_saom_simulate_period_replay", "cusna_pack_bits",
+    "cusna_philox4x32", "cusna_network_stats", "cusna_network_stats_packed",
+    "cusna_behavior_stats", "cusna_abi_version", "cusna_version_string", "cusna_has_cuda",
+    "cusna_last_error", "cusna_period_plan_create", "cusna_period_plan_set_params",
+    "cusna_period_plan_run", "cusna_period_plan_sync", "cusna_period_plan_fetch",
+    "cusna_period_plan_kernel_ms", "cusna_period_plan_shape", "cusna_period_plan_destroy",
+)
+
+_lib = None
+
+
+def load(path: str | None = None) -> C.CDLL:
+    """Load the gfx950 library (built by ``cusna_amd.build``); raise if absent."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(f"{p} is missing: build it with `python -m cusna_amd.build` "
+                           "(the GPU library has no CPU fallback)")
+    L = C.CDLL(p)
+    L.cusna_saom_simulate_period.restype = C.c_int
+    L.cusna_saom_simulate_period.argtypes = [_vp, _vp]
+    L.cusna_saom_simulate_period_replay.restype = C.c_int
+    L.cusna_saom_simulate_period_replay.argtypes = [_vp, _vp, _vp, _vp, C.c_int32]
+    L.cusna_pack_bits.restype = C.c_int
+    L.cusna_pack_bits.argtypes = [_vp, C.c_int32, _vp, _vp, _vp]
+    L.cusna_philox4x32.restype = C.c_int
+    L.cusna_philox4x32.argtypes = [_vp, _vp, _vp]
+    L.cusna_network_stats.restype = C.c_int
+    L.cusna_network_stats.argtypes = [_vp, C.c_int32, C.c_int32] + [_vp] * 10 + [C.c_int32, _vp]
+    L.cusna_network_stats_packed.restype = C.c_int
+    L.cusna_network_stats_packed.argtypes = [_vp, C.c_int32, C.c_int32, C.c_int32] + [_vp] * 10 + [
+        C.c_int32, _vp, _vp, _vp]
+    L.cusna_behavior_stats.restype = C.c_int
+    L.cusna_behavior_stats.argtypes = [_vp, C.c_int32, C.c_int32, C.c_int32, _vp, _vp, _vp,
+                                       C.c_double, C.c_double, C.c_double, _vp]
+    L.cusna_abi_version.restype = C.c_int32
+    L.cusna_version_string.restype = C.c_char_p
+    L.cusna_has_cuda.restype = C.c_int32
+    L.cusna_last_error.restype = C.c_char_p
+    L.cusna_period_plan_create.restype = C.c_int
+    L.cusna_period_plan_create.argtypes = [_vp, C.c_int32, C.POINTER(_vp)]
+    L.cusna_period_plan_set_params.restype = C.c_int
+    L.cusna_period_plan_set_params.argtypes = [_vp, _vp, _vp, _vp, _vp]
+    L.cusna_period_plan_run.restype = C.c_int
+    L.cusna_period_plan_run.argtypes = [_vp, C.c_uint64, C.c_int32]
+    L.cusna_period_plan_sync.restype = C.c_int
+    L.cusna_period_plan_sync.argtypes = [_vp]
+    L.cusna_period_plan_fetch.restype = C.c_int
+    L.cusna_period_plan_fetch.argtypes = [_vp, _vp]
+    L.cusna_period_plan_kernel_ms.restype = C.c_double
+    L.cusna_period_plan_kernel_ms.argtypes = [_vp]
+    L.cusna_period_plan_shape.restype = C.c_int32
+    L.cusna_period_plan_shape.argtypes = [_vp]
+    L.cusna_period_plan_destroy.restype = None
+    L.cusna_period_plan_destroy.argtypes = [_vp]
+    if path is None:
+        _lib = L
+    return L
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def _check(name: str, st: int) -> None:
+    if st != 0:
+        detail = load().cusna_last_error().decode(errors="replace")
+        raise CusnaError(name, st, detail)
+
+
+def has_device() -> bool:
+    return bool(load().cusna_has_cuda())
+
+
+def simulate_period(prob: SaomProblem, want_end_nets: bool = False) -> dict:
+    """One period for prob.M chains on the GPU (cusna_saom_simulate_period)."""
+    L = load()
+    out = run_period(L.cusna_saom_simulate_period, prob, want_end_nets, "cusna_saom_simulate_period")
+    return out
+
+
+def simulate_period_replay(prob: SaomProblem, log_u0: np.ndarray, loglog_u: np.ndarray,
+                           want_end_nets: bool = False) -> dict:
+    L = load()
+    lu0 = np.ascontiguousarray(log_u0, np.float64)
+    llu = np.ascontiguousarray(loglog_u, np.float64)
+    M, smax = lu0.shape
+    if llu.shape != (M, smax, prob.n) or M != prob.M:
+        raise ValueError("replay arrays must be (M, smax) and (M, smax, n)")
+    return run_period(L.cusna_saom_simulate_period_replay, prob, want_end_nets,
+                      "cusna_saom_simulate_period_replay",
+                      extra_args=(_p(lu0), _p(llu), int(smax)))
+
+
+def network_stats(a, type_ids, p1=None, p2=None, vmat=None, vmiss=None, dyn_flags=None, vb=None,
+                  od0=None, id0=None, wtab=None, symflag=0) -> np.ndarray:
+    """Statistics of one dense network (cusna_network_stats)."""
+    a = np.ascontiguousarray(np.asarray(a) != 0, np.uint8)
+    n = a.shape[0]
+    t = np.ascontiguousarray(type_ids, np.int32)
+    k = len(t)
+    arr = lambda x, dt, shape: None if x is None else np.ascontiguousarray(
+        np.asarray(x, dt).reshape(shape))
+    p1 = arr(p1 if p1 is not None else np.zeros(k), np.float64, (k,))
+    p2 = arr(p2 if p2 is not None else np.zeros(k), np.float64, (k,))
+    vmat = arr(vmat if vmat is not None else np.zeros((k, n)), np.float64, (k, n))
+    wtab = arr(wtab if wtab is not None else np.zeros((k, n + 1)), np.float64, (k, n + 1))
+    vmiss = arr(None if vmiss is None else (np.asarray(vmiss) != 0), np.uint8, (k, n))
+    dyn = arr(dyn_flags, np.int32, (k,))
+    vb = arr(vb, np.float64, (n,))
+    od0 = arr(od0, np.int32, (n,))
+    id0 = arr(id0, np.int32, (n,))
+    out = np.zeros(k, np.float64)
+    st = load().cusna_network_stats(_p(a), n, k, _p(t), _p(p1), _p(p2), _p(vmat), _p(vmiss),
+                                    _p(dyn), _p(vb), _p(od0), _p(id0), _p(wtab), int(symflag),
+                                    _p(out))
+    _check("cusna_network_stats", st)
+    return out
+
+
+def network_stats_packed(nets, type_ids, p1=None, p2=None, vmat=None, vmiss=None,
+                         dyn_flags=None, vb=None, od0=None, id0=None, wtab=None, symflag=0,
+                         keep=None, force=None) -> np.ndarray:
+    """Statistics of M networks (M, n, n) in one launch (cusna_network_stats_packed)."""
+    nets = np.asarray(nets)
+    M, n, _ = nets.shape
+    rows = np.ascontiguousarray(np.stack([pack_rows(nets[c]) for c in range(M)]), np.uint32)
+    t = np.ascontiguousarray(type_ids, np.int32)
+    k = len(t)
+    f64 = lambda x, shape: np.ascontiguousarray(np.asarray(x, np.float64).reshape(shape))
+    p1 = f64(p1 if p1 is not None else np.zeros(k), (k,))
+    p2 = f64(p2 if p2 is not None else np.zeros(k), (k,))
+    vmat = f64(vmat if vmat is not None else np.zeros((k, n)), (k, n))
+    wtab = f64(wtab if wtab is not None else np.zeros((k, n + 1)), (k, n + 1))
+    vm = None if vmiss is None else np.ascontiguousarray(np.asarray(vmiss) != 0, np.uint8)
+    dyn = None if dyn_flags is None else np.ascontiguousarray(dyn_flags, np.int32)
+    vbb = None if vb is None else f64(vb, (n,))
+    o0 = None if od0 is None else np.ascontiguousarray(od0, np.int32)
+    i0 = None if id0 is None else np.ascontiguousarray(id0, np.int32)
+    kp = None if keep is None else pack_rows(keep)
+    fc = None if force is None else pack_rows(force)
+    out = np.zeros((M, k), np.float64)
+    st = load().cusna_network_stats_packed(_p(rows), n, M, k, _p(t), _p(p1), _p(p2), _p(vmat),
+                                           _p(vm), _p(dyn), _p(vbb), _p(o0), _p(i0), _p(wtab),
+                                           int(symflag), _p(kp), _p(fc), _p(out))
+    _check("cusna_network_stats_packed", st)
+    return out
+
+
+def behavior_stats(z, bty, xnet, bkdist, zbar, brange, bsimmean) -> np.ndarray:
+    """Behaviour statistics of M behaviour vectors (cusna_behavior_stats)."""
+    z = np.ascontiguousarray(np.atleast_2d(z), np.int32)
+    M, n = z.shape
+    bty = np.ascontiguousarray(bty, np.int32)
+    xnet = np.ascontiguousarray(np.asarray(xnet) != 0, np.uint8)
+    bk = None if bkdist is None else np.ascontiguousarray(bkdist, np.int32)
+    out = np.zeros((M, len(bty)), np.float64)
+    st = load().cusna_behavior_stats(_p(z), n, M, len(bty), _p(bty), _p(xnet), _p(bk),
+                                     float(zbar), float(brange), float(bsimmean), _p(out))
+    _check("cusna_behavior_stats", st)
+    return out
+
+
+def philox4x32(counter, key) -> np.ndarray:
+    c = np.ascontiguousarray(counter, np.uint32)
+    k = np.ascontiguousarray(key, np.uint32)
+    out = np.zeros(4, np.uint32)
+    _check("cusna_philox4x32", load().cusna_philox4x32(_p(c), _p(k), _p(out)))
+    return out
+
+
+class PeriodPlan:
+    """Device-resident period problem (cusna_period_plan_*)."""
+
+    def __init__(self, prob: SaomProblem, want_end_nets: bool = False):
+        self.L = load()
+        self.prob = prob
+        self._c = prob.build()
+        self.h = _vp()
+        _check("cusna_period_plan_create",
+               self.L.cusna_period_plan_create(C.byref(self._c), int(want_end_nets),
+                                               C.byref(self.h)))
+        self.want_end_nets = want_end_nets
+
+    def set_params(self, theta, lam, thetab=None, lamb=None) -> None:
+        M, k = self.prob.M, self.prob.k
+        th = np.ascontiguousarray(np.broadcast_to(np.asarray(theta, np.float32), (M, k)))
+        lm = np.ascontiguousarray(np.broadcast_to(np.asarray(lam, np.float32), (M,)))
+        thb = lmb = None
+        if self.prob.kb:
+            thb = np.ascontiguousarray(np.broadcast_to(np.asarray(thetab, np.float32),
+                                                       (M, self.prob.kb)))
+            lmb = np.ascontiguousarray(np.broadcast_to(np.asarray(lamb, np.float32), (M,)))
+        _check("cusna_period_plan_set_params",
+               self.L.cusna_period_plan_set_params(self.h, _p(th), _p(lm), _p(thb), _p(lmb)))
+
+    def run(self, seed: int, want_stats: bool = True) -> None:
+        _check("cusna_period_plan_run",
+               self.L.cusna_period_plan_run(self.h, int(seed) & 0xFFFFFFFFFFFFFFFF,
+                                            int(want_stats)))
+
+    def sync(self) -> None:
+        _check("cusna_period_plan_sync", self.L.cusna_period_plan_sync(self.h))
+
+    def fetch(self, want_end_nets: bool = False) -> dict:
+        res = SaomResult(self.prob.n, self.prob.k, self.prob.M, self.prob.kb, want_end_nets)
+        _check("cusna_period_plan_fetch", self.L.cusna_period_plan_fetch(self.h, C.byref(res.c)))
+        return res.as_dict()
+
+    def kernel_ms(self) -> float:
+        return float(self.L.cusna_period_plan_kernel_ms(self.h))
+
+    @property
+    def block_threads(self) -> int:
+        return int(self.L.cusna_period_plan_shape(self.h))
+
+    def close(self) -> None:
+        if self.h:
+            self.L.cusna_period_plan_destroy(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,118 @@
+"""Synthetic SAOM workloads of the benchmark configurations.
+
+Each builder returns the inputs of one period call (a ``SaomProblem``) from a
+fixed seed, using the same marshalling as the R glue (float64 descriptors,
+NULL masks, float32 theta/lambda):
+
+  s50_like_panel   50 actors, 3 waves: wave 1 Poisson(2.3) out-ties capped at
+                   6, reciprocated with p = 0.4; waves 2-3 simulated.
+  c2_problem       500 actors, directed, rate 8 per period, mean outdegree 5;
+                   density -2.8, recip 1.8, transTrip 0.3, cycle3 -0.1,
+                   egoX/altX/simX 0.1/0.1/0.6 on a covariate uniform on 1..5.
+  c4_problem       300 actors with an ordinal behaviour 1..5 = Binomial(4, 0.4)+1;
+                   network density/recip/transTrip/simX(behaviour), behaviour
+                   rate 4, linear -0.2, quad -0.3, avAlt 0.8.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .abi import SaomProblem
+from .data import build_saom_data
+from .effects import BehEffect, Effect, prepare_effects
+
+C2_THETA = [-2.8, 1.8, 0.3, -0.1, 0.1, 0.1, 0.6]
+C2_EFFECTS = ["density", "recip", "transTrip", "cycle3", "egoX", "altX", "simX"]
+
+
+def bernoulli_network(n: int, mean_outdeg: float, rng: np.random.Generator) -> np.ndarray:
+    """Directed Bernoulli graph with P(tie) = mean_outdeg / (n - 1), no loops."""
+    a = (rng.random((n, n)) < mean_outdeg / (n - 1)).astype(np.uint8)
+    np.fill_diagonal(a, 0)
+    return a
+
+
+def poisson_capped_network(n: int, lam: float, cap: int, p_recip: float,
+                           rng: np.random.Generator) -> np.ndarray:
+    """Each actor sends min(Poisson(lam), cap) ties to uniform alters; each tie
+    is reciprocated with probability p_recip."""
+    a = np.zeros((n, n), dtype=np.uint8)
+    for i in range(n):
+        d = min(int(rng.poisson(lam)), cap)
+        alters = rng.choice(np.delete(np.arange(n), i), size=d, replace=False)
+        a[i, alters] = 1
+    ii, jj = np.nonzero(a)
+    back = rng.random(len(ii)) < p_recip
+    a[jj[back], ii[back]] = 1
+    np.fill_diagonal(a, 0)
+    return a
+
+
+def effect_problem(x0: np.ndarray, effects: list, theta, lam: float, M: int, seed: int,
+                   beh=None, z0=None, beh_effects=None, thetab=None, lamb: float = 0.0,
+                   conditional: bool = False, target_dist: int = 0) -> SaomProblem:
+    """SaomProblem for a start network and an effect list (descriptors as the
+    R layer prepares them for period 1)."""
+    n = x0.shape[0]
+    z_start = None if z0 is None else np.asarray(z0, float)
+    d = prepare_effects(effects, n, 1, beh, z_start)
+    prob = SaomProblem(
+        x0=x0, theta=np.asarray(theta, float), lam=lam, type_ids=d.type_ids, M=M, seed=seed,
+        etypes=d.etypes, p1=d.p1, p2=d.p2, vmat=d.vmat, vmiss=d.vmiss, dyn_flags=d.dyn_flags,
+        dyc_idx=d.dyc_idx, wtab=d.wtab, int_comp=d.int_comp, vb=d.v_dyn,
+        conditional=1 if conditional else 0, target_dist=target_dist)
+    if beh_effects:
+        prob.kb = len(beh_effects)
+        prob.lamb = lamb
+        prob.thetab = np.asarray(thetab, float)
+        prob.bty = np.array([e.type_id for e in beh_effects], np.int32)
+        prob.z0 = np.asarray(z0, np.int32)
+        prob.bkdist = np.ones(n, np.int32)
+        prob.zmin, prob.zmax = beh.zmin, beh.zmax
+        prob.zbar, prob.brange, prob.bsimmean = beh.overall_mean, beh.range, beh.sim_mean
+    return prob
+
+
+def c2_covariate(n: int = 500, seed: int = 2) -> np.ndarray:
+    rng = np.random.default_rng(seed + 1000)
+    return rng.integers(1, 6, size=n).astype(float)
+
+
+def c2_effects(x: np.ndarray) -> list:
+    return [Effect("density"), Effect("recip"), Effect("transTrip"), Effect("cycle3"),
+            Effect("egoX", x), Effect("altX", x), Effect("simX", x)]
+
+
+def c2_problem(M: int = 4096, n: int = 500, seed: int = 2, x0: np.ndarray | None = None,
+               period_seed: int = 11) -> SaomProblem:
+    """Config-2 period problem from wave 1 (or from the given start network)."""
+    rng = np.random.default_rng(seed)
+    w1 = bernoulli_network(n, 5.0, rng)
+    x = c2_covariate(n, seed)
+    start = w1 if x0 is None else x0
+    return effect_problem(start, c2_effects(x), C2_THETA, 8.0, M, period_seed)
+
+
+def c2_waves(simulate, n: int = 500, seed: int = 2) -> list:
+    """Waves 1 and 2 of config 2; wave 2 is one seeded simulated period from
+    wave 1 (``simulate`` runs a SaomProblem and returns a_ends)."""
+    rng = np.random.default_rng(seed)
+    w1 = bernoulli_network(n, 5.0, rng)
+    prob = c2_problem(M=1, n=n, seed=seed, x0=w1, period_seed=seed + 7)
+    w2 = simulate(prob)[0]
+    return [w1, np.asarray(w2, np.uint8)]
+
+
+def c4_problem(M: int = 2048, n: int = 300, seed: int = 4, period_seed: int = 13) -> SaomProblem:
+    """Config-4 co-evolution period problem (period 1 from wave 1)."""
+    rng = np.random.default_rng(seed)
+    w1 = bernoulli_network(n, 4.0, rng)
+    z = rng.binomial(4, 0.4, size=(n, 3)) + 1
+    w2 = bernoulli_network(n, 4.0, rng)
+    d = build_saom_data([w1, w2, w2], behavior=z.astype(float))
+    effects = [Effect("density"), Effect("recip"), Effect("transTrip"),
+               Effect("simX", dyn=True)]
+    beh_effects = [BehEffect("linear"), BehEffect("quad"), BehEffect("avAlt")]
+    return effect_problem(w1, effects, [-2.6, 1.6, 0.3, 0.8], 6.0, M, period_seed, beh=d.beh,
+                          z0=d.beh.imputed[:, 0], beh_effects=beh_effects,
+                          thetab=[-0.2, -0.3, 0.8], lamb=4.0)

@@ -233,6 +233,36 @@ CUSNA_API cusna_status cusna_network_stats_packed(const uint32_t *rows, int32_t 
                                                   const uint32_t *keep, const uint32_t *force,
                                                   double *out);
 
+/*
+ * Device-resident period plans. No counterpart in the reference, whose period
+ * call re-marshals everything every time (cusna_saom_glue.cpp:57-189); the
+ * reserved Layer-2 estimator types (libcusna.h:541-542) are where the
+ * reference meant such state to live. A plan uploads one problem's
+ * chain-invariant inputs once; each run launches the period kernels on the
+ * plan's own HIP stream and leaves the outputs in device memory until fetched.
+ *
+ *   create        validate + upload problem (problem->theta/lam used as the
+ *                 initial per-chain parameters); want_end_nets reserves the
+ *                 dense a_ends buffer
+ *   set_params    replace per-chain theta[M*k], lam[M] (thetab/lamb with kb>0)
+ *   run           launch one period with the given Philox seed (asynchronous)
+ *   sync / fetch  wait; copy outputs into a cusna_saom_result
+ *   kernel_ms     duration of the last period kernel (HIP events)
+ */
+typedef struct cusna_period_plan cusna_period_plan;
+CUSNA_API cusna_status cusna_period_plan_create(const cusna_saom_problem *problem,
+                                                int32_t want_end_nets, cusna_period_plan **plan);
+CUSNA_API cusna_status cusna_period_plan_set_params(cusna_period_plan *plan, const float *theta,
+                                                    const float *lam, const float *thetab,
+                                                    const float *lamb);
+CUSNA_API cusna_status cusna_period_plan_run(cusna_period_plan *plan, uint64_t seed,
+                                             int32_t want_stats);
+CUSNA_API cusna_status cusna_period_plan_sync(cusna_period_plan *plan);
+CUSNA_API cusna_status cusna_period_plan_fetch(cusna_period_plan *plan, cusna_saom_result *result);
+CUSNA_API double cusna_period_plan_kernel_ms(cusna_period_plan *plan);
+CUSNA_API int32_t cusna_period_plan_shape(const cusna_period_plan *plan);
+CUSNA_API void cusna_period_plan_destroy(cusna_period_plan *plan);
+
 #ifdef __cplusplus
 }
 #endif

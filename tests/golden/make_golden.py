@@ -1,0 +1,62 @@
+"""Writes the golden fixtures under tests/golden/ (run in a container that has
+the reference sources; the fixtures, not the reference, travel).
+
+  sim.json.gz       inputs and outputs of the reference's own period-simulator
+                    fixture (cran/cusna tests/testthat/fixtures/sim/sim.json.gz):
+                    s50 waves, alcohol covariate, three configurations x 8 chains
+  ref_cases.json.gz outputs of the reference engine compiled from its source
+                    (oracle/_ref/libcusna_ref.so) on synthetic inputs that the
+                    reference's fixtures do not cover: cycle3/inPop/outAct and
+                    every other effect id, behaviour avAlt/totSim, symmetric and
+                    conditional runs, and statistics of random networks
+  abi_offsets.txt   field offsets of the reference ABI structs
+  targets.json.gz, effects.json.gz, estimate.json.gz, data_s50.json.gz:
+                    copies of the reference's host-layer fixtures (inputs and
+                    expected outputs)
+"""
+import gzip
+import json
+import os
+import shutil
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+REF_FIX = "/root/reference/tests/testthat/fixtures"
+
+
+def dump(name, obj):
+    with gzip.open(os.path.join(HERE, name), "wt") as f:
+        json.dump(obj, f, separators=(",", ":"))
+
+
+def main():
+    import oracle as O
+    from ref_cases import build_cases
+
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+    assert O.reference() is not None, "oracle/_ref not built"
+    # 1. the reference's simulator fixture, verbatim data
+    with gzip.open(os.path.join(REF_FIX, "sim", "sim.json.gz"), "rt") as f:
+        dump("sim.json.gz", json.load(f))
+    for src, dst in [("targets/targets.json.gz", "targets.json.gz"),
+                     ("effects/effects.json.gz", "effects.json.gz"),
+                     ("estimate/estimate.json.gz", "estimate.json.gz"),
+                     ("data/s50.json.gz", "data_s50.json.gz")]:
+        shutil.copyfile(os.path.join(REF_FIX, src), os.path.join(HERE, dst))
+    # 2. reference-engine outputs on synthetic cases
+    dump("ref_cases.json.gz", build_cases(O))
+    # 3. ABI layout of the reference header
+    out = subprocess.run([os.path.join(ROOT, "oracle", "_ref", "abi_offsets_ref")],
+                         capture_output=True, text=True, check=True).stdout
+    with open(os.path.join(HERE, "abi_offsets.txt"), "w") as f:
+        f.write(out)
+
+
+if __name__ == "__main__":
+    main()
