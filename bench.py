@@ -1,0 +1,160 @@
+"""Benchmark: SAOM chain-periods per second at n = 500 on one MI355X.
+
+Workload (config 2): 500 directed actors, 4096 replicate chains, 2 periods,
+rate 8 per period; wave 1 Bernoulli with mean outdegree 5; effects density
+-2.8, recip 1.8, transTrip 0.3, cycle3 -0.1, egoX/altX/simX 0.1/0.1/0.6 on a
+covariate uniform on 1..5; wave 2 is one seeded simulated period. Choice in
+float64, statistics exact.
+
+A step is one pass over the batch: both periods for all 4096 chains (8192
+chain-periods), each period one launch of the period kernel plus the
+statistics kernel, from inputs already resident in device memory (period
+plans). The timed region is bracketed by device synchronisation.
+
+Also reported: the period kernel against its VALU roof (HIP events on the
+plan's stream), and the reference CPU engine (compiled from the reference's
+own source under oracle/_ref) timed on this host's cores on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("SAOM chain-periods/sec at n=500 (GPU vs host CPU); MoM wall time to "
+          "max|t|≤0.1")
+# MI355X: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz = 39.3e12 VALU lane-ops/s
+# (the FP64 vector peak of 78.6 TFLOP/s counts an FMA as two flops).
+VALU_PEAK_OPS = 256 * 4 * 16 * 2.4e9
+
+
+def algorithmic_ops_per_alternative(n: int, k: int) -> float:
+    """VALU operations one alternative evaluation needs, counted from the
+    algorithm (not from the compiled code):
+      Philox4x32-10: 10 rounds x (2 mul-hi + 2 mul-lo + 4 xor + 2 key add) = 100
+      two u64 AND + popcount features over W64 words: 8 lane-ops per word
+      linear predictor: k multiplies + k adds; change statistics ~2 per effect
+      Gumbel noise and softmax weight: 2 log + 1 exp, counted as 1 op each,
+      plus 4 flops (subtract, compare, scale)."""
+    w64 = (n + 63) // 64
+    return 100 + 8 * w64 + 4 * k + 3 + 4
+
+
+def cpu_baseline(prob_factory, threads: int, chains: int) -> dict:
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    lib = O.reference()
+    kind = "reference"
+    if lib is None:
+        lib, kind = O.oracle(), "port"
+    prob = prob_factory(chains)
+    setter = getattr(O.oracle(), "oracle_set_threads")
+    setter(threads)
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    t0 = time.perf_counter()
+    if kind == "reference":
+        r = O.simulate_reference(prob)
+    else:
+        r = O.simulate(prob)
+    wall = time.perf_counter() - t0
+    return {"value": round(chains / wall, 4), "unit": "chain-periods/s", "cores": threads,
+            "kind": kind,
+            "sample": (f"{chains} chains of period 1 of the same workload (n=500, rate 8, 7 "
+                       f"effects), OpenMP static schedule over chains, {threads} threads, "
+                       f"{wall:.1f} s wall, mean ministeps {float(r['steps'].mean()):.0f}")}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--chains", type=int, default=4096)
+    ap.add_argument("--n", type=int, default=500)
+    ap.add_argument("--threads", type=int, default=0, help="block size (0 = library default)")
+    ap.add_argument("--cpu-chains", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box in this setup has one GPU, and the "
+              "path shards over chains without cross-GPU traffic (see DESIGN.md).",
+              file=sys.stderr)
+        return 2
+
+    import numpy as np
+
+    from cusna_amd import lib
+    from cusna_amd.workloads import c2_problem, c2_waves
+
+    if not lib.has_device():
+        print("no HIP device", file=sys.stderr)
+        return 3
+    n, M = args.n, args.chains
+    w1, w2 = c2_waves(lambda p: lib.simulate_period(p, want_end_nets=True)["a_ends"], n=n)
+    probs = [c2_problem(M=M, n=n, x0=w, period_seed=11 + q) for q, w in enumerate((w1, w2))]
+    for p in probs:
+        p.threads = args.threads
+    plans = [lib.PeriodPlan(p) for p in probs]
+
+    def step(s: int) -> list:
+        ms = []
+        for q, pl in enumerate(plans):
+            pl.run(1_000_003 * s + q)
+            pl.sync()
+            ms.append(pl.kernel_ms())
+        return ms
+
+    for s in range(args.warmup):
+        step(10_000 + s)
+    for pl in plans:
+        pl.sync()
+    kern = []
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        kern += step(s)
+    for pl in plans:
+        pl.sync()
+    elapsed = time.perf_counter() - t0
+    outs = [pl.fetch() for pl in plans]
+    chain_periods = 2 * M * args.steps
+    value = chain_periods / elapsed
+    # algorithmic work of the last launches: alternatives = ministeps x n
+    units = [float(o["steps"].astype(np.int64).sum()) * n for o in outs]
+    ops_unit = algorithmic_ops_per_alternative(n, probs[0].k)
+    kms = float(np.mean(kern))
+    achieved = float(np.mean(units)) * ops_unit / (kms * 1e-3)
+    line = {
+        "metric": METRIC, "value": round(value, 2), "unit": "chain-periods/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"config-2 period simulation: n={n}, {M} chains x 2 periods, "
+                               "rate 8, density/recip/transTrip/cycle3/egoX/altX/simX, "
+                               "unconditional", "n": n, "chains": M, "periods": 2,
+                   "block_threads": plans[0].block_threads},
+        "roofline": {"bound": "valu", "achieved": round(achieved / 1e12, 3),
+                     "peak": round(VALU_PEAK_OPS / 1e12, 3), "unit": "Tops/s",
+                     "frac": round(achieved / VALU_PEAK_OPS, 4), "traffic": None,
+                     "kernel": "saom_period_kernel", "kernel_ms": round(kms, 3),
+                     "alternatives_per_launch": float(np.mean(units)),
+                     "ops_per_alternative": ops_unit},
+    }
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+        chains = args.cpu_chains or 4 * threads
+        line["cpu_baseline"] = cpu_baseline(
+            lambda m: c2_problem(M=m, n=n, x0=w1, period_seed=11), threads, chains)
+    for pl in plans:
+        pl.close()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
