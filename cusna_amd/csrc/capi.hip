@@ -211,7 +211,7 @@ struct cusna_period_plan {
   hipStream_t s = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   size_t o_keep = 0, o_force = 0, o_vb = 0, o_vmiss = 0, o_od0 = 0, o_id0 = 0, o_stats = 0;
-  size_t o_dense = 0, o_th = 0, o_lam = 0, o_thb = 0, o_lamb = 0;
+  size_t o_dense = 0, o_th = 0, o_lam = 0, o_thb = 0, o_lamb = 0, o_params = 0;
   bool stats_valid = false;
   ~cusna_period_plan() {
     if (ev0) (void)hipEventDestroy(ev0);
@@ -258,7 +258,7 @@ cusna_status plan_create(const cusna_saom_problem* pb, bool want_dense, const do
   P.kb = kb;
   P.M = M;
   P.W64 = W64;
-  P.RS = W64;
+  P.RS = n;  // word-major LDS rows: word w of actor h at w * n + h
   P.nact = pb->act ? pb->nact : n;
   P.has_comp = pb->has_comp;
   P.symflag = pb->symflag;
@@ -289,7 +289,7 @@ cusna_status plan_create(const cusna_saom_problem* pb, bool want_dense, const do
   }
   for (int e = 0; e < kb; ++e) P.bty[e] = pb->bty[e];
   pl->nw = pick_nw(wmax, pb->threads);
-  if (saom::sim_lds_bytes(P, pl->nw) > kLdsLimit) {
+  if (saom::sim_lds_bytes(P, pl->nw, wmax) > kLdsLimit) {
     set_error("chain state exceeds the 160 KB LDS of a CU");
     return CUSNA_ERR_UNSUPPORTED;
   }
@@ -358,6 +358,7 @@ cusna_status plan_create(const cusna_saom_problem* pb, bool want_dense, const do
   const size_t o_steps = D.add((size_t)M * 4);
   const size_t o_bdist = D.add((size_t)M * 4), o_zend = D.add((size_t)M * n * 4);
   pl->o_stats = D.add((size_t)M * k * 8);
+  pl->o_params = D.add(sizeof(saom::SimParams));
   if (want_dense) pl->o_dense = D.add((size_t)M * n * n);
   size_t o_lu0 = 0, o_llu = 0;
   if (replay) {
@@ -429,6 +430,8 @@ cusna_status plan_create(const cusna_saom_problem* pb, bool want_dense, const do
   P.bdist = D.at<int32_t>(o_bdist);
   P.z_end = kb > 0 ? D.at<int32_t>(o_zend) : nullptr;
   P.actor_dist = nullptr;  // written by the reference only when nrate > 0 (:406-409)
+  HIP_TRY(hipMemcpyAsync(D.at<char>(pl->o_params), &P, sizeof(P), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipStreamSynchronize(s));
   *out = pl.release();
   return CUSNA_OK;
 }
@@ -463,7 +466,9 @@ cusna_status plan_run(cusna_period_plan* pl, uint64_t seed, bool want_stats) {
   P.key0 = (uint32_t)(seed & 0xFFFFFFFFu);
   P.key1 = (uint32_t)(seed >> 32);
   HIP_TRY(hipEventRecord(pl->ev0, pl->s));
-  int le = saom::launch_period(P, saom::LaunchShape{pl->nw, pl->wmax}, pl->replay, pl->s);
+  int le = saom::launch_period(P, pl->D.at<saom::SimParams>(pl->o_params),
+                               saom::LaunchShape{pl->nw, pl->wmax}, pl->replay, P.key0, P.key1,
+                               pl->s);
   if (le != 0) {
     set_error(std::string("period kernel launch: ") + hipGetErrorString((hipError_t)le));
     return CUSNA_ERR_CUDA;
@@ -698,6 +703,22 @@ CUSNA_API cusna_status cusna_philox4x32(const uint32_t counter[4], const uint32_
   if (saom::launch_philox(D.at<uint32_t>(o_c), D.at<uint32_t>(o_k), D.at<uint32_t>(o_o), s))
     return CUSNA_ERR_CUDA;
   HIP_TRY(hipMemcpyAsync(out, D.at<char>(o_o), 16, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return CUSNA_OK;
+}
+
+CUSNA_API cusna_status cusna_gumbel_bound_check(uint32_t x0, uint32_t stride, uint32_t count,
+                                               double* max_abs_err) {
+  if (!max_abs_err) return CUSNA_ERR_NULL_ARG;
+  if (device_count() < 1) return CUSNA_ERR_NO_CUDA;
+  DeviceBlock D;
+  const size_t o = D.add(8);
+  HIP_TRY(D.alloc());
+  hipStream_t s = thread_stream();
+  HIP_TRY(hipMemsetAsync(D.at<char>(o), 0, 8, s));
+  if (saom::launch_gumbel_check(x0, stride, count, D.at<unsigned long long>(o), s))
+    return CUSNA_ERR_CUDA;
+  HIP_TRY(hipMemcpyAsync(max_abs_err, D.at<char>(o), 8, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   return CUSNA_OK;
 }

@@ -51,10 +51,17 @@ struct LaunchShape {
 };
 
 // LDS bytes one chain needs with the given shape.
-size_t sim_lds_bytes(const SimParams& P, int nw);
+size_t sim_lds_bytes(const SimParams& P, int nw, int wmax);
 
-// Returns 0 on success, a hipError_t value otherwise.
-int launch_period(const SimParams& P, LaunchShape s, bool replay, void* stream);
+// Effects whose change statistics need no per-ministep arrays beyond the
+// two-path counts (no transTies, transRecTrip, dyadic X, gwesp/gwdsp family,
+// interactions): they run in the SIMPLE specialisation of the kernel.
+bool simple_effects(const int* type_ids, int k);
+
+// Launch the period kernel. P is the host copy (shape, LDS size); Pd the same
+// struct in device memory, read by the kernel. Returns 0 or a hipError_t.
+int launch_period(const SimParams& P, const SimParams* Pd, LaunchShape s, bool replay,
+                  uint32_t key0, uint32_t key1, void* stream);
 
 // Statistics of M packed networks (rows u64, n*W64 per network).
 struct StatParams {
@@ -87,5 +94,8 @@ int launch_expand_dense(const uint64_t* rows, int n, int W64, int M, uint8_t* de
                         void* stream);
 
 int launch_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out, void* stream);
+
+int launch_gumbel_check(uint32_t x0, uint32_t stride, uint32_t count,
+                        unsigned long long* max_bits, void* stream);
 
 }  // namespace saom

@@ -5,22 +5,42 @@
 
 namespace saom {
 
-int launch_period_w1(const SimParams& P, int nw, bool replay, void* stream);
-int launch_period_w2(const SimParams& P, int nw, bool replay, void* stream);
-int launch_period_w4(const SimParams& P, int nw, bool replay, void* stream);
-int launch_period_w8(const SimParams& P, int nw, bool replay, void* stream);
-int launch_period_w16(const SimParams& P, int nw, bool replay, void* stream);
-size_t sim_lds_bytes_w1(const SimParams& P, int nw);
+#define SAOM_DECL(W)                                                                     \
+  int launch_period_w##W(const SimParams& P, const SimParams* Pd, int nw, bool replay,   \
+                         bool simple, uint32_t k0, uint32_t k1, void* stream);
+SAOM_DECL(1)
+SAOM_DECL(2)
+SAOM_DECL(4)
+SAOM_DECL(8)
+SAOM_DECL(16)
+#undef SAOM_DECL
+size_t sim_lds_bytes_w1(const SimParams& P, int nw, int wmax);
 
-size_t sim_lds_bytes(const SimParams& P, int nw) { return sim_lds_bytes_w1(P, nw); }
+size_t sim_lds_bytes(const SimParams& P, int nw, int wmax) {
+  return sim_lds_bytes_w1(P, nw, wmax);
+}
 
-int launch_period(const SimParams& P, LaunchShape s, bool replay, void* stream) {
+bool simple_effects(const int* t, int k) {
+  for (int e = 0; e < k; ++e) {
+    switch (t[e]) {
+      case 5: case 6: case 22: case 24: case 26: case 27: case 28: case 29: case 30: case 32:
+        return false;
+      default:
+        break;
+    }
+  }
+  return true;
+}
+
+int launch_period(const SimParams& P, const SimParams* Pd, LaunchShape s, bool replay,
+                  uint32_t k0, uint32_t k1, void* stream) {
+  const bool simple = simple_effects(P.tid, P.k);
   switch (s.wmax) {
-    case 1: return launch_period_w1(P, s.nw, replay, stream);
-    case 2: return launch_period_w2(P, s.nw, replay, stream);
-    case 4: return launch_period_w4(P, s.nw, replay, stream);
-    case 8: return launch_period_w8(P, s.nw, replay, stream);
-    case 16: return launch_period_w16(P, s.nw, replay, stream);
+    case 1: return launch_period_w1(P, Pd, s.nw, replay, simple, k0, k1, stream);
+    case 2: return launch_period_w2(P, Pd, s.nw, replay, simple, k0, k1, stream);
+    case 4: return launch_period_w4(P, Pd, s.nw, replay, simple, k0, k1, stream);
+    case 8: return launch_period_w8(P, Pd, s.nw, replay, simple, k0, k1, stream);
+    case 16: return launch_period_w16(P, Pd, s.nw, replay, simple, k0, k1, stream);
     default: return (int)hipErrorInvalidValue;
   }
 }

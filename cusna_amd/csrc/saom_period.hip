@@ -3,7 +3,8 @@
 // One workgroup of NW wave64s runs one chain of one period, i.e. the
 // reference's simulate_one (cran/cusna src/libcusna_saom_cpu.cpp:142-303).
 //
-// State in LDS: the network as u64 bit rows (n rows of RS words), in/out
+// State in LDS: the network as u64 bit rows stored word-major (word w of
+// actor h at w * n + h, so a wave reading word w of 64 actors is conflict-free), in/out
 // degrees, behaviour values. Alternative j of a network ministep belongs to
 // lane (j & 63) of wave ((j >> 6) % NW) in round (j >> 6) / NW, so round r of
 // wave w covers u64 word NW*r + w of every bit row: a lane's own j is always
@@ -27,6 +28,7 @@
 // Behaviour ministeps follow libcusna_saom_cpu.cpp:202-227.
 #include <hip/hip_runtime.h>
 
+#include "saom_device.h"
 #include "saom_kernels.h"
 
 namespace saom {
@@ -34,22 +36,55 @@ namespace {
 
 __device__ __forceinline__ int popc64(uint64_t x) { return __popcll(x); }
 
+// Wave64 all-reductions: four DPP steps combine within each row of 16 lanes
+// (quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror, row_mirror), then
+// the four row results are read into scalar registers and combined. Every lane
+// ends with the same bits (each step combines a lane with a partner holding the
+// same kind of partial, and a + b == b + a exactly).
+template <int CTRL>
+__device__ __forceinline__ int dpp_i32(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = dpp_i32<CTRL>((int)b);
+  const int hi = dpp_i32<CTRL>((int)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  v = v + dpp_f64<0xB1>(v);
+  v = v + dpp_f64<0x4E>(v);
+  v = v + dpp_f64<0x141>(v);
+  v = v + dpp_f64<0x140>(v);
+  return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
 }
 
 __device__ __forceinline__ long long wave_sum_ll(long long v) {
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  // counts per lane fit in 32 bits
+  int x = (int)v;
+  x += dpp_i32<0xB1>(x);
+  x += dpp_i32<0x4E>(x);
+  x += dpp_i32<0x141>(x);
+  x += dpp_i32<0x140>(x);
+  return (long long)__builtin_amdgcn_readlane(x, 0) + __builtin_amdgcn_readlane(x, 16) +
+         __builtin_amdgcn_readlane(x, 32) + __builtin_amdgcn_readlane(x, 48);
 }
 
 __device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-  return v;
+  v = fmax(v, dpp_f64<0xB1>(v));
+  v = fmax(v, dpp_f64<0x4E>(v));
+  v = fmax(v, dpp_f64<0x141>(v));
+  v = fmax(v, dpp_f64<0x140>(v));
+  return fmax(fmax(readlane_f64(v, 0), readlane_f64(v, 16)),
+              fmax(readlane_f64(v, 32), readlane_f64(v, 48)));
 }
 
 // lexicographic max of (g, -j): larger g wins, lower j breaks ties.
@@ -60,19 +95,59 @@ __device__ __forceinline__ void arg_combine(double& g, int& j, double g2, int j2
   }
 }
 
-__device__ __forceinline__ void wave_argmax(double& g, int& j) {
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) {
-    const double g2 = __shfl_xor(g, o, 64);
-    const int j2 = __shfl_xor(j, o, 64);
-    arg_combine(g, j, g2, j2);
-  }
+template <int CTRL>
+__device__ __forceinline__ void argmax_step(double& g, int& j) {
+  const double g2 = dpp_f64<CTRL>(g);
+  const int j2 = dpp_i32<CTRL>(j);
+  arg_combine(g, j, g2, j2);
 }
 
-__device__ __forceinline__ uint32_t philox_word0(uint32_t c0, uint32_t c1, uint32_t c2,
-                                                 uint32_t c3, uint32_t k0, uint32_t k1) {
-  philox10(c0, c1, c2, c3, k0, k1);
-  return c0;
+__device__ __forceinline__ void wave_argmax(double& g, int& j) {
+  argmax_step<0xB1>(g, j);
+  argmax_step<0x4E>(g, j);
+  argmax_step<0x141>(g, j);
+  argmax_step<0x140>(g, j);
+  double gb = readlane_f64(g, 0);
+  int jb = __builtin_amdgcn_readlane(j, 0);
+  arg_combine(gb, jb, readlane_f64(g, 16), __builtin_amdgcn_readlane(j, 16));
+  arg_combine(gb, jb, readlane_f64(g, 32), __builtin_amdgcn_readlane(j, 32));
+  arg_combine(gb, jb, readlane_f64(g, 48), __builtin_amdgcn_readlane(j, 48));
+  g = gb;
+  j = jb;
+}
+
+// Word 0 of the Philox4x32-10 blocks (c0, c1, 1, j[r]) of the lane's R
+// alternatives: the round loop stays rolled (two key registers) and the R
+// independent blocks are interleaved inside it.
+template <int R>
+__device__ __forceinline__ void philox_alternatives(uint32_t c0, uint32_t c1, const int (&j)[R],
+                                                    uint32_t k0, uint32_t k1, uint32_t (&u)[R]) {
+  uint32_t a[R], b[R], c[R], d[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    a[r] = c0;
+    b[r] = c1;
+    c[r] = 1u;
+    d[r] = (uint32_t)j[r];
+  }
+#pragma unroll 1
+  for (int round = 0; round < 10; ++round) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const uint64_t m0 = (uint64_t)0xD2511F53u * a[r];
+      const uint64_t m1 = (uint64_t)0xCD9E8D57u * c[r];
+      const uint32_t y0 = (uint32_t)(m1 >> 32) ^ b[r] ^ k0;
+      const uint32_t y2 = (uint32_t)(m0 >> 32) ^ d[r] ^ k1;
+      b[r] = (uint32_t)m1;
+      d[r] = (uint32_t)m0;
+      a[r] = y0;
+      c[r] = y2;
+    }
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) u[r] = a[r];
 }
 
 // Per-alternative features of one lane (R alternatives). Alternative r of
@@ -113,7 +188,7 @@ struct ChainLds {
 // alternatives (libcusna_saom_cpu.cpp:49-101), creation/endowment masking
 // applied (:258-259). Covariate values: vmat row e, or the current behaviour
 // minus zbar for behaviour-dynamic effects (:233, 244, 252-253).
-template <int R, int WMAX>
+template <int R, int WMAX, bool SIMPLE>
 __device__ __forceinline__ void eff_base(const SimParams& P, int e, const Alt<R>& A,
                                          const Ego<WMAX>& U, const ChainLds& S, double (&d)[R]) {
   const uint64_t* rows = S.rows;
@@ -154,29 +229,6 @@ __device__ __forceinline__ void eff_base(const SimParams& P, int e, const Alt<R>
     case 4:
 #pragma unroll
       for (int r = 0; r < R; ++r) d[r] = A.dl(r) * (double)(indeg[A.j[r]] + (1 - A.bij(r)));
-      break;
-    case 5:  // transTies: |row_i & row_j & {h: tp_h == x_ij}| + [tp_j > 0]
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const uint64_t* rj = rows + (size_t)A.j[r] * RS;
-        const uint64_t* tm = A.bij(r) ? S.tpm1 : S.tpm0;
-        int c = 0;
-#pragma unroll
-        for (int wd = 0; wd < WMAX; ++wd)
-          if (wd < S.W64) c += popc64(U.rowi[wd] & rj[wd] & tm[wd]);
-        d[r] = A.dl(r) * (double)(c + (A.tp[r] > 0 ? 1 : 0));
-      }
-      break;
-    case 6:  // transRecTrip: x_ji tp_j + |row_i & col_i & row_j|
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const uint64_t* rj = rows + (size_t)A.j[r] * RS;
-        int c = 0;
-#pragma unroll
-        for (int wd = 0; wd < WMAX; ++wd)
-          if (wd < S.W64) c += popc64(U.rowi[wd] & U.coli[wd] & rj[wd]);
-        d[r] = A.dl(r) * (double)((A.bji(r) ? A.tp[r] : 0) + c);
-      }
       break;
     case 7:
 #pragma unroll
@@ -252,93 +304,121 @@ __device__ __forceinline__ void eff_base(const SimParams& P, int e, const Alt<R>
 #pragma unroll
       for (int r = 0; r < R; ++r) d[r] = A.dl(r) * vi * VJ(r);
       break;
-    case 22: {
-      const int dc = P.dci[e];
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const double dyw =
-            (dc >= 0 && P.dyc) ? P.dyc[((size_t)dc * n + U.i) * n + A.j[r]] : 0.0;
-        d[r] = A.dl(r) * dyw;
-      }
-      break;
-    }
-    case 24:
-#pragma unroll
-      for (int r = 0; r < R; ++r) d[r] = A.dl(r) * w[A.tp[r]];
-      break;
     case 25:
 #pragma unroll
       for (int r = 0; r < R; ++r) d[r] = A.dl(r) * (vi + VJ(r));
-      break;
-    case 26:
-#pragma unroll
-      for (int r = 0; r < R; ++r) d[r] = A.dl(r) * w[A.so[r]];
-      break;
-    case 27: {  // gwespBF: common in-neighbours #{h in in(i): h -> j}
-      int c[R];
-#pragma unroll
-      for (int r = 0; r < R; ++r) c[r] = 0;
-#pragma unroll
-      for (int wd = 0; wd < WMAX; ++wd) {
-        uint64_t m = wd < S.W64 ? U.coli[wd] : 0ull;
-        while (m) {
-          const int h = wd * 64 + __ffsll((long long)m) - 1;
-          m &= m - 1;
-          const uint64_t* rh = rows + (size_t)h * RS;
-#pragma unroll
-          for (int r = 0; r < R; ++r) {
-            const int word = S.NW * r + S.wv;
-            const uint64_t x = word < S.W64 ? rh[word] : 0ull;
-            c[r] += (int)((x >> S.lane) & 1);
-          }
-        }
-      }
-#pragma unroll
-      for (int r = 0; r < R; ++r) d[r] = A.dl(r) * w[c[r]];
-      break;
-    }
-    case 28:
-#pragma unroll
-      for (int r = 0; r < R; ++r) d[r] = A.dl(r) * w[A.cy[r]];
-      break;
-    case 29:  // gwdspFF: out-neighbours h of j (h != i), in ascending order
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        double s = 0.0;
-        const uint64_t* rj = rows + (size_t)A.j[r] * RS;
-        for (int wd = 0; wd < P.W64; ++wd) {
-          uint64_t m = rj[wd];
-          while (m) {
-            const int h = wd * 64 + __ffsll((long long)m) - 1;
-            m &= m - 1;
-            if (h == U.i) continue;
-            const int p = tpall[h];
-            s += A.bij(r) ? (w[p] - w[p - 1]) : (w[p + 1] - w[p]);
-          }
-        }
-        d[r] = A.dl(r) * s;
-      }
-      break;
-    case 30:  // gwdspFB: in-neighbours h of j (h != i), in ascending order
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        double s = 0.0;
-        const int jw = A.j[r] >> 6, jb = A.j[r] & 63;
-        for (int h = 0; h < n; ++h) {
-          if (h == U.i || !((rows[(size_t)h * RS + jw] >> jb) & 1)) continue;
-          const int p = soall[h];
-          s += A.bij(r) ? (w[p] - w[p - 1]) : (w[p + 1] - w[p]);
-        }
-        d[r] = A.dl(r) * s;
-      }
       break;
     case 31:
 #pragma unroll
       for (int r = 0; r < R; ++r) d[r] = A.dl(r) * (VJ(r) >= p1 - 1e-12 ? 1.0 : 0.0);
       break;
     default:
+      if constexpr (!SIMPLE) {
+        switch (t) {
+      case 5:  // transTies: |row_i & row_j & {h: tp_h == x_ij}| + [tp_j > 0]
 #pragma unroll
-      for (int r = 0; r < R; ++r) d[r] = 0.0;
+        for (int r = 0; r < R; ++r) {
+          const uint64_t* tm = A.bij(r) ? S.tpm1 : S.tpm0;
+          int c = 0;
+#pragma unroll
+          for (int wd = 0; wd < WMAX; ++wd)
+            c += popc64(U.rowi[wd] & rows[(size_t)(wd) * RS + A.j[r]] & tm[wd]);
+          d[r] = A.dl(r) * (double)(c + (A.tp[r] > 0 ? 1 : 0));
+        }
+        break;
+      case 6:  // transRecTrip: x_ji tp_j + |row_i & col_i & row_j|
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          int c = 0;
+#pragma unroll
+          for (int wd = 0; wd < WMAX; ++wd)
+            c += popc64(U.rowi[wd] & U.coli[wd] & rows[(size_t)(wd) * RS + A.j[r]]);
+          d[r] = A.dl(r) * (double)((A.bji(r) ? A.tp[r] : 0) + c);
+        }
+        break;
+      case 22: {
+        const int dc = P.dci[e];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const double dyw =
+              (dc >= 0 && P.dyc) ? P.dyc[((size_t)dc * n + U.i) * n + A.j[r]] : 0.0;
+          d[r] = A.dl(r) * dyw;
+        }
+        break;
+      }
+      case 24:
+#pragma unroll
+        for (int r = 0; r < R; ++r) d[r] = A.dl(r) * w[A.tp[r]];
+        break;
+      case 26:
+#pragma unroll
+        for (int r = 0; r < R; ++r) d[r] = A.dl(r) * w[A.so[r]];
+        break;
+      case 27: {  // gwespBF: common in-neighbours #{h in in(i): h -> j}
+        int c[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) c[r] = 0;
+#pragma unroll
+        for (int wd = 0; wd < WMAX; ++wd) {
+          uint64_t m = U.coli[wd];
+          while (m) {
+            const int h = wd * 64 + __ffsll((long long)m) - 1;
+            m &= m - 1;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+              const int word = S.NW * r + S.wv;
+              const uint64_t x = rows[(size_t)(word) * RS + h];
+              c[r] += (int)((x >> S.lane) & 1);
+            }
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) d[r] = A.dl(r) * w[c[r]];
+        break;
+      }
+      case 28:
+#pragma unroll
+        for (int r = 0; r < R; ++r) d[r] = A.dl(r) * w[A.cy[r]];
+        break;
+      case 29:  // gwdspFF: out-neighbours h of j (h != i), in ascending order
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          double s = 0.0;
+          for (int wd = 0; wd < P.W64; ++wd) {
+            uint64_t m = rows[(size_t)(wd) * RS + A.j[r]];
+            while (m) {
+              const int h = wd * 64 + __ffsll((long long)m) - 1;
+              m &= m - 1;
+              if (h == U.i) continue;
+              const int p = tpall[h];
+              s += A.bij(r) ? (w[p] - w[p - 1]) : (w[p + 1] - w[p]);
+            }
+          }
+          d[r] = A.dl(r) * s;
+        }
+        break;
+      case 30:  // gwdspFB: in-neighbours h of j (h != i), in ascending order
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          double s = 0.0;
+          const int jw = A.j[r] >> 6, jb = A.j[r] & 63;
+          for (int h = 0; h < n; ++h) {
+            if (h == U.i || !((rows[(size_t)jw * RS + h] >> jb) & 1)) continue;
+            const int p = soall[h];
+            s += A.bij(r) ? (w[p] - w[p - 1]) : (w[p + 1] - w[p]);
+          }
+          d[r] = A.dl(r) * s;
+        }
+        break;
+          default:
+#pragma unroll
+            for (int r = 0; r < R; ++r) d[r] = 0.0;
+            break;
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) d[r] = 0.0;
+      }
       break;
   }
   const int et = P.ety[e];
@@ -352,25 +432,29 @@ __device__ __forceinline__ void eff_base(const SimParams& P, int e, const Alt<R>
 // Full change statistic of effect e, including interactions (type 32):
 // d = (c2 < 0 ? delta : 1) * (d_c0 * d_c1) [* d_c2] (:246-251). Alternatives
 // j == i get 0 (:238).
-template <int R, int WMAX>
+template <int R, int WMAX, bool SIMPLE>
 __device__ __forceinline__ void eff_full(const SimParams& P, int e, const Alt<R>& A,
                                          const Ego<WMAX>& U, const ChainLds& S, double (&d)[R]) {
-  // One eff_base call site: a plain effect runs the loop once; an interaction
-  // runs it over its two or three components.
-  const bool inter = P.tid[e] == 32;
-  const int c2 = P.c2[e];
-  double prod[R];
-  for (int q = 0; q < 3; ++q) {
-    const int ce = !inter ? (q == 0 ? e : -1) : (q == 0 ? P.c0[e] : (q == 1 ? P.c1[e] : c2));
-    if (ce < 0) break;
-    double tmp[R];
-    eff_base<R, WMAX>(P, ce, A, U, S, tmp);
+  if constexpr (SIMPLE) {
+    eff_base<R, WMAX, true>(P, e, A, U, S, d);
+  } else {
+    // One eff_base call site: a plain effect runs the loop once; an
+    // interaction runs it over its two or three components.
+    const bool inter = P.tid[e] == 32;
+    const int c2 = P.c2[e];
+    double prod[R];
+    for (int q = 0; q < 3; ++q) {
+      const int ce = !inter ? (q == 0 ? e : -1) : (q == 0 ? P.c0[e] : (q == 1 ? P.c1[e] : c2));
+      if (ce < 0) break;
+      double tmp[R];
+      eff_base<R, WMAX, false>(P, ce, A, U, S, tmp);
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      if (!inter) d[r] = tmp[r];
-      else if (q == 0) prod[r] = tmp[r];
-      else if (q == 1) d[r] = (c2 < 0 ? A.dl(r) : 1.0) * (prod[r] * tmp[r]);
-      else d[r] = d[r] * tmp[r];
+      for (int r = 0; r < R; ++r) {
+        if (!inter) d[r] = tmp[r];
+        else if (q == 0) prod[r] = tmp[r];
+        else if (q == 1) d[r] = (c2 < 0 ? A.dl(r) : 1.0) * (prod[r] * tmp[r]);
+        else d[r] = d[r] * tmp[r];
+      }
     }
   }
 #pragma unroll
@@ -407,19 +491,23 @@ struct LdsLayout {
 
 __host__ __device__ inline size_t al8(size_t x) { return (x + 7) & ~(size_t)7; }
 
-__host__ __device__ inline LdsLayout lds_layout(int n, int k, int kb, int W64, int RS, int nw) {
+// Rows are padded to wmax words (zero above W64) so the unrolled word loops of
+// the ministep need no bounds tests.
+__host__ __device__ inline LdsLayout lds_layout(int n, int k, int kb, int wmax, int RS, int nw,
+                                                int flags) {
+  const int W64 = wmax;
   LdsLayout L;
   size_t o = 0;
-  L.rows = o; o += (size_t)n * RS * 8;
+  L.rows = o; o += (size_t)W64 * RS * 8;
   L.indeg = o; o += (size_t)n * 4;
   L.outdeg = o; o += (size_t)n * 4;
-  L.z = o; o += (size_t)n * 4;
-  L.tpall = o; o += (size_t)n * 4;
-  L.soall = o; o += (size_t)n * 4;
+  L.z = o; if (flags & F_Z) o += (size_t)n * 4;
+  L.tpall = o; if (flags & (F_TPALL | F_SOALL)) o += (size_t)n * 4;
+  L.soall = o; if (flags & (F_TPALL | F_SOALL)) o += (size_t)n * 4;
   o = al8(o);
   L.coli = o; o += (size_t)W64 * 8;
-  L.tpm0 = o; o += (size_t)W64 * 8;
-  L.tpm1 = o; o += (size_t)W64 * 8;
+  L.tpm0 = o; if (flags & (F_TPALL | F_SOALL)) o += (size_t)W64 * 8;
+  L.tpm1 = o; if (flags & (F_TPALL | F_SOALL)) o += (size_t)W64 * 8;
   L.red = o; o += (size_t)nw * 4 * 8;
   L.tsl = o; o += (size_t)nw * (k + 1) * 8;
   L.dstar = o; o += (size_t)k * 8;
@@ -430,14 +518,21 @@ __host__ __device__ inline LdsLayout lds_layout(int n, int k, int kb, int W64, i
   return L;
 }
 
-template <int NW, int R, int WMAX, bool REPLAY>
-__global__ void __launch_bounds__(64 * NW) saom_period_kernel(const SimParams P) {
+// Register budget: two waves per SIMD for the two-wave shape, three for the
+// four-wave shape (LDS admits four chains of n = 500 per CU).
+template <int NW, int R, int WMAX, bool REPLAY, bool SIMPLE>
+__global__ void __launch_bounds__(64 * NW)
+    __attribute__((amdgpu_waves_per_eu(NW == 1 ? 1 : (NW == 2 ? 2 : 3))))
+    saom_period_kernel(const SimParams* __restrict__ Pp, uint32_t key0, uint32_t key1) {
+  const SimParams& P = *Pp;
   constexpr int NT = 64 * NW;
   extern __shared__ __align__(16) unsigned char smem[];
+  // word-major bit rows: word w of actor h at rows[w * RS + h], RS = n, so the
+  // lanes of a wave reading word w of consecutive actors hit consecutive words
   const int n = P.n, k = P.k, kb = P.kb, W64 = P.W64, RS = P.RS;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int chain = blockIdx.x;
-  const LdsLayout L = lds_layout(n, k, kb, W64, RS, NW);
+  const LdsLayout L = lds_layout(n, k, kb, WMAX, RS, NW, P.flags);
   uint64_t* rows = (uint64_t*)(smem + L.rows);
   int* indeg = (int*)(smem + L.indeg);
   int* outdeg = (int*)(smem + L.outdeg);
@@ -453,17 +548,18 @@ __global__ void __launch_bounds__(64 * NW) saom_period_kernel(const SimParams P)
   double* sc = (double*)(smem + L.sc);
   double* bsc = (double*)(smem + L.bsc);
   long long* cnt = (long long*)(smem + L.cnt);
-  const int flags = P.flags;
+  const int flags = SIMPLE ? (P.flags & (F_TP | F_Z | F_RDEG)) : P.flags;
   const ChainLds S{rows, RS, W64, NW, wv, lane, indeg, outdeg, z, tpall, soall, tpm0, tpm1};
 
   // ---- chain set-up (libcusna_saom_cpu.cpp:146-160)
-  for (int q = tid; q < n * W64; q += NT) {
-    const int h = q / W64, wd = q - h * W64;
-    rows[(size_t)h * RS + wd] = P.xs[q];
+  for (int q = tid; q < n * WMAX; q += NT) {
+    const int h = q / WMAX, wd = q - h * WMAX;
+    rows[(size_t)wd * RS + h] = wd < W64 ? P.xs[(size_t)h * W64 + wd] : 0ull;
   }
+  for (int q = tid; q < WMAX; q += NT) coli_s[q] = 0ull;
   for (int q = tid; q < n; q += NT) {
     indeg[q] = 0;
-    z[q] = (flags & F_Z) ? P.z0[q] : 0;
+    if (flags & F_Z) z[q] = P.z0[q];
   }
   for (int q = tid; q < k; q += NT) sc[q] = 0.0;
   for (int q = tid; q < kb; q += NT) bsc[q] = 0.0;
@@ -471,7 +567,7 @@ __global__ void __launch_bounds__(64 * NW) saom_period_kernel(const SimParams P)
   for (int h = tid; h < n; h += NT) {
     int od = 0;
     for (int wd = 0; wd < W64; ++wd) {
-      uint64_t m = rows[(size_t)h * RS + wd];
+      uint64_t m = rows[(size_t)wd * RS + h];
       od += popc64(m);
       while (m) {
         const int j = wd * 64 + __ffsll((long long)m) - 1;
@@ -496,7 +592,7 @@ __global__ void __launch_bounds__(64 * NW) saom_period_kernel(const SimParams P)
   for (;;) {
     // 1. ministep draw (:167, 194-200)
     uint32_t c0 = (uint32_t)steps, c1 = (uint32_t)chain, c2 = 0u, c3 = 0u;
-    philox10(c0, c1, c2, c3, P.key0, P.key1);
+    philox10(c0, c1, c2, c3, key0, key1);
     double lu0;
     if (REPLAY) lu0 = P.lu0[(size_t)chain * P.smax + steps];
     else lu0 = log(unit_open(c0));
@@ -506,7 +602,7 @@ __global__ void __launch_bounds__(64 * NW) saom_period_kernel(const SimParams P)
     long long ix = (long long)(unit_open(c2) * nact);
     if (ix >= nact) ix = nact - 1;
     const int i = P.act ? P.act[ix] : (int)ix;
-    const uint64_t* rowi_p = rows + (size_t)i * RS;
+    auto ROWI = [&](int w) -> uint64_t { return rows[(size_t)w * RS + i]; };
 
     if (!net) {
       // ---- behaviour ministep (:202-227)
@@ -515,7 +611,7 @@ __global__ void __launch_bounds__(64 * NW) saom_period_kernel(const SimParams P)
       long long nb = 0, nH = 0, nE = 0, nL = 0;
       double zs = 0.0;  // exact in any order: terms are multiples of ulp(float zbar)
       for (int q = tid; q < n; q += NT) {
-        if ((rowi_p[q >> 6] >> (q & 63)) & 1) {
+        if ((ROWI(q >> 6) >> (q & 63)) & 1) {
           nb++;
           const int zj = z[q];
           zs += (double)zj - P.zbar;
@@ -604,7 +700,7 @@ __global__ void __launch_bounds__(64 * NW) saom_period_kernel(const SimParams P)
     Ego<WMAX> U;
     U.i = i;
 #pragma unroll
-    for (int wd = 0; wd < WMAX; ++wd) U.rowi[wd] = wd < W64 ? rowi_p[wd] : 0ull;
+    for (int wd = 0; wd < WMAX; ++wd) U.rowi[wd] = ROWI(wd);
     Alt<R> A;
     A.in = A.ok = A.xij = A.xji = 0u;
     const int iw = i >> 6, ib = i & 63;
@@ -614,10 +710,10 @@ __global__ void __launch_bounds__(64 * NW) saom_period_kernel(const SimParams P)
       const int j = word * 64 + lane;
       const bool in = j < n;
       A.j[r] = in ? j : i;
-      const int xji = in ? (int)((rows[(size_t)A.j[r] * RS + iw] >> ib) & 1) : 0;
+      const int xji = in ? (int)((rows[(size_t)iw * RS + A.j[r]] >> ib) & 1) : 0;
       const uint64_t bal = __ballot(xji);
-      if (lane == 0 && word < W64) coli_s[word] = bal;
-      const uint64_t rw = word < W64 ? rowi_p[word] : 0ull;
+      if (lane == 0) coli_s[word] = bal;
+      const uint64_t rw = ROWI(word);
       const int xij = (int)((rw >> lane) & 1);
       A.in |= (uint32_t)in << r;
       A.xji |= (uint32_t)xji << r;
@@ -625,10 +721,10 @@ __global__ void __launch_bounds__(64 * NW) saom_period_kernel(const SimParams P)
     }
     __syncthreads();
 #pragma unroll
-    for (int wd = 0; wd < WMAX; ++wd) U.coli[wd] = wd < W64 ? coli_s[wd] : 0ull;
+    for (int wd = 0; wd < WMAX; ++wd) U.coli[wd] = coli_s[wd];
     U.outdeg_i = outdeg[i];
     U.indeg_i = indeg[i];
-    U.zvi = (double)z[i] - P.zbar;
+    U.zvi = (flags & F_Z) ? (double)z[i] - P.zbar : -P.zbar;
     U.rdeg_i = 0;
     if (flags & F_RDEG) {
 #pragma unroll
@@ -646,11 +742,10 @@ __global__ void __launch_bounds__(64 * NW) saom_period_kernel(const SimParams P)
         while (m) {
           const int h = wd * 64 + __ffsll((long long)m) - 1;
           m &= m - 1;
-          const uint64_t* rh = rows + (size_t)h * RS;
 #pragma unroll
           for (int r = 0; r < R; ++r) {
             const int word = NW * r + wv;
-            const uint64_t x = word < W64 ? rh[word] : 0ull;
+            const uint64_t x = rows[(size_t)(word) * RS + h];
             A.tp[r] += (int)((x >> lane) & 1);
           }
         }
@@ -658,12 +753,11 @@ __global__ void __launch_bounds__(64 * NW) saom_period_kernel(const SimParams P)
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      const uint64_t* rj = rows + (size_t)A.j[r] * RS;
       int so = 0, cy = 0;
 #pragma unroll
       for (int wd = 0; wd < WMAX; ++wd) {
-        if (wd < W64) {
-          const uint64_t x = rj[wd];
+        {
+          const uint64_t x = rows[(size_t)(wd) * RS + A.j[r]];
           so += popc64(U.rowi[wd] & x);
           cy += popc64(x & U.coli[wd]);
         }
@@ -682,7 +776,7 @@ __global__ void __launch_bounds__(64 * NW) saom_period_kernel(const SimParams P)
         }
         const uint64_t b0 = __ballot(in && A.tp[r] == 0);
         const uint64_t b1 = __ballot(in && A.tp[r] == 1);
-        if (lane == 0 && word < W64) {
+        if (lane == 0) {
           tpm0[word] = b0;
           tpm1[word] = b1;
         }
@@ -710,7 +804,7 @@ __global__ void __launch_bounds__(64 * NW) saom_period_kernel(const SimParams P)
     for (int r = 0; r < R; ++r) f[r] = 0.0;
     for (int e = 0; e < k; ++e) {
       double d[R];
-      eff_full<R, WMAX>(P, e, A, U, S, d);
+      eff_full<R, WMAX, SIMPLE>(P, e, A, U, S, d);
       const double te = th[e];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
@@ -721,20 +815,47 @@ __global__ void __launch_bounds__(64 * NW) saom_period_kernel(const SimParams P)
     // 5. Gumbel-max choice (:265-268) and lmax
     double gb = -1e30, lm = -1e30;
     int jb = i;
+    if constexpr (REPLAY) {
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      if (A.j[r] == i) f[r] = 0.0;
-      if (!((A.ok >> r) & 1)) continue;
-      if (f[r] > lm) lm = f[r];
-      double ll;
-      if (REPLAY) {
-        ll = P.llu[((size_t)chain * P.smax + steps) * n + A.j[r]];
-      } else {
-        const uint32_t u = philox_word0((uint32_t)steps, (uint32_t)chain, 1u, (uint32_t)A.j[r],
-                                        P.key0, P.key1);
-        ll = log(-log(unit_open(u)));
+      for (int r = 0; r < R; ++r) {
+        if (A.j[r] == i) f[r] = 0.0;
+        if (!((A.ok >> r) & 1)) continue;
+        if (f[r] > lm) lm = f[r];
+        const double ll = P.llu[((size_t)chain * P.smax + steps) * n + A.j[r]];
+        arg_combine(gb, jb, f[r] - ll, A.j[r]);
       }
-      arg_combine(gb, jb, f[r] - ll, A.j[r]);
+    } else {
+      // Bound every g_j = f_j - log(-log u_j) in float, then evaluate the
+      // exact float64 noise only where g_j can reach the maximum: every j
+      // with the largest exact g (ties included) satisfies approx >= M - 2e.
+      uint32_t uj[R];
+      philox_alternatives<R>((uint32_t)steps, (uint32_t)chain, A.j, key0, key1, uj);
+      double ga[R];
+      double M = -INFINITY;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if (A.j[r] == i) f[r] = 0.0;
+        ga[r] = -INFINITY;
+        if (!((A.ok >> r) & 1)) continue;
+        if (f[r] > lm) lm = f[r];
+        ga[r] = f[r] - (double)approx_loglog(uj[r]);
+        M = fmax(M, ga[r]);
+      }
+      M = wave_max(M);
+      if (NW > 1) {
+        if (lane == 0) red[wv * 4 + 3] = M;
+        __syncthreads();
+        M = red[3];
+        for (int q = 1; q < NW; ++q) M = fmax(M, red[q * 4 + 3]);
+      }
+      const double cut = M - 2.0 * kGumbelSlack;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if (ga[r] >= cut) {
+          const double ll = log(-log(unit_open(uj[r])));
+          arg_combine(gb, jb, f[r] - ll, A.j[r]);
+        }
+      }
     }
     wave_argmax(gb, jb);
     lm = wave_max(lm);
@@ -767,7 +888,7 @@ __global__ void __launch_bounds__(64 * NW) saom_period_kernel(const SimParams P)
     if (lane == 0) tsl[wv * (k + 1) + k] = Sw;
     for (int e = 0; e < k; ++e) {
       double d[R];
-      eff_full<R, WMAX>(P, e, A, U, S, d);
+      eff_full<R, WMAX, SIMPLE>(P, e, A, U, S, d);
       double T = 0.0;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
@@ -781,26 +902,27 @@ __global__ void __launch_bounds__(64 * NW) saom_period_kernel(const SimParams P)
     int aij = 0, x0ij = 0, kdij = 0;
     if (jstar != i) {
       const int jw = jstar >> 6, jbit = jstar & 63;
-      aij = (int)((rowi_p[jw] >> jbit) & 1);
+      aij = (int)((ROWI(jw) >> jbit) & 1);
       x0ij = (int)((P.x0[(size_t)i * W64 + jw] >> jbit) & 1);
       kdij = (int)((P.kd[(size_t)i * W64 + jw] >> jbit) & 1);
     }
     __syncthreads();
-    if (tid == 0) {
-      double St = 0.0;
-      for (int q = 0; q < NW; ++q) St += tsl[q * (k + 1) + k];
-      for (int e = 0; e < k; ++e) {
-        double Tt = 0.0;
-        for (int q = 0; q < NW; ++q) Tt += tsl[q * (k + 1) + e];
-        sc[e] += dstar[e] - Tt / St;
+    if (tid < k) {  // one lane per effect: score_e += d_e(j*) - T_e / S
+      double St = 0.0, Tt = 0.0;
+      for (int q = 0; q < NW; ++q) {
+        St += tsl[q * (k + 1) + k];
+        Tt += tsl[q * (k + 1) + tid];
       }
+      sc[tid] += dstar[tid] - Tt / St;
+    }
+    if (tid == 0) {
       if (jstar != i) {
         const int dd = 1 - 2 * aij;
-        rows[(size_t)i * RS + (jstar >> 6)] ^= (1ull << (jstar & 63));
+        rows[(size_t)(jstar >> 6) * RS + i] ^= (1ull << (jstar & 63));
         indeg[jstar] += dd;
         outdeg[i] += dd;
         if (P.symflag) {
-          rows[(size_t)jstar * RS + (i >> 6)] ^= (1ull << (i & 63));
+          rows[(size_t)(i >> 6) * RS + jstar] ^= (1ull << (i & 63));
           indeg[i] += dd;
           outdeg[jstar] += dd;
         }
@@ -824,8 +946,8 @@ __global__ void __launch_bounds__(64 * NW) saom_period_kernel(const SimParams P)
       for (int wd = 0; wd < W64; ++wd) {
         const uint64_t x0 = P.x0[(size_t)h * W64 + wd];
         const uint64_t im = P.inactw[wd];
-        const uint64_t v = rows[(size_t)h * RS + wd];
-        rows[(size_t)h * RS + wd] = hin ? x0 : ((v & ~im) | (x0 & im));
+        const uint64_t v = rows[(size_t)wd * RS + h];
+        rows[(size_t)wd * RS + h] = hin ? x0 : ((v & ~im) | (x0 & im));
       }
     }
     __syncthreads();
@@ -835,7 +957,7 @@ __global__ void __launch_bounds__(64 * NW) saom_period_kernel(const SimParams P)
   for (int h = tid; h < n; h += NT) {
     int c = 0;
     for (int wd = 0; wd < W64; ++wd) {
-      const uint64_t v = rows[(size_t)h * RS + wd];
+      const uint64_t v = rows[(size_t)wd * RS + h];
       out_rows[(size_t)h * W64 + wd] = v;
       c += popc64((v ^ P.x0[(size_t)h * W64 + wd]) & P.kd[(size_t)h * W64 + wd]);
     }
@@ -865,17 +987,25 @@ __global__ void __launch_bounds__(64 * NW) saom_period_kernel(const SimParams P)
   }
 }
 
-template <int NW, int WMAX>
-hipError_t launch_shape(const SimParams& P, bool replay, hipStream_t st) {
+template <int NW, int WMAX, bool REPLAY, bool SIMPLE>
+hipError_t launch_shape(const SimParams& P, const SimParams* Pd, uint32_t k0, uint32_t k1,
+                        hipStream_t st) {
   constexpr int R = WMAX / NW;
-  const size_t lds = lds_layout(P.n, P.k, P.kb, P.W64, P.RS, NW).total;
-  auto kern = replay ? saom_period_kernel<NW, R, WMAX, true>
-                     : saom_period_kernel<NW, R, WMAX, false>;
+  const size_t lds = lds_layout(P.n, P.k, P.kb, WMAX, P.RS, NW, P.flags).total;
+  auto kern = saom_period_kernel<NW, R, WMAX, REPLAY, SIMPLE>;
   hipError_t e = hipFuncSetAttribute((const void*)kern,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(kern, dim3(P.M), dim3(64 * NW), lds, st, P);
+  hipLaunchKernelGGL(kern, dim3(P.M), dim3(64 * NW), lds, st, Pd, k0, k1);
   return hipGetLastError();
+}
+
+template <int NW, int WMAX>
+hipError_t launch_variant(const SimParams& P, const SimParams* Pd, bool replay, bool simple,
+                          uint32_t k0, uint32_t k1, hipStream_t st) {
+  if (replay) return launch_shape<NW, WMAX, true, false>(P, Pd, k0, k1, st);
+  if (simple) return launch_shape<NW, WMAX, false, true>(P, Pd, k0, k1, st);
+  return launch_shape<NW, WMAX, false, false>(P, Pd, k0, k1, st);
 }
 
 }  // namespace
@@ -888,30 +1018,32 @@ hipError_t launch_shape(const SimParams& P, bool replay, hipStream_t st) {
 #define SAOM_CAT2(a, b) a##b
 #define SAOM_CAT(a, b) SAOM_CAT2(a, b)
 
-int SAOM_CAT(launch_period_w, SAOM_WMAX)(const SimParams& P, int nw, bool replay, void* stream) {
+int SAOM_CAT(launch_period_w, SAOM_WMAX)(const SimParams& P, const SimParams* Pd, int nw,
+                                         bool replay, bool simple, uint32_t k0, uint32_t k1,
+                                         void* stream) {
   hipStream_t st = (hipStream_t)stream;
   constexpr int WM = SAOM_WMAX;
   hipError_t e = hipErrorInvalidValue;
   if constexpr (WM <= 8) {
-    if (nw == 1) e = launch_shape<1, WM>(P, replay, st);
+    if (nw == 1) e = launch_variant<1, WM>(P, Pd, replay, simple, k0, k1, st);
   }
   if constexpr (WM >= 2) {
-    if (nw == 2) e = launch_shape<2, WM>(P, replay, st);
+    if (nw == 2) e = launch_variant<2, WM>(P, Pd, replay, simple, k0, k1, st);
   }
   if constexpr (WM >= 4) {
-    if (nw == 4) e = launch_shape<4, WM>(P, replay, st);
+    if (nw == 4) e = launch_variant<4, WM>(P, Pd, replay, simple, k0, k1, st);
   }
   if constexpr (WM >= 8) {
-    if (nw == 8) e = launch_shape<8, WM>(P, replay, st);
+    if (nw == 8) e = launch_variant<8, WM>(P, Pd, replay, simple, k0, k1, st);
   }
   if constexpr (WM >= 16) {
-    if (nw == 16) e = launch_shape<16, WM>(P, replay, st);
+    if (nw == 16) e = launch_variant<16, WM>(P, Pd, replay, simple, k0, k1, st);
   }
   return (int)e;
 }
 
-size_t SAOM_CAT(sim_lds_bytes_w, SAOM_WMAX)(const SimParams& P, int nw) {
-  return lds_layout(P.n, P.k, P.kb, P.W64, P.RS, nw).total;
+size_t SAOM_CAT(sim_lds_bytes_w, SAOM_WMAX)(const SimParams& P, int nw, int wmax) {
+  return lds_layout(P.n, P.k, P.kb, wmax, P.RS, nw, P.flags).total;
 }
 
 }  // namespace saom

@@ -12,6 +12,7 @@
 // beh_stats_kernel restates cusna_behavior_stats (:767-817).
 #include <hip/hip_runtime.h>
 
+#include "saom_device.h"
 #include "saom_kernels.h"
 
 namespace saom {
@@ -459,7 +460,29 @@ __global__ void philox_kernel(const uint32_t* ctr, const uint32_t* key, uint32_t
   }
 }
 
+// max |approx_loglog(x) - log(-log u(x))| over x = x0 + q*stride, q < count;
+// the float64 reference is the same device log() the period kernel uses.
+__global__ void gumbel_check_kernel(uint32_t x0, uint32_t stride, uint32_t count,
+                                    unsigned long long* max_bits) {
+  double m = 0.0;
+  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < count;
+       q += gridDim.x * blockDim.x) {
+    const uint32_t x = x0 + q * stride;
+    const double exact = log(-log(unit_open(x)));
+    const double err = fabs((double)approx_loglog(x) - exact);
+    m = fmax(m, err);
+  }
+  atomicMax(max_bits, (unsigned long long)__double_as_longlong(m));
+}
+
 }  // namespace
+
+int launch_gumbel_check(uint32_t x0, uint32_t stride, uint32_t count,
+                        unsigned long long* max_bits, void* stream) {
+  hipLaunchKernelGGL(gumbel_check_kernel, dim3(1024), dim3(256), 0, (hipStream_t)stream, x0,
+                     stride, count, max_bits);
+  return (int)hipGetLastError();
+}
 
 int launch_net_stats(const StatParams& P, void* stream) {
   bool need_cols = false;

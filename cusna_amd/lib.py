@@ -24,6 +24,7 @@ EXPORTS = (
     "cusna_last_error", "cusna_period_plan_create", "cusna_period_plan_set_params",
     "cusna_period_plan_run", "cusna_period_plan_sync", "cusna_period_plan_fetch",
     "cusna_period_plan_kernel_ms", "cusna_period_plan_shape", "cusna_period_plan_destroy",
+    "cusna_gumbel_bound_check",
 )
 
 _lib = None
@@ -73,6 +74,9 @@ def load(path: str | None = None) -> C.CDLL:
     L.cusna_period_plan_kernel_ms.argtypes = [_vp]
     L.cusna_period_plan_shape.restype = C.c_int32
     L.cusna_period_plan_shape.argtypes = [_vp]
+    L.cusna_gumbel_bound_check.restype = C.c_int
+    L.cusna_gumbel_bound_check.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32,
+                                           C.POINTER(C.c_double)]
     L.cusna_period_plan_destroy.restype = None
     L.cusna_period_plan_destroy.argtypes = [_vp]
     if path is None:
@@ -181,6 +185,16 @@ def behavior_stats(z, bty, xnet, bkdist, zbar, brange, bsimmean) -> np.ndarray:
                                      float(zbar), float(brange), float(bsimmean), _p(out))
     _check("cusna_behavior_stats", st)
     return out
+
+
+def gumbel_bound_error(x0: int, stride: int, count: int) -> float:
+    """Largest |float32 bound - float64 log(-log u)| on the device over a range
+    of 32-bit uniforms (cusna_gumbel_bound_check)."""
+    out = C.c_double()
+    _check("cusna_gumbel_bound_check",
+           load().cusna_gumbel_bound_check(x0 & 0xFFFFFFFF, stride & 0xFFFFFFFF, count,
+                                           C.byref(out)))
+    return out.value
 
 
 def philox4x32(counter, key) -> np.ndarray:

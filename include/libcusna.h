@@ -234,6 +234,17 @@ CUSNA_API cusna_status cusna_network_stats_packed(const uint32_t *rows, int32_t 
                                                   double *out);
 
 /*
+ * Verification hook, no counterpart in the reference. The period kernel
+ * discards alternatives whose float32 bound on the Gumbel noise log(-log u)
+ * cannot reach the maximum, and evaluates the float64 noise only for the
+ * rest; that is exact as long as the float32 value is within 2^-8 of the
+ * float64 one. This returns the largest difference over the uniforms
+ * u = (x + 1/2) 2^-32, x = x0 + q*stride (mod 2^32), q < count.
+ */
+CUSNA_API cusna_status cusna_gumbel_bound_check(uint32_t x0, uint32_t stride, uint32_t count,
+                                               double *max_abs_err);
+
+/*
  * Device-resident period plans. No counterpart in the reference, whose period
  * call re-marshals everything every time (cusna_saom_glue.cpp:57-189); the
  * reserved Layer-2 estimator types (libcusna.h:541-542) are where the

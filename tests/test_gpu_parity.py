@@ -243,3 +243,14 @@ def test_period_plan_reuse(gpu_lib, oracle_lib):
     assert_period_equal(b, ref2, label="plan run 2")
     assert plan.kernel_ms() > 0.0
     plan.close()
+
+
+def test_gumbel_float_bound(gpu_lib):
+    """The float32 Gumbel bound that prunes alternatives stays far inside its
+    2^-8 slack over the whole 32-bit uniform range: every x in steps of 97
+    (44 million values) plus both ends and the u = 1/2 switch-over."""
+    worst = gpu_lib.gumbel_bound_error(0, 97, (1 << 32) // 97)
+    for x0 in (0, 0x7FFFF000, 0x80000000 - 4096, 0xFFFFF000, 0xFEFFF000):
+        worst = max(worst, gpu_lib.gumbel_bound_error(x0, 1, 4096))
+    print("max |approx - exact| =", worst)
+    assert worst < 2.0 ** -12
