@@ -78,6 +78,30 @@ class cusna_saom_result(C.Structure):
     ]
 
 
+class cusna_mom_options(C.Structure):
+    """Mirror of struct cusna_mom_options (include/libcusna.h)."""
+    _fields_ = [
+        ("struct_version", C.c_int32), ("mode", C.c_int32), ("P", C.c_int32),
+        ("nsub", C.c_int32), ("n2", C.c_int32 * 8),
+        ("n1", C.c_int32), ("nD", C.c_int32), ("batch2", C.c_int32), ("n3", C.c_int32),
+        ("firstg", C.c_double), ("diagonalize", C.c_double), ("seed", C.c_uint64),
+        ("use_graph", C.c_int32), ("max_repeats", C.c_int32), ("conv_tol", C.c_double),
+    ]
+
+
+class cusna_mom_result(C.Structure):
+    """Mirror of struct cusna_mom_result (include/libcusna.h)."""
+    _fields_ = [
+        ("struct_version", C.c_int32), ("p", C.c_int32),
+        ("theta", _p), ("se", _p), ("tconv", _p), ("sim_means", _p),
+        ("D", _p), ("msf", _p), ("rate_est", _p), ("rate_sd", _p), ("phase3_stats", _p),
+        ("phase3_sp", _p), ("phase3_scp", _p),
+        ("tconv_max", C.c_double), ("max_abs_tconv", C.c_double), ("n_sims", C.c_int64),
+        ("repeats", C.c_int32), ("reserved", C.c_int32),
+        ("phase_seconds", C.c_double * 3), ("wall_seconds", C.c_double),
+    ]
+
+
 def pack_rows(dense: np.ndarray) -> np.ndarray:
     """Dense (n, n) 0/1 -> packed out-rows (n, W) uint32, bit j&31 of word j>>5."""
     d = np.ascontiguousarray(dense != 0)

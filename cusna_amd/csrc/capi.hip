@@ -18,8 +18,11 @@
 
 #include "../../include/libcusna.h"
 #include "saom_kernels.h"
+#include "plan.h"
 
-namespace {
+using namespace cusna_impl;
+
+namespace cusna_impl {
 
 thread_local std::string g_last_error;
 
@@ -52,41 +55,6 @@ hipStream_t thread_stream() {
   }
   return g_stream.s;
 }
-
-// Device allocation owned by one call: one hipMalloc, carved in 256-byte
-// aligned pieces, freed on scope exit.
-class DeviceBlock {
- public:
-  size_t add(size_t bytes) {
-    const size_t off = total_;
-    total_ += (bytes + 255) & ~(size_t)255;
-    return off;
-  }
-  hipError_t alloc() {
-    if (total_ == 0) total_ = 256;
-    return hipMalloc(&base_, total_);
-  }
-  template <class T>
-  T* at(size_t off) const {
-    return reinterpret_cast<T*>(static_cast<char*>(base_) + off);
-  }
-  ~DeviceBlock() {
-    if (base_) (void)hipFree(base_);
-  }
-
- private:
-  void* base_ = nullptr;
-  size_t total_ = 0;
-};
-
-#define HIP_TRY(expr)                                                                    \
-  do {                                                                                   \
-    hipError_t e_ = (expr);                                                              \
-    if (e_ != hipSuccess) {                                                              \
-      set_error(std::string(#expr) + ": " + hipGetErrorString(e_));                      \
-      return CUSNA_ERR_CUDA;                                                             \
-    }                                                                                    \
-  } while (0)
 
 // ABI bit rows (n*W u32) -> kernel rows (n*W64 u64); NULL -> off-diagonal
 // (the reference's default masks, libcusna_saom_cpu.cpp:326-331) or zero.
@@ -195,32 +163,9 @@ cusna_status validate(const cusna_saom_problem* pb, const cusna_saom_result* r) 
   return CUSNA_OK;
 }
 
-}  // namespace
+}  // namespace cusna_impl
 
-// A period plan owns the device copy of one problem: chain-invariant inputs
-// are uploaded once, per-chain parameters can be replaced between runs, and
-// outputs stay in HBM until fetched. The one-shot ABI call is create, run,
-// fetch, destroy.
-struct cusna_period_plan {
-  int n = 0, k = 0, kb = 0, M = 0, W64 = 0, wmax = 0, nw = 0, nd = 0;
-  bool replay = false, want_dense = false, own_stream = false;
-  int symflag = 0;
-  bool has_vmiss = false, has_od0 = false, has_id0 = false;
-  saom::SimParams P{};
-  DeviceBlock D;
-  hipStream_t s = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  size_t o_keep = 0, o_force = 0, o_vb = 0, o_vmiss = 0, o_od0 = 0, o_id0 = 0, o_stats = 0;
-  size_t o_dense = 0, o_th = 0, o_lam = 0, o_thb = 0, o_lamb = 0, o_params = 0;
-  bool stats_valid = false;
-  ~cusna_period_plan() {
-    if (ev0) (void)hipEventDestroy(ev0);
-    if (ev1) (void)hipEventDestroy(ev1);
-    if (own_stream && s) (void)hipStreamDestroy(s);
-  }
-};
-
-namespace {
+namespace cusna_impl {
 
 cusna_status plan_create(const cusna_saom_problem* pb, bool want_dense, const double* log_u0,
                          const double* loglog_u, int32_t smax, hipStream_t stream,
@@ -410,6 +355,10 @@ cusna_status plan_create(const cusna_saom_problem* pb, bool want_dense, const do
   P.inactw = inact.empty() ? nullptr : D.at<uint64_t>(o_inact);
   P.theta = D.at<double>(pl->o_th);
   P.lam = D.at<double>(pl->o_lam);
+  P.th_stride = k;
+  P.thb_stride = kb;
+  P.lam_stride = 1;
+  P.keys = nullptr;
   P.thetab = D.at<double>(pl->o_thb);
   P.lamb = D.at<double>(pl->o_lamb);
   P.vmat = D.at<double>(o_vmat);
@@ -461,26 +410,29 @@ cusna_status plan_set_params(cusna_period_plan* pl, const float* theta, const fl
   return CUSNA_OK;
 }
 
-cusna_status plan_run(cusna_period_plan* pl, uint64_t seed, bool want_stats) {
-  saom::SimParams& P = pl->P;
+cusna_status plan_run(cusna_period_plan* pl, uint64_t seed, bool want_stats, int M_run,
+                      hipStream_t stream, bool timed) {
+  saom::SimParams P = pl->P;  // host copy: grid size and LDS of this launch
   P.key0 = (uint32_t)(seed & 0xFFFFFFFFu);
   P.key1 = (uint32_t)(seed >> 32);
-  HIP_TRY(hipEventRecord(pl->ev0, pl->s));
+  if (M_run > 0) P.M = M_run;
+  hipStream_t s = stream ? stream : pl->s;
+  if (timed) HIP_TRY(hipEventRecord(pl->ev0, s));
   int le = saom::launch_period(P, pl->D.at<saom::SimParams>(pl->o_params),
                                saom::LaunchShape{pl->nw, pl->wmax}, pl->replay, P.key0, P.key1,
-                               pl->s);
+                               s);
   if (le != 0) {
     set_error(std::string("period kernel launch: ") + hipGetErrorString((hipError_t)le));
     return CUSNA_ERR_CUDA;
   }
-  HIP_TRY(hipEventRecord(pl->ev1, pl->s));
+  if (timed) HIP_TRY(hipEventRecord(pl->ev1, s));
   pl->stats_valid = false;
   if (want_stats) {
     saom::StatParams SP;
     std::memset(&SP, 0, sizeof(SP));
     SP.n = pl->n;
     SP.k = pl->k;
-    SP.M = pl->M;
+    SP.M = P.M;
     SP.W64 = pl->W64;
     SP.symflag = pl->symflag;
     for (int e = 0; e < pl->k; ++e) {
@@ -500,7 +452,7 @@ cusna_status plan_run(cusna_period_plan* pl, uint64_t seed, bool want_stats) {
     SP.od0 = pl->has_od0 ? pl->D.at<int32_t>(pl->o_od0) : nullptr;
     SP.id0 = pl->has_id0 ? pl->D.at<int32_t>(pl->o_id0) : nullptr;
     SP.out = pl->D.at<double>(pl->o_stats);
-    le = saom::launch_net_stats(SP, pl->s);
+    le = saom::launch_net_stats(SP, s);
     if (le != 0) {
       set_error(std::string("stats kernel launch: ") + hipGetErrorString((hipError_t)le));
       return CUSNA_ERR_CUDA;
@@ -562,7 +514,7 @@ cusna_status simulate(const cusna_saom_problem* pb, cusna_saom_result* r, const 
   std::unique_ptr<cusna_period_plan> pl(raw);
   st = plan_set_params(pl.get(), pb->theta, pb->lam, pb->thetab, pb->lamb);
   if (st != CUSNA_OK) return st;
-  st = plan_run(pl.get(), pb->seed, r->stats != nullptr);
+  st = plan_run(pl.get(), pb->seed, r->stats != nullptr, -1, nullptr, true);
   if (st != CUSNA_OK) return st;
   return plan_fetch(pl.get(), r);
 }
@@ -643,7 +595,7 @@ cusna_status net_stats_packed(const uint64_t* rows64_host, int n, int M, int k,
   return CUSNA_OK;
 }
 
-}  // namespace
+}  // namespace cusna_impl
 
 extern "C" {
 
@@ -833,7 +785,7 @@ CUSNA_API cusna_status cusna_period_plan_set_params(cusna_period_plan* plan, con
 CUSNA_API cusna_status cusna_period_plan_run(cusna_period_plan* plan, uint64_t seed,
                                              int32_t want_stats) {
   if (!plan) return CUSNA_ERR_NULL_ARG;
-  return plan_run(plan, seed, want_stats != 0);
+  return plan_run(plan, seed, want_stats != 0, -1, nullptr, true);
 }
 
 CUSNA_API cusna_status cusna_period_plan_sync(cusna_period_plan* plan) {

@@ -1,0 +1,709 @@
+// mom.hip -- method-of-moments (Robbins-Monro) estimator on device-resident
+// period plans.
+//
+// Restates the reference's R estimator (cran/cusna R/mom.R:91-269): phase 1
+// (n1 chains, derivative, first step), phase 2 (nsub sub-phases: nD chains for
+// the derivative, then n2[sub] Robbins-Monro iterations of batch2 chains with
+// gain firstg * 0.5^(sub-1), theta averaged), phase 3 (n3 chains: derivative,
+// moment covariance, convergence t-ratios, standard errors). Parameter layouts
+// (R/mom.R:39, 131-138): unconditional [rate x P, eval x k]; conditional
+// [eval x k] (internal rate 1); co-evolution [net rate x P, beh rate x P,
+// eval x k, beh x kb].
+//
+// MI355X layout: every simulation runs on the device; the moment vector,
+// its column means, the score-covariance derivative D and the moment
+// covariance are reduced on the device; a phase-2 iteration (seed draw,
+// P period kernels, statistics, moment reduction, guarded theta update) is
+// captured once per sub-phase in a hipGraph and replayed n2 times with no host
+// round trip. The host only inverts the p x p matrices between sub-phases.
+// theta is rounded to float32 before the kernels read it, as the R glue
+// rounds it for every period call (cusna_saom_glue.cpp:75-78).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <memory>
+#include <vector>
+
+#include "plan.h"
+
+namespace {
+
+using namespace cusna_impl;
+
+constexpr int kMomThreads = 256;
+
+struct PeriodOut {  // device pointers of one period's outputs
+  const int32_t* dist_stat;
+  const double* scores;  // M*(k+1)
+  const double* stats;   // M*k
+  const int32_t* bdist;
+  const double* bscores;  // M*(kb+1)
+  const double* bstats;   // M*kb
+};
+
+struct GatherArgs {
+  int M, P, p, k, kb, mode;
+  const PeriodOut* per;      // P entries (device)
+  const double* rate_consts;  // P
+  double* S;                  // M*p
+  double* SP;                 // P*M*p
+  double* SC;                 // P*M*p
+};
+
+__device__ inline uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+// Philox keys of this simulation round: one per period, from (seed, round).
+__global__ void mom_seed_kernel(uint32_t* keys, unsigned long long* counter, uint64_t seed,
+                                int P) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const unsigned long long it = counter[0];
+  for (int pp = 0; pp < P; ++pp) {
+    const uint64_t x = splitmix64(seed ^ splitmix64((uint64_t)it * 64u + (uint64_t)pp));
+    keys[2 * pp] = (uint32_t)x;
+    keys[2 * pp + 1] = (uint32_t)(x >> 32);
+  }
+  counter[0] = it + 1;
+}
+
+// Per chain: the period moment rows sp, score rows scp (R/mom.R:152-166) and
+// their sum over periods s. Scores pass through float32, the type of the ABI's
+// score outputs (libcusna.h:185-208) that the R estimator reads.
+__global__ void mom_gather_kernel(const GatherArgs G) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= G.M) return;
+  const int P = G.P, p = G.p, k = G.k, kb = G.kb;
+  const int off = G.mode == 2 ? 2 * P : (G.mode == 1 ? 0 : P);
+  double* s = G.S + (size_t)c * p;
+  for (int a = 0; a < p; ++a) s[a] = 0.0;
+  for (int pp = 0; pp < P; ++pp) {
+    const PeriodOut o = G.per[pp];
+    double* sp = G.SP + ((size_t)pp * G.M + c) * p;
+    double* sc = G.SC + ((size_t)pp * G.M + c) * p;
+    for (int a = 0; a < p; ++a) {
+      sp[a] = 0.0;
+      sc[a] = 0.0;
+    }
+    if (G.mode != 1) {
+      sp[pp] = (double)o.dist_stat[c] + G.rate_consts[pp];
+      sc[pp] = (double)(float)o.scores[(size_t)c * (k + 1)];
+    }
+    if (G.mode == 2) {
+      sp[P + pp] = (double)o.bdist[c];
+      sc[P + pp] = (double)(float)o.bscores[(size_t)c * (kb + 1)];
+      for (int e = 0; e < kb; ++e) {
+        sp[off + k + e] = o.bstats[(size_t)c * kb + e];
+        sc[off + k + e] = (double)(float)o.bscores[(size_t)c * (kb + 1) + 1 + e];
+      }
+    }
+    for (int e = 0; e < k; ++e) {
+      sp[off + e] = o.stats[(size_t)c * k + e];
+      sc[off + e] = (double)(float)o.scores[(size_t)c * (k + 1) + 1 + e];
+    }
+    for (int a = 0; a < p; ++a) s[a] += sp[a];
+  }
+}
+
+__device__ inline double block_sum(double v, double* sh) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (t < o) sh[t] += sh[t + o];
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// Column means of an M x p matrix, one block per column (fixed tree order).
+__global__ void mom_mean_kernel(const double* X, int M, int p, double* mean) {
+  __shared__ double sh[kMomThreads];
+  const int a = blockIdx.x;
+  double v = 0.0;
+  for (int c = threadIdx.x; c < M; c += blockDim.x) v += X[(size_t)c * p + a];
+  v = block_sum(v, sh);
+  if (threadIdx.x == 0) mean[a] = v / (double)M;
+}
+
+// out[a][b] (+)= sum_c (A[c][a] - mA[a]) (B[c][b] - mB[b]) / div, one block
+// per entry.
+__global__ void mom_cov_kernel(const double* A, const double* mA, const double* B,
+                               const double* mB, int M, int p, double div, int accumulate,
+                               double* out) {
+  __shared__ double sh[kMomThreads];
+  const int a = blockIdx.x / p, b = blockIdx.x - a * p;
+  double v = 0.0;
+  for (int c = threadIdx.x; c < M; c += blockDim.x)
+    v += (A[(size_t)c * p + a] - mA[a]) * (B[(size_t)c * p + b] - mB[b]);
+  v = block_sum(v, sh);
+  if (threadIdx.x == 0) out[blockIdx.x] = (accumulate ? out[blockIdx.x] : 0.0) + v / div;
+}
+
+// Guarded Robbins-Monro step (R/mom.R:189-197, 236-243):
+// theta -= guard(gain * dinv (mean - target)).
+__global__ void mom_update_kernel(double* theta, double* theta_sim, double* thav,
+                                  const double* dinv, const double* mean, const double* target,
+                                  int p, double gain, int n_rates, int accumulate) {
+  __shared__ double dev[64];
+  const int a = threadIdx.x;
+  if (a < p) dev[a] = mean[a] - target[a];
+  __syncthreads();
+  if (a >= p) return;
+  double st = 0.0;
+  for (int b = 0; b < p; ++b) st += dinv[a * p + b] * dev[b];
+  st *= gain;
+  const double th = theta[a];
+  const double cap = fmax(1.0, 0.5 * fabs(th));
+  st = fmin(fmax(st, -cap), cap);
+  if (a < n_rates && th - st < 0.1 * th) st = 0.5 * th;
+  const double nt = th - st;
+  theta[a] = nt;
+  theta_sim[a] = (double)(float)nt;
+  if (accumulate) thav[a] += nt;
+}
+
+// End of a sub-phase (R/mom.R:245): theta = thav / n, thav reset, the
+// float32 copy the kernels read refreshed; slot p holds the internal rate 1
+// of conditional runs.
+__global__ void mom_average_kernel(double* theta, double* theta_sim, double* thav, int p,
+                                   double n) {
+  const int a = threadIdx.x;
+  if (a < p) {
+    theta[a] = thav[a] / n;
+    theta_sim[a] = (double)(float)theta[a];
+    thav[a] = 0.0;
+  }
+  if (a == 0) theta_sim[p] = 1.0;
+}
+
+// ---------------------------------------------------------------- host algebra
+
+using Mat = std::vector<double>;
+
+// Inverse by Gauss-Jordan with partial pivoting; false when (numerically)
+// singular: reciprocal condition (1-norm) below machine epsilon, the point at
+// which R's solve() refuses (mom.R:20 then falls back to the pseudo-inverse).
+bool invert(const Mat& A, int n, Mat& inv) {
+  Mat a = A;
+  inv.assign((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) inv[(size_t)i * n + i] = 1.0;
+  double anorm = 0.0;
+  for (int j = 0; j < n; ++j) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += std::fabs(A[(size_t)i * n + j]);
+    anorm = std::max(anorm, s);
+  }
+  for (int c = 0; c < n; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < n; ++r)
+      if (std::fabs(a[(size_t)r * n + c]) > std::fabs(a[(size_t)piv * n + c])) piv = r;
+    if (a[(size_t)piv * n + c] == 0.0) return false;
+    if (piv != c)
+      for (int j = 0; j < n; ++j) {
+        std::swap(a[(size_t)c * n + j], a[(size_t)piv * n + j]);
+        std::swap(inv[(size_t)c * n + j], inv[(size_t)piv * n + j]);
+      }
+    const double d = a[(size_t)c * n + c];
+    for (int j = 0; j < n; ++j) {
+      a[(size_t)c * n + j] /= d;
+      inv[(size_t)c * n + j] /= d;
+    }
+    for (int r = 0; r < n; ++r) {
+      if (r == c) continue;
+      const double f = a[(size_t)r * n + c];
+      if (f == 0.0) continue;
+      for (int j = 0; j < n; ++j) {
+        a[(size_t)r * n + j] -= f * a[(size_t)c * n + j];
+        inv[(size_t)r * n + j] -= f * inv[(size_t)c * n + j];
+      }
+    }
+  }
+  double inorm = 0.0;
+  for (int j = 0; j < n; ++j) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += std::fabs(inv[(size_t)i * n + j]);
+    inorm = std::max(inorm, s);
+  }
+  if (!(anorm * inorm < 1.0 / 2.220446049250313e-16)) return false;
+  return true;
+}
+
+// Moore-Penrose pseudo-inverse by one-sided Jacobi SVD, singular values below
+// sqrt(eps) * max dropped (mom.R:15-19).
+Mat pinv(const Mat& A, int n) {
+  Mat U = A, V((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) V[(size_t)i * n + i] = 1.0;
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    double off = 0.0;
+    for (int p = 0; p < n - 1; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        double al = 0, be = 0, ga = 0;
+        for (int i = 0; i < n; ++i) {
+          const double up = U[(size_t)i * n + p], uq = U[(size_t)i * n + q];
+          al += up * up;
+          be += uq * uq;
+          ga += up * uq;
+        }
+        if (ga == 0.0) continue;
+        off = std::max(off, std::fabs(ga) / std::sqrt(al * be));
+        const double zeta = (be - al) / (2.0 * ga);
+        const double t = (zeta >= 0 ? 1.0 : -1.0) / (std::fabs(zeta) + std::sqrt(1.0 + zeta * zeta));
+        const double cs = 1.0 / std::sqrt(1.0 + t * t), sn = cs * t;
+        for (int i = 0; i < n; ++i) {
+          const double up = U[(size_t)i * n + p], uq = U[(size_t)i * n + q];
+          U[(size_t)i * n + p] = cs * up - sn * uq;
+          U[(size_t)i * n + q] = sn * up + cs * uq;
+          const double vp = V[(size_t)i * n + p], vq = V[(size_t)i * n + q];
+          V[(size_t)i * n + p] = cs * vp - sn * vq;
+          V[(size_t)i * n + q] = sn * vp + cs * vq;
+        }
+      }
+    if (off < 1e-15) break;
+  }
+  std::vector<double> sv(n);
+  double smax = 0.0;
+  for (int j = 0; j < n; ++j) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += U[(size_t)i * n + j] * U[(size_t)i * n + j];
+    sv[j] = std::sqrt(s);
+    smax = std::max(smax, sv[j]);
+  }
+  const double tol = std::sqrt(2.220446049250313e-16) * smax;
+  Mat out((size_t)n * n, 0.0);
+  for (int j = 0; j < n; ++j) {
+    if (!(sv[j] > tol)) continue;
+    for (int r = 0; r < n; ++r)
+      for (int c = 0; c < n; ++c)
+        out[(size_t)r * n + c] +=
+            V[(size_t)r * n + j] * (U[(size_t)c * n + j] / sv[j]) / sv[j];
+  }
+  return out;
+}
+
+Mat safe_solve(const Mat& A, int n) {
+  Mat inv;
+  if (invert(A, n, inv)) return inv;
+  return pinv(A, n);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ engine
+struct cusna_mom_engine {
+  int P = 0, k = 0, kb = 0, p = 0, mode = 0, n = 0, Mmax = 0, n_rates = 0;
+  std::vector<std::unique_ptr<cusna_period_plan>> plans;
+  DeviceBlock D;
+  hipStream_t s = nullptr;
+  double *theta = nullptr, *theta_sim = nullptr, *thav = nullptr, *dinv = nullptr;
+  double *targets = nullptr, *rate_consts = nullptr, *mean = nullptr, *S = nullptr;
+  double *SP = nullptr, *SC = nullptr, *mSP = nullptr, *mSC = nullptr, *Dm = nullptr,
+         *msf = nullptr;
+  double* bstats = nullptr;
+  uint64_t* xnet = nullptr;
+  int32_t* bkdist = nullptr;
+  uint32_t* keys = nullptr;
+  unsigned long long* counter = nullptr;
+  PeriodOut* per = nullptr;
+  uint64_t seed = 0;
+  long long n_sims = 0;
+  ~cusna_mom_engine() {
+    if (s) (void)hipStreamDestroy(s);
+  }
+};
+
+namespace {
+
+cusna_status engine_simulate(cusna_mom_engine* E, int M) {
+  hipStream_t s = E->s;
+  hipLaunchKernelGGL(mom_seed_kernel, dim3(1), dim3(64), 0, s, E->keys, E->counter, E->seed,
+                     E->P);
+  for (int pp = 0; pp < E->P; ++pp) {
+    cusna_period_plan* pl = E->plans[pp].get();
+    cusna_status st = plan_run(pl, 0, true, M, s, false);
+    if (st != CUSNA_OK) return st;
+    if (E->mode == 2) {
+      saom::BehStatParams BP;
+      std::memset(&BP, 0, sizeof(BP));
+      BP.n = E->n;
+      BP.M = M;
+      BP.kb = E->kb;
+      BP.W64 = (E->n + 63) / 64;
+      for (int e = 0; e < E->kb; ++e) BP.bty[e] = pl->P.bty[e];
+      BP.z = pl->P.z_end;
+      BP.xnet = E->xnet + (size_t)pp * E->n * BP.W64;
+      BP.bkdist = E->bkdist + (size_t)pp * E->n;
+      BP.zbar = pl->P.zbar;
+      BP.brange = pl->P.brange;
+      BP.bsim = pl->P.bsim;
+      BP.out = E->bstats + (size_t)pp * E->Mmax * E->kb;
+      if (saom::launch_beh_stats(BP, s)) return CUSNA_ERR_CUDA;
+    }
+  }
+  GatherArgs G{M, E->P, E->p, E->k, E->kb, E->mode, E->per, E->rate_consts, E->S, E->SP, E->SC};
+  hipLaunchKernelGGL(mom_gather_kernel, dim3((M + 127) / 128), dim3(128), 0, s, G);
+  hipLaunchKernelGGL(mom_mean_kernel, dim3(E->p), dim3(kMomThreads), 0, s, E->S, M, E->p,
+                     E->mean);
+  E->n_sims += M;
+  return hipGetLastError() == hipSuccess ? CUSNA_OK : CUSNA_ERR_CUDA;
+}
+
+// D = sum over periods of crossprod(centred sp, centred scp) / M (mom.R:173-182)
+void engine_derivative(cusna_mom_engine* E, int M) {
+  hipStream_t s = E->s;
+  const int p = E->p;
+  for (int pp = 0; pp < E->P; ++pp) {
+    const double* sp = E->SP + (size_t)pp * M * p;  // gather's layout: [period][chain][a]
+    const double* sc = E->SC + (size_t)pp * M * p;
+    hipLaunchKernelGGL(mom_mean_kernel, dim3(p), dim3(kMomThreads), 0, s, sp, M, p, E->mSP);
+    hipLaunchKernelGGL(mom_mean_kernel, dim3(p), dim3(kMomThreads), 0, s, sc, M, p, E->mSC);
+    hipLaunchKernelGGL(mom_cov_kernel, dim3(p * p), dim3(kMomThreads), 0, s, sp, E->mSP, sc,
+                       E->mSC, M, p, (double)M, pp > 0 ? 1 : 0, E->Dm);
+  }
+}
+
+struct GraphHolder {  // a captured phase-2 iteration and its executable
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ge = nullptr;
+  ~GraphHolder() {
+    if (ge) (void)hipGraphExecDestroy(ge);
+    if (g) (void)hipGraphDestroy(g);
+  }
+};
+
+cusna_status fetch(cusna_mom_engine* E, double* dst, const double* src, size_t n) {
+  HIP_TRY(hipMemcpyAsync(dst, src, n * 8, hipMemcpyDeviceToHost, E->s));
+  HIP_TRY(hipStreamSynchronize(E->s));
+  return CUSNA_OK;
+}
+
+Mat dinv_of(const Mat& Dm, int p, double diag) {
+  Mat mix(Dm);
+  for (int a = 0; a < p; ++a)
+    for (int b = 0; b < p; ++b)
+      mix[(size_t)a * p + b] = (1.0 - diag) * Dm[(size_t)a * p + b] + (a == b ? diag * Dm[(size_t)a * p + b] : 0.0);
+  return safe_solve(mix, p);
+}
+
+}  // namespace
+
+extern "C" {
+
+CUSNA_API cusna_status cusna_mom_estimate(const cusna_saom_problem* const* periods,
+                                          const uint8_t* const* xnet, const double* targets,
+                                          const double* rate_consts, const double* theta0,
+                                          const cusna_mom_options* opt, cusna_mom_result* res) {
+  using clk = std::chrono::steady_clock;
+  const auto t_start = clk::now();
+  if (!periods || !targets || !theta0 || !opt || !res) return CUSNA_ERR_NULL_ARG;
+  if (opt->struct_version != CUSNA_ABI_VERSION || res->struct_version != CUSNA_ABI_VERSION)
+    return CUSNA_ERR_SHAPE;
+  const int P = opt->P, mode = opt->mode;
+  if (P < 1 || P > 16 || mode < 0 || mode > 2 || opt->nsub < 0 || opt->nsub > 8)
+    return CUSNA_ERR_SHAPE;
+  if (device_count() < 1) return CUSNA_ERR_NO_CUDA;
+  const cusna_saom_problem* p0 = periods[0];
+  if (!p0) return CUSNA_ERR_NULL_ARG;
+  const int k = p0->k, kb = mode == 2 ? p0->kb : 0, n = p0->n;
+  const int p = mode == 2 ? 2 * P + k + kb : (mode == 1 ? k : P + k);
+  if (p > 64) {
+    set_error("cusna_mom_estimate: more than 64 parameters");
+    return CUSNA_ERR_SHAPE;
+  }
+  if (mode == 2 && (kb < 1 || !xnet)) return CUSNA_ERR_NULL_ARG;
+  if (opt->n1 < 2 || opt->nD < 2 || opt->batch2 < 1 || opt->n3 < 2) {
+    set_error("cusna_mom_estimate: n1, nD and n3 must be at least 2, batch2 at least 1");
+    return CUSNA_ERR_SHAPE;
+  }
+  for (int sub = 0; sub < opt->nsub; ++sub)
+    if (opt->n2[sub] < 0) {
+      set_error("cusna_mom_estimate: negative n2");
+      return CUSNA_ERR_SHAPE;
+    }
+  const int Mmax = std::max(std::max(opt->n1, opt->n3), std::max(opt->nD, opt->batch2));
+
+  std::unique_ptr<cusna_mom_engine> E(new cusna_mom_engine());
+  E->P = P;
+  E->k = k;
+  E->kb = kb;
+  E->p = p;
+  E->mode = mode;
+  E->n = n;
+  E->Mmax = Mmax;
+  E->n_rates = mode == 1 ? 0 : (mode == 2 ? 2 * P : P);
+  E->seed = opt->seed;
+  HIP_TRY(hipStreamCreateWithFlags(&E->s, hipStreamNonBlocking));
+  const int W64 = (n + 63) / 64;
+  DeviceBlock& D = E->D;
+  const size_t o_theta = D.add(p * 8), o_tsim = D.add((p + 1) * 8), o_thav = D.add(p * 8);
+  const size_t o_dinv = D.add((size_t)p * p * 8), o_tg = D.add(p * 8), o_rc = D.add(P * 8);
+  const size_t o_mean = D.add(p * 8), o_S = D.add((size_t)Mmax * p * 8);
+  const size_t o_SP = D.add((size_t)P * Mmax * p * 8), o_SC = D.add((size_t)P * Mmax * p * 8);
+  const size_t o_mSP = D.add(p * 8), o_mSC = D.add(p * 8), o_Dm = D.add((size_t)p * p * 8);
+  const size_t o_msf = D.add((size_t)p * p * 8);
+  const size_t o_bst = D.add((size_t)P * Mmax * (kb > 0 ? kb : 1) * 8);
+  const size_t o_xn = D.add((size_t)P * n * W64 * 8), o_bk = D.add((size_t)P * n * 4);
+  const size_t o_keys = D.add(P * 8), o_cnt = D.add(8), o_per = D.add(P * sizeof(PeriodOut));
+  HIP_TRY(D.alloc());
+  E->theta = D.at<double>(o_theta);
+  E->theta_sim = D.at<double>(o_tsim);
+  E->thav = D.at<double>(o_thav);
+  E->dinv = D.at<double>(o_dinv);
+  E->targets = D.at<double>(o_tg);
+  E->rate_consts = D.at<double>(o_rc);
+  E->mean = D.at<double>(o_mean);
+  E->S = D.at<double>(o_S);
+  E->SP = D.at<double>(o_SP);
+  E->SC = D.at<double>(o_SC);
+  E->mSP = D.at<double>(o_mSP);
+  E->mSC = D.at<double>(o_mSC);
+  E->Dm = D.at<double>(o_Dm);
+  E->msf = D.at<double>(o_msf);
+  E->bstats = D.at<double>(o_bst);
+  E->xnet = D.at<uint64_t>(o_xn);
+  E->bkdist = D.at<int32_t>(o_bk);
+  E->keys = D.at<uint32_t>(o_keys);
+  E->counter = D.at<unsigned long long>(o_cnt);
+  E->per = D.at<PeriodOut>(o_per);
+  hipStream_t s = E->s;
+
+  std::vector<double> rc(P, 0.0);
+  if (rate_consts)
+    for (int q = 0; q < P; ++q) rc[q] = rate_consts[q];
+  std::vector<double> th0(theta0, theta0 + p), zeros((size_t)p * p, 0.0);
+  std::vector<double> tsim(p + 1);
+  for (int a = 0; a < p; ++a) tsim[a] = (double)(float)th0[a];
+  tsim[p] = 1.0;
+  const unsigned long long zero = 0ull;
+  HIP_TRY(hipMemcpyAsync(E->theta, th0.data(), p * 8, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(E->theta_sim, tsim.data(), (p + 1) * 8, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(E->thav, zeros.data(), p * 8, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(E->targets, targets, p * 8, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(E->rate_consts, rc.data(), P * 8, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(E->counter, &zero, 8, hipMemcpyHostToDevice, s));
+
+  // one plan per period; the kernels read theta/lambda from the engine's
+  // float32-rounded theta (shared row, stride 0) and the keys from memory
+  std::vector<PeriodOut> per(P);
+  for (int pp = 0; pp < P; ++pp) {
+    const cusna_saom_problem* pb = periods[pp];
+    if (!pb || pb->n != n || pb->k != k || (mode == 2 && pb->kb != kb)) return CUSNA_ERR_SHAPE;
+    cusna_saom_problem q = *pb;
+    q.M = Mmax;
+    std::vector<float> thf((size_t)Mmax * k, 0.0f), lmf(Mmax, 1.0f),
+        thbf((size_t)Mmax * (kb > 0 ? kb : 1), 0.0f), lmbf(Mmax, 1.0f);
+    q.theta = thf.data();
+    q.lam = lmf.data();
+    if (kb > 0) {
+      q.thetab = thbf.data();
+      q.lamb = lmbf.data();
+    }
+    q.conditional = mode == 1 ? 1 : 0;
+    cusna_period_plan* raw = nullptr;
+    cusna_status st = plan_create(&q, false, nullptr, nullptr, 0, s, &raw);
+    if (st != CUSNA_OK) return st;
+    E->plans.emplace_back(raw);
+    saom::SimParams& SPm = raw->P;
+    const int off = mode == 2 ? 2 * P : (mode == 1 ? 0 : P);
+    SPm.theta = E->theta_sim + off;
+    SPm.th_stride = 0;
+    SPm.lam = mode == 1 ? E->theta_sim + p : E->theta_sim + pp;
+    SPm.lam_stride = 0;
+    if (mode == 2) {
+      SPm.lamb = E->theta_sim + P + pp;
+      SPm.thetab = E->theta_sim + 2 * P + k;
+      SPm.thb_stride = 0;
+    }
+    SPm.keys = E->keys + 2 * pp;
+    HIP_TRY(hipMemcpyAsync(raw->D.at<char>(raw->o_params), &SPm, sizeof(SPm),
+                           hipMemcpyHostToDevice, s));
+    per[pp].dist_stat = SPm.dist_stat;
+    per[pp].scores = SPm.scores;
+    per[pp].stats = raw->D.at<double>(raw->o_stats);
+    per[pp].bdist = SPm.bdist;
+    per[pp].bscores = SPm.bscores;
+    per[pp].bstats = E->bstats + (size_t)pp * Mmax * (kb > 0 ? kb : 1);
+    if (mode == 2) {
+      std::vector<uint64_t> xr((size_t)n * W64, 0ull);
+      for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j)
+          if (xnet[pp][(size_t)i * n + j]) xr[(size_t)i * W64 + (j >> 6)] |= 1ull << (j & 63);
+      HIP_TRY(hipMemcpyAsync(E->xnet + (size_t)pp * n * W64, xr.data(), xr.size() * 8,
+                             hipMemcpyHostToDevice, s));
+      std::vector<int32_t> bk(n, 1);
+      if (pb->bkdist)
+        for (int i = 0; i < n; ++i) bk[i] = pb->bkdist[i];
+      HIP_TRY(hipMemcpyAsync(E->bkdist + (size_t)pp * n, bk.data(), n * 4, hipMemcpyHostToDevice,
+                             s));
+      HIP_TRY(hipStreamSynchronize(s));
+    }
+  }
+  HIP_TRY(hipMemcpyAsync(E->per, per.data(), P * sizeof(PeriodOut), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipStreamSynchronize(s));
+
+  Mat Dm((size_t)p * p), mean(p);
+  std::vector<double> tgt(targets, targets + p);
+  auto upload_dinv = [&](const Mat& di) -> cusna_status {
+    HIP_TRY(hipMemcpyAsync(E->dinv, di.data(), (size_t)p * p * 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return CUSNA_OK;
+  };
+  auto t_phase = clk::now();
+  double phase_t[3] = {0, 0, 0};
+  cusna_status st;
+
+  // ---- phase 1 (mom.R:224-229)
+  if ((st = engine_simulate(E.get(), opt->n1)) != CUSNA_OK) return st;
+  engine_derivative(E.get(), opt->n1);
+  if ((st = fetch(E.get(), Dm.data(), E->Dm, (size_t)p * p)) != CUSNA_OK) return st;
+  if ((st = upload_dinv(dinv_of(Dm, p, opt->diagonalize))) != CUSNA_OK) return st;
+  hipLaunchKernelGGL(mom_update_kernel, dim3(1), dim3(64), 0, s, E->theta, E->theta_sim,
+                     E->thav, E->dinv, E->mean, E->targets, p, 0.5 * opt->firstg, E->n_rates, 0);
+  phase_t[0] = std::chrono::duration<double>(clk::now() - t_phase).count();
+  t_phase = clk::now();
+
+  // ---- phase 2 (mom.R:232-248), sub-phases replayed from a captured graph
+  auto subphase = [&](double gain, int n2) -> cusna_status {
+    cusna_status st2 = engine_simulate(E.get(), opt->nD);
+    if (st2 != CUSNA_OK) return st2;
+    engine_derivative(E.get(), opt->nD);
+    if ((st2 = fetch(E.get(), Dm.data(), E->Dm, (size_t)p * p)) != CUSNA_OK) return st2;
+    if ((st2 = upload_dinv(dinv_of(Dm, p, opt->diagonalize))) != CUSNA_OK) return st2;
+    if (n2 < 1) return CUSNA_OK;
+    auto iteration = [&]() -> cusna_status {
+      cusna_status st3 = engine_simulate(E.get(), opt->batch2);
+      if (st3 != CUSNA_OK) return st3;
+      hipLaunchKernelGGL(mom_update_kernel, dim3(1), dim3(64), 0, s, E->theta, E->theta_sim,
+                         E->thav, E->dinv, E->mean, E->targets, p, gain, E->n_rates, 1);
+      return hipGetLastError() == hipSuccess ? CUSNA_OK : CUSNA_ERR_CUDA;
+    };
+    if (opt->use_graph) {
+      GraphHolder G;
+      HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      const long long before = E->n_sims;
+      cusna_status cst = iteration();
+      hipError_t ce = hipStreamEndCapture(s, &G.g);
+      if (cst != CUSNA_OK) return cst;
+      if (ce != hipSuccess) {
+        set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+        return CUSNA_ERR_CUDA;
+      }
+      E->n_sims = before;
+      HIP_TRY(hipGraphInstantiate(&G.ge, G.g, nullptr, nullptr, 0));
+      for (int it = 0; it < n2; ++it) {
+        HIP_TRY(hipGraphLaunch(G.ge, s));
+        E->n_sims += opt->batch2;
+      }
+      HIP_TRY(hipStreamSynchronize(s));
+    } else {
+      for (int it = 0; it < n2; ++it) {
+        cusna_status st3 = iteration();
+        if (st3 != CUSNA_OK) return st3;
+      }
+    }
+    hipLaunchKernelGGL(mom_average_kernel, dim3(1), dim3(64), 0, s, E->theta, E->theta_sim,
+                       E->thav, p, (double)n2);
+    return CUSNA_OK;
+  };
+  for (int sub = 0; sub < opt->nsub; ++sub) {
+    if ((st = subphase(opt->firstg * std::pow(0.5, sub), opt->n2[sub])) != CUSNA_OK) return st;
+  }
+  phase_t[1] = std::chrono::duration<double>(clk::now() - t_phase).count();
+
+  // ---- phase 3 (mom.R:250-258), optionally repeated until max |t| <= tol
+  std::vector<double> sim_means(p), tconv(p), se(p);
+  double tconv_max = 0.0, max_abs_t = 0.0;
+  Mat msf((size_t)p * p);
+  int repeats = 0;
+  std::vector<double> theta_h(p);
+  for (;;) {
+    t_phase = clk::now();
+    if ((st = engine_simulate(E.get(), opt->n3)) != CUSNA_OK) return st;
+    engine_derivative(E.get(), opt->n3);
+    hipLaunchKernelGGL(mom_cov_kernel, dim3(p * p), dim3(kMomThreads), 0, s, E->S, E->mean, E->S,
+                       E->mean, opt->n3, p, (double)(opt->n3 - 1), 0, E->msf);
+    if ((st = fetch(E.get(), Dm.data(), E->Dm, (size_t)p * p)) != CUSNA_OK) return st;
+    if ((st = fetch(E.get(), msf.data(), E->msf, (size_t)p * p)) != CUSNA_OK) return st;
+    if ((st = fetch(E.get(), sim_means.data(), E->mean, p)) != CUSNA_OK) return st;
+    if ((st = fetch(E.get(), theta_h.data(), E->theta, p)) != CUSNA_OK) return st;
+    const Mat dinv = safe_solve(Dm, p);
+    std::vector<double> dev(p);
+    max_abs_t = 0.0;
+    for (int a = 0; a < p; ++a) {
+      dev[a] = sim_means[a] - tgt[a];
+      tconv[a] = dev[a] / std::sqrt(msf[(size_t)a * p + a]);
+      max_abs_t = std::max(max_abs_t, std::fabs(tconv[a]));
+    }
+    const Mat msfi = safe_solve(msf, p);
+    double q = 0.0;
+    for (int a = 0; a < p; ++a)
+      for (int b = 0; b < p; ++b) q += dev[a] * msfi[(size_t)a * p + b] * dev[b];
+    tconv_max = std::sqrt(q);
+    for (int a = 0; a < p; ++a) {
+      double v = 0.0;  // diag(dinv msf dinv^T)
+      for (int b = 0; b < p; ++b)
+        for (int c = 0; c < p; ++c)
+          v += dinv[(size_t)a * p + b] * msf[(size_t)b * p + c] * dinv[(size_t)a * p + c];
+      se[a] = std::sqrt(std::fabs(v));
+    }
+    phase_t[2] += std::chrono::duration<double>(clk::now() - t_phase).count();
+    if (max_abs_t <= opt->conv_tol || repeats >= opt->max_repeats) break;
+    ++repeats;
+    t_phase = clk::now();
+    const int last = opt->nsub > 0 ? opt->nsub - 1 : 0;
+    const double gain = opt->firstg * std::pow(0.5, last);
+    if ((st = subphase(gain, opt->nsub > 0 ? opt->n2[last] : 10)) != CUSNA_OK) return st;
+    phase_t[1] += std::chrono::duration<double>(clk::now() - t_phase).count();
+  }
+
+  // conditional runs estimate the rates from the simulated times (mom.R:260, 265)
+  if (mode == 1 && (res->rate_est || res->rate_sd)) {
+    for (int pp = 0; pp < P; ++pp) {
+      std::vector<double> nt(opt->n3);
+      if ((st = fetch(E.get(), nt.data(), E->plans[pp]->P.ntim, opt->n3)) != CUSNA_OK) return st;
+      double m = 0.0;
+      for (double x : nt) m += x;
+      m /= opt->n3;
+      double v = 0.0;
+      for (double x : nt) v += (x - m) * (x - m);
+      if (res->rate_est) res->rate_est[pp] = m;
+      if (res->rate_sd) res->rate_sd[pp] = std::sqrt(v / (opt->n3 - 1));
+    }
+  }
+  if (res->phase3_stats) {
+    if ((st = fetch(E.get(), res->phase3_stats, E->S, (size_t)opt->n3 * p)) != CUSNA_OK) return st;
+  }
+  if (res->phase3_sp) {
+    if ((st = fetch(E.get(), res->phase3_sp, E->SP, (size_t)P * opt->n3 * p)) != CUSNA_OK)
+      return st;
+  }
+  if (res->phase3_scp) {
+    if ((st = fetch(E.get(), res->phase3_scp, E->SC, (size_t)P * opt->n3 * p)) != CUSNA_OK)
+      return st;
+  }
+  res->p = p;
+  for (int a = 0; a < p; ++a) {
+    if (res->theta) res->theta[a] = theta_h[a];
+    if (res->se) res->se[a] = se[a];
+    if (res->tconv) res->tconv[a] = tconv[a];
+    if (res->sim_means) res->sim_means[a] = sim_means[a];
+  }
+  if (res->D) std::copy(Dm.begin(), Dm.end(), res->D);
+  if (res->msf) std::copy(msf.begin(), msf.end(), res->msf);
+  res->tconv_max = tconv_max;
+  res->max_abs_tconv = max_abs_t;
+  res->n_sims = E->n_sims;
+  res->repeats = repeats;
+  for (int q = 0; q < 3; ++q) res->phase_seconds[q] = phase_t[q];
+  res->wall_seconds = std::chrono::duration<double>(clk::now() - t_start).count();
+  return CUSNA_OK;
+}
+
+}  // extern "C"

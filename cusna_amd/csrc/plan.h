@@ -1,0 +1,93 @@
+// plan.h -- internal: device-resident period plans shared by the C ABI
+// (capi.hip) and the method-of-moments engine (mom.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+
+#include "../../include/libcusna.h"
+#include "saom_kernels.h"
+
+namespace cusna_impl {
+
+// Device allocation owned by one call: one hipMalloc, carved in 256-byte
+// aligned pieces, freed on scope exit.
+class DeviceBlock {
+ public:
+  size_t add(size_t bytes) {
+    const size_t off = total_;
+    total_ += (bytes + 255) & ~(size_t)255;
+    return off;
+  }
+  hipError_t alloc() {
+    if (total_ == 0) total_ = 256;
+    return hipMalloc(&base_, total_);
+  }
+  template <class T>
+  T* at(size_t off) const {
+    return reinterpret_cast<T*>(static_cast<char*>(base_) + off);
+  }
+  ~DeviceBlock() {
+    if (base_) (void)hipFree(base_);
+  }
+
+ private:
+  void* base_ = nullptr;
+  size_t total_ = 0;
+};
+
+void set_error(const std::string& s);
+int device_count();
+hipStream_t thread_stream();
+
+}  // namespace cusna_impl
+
+#define HIP_TRY(expr)                                                                    \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess) {                                                              \
+      cusna_impl::set_error(std::string(#expr) + ": " + hipGetErrorString(e_));                      \
+      return CUSNA_ERR_CUDA;                                                             \
+    }                                                                                    \
+  } while (0)
+
+// A period plan owns the device copy of one problem: chain-invariant inputs
+// are uploaded once, per-chain parameters can be replaced between runs, and
+// outputs stay in HBM until fetched. The one-shot ABI call is create, run,
+// fetch, destroy.
+struct cusna_period_plan {
+  int n = 0, k = 0, kb = 0, M = 0, W64 = 0, wmax = 0, nw = 0, nd = 0;
+  bool replay = false, want_dense = false, own_stream = false;
+  int symflag = 0;
+  bool has_vmiss = false, has_od0 = false, has_id0 = false;
+  saom::SimParams P{};
+  cusna_impl::DeviceBlock D;
+  hipStream_t s = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  size_t o_keep = 0, o_force = 0, o_vb = 0, o_vmiss = 0, o_od0 = 0, o_id0 = 0, o_stats = 0;
+  size_t o_dense = 0, o_th = 0, o_lam = 0, o_thb = 0, o_lamb = 0, o_params = 0;
+  bool stats_valid = false;
+  ~cusna_period_plan() {
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (own_stream && s) (void)hipStreamDestroy(s);
+  }
+};
+
+
+namespace cusna_impl {
+
+cusna_status plan_create(const cusna_saom_problem* pb, bool want_dense, const double* log_u0,
+                         const double* loglog_u, int32_t smax, hipStream_t stream,
+                         cusna_period_plan** out);
+cusna_status plan_set_params(cusna_period_plan* pl, const float* theta, const float* lam,
+                             const float* thetab, const float* lamb);
+// Launch one period (and the statistics kernel) for the first M_run chains
+// (M_run <= 0: all) on `stream` (nullptr: the plan's stream); `timed` records
+// the HIP events around the period kernel.
+cusna_status plan_run(cusna_period_plan* pl, uint64_t seed, bool want_stats, int M_run,
+                      hipStream_t stream, bool timed);
+cusna_status plan_fetch(cusna_period_plan* pl, cusna_saom_result* r);
+
+}  // namespace cusna_impl

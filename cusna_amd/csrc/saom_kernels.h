@@ -35,7 +35,9 @@ struct SimParams {
   int bty[kMaxBeff];
   const uint64_t *x0, *xs, *kd, *pm;  // n*W64 words each
   const uint64_t *inactw;             // W64 words or null
-  const double *theta, *lam, *thetab, *lamb;  // per chain: M*k, M, M*kb, M
+  const double *theta, *lam, *thetab, *lamb;  // chain c reads row c * stride
+  int th_stride, thb_stride, lam_stride;      // k, kb, 1 per chain; 0 = shared row
+  const uint32_t* keys;                       // [opt] Philox key in device memory
   const double *vmat, *wtab, *dyc;            // k*n, k*(n+1), nd*n*n (or null)
   const int32_t *z0, *bkd, *act;              // n each (act: nact used)
   const double *lu0, *llu;                    // replay inputs (or null)

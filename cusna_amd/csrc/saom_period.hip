@@ -523,8 +523,12 @@ __host__ __device__ inline LdsLayout lds_layout(int n, int k, int kb, int wmax, 
 template <int NW, int R, int WMAX, bool REPLAY, bool SIMPLE>
 __global__ void __launch_bounds__(64 * NW)
     __attribute__((amdgpu_waves_per_eu(NW == 1 ? 1 : (NW == 2 ? 2 : 3))))
-    saom_period_kernel(const SimParams* __restrict__ Pp, uint32_t key0, uint32_t key1) {
+    saom_period_kernel(const SimParams* __restrict__ Pp, uint32_t key0_arg, uint32_t key1_arg) {
   const SimParams& P = *Pp;
+  // the Philox key comes from the launch, or from device memory when a
+  // captured graph replays the launch with keys a device kernel has written
+  const uint32_t key0 = P.keys ? P.keys[0] : key0_arg;
+  const uint32_t key1 = P.keys ? P.keys[1] : key1_arg;
   constexpr int NT = 64 * NW;
   extern __shared__ __align__(16) unsigned char smem[];
   // word-major bit rows: word w of actor h at rows[w * RS + h], RS = n, so the
@@ -579,10 +583,10 @@ __global__ void __launch_bounds__(64 * NW)
   }
   __syncthreads();
 
-  const double lam = P.lam[chain];
-  const double lamb = kb > 0 ? P.lamb[chain] : 0.0;
-  const double* th = P.theta + (size_t)chain * k;
-  const double* thb = kb > 0 ? P.thetab + (size_t)chain * kb : nullptr;
+  const double lam = P.lam[(size_t)chain * P.lam_stride];
+  const double lamb = kb > 0 ? P.lamb[(size_t)chain * P.lam_stride] : 0.0;
+  const double* th = P.theta + (size_t)chain * P.th_stride;
+  const double* thb = kb > 0 ? P.thetab + (size_t)chain * P.thb_stride : nullptr;
   const int nact = P.nact;
   long long steps = 0, steps_beh = 0, dist = 0, bdist = 0;
   long long cap = kMaxSteps;
@@ -993,9 +997,11 @@ hipError_t launch_shape(const SimParams& P, const SimParams* Pd, uint32_t k0, ui
   constexpr int R = WMAX / NW;
   const size_t lds = lds_layout(P.n, P.k, P.kb, WMAX, P.RS, NW, P.flags).total;
   auto kern = saom_period_kernel<NW, R, WMAX, REPLAY, SIMPLE>;
-  hipError_t e = hipFuncSetAttribute((const void*)kern,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (e != hipSuccess) return e;
+  // raise the dynamic LDS limit once per instantiation (not inside a graph
+  // capture); 160 KB is the whole LDS of a gfx950 CU
+  static const hipError_t attr = hipFuncSetAttribute(
+      (const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  if (attr != hipSuccess) return attr;
   hipLaunchKernelGGL(kern, dim3(P.M), dim3(64 * NW), lds, st, Pd, k0, k1);
   return hipGetLastError();
 }

@@ -489,9 +489,9 @@ int launch_net_stats(const StatParams& P, void* stream) {
   for (int e = 0; e < P.k; ++e)
     if (P.tid[e] == 27 || P.tid[e] == 28 || P.tid[e] == 30) need_cols = true;
   const size_t lds = net_stats_lds(P.n, P.W64, need_cols);
-  hipError_t e = hipFuncSetAttribute((const void*)net_stats_kernel,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (e != hipSuccess) return (int)e;
+  static const hipError_t attr = hipFuncSetAttribute(
+      (const void*)net_stats_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  if (attr != hipSuccess) return (int)attr;
   hipLaunchKernelGGL(net_stats_kernel, dim3(P.M), dim3(kStatThreads), lds, (hipStream_t)stream,
                      P);
   return (int)hipGetLastError();
@@ -503,9 +503,9 @@ size_t net_stats_lds(int n, int W64, bool need_cols) {
 
 int launch_beh_stats(const BehStatParams& P, void* stream) {
   const size_t lds = (size_t)P.n * 5 * 8;
-  hipError_t e = hipFuncSetAttribute((const void*)beh_stats_kernel,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (e != hipSuccess) return (int)e;
+  static const hipError_t attr = hipFuncSetAttribute(
+      (const void*)beh_stats_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  if (attr != hipSuccess) return (int)attr;
   hipLaunchKernelGGL(beh_stats_kernel, dim3(P.M), dim3(kStatThreads), lds, (hipStream_t)stream,
                      P);
   return (int)hipGetLastError();

@@ -10,7 +10,8 @@ import os
 
 import numpy as np
 
-from .abi import CusnaError, SaomProblem, SaomResult, pack_rows, run_period
+from .abi import (CusnaError, SaomProblem, SaomResult, cusna_mom_options, cusna_mom_result,
+                  pack_rows, run_period)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_lib", "libcusna_amd.so")
@@ -24,7 +25,7 @@ EXPORTS = (
     "cusna_last_error", "cusna_period_plan_create", "cusna_period_plan_set_params",
     "cusna_period_plan_run", "cusna_period_plan_sync", "cusna_period_plan_fetch",
     "cusna_period_plan_kernel_ms", "cusna_period_plan_shape", "cusna_period_plan_destroy",
-    "cusna_gumbel_bound_check",
+    "cusna_gumbel_bound_check", "cusna_mom_estimate",
 )
 
 _lib = None
@@ -79,6 +80,9 @@ def load(path: str | None = None) -> C.CDLL:
                                            C.POINTER(C.c_double)]
     L.cusna_period_plan_destroy.restype = None
     L.cusna_period_plan_destroy.argtypes = [_vp]
+    L.cusna_mom_estimate.restype = C.c_int
+    L.cusna_mom_estimate.argtypes = [_vp, _vp, _vp, _vp, _vp, C.POINTER(cusna_mom_options),
+                                     C.POINTER(cusna_mom_result)]
     if path is None:
         _lib = L
     return L
@@ -184,6 +188,64 @@ def behavior_stats(z, bty, xnet, bkdist, zbar, brange, bsimmean) -> np.ndarray:
     st = load().cusna_behavior_stats(_p(z), n, M, len(bty), _p(bty), _p(xnet), _p(bk),
                                      float(zbar), float(brange), float(bsimmean), _p(out))
     _check("cusna_behavior_stats", st)
+    return out
+
+
+def mom_estimate(problems: list, targets, theta0, mode: int, n1: int, nD: int, batch2: int,
+                 n3: int, n2: list, firstg: float = 0.4, diagonalize: float = 0.2,
+                 seed: int = 1234, xnets: list | None = None, rate_consts=None,
+                 use_graph: bool = True, max_repeats: int = 0, conv_tol: float = 0.1) -> dict:
+    """Method-of-moments estimate on the device (cusna_mom_estimate).
+
+    ``problems``: one SaomProblem per period (inputs only); ``xnets``: the
+    co-evolution networks of the behaviour statistics, one dense (n, n) per
+    period (mode 2)."""
+    L = load()
+    P = len(problems)
+    targets = np.ascontiguousarray(targets, dtype=np.float64)
+    p = targets.shape[0]
+    built = [pr.build() for pr in problems]
+    parr = (_vp * P)(*[C.addressof(b) for b in built])
+    xarr = None
+    xs = []
+    if xnets is not None:
+        xs = [np.ascontiguousarray(np.asarray(x) != 0, dtype=np.uint8) for x in xnets]
+        xarr = (_vp * P)(*[x.ctypes.data for x in xs])
+    rc = None if rate_consts is None else np.ascontiguousarray(rate_consts, dtype=np.float64)
+    th0 = np.ascontiguousarray(theta0, dtype=np.float64)
+    if th0.shape != (p,):
+        raise ValueError(f"theta0 has shape {th0.shape}, expected ({p},)")
+    if len(n2) > 8:
+        raise ValueError("at most 8 phase-2 sub-phases")
+    o = cusna_mom_options()
+    o.struct_version = 1
+    o.mode, o.P, o.nsub = int(mode), P, len(n2)
+    for i, v in enumerate(n2):
+        o.n2[i] = int(v)
+    o.n1, o.nD, o.batch2, o.n3 = int(n1), int(nD), int(batch2), int(n3)
+    o.firstg, o.diagonalize = float(firstg), float(diagonalize)
+    o.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+    o.use_graph, o.max_repeats, o.conv_tol = int(use_graph), int(max_repeats), float(conv_tol)
+    out = {name: np.zeros(p) for name in ("theta", "se", "tconv", "sim_means")}
+    out["D"] = np.zeros((p, p))
+    out["msf"] = np.zeros((p, p))
+    out["phase3_stats"] = np.zeros((int(n3), p))
+    out["phase3_sp"] = np.zeros((P, int(n3), p))
+    out["phase3_scp"] = np.zeros((P, int(n3), p))
+    if mode == 1:
+        out["rate_est"] = np.zeros(P)
+        out["rate_sd"] = np.zeros(P)
+    r = cusna_mom_result()
+    r.struct_version = 1
+    for name, a in out.items():
+        setattr(r, name, a.ctypes.data)
+    _check("cusna_mom_estimate",
+           L.cusna_mom_estimate(parr, xarr, targets.ctypes.data, _p(rc), th0.ctypes.data,
+                                C.byref(o), C.byref(r)))
+    del built, xs
+    out.update(tconv_max=r.tconv_max, max_abs_tconv=r.max_abs_tconv, n_sims=int(r.n_sims),
+               repeats=int(r.repeats), phase_seconds=list(r.phase_seconds),
+               wall_seconds=r.wall_seconds)
     return out
 
 

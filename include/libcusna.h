@@ -274,6 +274,62 @@ CUSNA_API double cusna_period_plan_kernel_ms(cusna_period_plan *plan);
 CUSNA_API int32_t cusna_period_plan_shape(const cusna_period_plan *plan);
 CUSNA_API void cusna_period_plan_destroy(cusna_period_plan *plan);
 
+/*
+ * Method-of-moments estimator (the reference's Layer-2 estimator, reserved as
+ * opaque types at libcusna.h:541-542 and implemented in R at R/mom.R:91-269).
+ * The caller prepares one cusna_saom_problem per period (inputs only; M,
+ * theta, lam and seed are ignored) and the theta-ordered target vector
+ * (R/targets.R:99-126). Parameter layouts, by mode:
+ *   0 unconditional  [rate x P, eval x k]
+ *   1 conditional    [eval x k]                 (internal rate 1)
+ *   2 co-evolution   [net rate x P, beh rate x P, eval x k, beh x kb]
+ * Phase 1 (n1 chains), phase 2 (nsub sub-phases of nD derivative chains and
+ * n2[s] Robbins-Monro iterations of batch2 chains, gain firstg * 0.5^s) and
+ * phase 3 (n3 chains) follow R/mom.R:218-269; every simulation and moment
+ * reduction runs on the device, phase-2 iterations replayed from a HIP graph
+ * when use_graph is set. With max_repeats > 0, a phase-3 result with
+ * max|tconv| > conv_tol is followed by one more last-gain sub-phase and a new
+ * phase 3, up to max_repeats times (no counterpart in the reference).
+ */
+typedef struct cusna_mom_options {
+  int32_t struct_version; /* CUSNA_ABI_VERSION */
+  int32_t mode;           /* 0 unconditional, 1 conditional, 2 co-evolution */
+  int32_t P;              /* periods, 1..16 */
+  int32_t nsub;           /* phase-2 sub-phases, 0..8 */
+  int32_t n2[8];          /* iterations per sub-phase */
+  int32_t n1, nD, batch2, n3;
+  double firstg;          /* 0.4 in the reference (mom.R:26) */
+  double diagonalize;     /* 0.2 */
+  uint64_t seed;          /* Philox keys of round r, period q derive from (seed, r, q) */
+  int32_t use_graph;      /* replay phase-2 iterations from a captured HIP graph */
+  int32_t max_repeats;    /* extra sub-phase + phase 3 rounds until converged */
+  double conv_tol;        /* max|tconv| bound for max_repeats (0.1) */
+} cusna_mom_options;
+
+typedef struct cusna_mom_result {
+  int32_t struct_version; /* CUSNA_ABI_VERSION */
+  int32_t p;              /* out: parameter count */
+  /* caller-owned outputs, each optional (NULL = not wanted) */
+  double *theta, *se, *tconv, *sim_means; /* p */
+  double *D, *msf;                        /* p*p, row-major */
+  double *rate_est, *rate_sd;             /* P, conditional mode (mom.R:260, 265) */
+  double *phase3_stats;                   /* n3*p moment rows of phase 3 */
+  double *phase3_sp, *phase3_scp;         /* P*n3*p per-period moment / score rows */
+  double tconv_max;                       /* sqrt(dev' msf^-1 dev) */
+  double max_abs_tconv;
+  int64_t n_sims;                         /* chains simulated (each over all periods) */
+  int32_t repeats;
+  int32_t reserved;
+  double phase_seconds[3];
+  double wall_seconds;
+} cusna_mom_result;
+
+CUSNA_API cusna_status cusna_mom_estimate(const cusna_saom_problem *const *periods,
+                                          const uint8_t *const *xnet, const double *targets,
+                                          const double *rate_consts, const double *theta0,
+                                          const cusna_mom_options *options,
+                                          cusna_mom_result *result);
+
 #ifdef __cplusplus
 }
 #endif

@@ -128,3 +128,29 @@ def test_pack_bits_layout():
     # bit (j & 31) of word (j >> 5) is the tie to j
     i, j = np.argwhere(a)[0]
     assert (out[i, j >> 5] >> (j & 31)) & 1 == 1
+
+
+def test_mom_struct_layout_matches_header(tmp_path):
+    """ctypes mirrors of cusna_mom_options / cusna_mom_result against the C
+    layout of include/libcusna.h (no reference counterpart: the reference
+    leaves these types opaque, libcusna.h:541-542)."""
+    from cusna_amd.abi import cusna_mom_options, cusna_mom_result
+
+    lines = []
+    for T in (cusna_mom_options, cusna_mom_result):
+        lines.append(f'printf("sizeof.{T.__name__} %zu\\n", sizeof({T.__name__}));')
+        for name, _ in T._fields_:
+            lines.append(f'printf("{T.__name__}.{name} %zu\\n", offsetof({T.__name__}, {name}));')
+    src = tmp_path / "mom_offsets.c"
+    src.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "libcusna.h"\n'
+                   "int main(void) {\n" + "\n".join(lines) + "\nreturn 0;\n}\n")
+    exe = tmp_path / "mom_offsets"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                    str(exe)], check=True)
+    got = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True,
+                                                       text=True, check=True).stdout.split("\n")
+               if line)
+    for T in (cusna_mom_options, cusna_mom_result):
+        assert ctypes.sizeof(T) == int(got[f"sizeof.{T.__name__}"])
+        for name, _ in T._fields_:
+            assert getattr(T, name).offset == int(got[f"{T.__name__}.{name}"]), name
