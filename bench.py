@@ -12,8 +12,10 @@ statistics kernel, from inputs already resident in device memory (period
 plans). The timed region is bracketed by device synchronisation.
 
 Also reported: the period kernel against its VALU roof (HIP events on the
-plan's stream), and the reference CPU engine (compiled from the reference's
-own source under oracle/_ref) timed on this host's cores on a bounded sample.
+plan's stream), the reference CPU engine (compiled from the reference's own
+source under oracle/_ref) timed on this host's cores on a bounded sample, and
+("mom") the wall time of a method-of-moments fit to max|t| <= 0.1 on the
+device next to one pass of the same schedule through the reference engine.
 """
 from __future__ import annotations
 
@@ -70,6 +72,56 @@ def cpu_baseline(prob_factory, threads: int, chains: int) -> dict:
                        f"{wall:.1f} s wall, mean ministeps {float(r['steps'].mean()):.0f}")}
 
 
+def mom_measure(lib, cpu: bool, threads: int) -> dict:
+    """Config-1 fit: 50 actors, 3 waves, conditional MoM on density/recip/
+    transTrip with the reference's default schedule (n3 = 1000), repeated
+    last-gain sub-phases until max|tconv| <= 0.1. Wall time of the whole
+    cusna_mom_estimate call (set-up, three phases, host solves)."""
+    import numpy as np
+
+    from cusna_amd.mom import MomControl, MomEstimator
+    from cusna_amd.workloads import c1_estimation
+
+    data, eff = c1_estimation(lambda p: lib.simulate_period(p, want_end_nets=True)["a_ends"])
+    est = MomEstimator(data, eff, None, True, MomControl(n3=1000))
+    est.run(max_repeats=0)  # warm-up: code objects and first-launch set-up
+    t0 = time.perf_counter()
+    fit = est.run(max_repeats=20, conv_tol=0.1)
+    wall = time.perf_counter() - t0
+    out = {"workload": "config-1 fit: n=50, 3 waves, conditional, density/recip/transTrip, "
+                       "schedule n1=nD=1000, n2=40/40/60/80 x 256, n3=1000, then last-gain "
+                       "sub-phase + phase 3 until max|t|<=0.1",
+           "wall_s": round(wall, 3), "max_abs_tconv": round(float(fit.max_abs_tconv), 4),
+           "repeats": fit.repeats, "chains_simulated": fit.n_sims,
+           "phase_s": [round(x, 3) for x in fit.phase_seconds],
+           "theta": [round(float(x), 4) for x in fit.theta]}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import mom_oracle as MO
+        import oracle as O
+
+        if O.reference() is not None:
+            sim, kind = O.simulate_reference, "reference"
+        else:
+            sim, kind = O.simulate, "port"
+        O.oracle().oracle_set_threads(threads)
+        os.environ["OMP_NUM_THREADS"] = str(threads)
+        c = est.control
+        rows = MO.reference_rows(est, sim)
+        t0 = time.perf_counter()
+        r = MO.mom_run(rows, est.P, est.p, est.targets, est.default_theta0(), est.mode, c.n1,
+                       c.nD, c.batch2, c.n3, c.n2, c.firstg, c.diagonalize, c.seed)
+        cw = time.perf_counter() - t0
+        out["cpu"] = {"kind": kind, "cores": threads, "one_pass_wall_s": round(cw, 3),
+                      "one_pass_chains": r["n_sims"],
+                      "one_pass_max_abs_tconv": round(float(np.abs(r["tconv"]).max()), 4),
+                      "wall_s_at_gpu_chains": round(cw * fit.n_sims / r["n_sims"], 3),
+                      "sample": "one pass of the same schedule through the reference "
+                                "period engine (Python driver of R/mom.R's algebra); "
+                                "scaled by chains to the GPU run's chain count"}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,6 +132,7 @@ def main() -> int:
     ap.add_argument("--threads", type=int, default=0, help="block size (0 = library default)")
     ap.add_argument("--cpu-chains", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mom", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has one GPU, and the "
@@ -152,6 +205,9 @@ def main() -> int:
             lambda m: c2_problem(M=m, n=n, x0=w1, period_seed=11), threads, chains)
     for pl in plans:
         pl.close()
+    if not args.no_mom:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+        line["mom"] = mom_measure(lib, not args.no_cpu, threads)
     print(json.dumps(line), flush=True)
     return 0
 

@@ -86,6 +86,7 @@ class cusna_mom_options(C.Structure):
         ("n1", C.c_int32), ("nD", C.c_int32), ("batch2", C.c_int32), ("n3", C.c_int32),
         ("firstg", C.c_double), ("diagonalize", C.c_double), ("seed", C.c_uint64),
         ("use_graph", C.c_int32), ("max_repeats", C.c_int32), ("conv_tol", C.c_double),
+        ("beh_zbar", C.c_double), ("beh_brange", C.c_double), ("beh_bsimmean", C.c_double),
     ]
 
 

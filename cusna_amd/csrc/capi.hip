@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -106,6 +107,27 @@ int effect_flags(const int32_t* t, int k, int kb, const int32_t* dyn) {
   }
   if (kb > 0) f |= saom::F_Z;
   return f;
+}
+
+// Block shapes chosen by cusna_period_plan_autotune, by (word bound, words,
+// effects, effect flags); plan_create uses them when the caller leaves
+// `threads` at 0.
+struct ShapeKey {
+  int wmax, W64, k, flags;
+  bool operator<(const ShapeKey& o) const {
+    if (wmax != o.wmax) return wmax < o.wmax;
+    if (W64 != o.W64) return W64 < o.W64;
+    if (k != o.k) return k < o.k;
+    return flags < o.flags;
+  }
+};
+std::mutex g_shape_mu;
+std::map<ShapeKey, int> g_shape_cache;
+
+int cached_nw(const ShapeKey& key) {
+  std::lock_guard<std::mutex> lk(g_shape_mu);
+  auto it = g_shape_cache.find(key);
+  return it == g_shape_cache.end() ? 0 : it->second;
 }
 
 int pick_nw(int wmax, int threads) {
@@ -234,6 +256,10 @@ cusna_status plan_create(const cusna_saom_problem* pb, bool want_dense, const do
   }
   for (int e = 0; e < kb; ++e) P.bty[e] = pb->bty[e];
   pl->nw = pick_nw(wmax, pb->threads);
+  if (pb->threads <= 0) {
+    const int tuned = cached_nw(ShapeKey{wmax, W64, k, P.flags});
+    if (tuned > 0) pl->nw = tuned;
+  }
   if (saom::sim_lds_bytes(P, pl->nw, wmax) > kLdsLimit) {
     set_error("chain state exceeds the 160 KB LDS of a CU");
     return CUSNA_ERR_UNSUPPORTED;
@@ -427,12 +453,18 @@ cusna_status plan_run(cusna_period_plan* pl, uint64_t seed, bool want_stats, int
   }
   if (timed) HIP_TRY(hipEventRecord(pl->ev1, s));
   pl->stats_valid = false;
-  if (want_stats) {
+  if (want_stats) return plan_stats(pl, P.M, s);
+  return CUSNA_OK;
+}
+
+cusna_status plan_stats(cusna_period_plan* pl, int M_run, hipStream_t s) {
+  const saom::SimParams& P = pl->P;
+  {
     saom::StatParams SP;
     std::memset(&SP, 0, sizeof(SP));
     SP.n = pl->n;
     SP.k = pl->k;
-    SP.M = P.M;
+    SP.M = M_run;
     SP.W64 = pl->W64;
     SP.symflag = pl->symflag;
     for (int e = 0; e < pl->k; ++e) {
@@ -452,7 +484,7 @@ cusna_status plan_run(cusna_period_plan* pl, uint64_t seed, bool want_stats, int
     SP.od0 = pl->has_od0 ? pl->D.at<int32_t>(pl->o_od0) : nullptr;
     SP.id0 = pl->has_id0 ? pl->D.at<int32_t>(pl->o_id0) : nullptr;
     SP.out = pl->D.at<double>(pl->o_stats);
-    le = saom::launch_net_stats(SP, s);
+    const int le = saom::launch_net_stats(SP, s);
     if (le != 0) {
       set_error(std::string("stats kernel launch: ") + hipGetErrorString((hipError_t)le));
       return CUSNA_ERR_CUDA;
@@ -517,6 +549,143 @@ cusna_status simulate(const cusna_saom_problem* pb, cusna_saom_result* r, const 
   st = plan_run(pl.get(), pb->seed, r->stats != nullptr, -1, nullptr, true);
   if (st != CUSNA_OK) return st;
   return plan_fetch(pl.get(), r);
+}
+
+// Several independent problems ("groups": networks, periods or parameter
+// sets) in as few period-kernel launches as there are distinct launch shapes.
+// Within a launch the items run longest-first: groups sorted by an estimate of
+// their per-chain work, so that the hardware's in-order block dispatch keeps
+// the CUs busy until the end.
+cusna_status simulate_groups(const cusna_saom_problem* const* pbs, int G,
+                             cusna_saom_result* const* rs) {
+  hipStream_t s = thread_stream();
+  std::vector<std::unique_ptr<cusna_period_plan>> plans(G);
+  for (int g = 0; g < G; ++g) {
+    if (!pbs[g] || !rs[g]) return CUSNA_ERR_NULL_ARG;
+    cusna_status st = validate(pbs[g], rs[g]);
+    if (st != CUSNA_OK) return st;
+    if (pbs[g]->M > (1 << 20)) {
+      set_error("cusna_saom_simulate_groups: at most 2^20 chains per group");
+      return CUSNA_ERR_SHAPE;
+    }
+    cusna_period_plan* raw = nullptr;
+    st = plan_create(pbs[g], rs[g]->want_end_nets && rs[g]->a_ends, nullptr, nullptr, 0, s, &raw);
+    if (st != CUSNA_OK) return st;
+    plans[g].reset(raw);
+    st = plan_set_params(raw, pbs[g]->theta, pbs[g]->lam, pbs[g]->thetab, pbs[g]->lamb);
+    if (st != CUSNA_OK) return st;
+  }
+  // buckets of one launch shape
+  struct Bucket {
+    int nw, wmax;
+    bool simple;
+    std::vector<int> groups;
+  };
+  std::vector<Bucket> buckets;
+  for (int g = 0; g < G; ++g) {
+    const cusna_period_plan* pl = plans[g].get();
+    const bool simple = saom::simple_effects(pl->P.tid, pl->P.k);
+    Bucket* b = nullptr;
+    for (auto& x : buckets)
+      if (x.nw == pl->nw && x.wmax == pl->wmax && x.simple == simple) b = &x;
+    if (!b) {
+      buckets.push_back(Bucket{pl->nw, pl->wmax, simple, {}});
+      b = &buckets.back();
+    }
+    b->groups.push_back(g);
+  }
+  // expected ministeps x alternatives x words per chain
+  auto cost = [&](int g) {
+    const cusna_saom_problem* pb = pbs[g];
+    const double lam = pb->lam ? (double)pb->lam[0] : 1.0;
+    const double nact = pb->act ? pb->nact : pb->n;
+    const double steps = pb->conditional ? 2.0 * pb->target_dist + 1.0 : lam * nact + 1.0;
+    return steps * pb->n * (plans[g]->W64 + pb->k + 8.0);
+  };
+  size_t n_items = 0;
+  for (int g = 0; g < G; ++g) n_items += (size_t)plans[g]->M;
+  DeviceBlock D;
+  const size_t o_grp = D.add((size_t)G * sizeof(saom::SimParams)), o_items = D.add(n_items * 4);
+  HIP_TRY(D.alloc());
+  std::vector<saom::SimParams> grp(G);
+  for (int g = 0; g < G; ++g) grp[g] = plans[g]->P;
+  HIP_TRY(hipMemcpyAsync(D.at<char>(o_grp), grp.data(), G * sizeof(saom::SimParams),
+                         hipMemcpyHostToDevice, s));
+  std::vector<uint32_t> items;
+  items.reserve(n_items);
+  for (auto& b : buckets) {
+    std::stable_sort(b.groups.begin(), b.groups.end(),
+                     [&](int a, int c) { return cost(a) > cost(c); });
+    const size_t first = items.size();
+    size_t lds = 0;
+    for (int g : b.groups) {
+      for (int c = 0; c < plans[g]->M; ++c) items.push_back(((uint32_t)g << 20) | (uint32_t)c);
+      lds = std::max(lds, saom::sim_lds_bytes(plans[g]->P, b.nw, b.wmax));
+    }
+    saom::PeriodLaunch L;
+    L.Pd = D.at<saom::SimParams>(o_grp);
+    L.items = D.at<uint32_t>(o_items) + first;
+    L.grid = (int)(items.size() - first);
+    L.lds = lds;
+    L.key0 = L.key1 = 0u;
+    HIP_TRY(hipMemcpyAsync(D.at<uint32_t>(o_items) + first, items.data() + first,
+                           (items.size() - first) * 4, hipMemcpyHostToDevice, s));
+    const int le =
+        saom::launch_period_ex(L, saom::LaunchShape{b.nw, b.wmax}, false, b.simple, s);
+    if (le != 0) {
+      set_error(std::string("period kernel launch: ") + hipGetErrorString((hipError_t)le));
+      return CUSNA_ERR_CUDA;
+    }
+  }
+  for (int g = 0; g < G; ++g) {
+    if (rs[g]->stats) {
+      cusna_status st = plan_stats(plans[g].get(), plans[g]->M, s);
+      if (st != CUSNA_OK) return st;
+    }
+  }
+  for (int g = 0; g < G; ++g) {
+    cusna_status st = plan_fetch(plans[g].get(), rs[g]);
+    if (st != CUSNA_OK) return st;
+  }
+  HIP_TRY(hipStreamSynchronize(s));
+  return CUSNA_OK;
+}
+
+// Time the period kernel of `pl` in every compiled block shape on its first
+// `probe` chains (one untimed and one timed launch each) and keep the fastest.
+cusna_status autotune(cusna_period_plan* pl, int probe, int* best_threads) {
+  if (pl->replay) return CUSNA_ERR_UNSUPPORTED;
+  if (probe <= 0) probe = std::min(pl->M, 2048);
+  probe = std::min(probe, pl->M);
+  const int keep = pl->nw;
+  int best = keep;
+  double best_ms = 1e300;
+  for (int nw = 1; nw <= 16; nw *= 2) {
+    if (nw > pl->wmax || (pl->wmax == 16 && nw < 2)) continue;
+    if (saom::sim_lds_bytes(pl->P, nw, pl->wmax) > kLdsLimit) continue;
+    pl->nw = nw;
+    float ms = 0.0f;
+    for (int rep = 0; rep < 2; ++rep) {
+      cusna_status st = plan_run(pl, 0x5eedull + (uint64_t)rep, false, probe, nullptr, true);
+      if (st != CUSNA_OK) {
+        pl->nw = keep;
+        return st;
+      }
+      HIP_TRY(hipEventSynchronize(pl->ev1));
+      HIP_TRY(hipEventElapsedTime(&ms, pl->ev0, pl->ev1));
+    }
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = nw;
+    }
+  }
+  pl->nw = best;
+  {
+    std::lock_guard<std::mutex> lk(g_shape_mu);
+    g_shape_cache[ShapeKey{pl->wmax, pl->W64, pl->k, pl->P.flags}] = best;
+  }
+  if (best_threads) *best_threads = 64 * best;
+  return CUSNA_OK;
 }
 
 cusna_status net_stats_packed(const uint64_t* rows64_host, int n, int M, int k,
@@ -613,6 +782,22 @@ CUSNA_API cusna_status cusna_saom_simulate_period(const cusna_saom_problem* prob
                                                   cusna_saom_result* result) {
   g_last_error.clear();
   return simulate(problem, result, nullptr, nullptr, 0);
+}
+
+CUSNA_API cusna_status cusna_saom_simulate_groups(const cusna_saom_problem* const* problems,
+                                                  int32_t ngroups,
+                                                  cusna_saom_result* const* results) {
+  g_last_error.clear();
+  if (!problems || !results) return CUSNA_ERR_NULL_ARG;
+  if (ngroups < 1 || ngroups > 4096) return CUSNA_ERR_SHAPE;
+  if (device_count() < 1) return CUSNA_ERR_NO_CUDA;
+  return simulate_groups(problems, ngroups, results);
+}
+
+CUSNA_API cusna_status cusna_period_plan_autotune(cusna_period_plan* plan, int32_t probe_chains,
+                                                  int32_t* best_threads) {
+  if (!plan) return CUSNA_ERR_NULL_ARG;
+  return autotune(plan, probe_chains, best_threads);
 }
 
 CUSNA_API cusna_status cusna_saom_simulate_period_replay(const cusna_saom_problem* problem,

@@ -311,9 +311,12 @@ struct cusna_mom_engine {
   int32_t* bkdist = nullptr;
   uint32_t* keys = nullptr;
   unsigned long long* counter = nullptr;
+  saom::SimParams* groups = nullptr;         // device parameters of each period
+  uint32_t* items = nullptr;                 // P * Mmax launch items
   PeriodOut* per = nullptr;
   uint64_t seed = 0;
   long long n_sims = 0;
+  double beh_zbar = 0.0, beh_brange = 1.0, beh_bsim = 0.0;
   ~cusna_mom_engine() {
     if (s) (void)hipStreamDestroy(s);
   }
@@ -325,9 +328,26 @@ cusna_status engine_simulate(cusna_mom_engine* E, int M) {
   hipStream_t s = E->s;
   hipLaunchKernelGGL(mom_seed_kernel, dim3(1), dim3(64), 0, s, E->keys, E->counter, E->seed,
                      E->P);
+  // all periods in one launch: item b = (period b % P, chain b / P), so the
+  // first P * M items are chains 0..M-1 of every period
+  {
+    cusna_period_plan* p0 = E->plans[0].get();
+    saom::PeriodLaunch L;
+    L.Pd = E->groups;
+    L.items = E->items;
+    L.grid = E->P * M;
+    L.lds = saom::sim_lds_bytes(p0->P, p0->nw, p0->wmax);
+    L.key0 = L.key1 = 0u;
+    const int le = saom::launch_period_ex(L, saom::LaunchShape{p0->nw, p0->wmax}, false,
+                                          saom::simple_effects(p0->P.tid, p0->P.k), s);
+    if (le != 0) {
+      set_error(std::string("period kernel launch: ") + hipGetErrorString((hipError_t)le));
+      return CUSNA_ERR_CUDA;
+    }
+  }
   for (int pp = 0; pp < E->P; ++pp) {
     cusna_period_plan* pl = E->plans[pp].get();
-    cusna_status st = plan_run(pl, 0, true, M, s, false);
+    cusna_status st = plan_stats(pl, M, s);
     if (st != CUSNA_OK) return st;
     if (E->mode == 2) {
       saom::BehStatParams BP;
@@ -340,9 +360,9 @@ cusna_status engine_simulate(cusna_mom_engine* E, int M) {
       BP.z = pl->P.z_end;
       BP.xnet = E->xnet + (size_t)pp * E->n * BP.W64;
       BP.bkdist = E->bkdist + (size_t)pp * E->n;
-      BP.zbar = pl->P.zbar;
-      BP.brange = pl->P.brange;
-      BP.bsim = pl->P.bsim;
+      BP.zbar = E->beh_zbar;
+      BP.brange = E->beh_brange;
+      BP.bsim = E->beh_bsim;
       BP.out = E->bstats + (size_t)pp * E->Mmax * E->kb;
       if (saom::launch_beh_stats(BP, s)) return CUSNA_ERR_CUDA;
     }
@@ -428,6 +448,10 @@ CUSNA_API cusna_status cusna_mom_estimate(const cusna_saom_problem* const* perio
       return CUSNA_ERR_SHAPE;
     }
   const int Mmax = std::max(std::max(opt->n1, opt->n3), std::max(opt->nD, opt->batch2));
+  if (Mmax > (1 << 20)) {
+    set_error("cusna_mom_estimate: at most 2^20 chains per simulation round");
+    return CUSNA_ERR_SHAPE;
+  }
 
   std::unique_ptr<cusna_mom_engine> E(new cusna_mom_engine());
   E->P = P;
@@ -439,6 +463,9 @@ CUSNA_API cusna_status cusna_mom_estimate(const cusna_saom_problem* const* perio
   E->Mmax = Mmax;
   E->n_rates = mode == 1 ? 0 : (mode == 2 ? 2 * P : P);
   E->seed = opt->seed;
+  E->beh_zbar = opt->beh_zbar;
+  E->beh_brange = opt->beh_brange;
+  E->beh_bsim = opt->beh_bsimmean;
   HIP_TRY(hipStreamCreateWithFlags(&E->s, hipStreamNonBlocking));
   const int W64 = (n + 63) / 64;
   DeviceBlock& D = E->D;
@@ -451,6 +478,7 @@ CUSNA_API cusna_status cusna_mom_estimate(const cusna_saom_problem* const* perio
   const size_t o_bst = D.add((size_t)P * Mmax * (kb > 0 ? kb : 1) * 8);
   const size_t o_xn = D.add((size_t)P * n * W64 * 8), o_bk = D.add((size_t)P * n * 4);
   const size_t o_keys = D.add(P * 8), o_cnt = D.add(8), o_per = D.add(P * sizeof(PeriodOut));
+  const size_t o_grp = D.add(P * sizeof(saom::SimParams)), o_items = D.add((size_t)P * Mmax * 4);
   HIP_TRY(D.alloc());
   E->theta = D.at<double>(o_theta);
   E->theta_sim = D.at<double>(o_tsim);
@@ -472,6 +500,8 @@ CUSNA_API cusna_status cusna_mom_estimate(const cusna_saom_problem* const* perio
   E->keys = D.at<uint32_t>(o_keys);
   E->counter = D.at<unsigned long long>(o_cnt);
   E->per = D.at<PeriodOut>(o_per);
+  E->groups = D.at<saom::SimParams>(o_grp);
+  E->items = D.at<uint32_t>(o_items);
   hipStream_t s = E->s;
 
   std::vector<double> rc(P, 0.0);
@@ -546,6 +576,21 @@ CUSNA_API cusna_status cusna_mom_estimate(const cusna_saom_problem* const* perio
     }
   }
   HIP_TRY(hipMemcpyAsync(E->per, per.data(), P * sizeof(PeriodOut), hipMemcpyHostToDevice, s));
+  std::vector<saom::SimParams> grp(P);
+  for (int pp = 0; pp < P; ++pp) {
+    cusna_period_plan* pl = E->plans[pp].get();
+    if (pl->nw != E->plans[0]->nw || pl->wmax != E->plans[0]->wmax) {
+      set_error("cusna_mom_estimate: periods need the same launch shape");
+      return CUSNA_ERR_SHAPE;
+    }
+    grp[pp] = pl->P;
+  }
+  std::vector<uint32_t> items((size_t)P * Mmax);
+  for (size_t b = 0; b < items.size(); ++b)
+    items[b] = ((uint32_t)(b % P) << 20) | (uint32_t)(b / P);
+  HIP_TRY(hipMemcpyAsync(E->groups, grp.data(), P * sizeof(saom::SimParams),
+                         hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(E->items, items.data(), items.size() * 4, hipMemcpyHostToDevice, s));
   HIP_TRY(hipStreamSynchronize(s));
 
   Mat Dm((size_t)p * p), mean(p);

@@ -88,6 +88,8 @@ cusna_status plan_set_params(cusna_period_plan* pl, const float* theta, const fl
 // the HIP events around the period kernel.
 cusna_status plan_run(cusna_period_plan* pl, uint64_t seed, bool want_stats, int M_run,
                       hipStream_t stream, bool timed);
+// The statistics kernel over the first M_run chains' end networks.
+cusna_status plan_stats(cusna_period_plan* pl, int M_run, hipStream_t stream);
 cusna_status plan_fetch(cusna_period_plan* pl, cusna_saom_result* r);
 
 }  // namespace cusna_impl

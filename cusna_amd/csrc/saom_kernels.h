@@ -65,6 +65,21 @@ bool simple_effects(const int* type_ids, int k);
 int launch_period(const SimParams& P, const SimParams* Pd, LaunchShape s, bool replay,
                   uint32_t key0, uint32_t key1, void* stream);
 
+// General form. One group: Pd, grid = chains, key0/key1 the Philox key.
+// Several groups: Pd[g] are group g's device parameters (its Philox key in
+// their key0/key1 or keys), items[b] = (g << 20) | chain names block b's
+// work; grid = number of items; lds = the largest of the groups' LDS sizes.
+// All groups of a launch share the word bound and the block shape.
+struct PeriodLaunch {
+  const SimParams* Pd;
+  const uint32_t* items;
+  int grid;
+  size_t lds;
+  uint32_t key0, key1;
+};
+int launch_period_ex(const PeriodLaunch& L, LaunchShape s, bool replay, bool simple,
+                     void* stream);
+
 // Statistics of M packed networks (rows u64, n*W64 per network).
 struct StatParams {
   int n, k, M, W64, symflag;

@@ -5,9 +5,8 @@
 
 namespace saom {
 
-#define SAOM_DECL(W)                                                                     \
-  int launch_period_w##W(const SimParams& P, const SimParams* Pd, int nw, bool replay,   \
-                         bool simple, uint32_t k0, uint32_t k1, void* stream);
+#define SAOM_DECL(W) \
+  int launch_period_w##W(const PeriodLaunch& L, int nw, bool replay, bool simple, void* stream);
 SAOM_DECL(1)
 SAOM_DECL(2)
 SAOM_DECL(4)
@@ -32,17 +31,29 @@ bool simple_effects(const int* t, int k) {
   return true;
 }
 
-int launch_period(const SimParams& P, const SimParams* Pd, LaunchShape s, bool replay,
-                  uint32_t k0, uint32_t k1, void* stream) {
-  const bool simple = simple_effects(P.tid, P.k);
+int launch_period_ex(const PeriodLaunch& L, LaunchShape s, bool replay, bool simple,
+                     void* stream) {
+  if (L.grid < 1) return 0;
   switch (s.wmax) {
-    case 1: return launch_period_w1(P, Pd, s.nw, replay, simple, k0, k1, stream);
-    case 2: return launch_period_w2(P, Pd, s.nw, replay, simple, k0, k1, stream);
-    case 4: return launch_period_w4(P, Pd, s.nw, replay, simple, k0, k1, stream);
-    case 8: return launch_period_w8(P, Pd, s.nw, replay, simple, k0, k1, stream);
-    case 16: return launch_period_w16(P, Pd, s.nw, replay, simple, k0, k1, stream);
+    case 1: return launch_period_w1(L, s.nw, replay, simple, stream);
+    case 2: return launch_period_w2(L, s.nw, replay, simple, stream);
+    case 4: return launch_period_w4(L, s.nw, replay, simple, stream);
+    case 8: return launch_period_w8(L, s.nw, replay, simple, stream);
+    case 16: return launch_period_w16(L, s.nw, replay, simple, stream);
     default: return (int)hipErrorInvalidValue;
   }
+}
+
+int launch_period(const SimParams& P, const SimParams* Pd, LaunchShape s, bool replay,
+                  uint32_t k0, uint32_t k1, void* stream) {
+  PeriodLaunch L;
+  L.Pd = Pd;
+  L.items = nullptr;
+  L.grid = P.M;
+  L.lds = sim_lds_bytes(P, s.nw, s.wmax);
+  L.key0 = k0;
+  L.key1 = k1;
+  return launch_period_ex(L, s, replay, simple_effects(P.tid, P.k), stream);
 }
 
 }  // namespace saom

@@ -523,19 +523,25 @@ __host__ __device__ inline LdsLayout lds_layout(int n, int k, int kb, int wmax, 
 template <int NW, int R, int WMAX, bool REPLAY, bool SIMPLE>
 __global__ void __launch_bounds__(64 * NW)
     __attribute__((amdgpu_waves_per_eu(NW == 1 ? 1 : (NW == 2 ? 2 : 3))))
-    saom_period_kernel(const SimParams* __restrict__ Pp, uint32_t key0_arg, uint32_t key1_arg) {
-  const SimParams& P = *Pp;
-  // the Philox key comes from the launch, or from device memory when a
-  // captured graph replays the launch with keys a device kernel has written
-  const uint32_t key0 = P.keys ? P.keys[0] : key0_arg;
-  const uint32_t key1 = P.keys ? P.keys[1] : key1_arg;
+    saom_period_kernel(const SimParams* __restrict__ Pp, const uint32_t* __restrict__ items,
+                       uint32_t key0_arg, uint32_t key1_arg) {
+  // One group: block b runs chain b of Pp[0]. Several groups in one launch:
+  // item b names the group g (high 12 bits, parameters Pp[g]) and the chain
+  // (low 20 bits).
+  const uint32_t it = items ? items[blockIdx.x] : (uint32_t)blockIdx.x;
+  const int chain = items ? (int)(it & 0xFFFFFu) : (int)blockIdx.x;
+  const SimParams& P = Pp[items ? (it >> 20) : 0u];
+  // the Philox key comes from the launch (one group) or the group's
+  // parameters, or from device memory when a captured graph replays the
+  // launch with keys a device kernel has written
+  const uint32_t key0 = P.keys ? P.keys[0] : (items ? P.key0 : key0_arg);
+  const uint32_t key1 = P.keys ? P.keys[1] : (items ? P.key1 : key1_arg);
   constexpr int NT = 64 * NW;
   extern __shared__ __align__(16) unsigned char smem[];
   // word-major bit rows: word w of actor h at rows[w * RS + h], RS = n, so the
   // lanes of a wave reading word w of consecutive actors hit consecutive words
   const int n = P.n, k = P.k, kb = P.kb, W64 = P.W64, RS = P.RS;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int chain = blockIdx.x;
   const LdsLayout L = lds_layout(n, k, kb, WMAX, RS, NW, P.flags);
   uint64_t* rows = (uint64_t*)(smem + L.rows);
   int* indeg = (int*)(smem + L.indeg);
@@ -992,26 +998,24 @@ __global__ void __launch_bounds__(64 * NW)
 }
 
 template <int NW, int WMAX, bool REPLAY, bool SIMPLE>
-hipError_t launch_shape(const SimParams& P, const SimParams* Pd, uint32_t k0, uint32_t k1,
-                        hipStream_t st) {
+hipError_t launch_shape(const PeriodLaunch& L, hipStream_t st) {
   constexpr int R = WMAX / NW;
-  const size_t lds = lds_layout(P.n, P.k, P.kb, WMAX, P.RS, NW, P.flags).total;
   auto kern = saom_period_kernel<NW, R, WMAX, REPLAY, SIMPLE>;
   // raise the dynamic LDS limit once per instantiation (not inside a graph
   // capture); 160 KB is the whole LDS of a gfx950 CU
   static const hipError_t attr = hipFuncSetAttribute(
       (const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
   if (attr != hipSuccess) return attr;
-  hipLaunchKernelGGL(kern, dim3(P.M), dim3(64 * NW), lds, st, Pd, k0, k1);
+  hipLaunchKernelGGL(kern, dim3(L.grid), dim3(64 * NW), L.lds, st, L.Pd, L.items, L.key0,
+                     L.key1);
   return hipGetLastError();
 }
 
 template <int NW, int WMAX>
-hipError_t launch_variant(const SimParams& P, const SimParams* Pd, bool replay, bool simple,
-                          uint32_t k0, uint32_t k1, hipStream_t st) {
-  if (replay) return launch_shape<NW, WMAX, true, false>(P, Pd, k0, k1, st);
-  if (simple) return launch_shape<NW, WMAX, false, true>(P, Pd, k0, k1, st);
-  return launch_shape<NW, WMAX, false, false>(P, Pd, k0, k1, st);
+hipError_t launch_variant(const PeriodLaunch& L, bool replay, bool simple, hipStream_t st) {
+  if (replay) return launch_shape<NW, WMAX, true, false>(L, st);
+  if (simple) return launch_shape<NW, WMAX, false, true>(L, st);
+  return launch_shape<NW, WMAX, false, false>(L, st);
 }
 
 }  // namespace
@@ -1024,26 +1028,25 @@ hipError_t launch_variant(const SimParams& P, const SimParams* Pd, bool replay, 
 #define SAOM_CAT2(a, b) a##b
 #define SAOM_CAT(a, b) SAOM_CAT2(a, b)
 
-int SAOM_CAT(launch_period_w, SAOM_WMAX)(const SimParams& P, const SimParams* Pd, int nw,
-                                         bool replay, bool simple, uint32_t k0, uint32_t k1,
-                                         void* stream) {
+int SAOM_CAT(launch_period_w, SAOM_WMAX)(const PeriodLaunch& L, int nw, bool replay,
+                                         bool simple, void* stream) {
   hipStream_t st = (hipStream_t)stream;
   constexpr int WM = SAOM_WMAX;
   hipError_t e = hipErrorInvalidValue;
   if constexpr (WM <= 8) {
-    if (nw == 1) e = launch_variant<1, WM>(P, Pd, replay, simple, k0, k1, st);
+    if (nw == 1) e = launch_variant<1, WM>(L, replay, simple, st);
   }
   if constexpr (WM >= 2) {
-    if (nw == 2) e = launch_variant<2, WM>(P, Pd, replay, simple, k0, k1, st);
+    if (nw == 2) e = launch_variant<2, WM>(L, replay, simple, st);
   }
   if constexpr (WM >= 4) {
-    if (nw == 4) e = launch_variant<4, WM>(P, Pd, replay, simple, k0, k1, st);
+    if (nw == 4) e = launch_variant<4, WM>(L, replay, simple, st);
   }
   if constexpr (WM >= 8) {
-    if (nw == 8) e = launch_variant<8, WM>(P, Pd, replay, simple, k0, k1, st);
+    if (nw == 8) e = launch_variant<8, WM>(L, replay, simple, st);
   }
   if constexpr (WM >= 16) {
-    if (nw == 16) e = launch_variant<16, WM>(P, Pd, replay, simple, k0, k1, st);
+    if (nw == 16) e = launch_variant<16, WM>(L, replay, simple, st);
   }
   return (int)e;
 }

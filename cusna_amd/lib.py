@@ -25,7 +25,8 @@ EXPORTS = (
     "cusna_last_error", "cusna_period_plan_create", "cusna_period_plan_set_params",
     "cusna_period_plan_run", "cusna_period_plan_sync", "cusna_period_plan_fetch",
     "cusna_period_plan_kernel_ms", "cusna_period_plan_shape", "cusna_period_plan_destroy",
-    "cusna_gumbel_bound_check", "cusna_mom_estimate",
+    "cusna_gumbel_bound_check", "cusna_mom_estimate", "cusna_period_plan_autotune",
+    "cusna_saom_simulate_groups",
 )
 
 _lib = None
@@ -80,6 +81,10 @@ def load(path: str | None = None) -> C.CDLL:
                                            C.POINTER(C.c_double)]
     L.cusna_period_plan_destroy.restype = None
     L.cusna_period_plan_destroy.argtypes = [_vp]
+    L.cusna_period_plan_autotune.restype = C.c_int
+    L.cusna_period_plan_autotune.argtypes = [_vp, C.c_int32, C.POINTER(C.c_int32)]
+    L.cusna_saom_simulate_groups.restype = C.c_int
+    L.cusna_saom_simulate_groups.argtypes = [_vp, C.c_int32, _vp]
     L.cusna_mom_estimate.restype = C.c_int
     L.cusna_mom_estimate.argtypes = [_vp, _vp, _vp, _vp, _vp, C.POINTER(cusna_mom_options),
                                      C.POINTER(cusna_mom_result)]
@@ -191,6 +196,18 @@ def behavior_stats(z, bty, xnet, bkdist, zbar, brange, bsimmean) -> np.ndarray:
     return out
 
 
+def simulate_groups(problems: list, want_end_nets: bool = False) -> list:
+    """Several period problems in one call (cusna_saom_simulate_groups)."""
+    L = load()
+    G = len(problems)
+    built = [pr.build() for pr in problems]
+    res = [SaomResult(pr.n, pr.k, pr.M, pr.kb, want_end_nets) for pr in problems]
+    parr = (_vp * G)(*[C.addressof(b) for b in built])
+    rarr = (_vp * G)(*[C.addressof(r.c) for r in res])
+    _check("cusna_saom_simulate_groups", L.cusna_saom_simulate_groups(parr, G, rarr))
+    return [r.as_dict() for r in res]
+
+
 def mom_estimate(problems: list, targets, theta0, mode: int, n1: int, nD: int, batch2: int,
                  n3: int, n2: list, firstg: float = 0.4, diagonalize: float = 0.2,
                  seed: int = 1234, xnets: list | None = None, rate_consts=None,
@@ -226,6 +243,10 @@ def mom_estimate(problems: list, targets, theta0, mode: int, n1: int, nD: int, b
     o.firstg, o.diagonalize = float(firstg), float(diagonalize)
     o.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
     o.use_graph, o.max_repeats, o.conv_tol = int(use_graph), int(max_repeats), float(conv_tol)
+    if mode == 2:
+        pr0 = problems[0]
+        o.beh_zbar, o.beh_brange, o.beh_bsimmean = (float(pr0.zbar), float(pr0.brange),
+                                                     float(pr0.bsimmean))
     out = {name: np.zeros(p) for name in ("theta", "se", "tconv", "sim_means")}
     out["D"] = np.zeros((p, p))
     out["msf"] = np.zeros((p, p))
@@ -304,6 +325,14 @@ class PeriodPlan:
         res = SaomResult(self.prob.n, self.prob.k, self.prob.M, self.prob.kb, want_end_nets)
         _check("cusna_period_plan_fetch", self.L.cusna_period_plan_fetch(self.h, C.byref(res.c)))
         return res.as_dict()
+
+    def autotune(self, probe_chains: int = 0) -> int:
+        """Pick the fastest block shape (cusna_period_plan_autotune); returns
+        threads per chain."""
+        out = C.c_int32()
+        _check("cusna_period_plan_autotune",
+               self.L.cusna_period_plan_autotune(self.h, int(probe_chains), C.byref(out)))
+        return int(out.value)
 
     def kernel_ms(self) -> float:
         return float(self.L.cusna_period_plan_kernel_ms(self.h))

@@ -4,7 +4,7 @@ Each builder returns the inputs of one period call (a ``SaomProblem``) from a
 fixed seed, using the same marshalling as the R glue (float64 descriptors,
 NULL masks, float32 theta/lambda):
 
-  s50_like_panel   50 actors, 3 waves: wave 1 Poisson(2.3) out-ties capped at
+  c1_waves         50 actors, 3 waves: wave 1 Poisson(2.3) out-ties capped at
                    6, reciprocated with p = 0.4; waves 2-3 simulated.
   c2_problem       500 actors, directed, rate 8 per period, mean outdegree 5;
                    density -2.8, recip 1.8, transTrip 0.3, cycle3 -0.1,
@@ -71,6 +71,29 @@ def effect_problem(x0: np.ndarray, effects: list, theta, lam: float, M: int, see
         prob.zmin, prob.zmax = beh.zmin, beh.zmax
         prob.zbar, prob.brange, prob.bsimmean = beh.overall_mean, beh.range, beh.sim_mean
     return prob
+
+
+C1_THETA = [-2.5, 2.0, 0.4]
+C1_EFFECTS = ["density", "recip", "transTrip"]
+
+
+def c1_waves(simulate, n: int = 50, seed: int = 1) -> list:
+    """Three waves of config 1: wave 1 with min(Poisson(2.3), 6) out-ties per
+    actor, each reciprocated with p = 0.4; waves 2 and 3 are seeded simulated
+    periods at rate 6 with density -2.5, recip 2.0, transTrip 0.4
+    (``simulate`` runs a SaomProblem and returns a_ends)."""
+    rng = np.random.default_rng(seed)
+    waves = [poisson_capped_network(n, 2.3, 6, 0.4, rng)]
+    for q in range(2):
+        prob = effect_problem(waves[-1], [Effect(e) for e in C1_EFFECTS], C1_THETA, 6.0, 1,
+                              seed * 1000 + q)
+        waves.append(np.asarray(simulate(prob)[0], np.uint8))
+    return waves
+
+
+def c1_estimation(simulate, n: int = 50, seed: int = 1):
+    """(data, effects) of the config-1 fit: conditional, density/recip/transTrip."""
+    return build_saom_data(c1_waves(simulate, n, seed)), [Effect(e) for e in C1_EFFECTS]
 
 
 def c2_covariate(n: int = 500, seed: int = 2) -> np.ndarray:

@@ -275,6 +275,33 @@ CUSNA_API int32_t cusna_period_plan_shape(const cusna_period_plan *plan);
 CUSNA_API void cusna_period_plan_destroy(cusna_period_plan *plan);
 
 /*
+ * Block-shape autotuner (no counterpart in the reference, whose CPU engine
+ * has one shape: an OpenMP loop over chains, libcusna_saom_cpu.cpp:384-387).
+ * Times the period kernel in each compiled block shape (64 to 1024 threads
+ * per chain) on the plan's first probe_chains chains (<= 0: min(M, 2048)),
+ * keeps the fastest for this plan and for later plans of the same network
+ * size and effect set created with threads = 0, and returns it in
+ * *best_threads. The probe runs overwrite the plan's outputs. Trajectories,
+ * statistics and end networks do not depend on the shape; scores can differ
+ * in the last float64 bits (their sums run in a shape-dependent order).
+ */
+CUSNA_API cusna_status cusna_period_plan_autotune(cusna_period_plan *plan, int32_t probe_chains,
+                                                  int32_t *best_threads);
+
+/*
+ * Several independent period problems ("groups": networks of different
+ * sizes, periods, parameter sets) in one call: results[g] is what
+ * cusna_saom_simulate_period(problems[g], results[g]) returns. The groups run
+ * in one period-kernel launch per distinct launch shape, items ordered
+ * longest-first. Up to 4096 groups of up to 2^20 chains each. No counterpart
+ * in the reference (whose R layer calls the period function once per group
+ * and period, mom.R:143-148).
+ */
+CUSNA_API cusna_status cusna_saom_simulate_groups(const cusna_saom_problem *const *problems,
+                                                  int32_t ngroups,
+                                                  cusna_saom_result *const *results);
+
+/*
  * Method-of-moments estimator (the reference's Layer-2 estimator, reserved as
  * opaque types at libcusna.h:541-542 and implemented in R at R/mom.R:91-269).
  * The caller prepares one cusna_saom_problem per period (inputs only; M,
@@ -304,6 +331,10 @@ typedef struct cusna_mom_options {
   int32_t use_graph;      /* replay phase-2 iterations from a captured HIP graph */
   int32_t max_repeats;    /* extra sub-phase + phase 3 rounds until converged */
   double conv_tol;        /* max|tconv| bound for max_repeats (0.1) */
+  /* co-evolution: the float64 behaviour mean, range and similarity mean the
+   * behaviour statistics take (mom.R:114-115); the period problems carry
+   * their float32 roundings for the simulator */
+  double beh_zbar, beh_brange, beh_bsimmean;
 } cusna_mom_options;
 
 typedef struct cusna_mom_result {
