@@ -131,6 +131,7 @@ int cached_nw(const ShapeKey& key) {
 }
 
 int pick_nw(int wmax, int threads) {
+  if (wmax > 16) return 16;  // streamed networks: one shape
   if (threads > 0) {
     int nw = threads / 64;
     if (nw < 1) nw = 1;
@@ -203,10 +204,11 @@ cusna_status plan_create(const cusna_saom_problem* pb, bool want_dense, const do
   const int n = pb->n, W = pb->W, k = pb->k, M = pb->M, kb = pb->kb;
   const int W64 = (n + 63) / 64;
   const int wmax = wmax_for(W64);
-  if (wmax > 16) {
-    set_error("n above 1024 is not supported by the LDS-resident kernel");
-    return CUSNA_ERR_UNSUPPORTED;
+  if (wmax > 128) {
+    set_error("n above 8192");
+    return CUSNA_ERR_SHAPE;
   }
+  const bool stream_rows = wmax > 16;  // network rows in device memory
   const bool replay = log_u0 != nullptr;
   std::unique_ptr<cusna_period_plan> pl(new cusna_period_plan());
   pl->n = n;
@@ -264,15 +266,7 @@ cusna_status plan_create(const cusna_saom_problem* pb, bool want_dense, const do
     set_error("chain state exceeds the 160 KB LDS of a CU");
     return CUSNA_ERR_UNSUPPORTED;
   }
-  {
-    bool need_cols = false;
-    for (int e = 0; e < k; ++e)
-      if (P.tid[e] == 27 || P.tid[e] == 28 || P.tid[e] == 30) need_cols = true;
-    if (saom::net_stats_lds(n, W64, need_cols) > kLdsLimit) {
-      set_error("statistics of this network size need more than the 160 KB LDS of a CU");
-      return CUSNA_ERR_UNSUPPORTED;
-    }
-  }
+  const size_t stat_words = saom::net_stats_scratch_words(n, W64, saom::stats_need_cols(P.tid, k));
 
   // host-side staging
   const std::vector<uint64_t> x0 = rows64(pb->x0_out, n, W, W64, 0);
@@ -330,6 +324,9 @@ cusna_status plan_create(const cusna_saom_problem* pb, bool want_dense, const do
   const size_t o_bdist = D.add((size_t)M * 4), o_zend = D.add((size_t)M * n * 4);
   pl->o_stats = D.add((size_t)M * k * 8);
   pl->o_params = D.add(sizeof(saom::SimParams));
+  const size_t o_work = D.add(stream_rows ? (size_t)M * rowsb : 0);
+  pl->o_sscr = D.add((size_t)M * stat_words * 8);
+  pl->has_sscr = stat_words > 0;
   if (want_dense) pl->o_dense = D.add((size_t)M * n * n);
   size_t o_lu0 = 0, o_llu = 0;
   if (replay) {
@@ -405,6 +402,7 @@ cusna_status plan_create(const cusna_saom_problem* pb, bool want_dense, const do
   P.bdist = D.at<int32_t>(o_bdist);
   P.z_end = kb > 0 ? D.at<int32_t>(o_zend) : nullptr;
   P.actor_dist = nullptr;  // written by the reference only when nrate > 0 (:406-409)
+  P.work = stream_rows ? D.at<uint64_t>(o_work) : nullptr;
   HIP_TRY(hipMemcpyAsync(D.at<char>(pl->o_params), &P, sizeof(P), hipMemcpyHostToDevice, s));
   HIP_TRY(hipStreamSynchronize(s));
   *out = pl.release();
@@ -484,6 +482,7 @@ cusna_status plan_stats(cusna_period_plan* pl, int M_run, hipStream_t s) {
     SP.od0 = pl->has_od0 ? pl->D.at<int32_t>(pl->o_od0) : nullptr;
     SP.id0 = pl->has_id0 ? pl->D.at<int32_t>(pl->o_id0) : nullptr;
     SP.out = pl->D.at<double>(pl->o_stats);
+    SP.scratch = pl->has_sscr ? pl->D.at<uint64_t>(pl->o_sscr) : nullptr;
     const int le = saom::launch_net_stats(SP, s);
     if (le != 0) {
       set_error(std::string("stats kernel launch: ") + hipGetErrorString((hipError_t)le));
@@ -700,13 +699,8 @@ cusna_status net_stats_packed(const uint64_t* rows64_host, int n, int M, int k,
   }
   if (k > CUSNA_MAXEFF) return CUSNA_ERR_SHAPE;
   const int W64 = (n + 63) / 64;
-  bool need_cols = false;
-  for (int e = 0; e < k; ++e)
-    if (type_ids[e] == 27 || type_ids[e] == 28 || type_ids[e] == 30) need_cols = true;
-  if (saom::net_stats_lds(n, W64, need_cols) > kLdsLimit) {
-    set_error("network too large for the LDS-resident statistics kernel");
-    return CUSNA_ERR_UNSUPPORTED;
-  }
+  const size_t stat_words =
+      saom::net_stats_scratch_words(n, W64, saom::stats_need_cols(type_ids, k));
   saom::StatParams SP;
   std::memset(&SP, 0, sizeof(SP));
   SP.n = n;
@@ -728,6 +722,7 @@ cusna_status net_stats_packed(const uint64_t* rows64_host, int n, int M, int k,
   const size_t o_wtab = D.add((size_t)k * (n + 1) * 8), o_vmiss = D.add((size_t)k * n);
   const size_t o_od0 = D.add((size_t)n * 4), o_id0 = D.add((size_t)n * 4);
   const size_t o_out = D.add((size_t)M * k * 8);
+  const size_t o_scr = D.add((size_t)M * stat_words * 8);
   HIP_TRY(D.alloc());
   hipStream_t s = thread_stream();
   std::vector<double> zk((size_t)k * (n + 1), 0.0);
@@ -754,6 +749,7 @@ cusna_status net_stats_packed(const uint64_t* rows64_host, int n, int M, int k,
   SP.od0 = od0 ? D.at<int32_t>(o_od0) : nullptr;
   SP.id0 = id0 ? D.at<int32_t>(o_id0) : nullptr;
   SP.out = D.at<double>(o_out);
+  SP.scratch = stat_words ? D.at<uint64_t>(o_scr) : nullptr;
   const int le = saom::launch_net_stats(SP, s);
   if (le != 0) {
     set_error(std::string("stats kernel launch: ") + hipGetErrorString((hipError_t)le));
@@ -913,7 +909,6 @@ CUSNA_API cusna_status cusna_behavior_stats(const int32_t* z_batch, int32_t n, i
     set_error("no HIP device");
     return CUSNA_ERR_NO_CUDA;
   }
-  if ((size_t)n * 5 * 8 > kLdsLimit) return CUSNA_ERR_UNSUPPORTED;
   const int W64 = (n + 63) / 64;
   const std::vector<uint64_t> xr = dense_rows64(xnet, n, W64);
   saom::BehStatParams BP;
@@ -929,6 +924,7 @@ CUSNA_API cusna_status cusna_behavior_stats(const int32_t* z_batch, int32_t n, i
   DeviceBlock D;
   const size_t o_z = D.add((size_t)M * n * 4), o_x = D.add(xr.size() * 8);
   const size_t o_bk = D.add((size_t)n * 4), o_out = D.add((size_t)M * kb * 8);
+  const size_t o_scr = D.add((size_t)M * saom::beh_stats_scratch_doubles(n) * 8);
   HIP_TRY(D.alloc());
   hipStream_t s = thread_stream();
   HIP_TRY(hipMemcpyAsync(D.at<char>(o_z), z_batch, (size_t)M * n * 4, hipMemcpyHostToDevice, s));
@@ -939,6 +935,7 @@ CUSNA_API cusna_status cusna_behavior_stats(const int32_t* z_batch, int32_t n, i
   BP.xnet = D.at<uint64_t>(o_x);
   BP.bkdist = bkdist ? D.at<int32_t>(o_bk) : nullptr;
   BP.out = D.at<double>(o_out);
+  BP.scratch = saom::beh_stats_scratch_doubles(n) ? D.at<double>(o_scr) : nullptr;
   if (saom::launch_beh_stats(BP, s)) return CUSNA_ERR_CUDA;
   HIP_TRY(hipMemcpyAsync(out, D.at<char>(o_out), (size_t)M * kb * 8, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));

@@ -307,6 +307,7 @@ struct cusna_mom_engine {
   double *SP = nullptr, *SC = nullptr, *mSP = nullptr, *mSC = nullptr, *Dm = nullptr,
          *msf = nullptr;
   double* bstats = nullptr;
+  double* bscratch = nullptr;  // behaviour statistics above the LDS (else null)
   uint64_t* xnet = nullptr;
   int32_t* bkdist = nullptr;
   uint32_t* keys = nullptr;
@@ -364,6 +365,7 @@ cusna_status engine_simulate(cusna_mom_engine* E, int M) {
       BP.brange = E->beh_brange;
       BP.bsim = E->beh_bsim;
       BP.out = E->bstats + (size_t)pp * E->Mmax * E->kb;
+      BP.scratch = E->bscratch;
       if (saom::launch_beh_stats(BP, s)) return CUSNA_ERR_CUDA;
     }
   }
@@ -479,6 +481,8 @@ CUSNA_API cusna_status cusna_mom_estimate(const cusna_saom_problem* const* perio
   const size_t o_xn = D.add((size_t)P * n * W64 * 8), o_bk = D.add((size_t)P * n * 4);
   const size_t o_keys = D.add(P * 8), o_cnt = D.add(8), o_per = D.add(P * sizeof(PeriodOut));
   const size_t o_grp = D.add(P * sizeof(saom::SimParams)), o_items = D.add((size_t)P * Mmax * 4);
+  const size_t bscr_d = mode == 2 ? saom::beh_stats_scratch_doubles(n) : 0;
+  const size_t o_bscr = D.add((size_t)Mmax * bscr_d * 8);
   HIP_TRY(D.alloc());
   E->theta = D.at<double>(o_theta);
   E->theta_sim = D.at<double>(o_tsim);
@@ -502,6 +506,7 @@ CUSNA_API cusna_status cusna_mom_estimate(const cusna_saom_problem* const* perio
   E->per = D.at<PeriodOut>(o_per);
   E->groups = D.at<saom::SimParams>(o_grp);
   E->items = D.at<uint32_t>(o_items);
+  E->bscratch = bscr_d ? D.at<double>(o_bscr) : nullptr;
   hipStream_t s = E->s;
 
   std::vector<double> rc(P, 0.0);

@@ -67,6 +67,8 @@ struct cusna_period_plan {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   size_t o_keep = 0, o_force = 0, o_vb = 0, o_vmiss = 0, o_od0 = 0, o_id0 = 0, o_stats = 0;
   size_t o_dense = 0, o_th = 0, o_lam = 0, o_thb = 0, o_lamb = 0, o_params = 0;
+  size_t o_sscr = 0;       // statistics scratch (networks above the LDS)
+  bool has_sscr = false;
   bool stats_valid = false;
   ~cusna_period_plan() {
     if (ev0) (void)hipEventDestroy(ev0);

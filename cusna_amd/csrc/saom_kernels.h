@@ -42,6 +42,7 @@ struct SimParams {
   const int32_t *z0, *bkd, *act;              // n each (act: nact used)
   const double *lu0, *llu;                    // replay inputs (or null)
   uint64_t *end_rows;                         // M*n*W64 end networks
+  uint64_t *work;                             // streamed: M*W64*n working rows
   double *ntim, *scores, *bscores;            // M, M*(k+1), M*(kb+1)
   int32_t *dist, *dist_stat, *steps, *bdist, *z_end, *actor_dist;
 };
@@ -49,7 +50,7 @@ struct SimParams {
 // Launch shape chosen for one call.
 struct LaunchShape {
   int nw;    // waves per chain (workgroup = 64*nw threads)
-  int wmax;  // compile-time bound on W64 (1, 2, 4, 8, 16)
+  int wmax;  // compile-time bound on W64 (1 .. 16 LDS-resident; 32, 64, 128 streamed)
 };
 
 // LDS bytes one chain needs with the given shape.
@@ -91,9 +92,15 @@ struct StatParams {
   const uint8_t* vmiss;   // k*n or null
   const int32_t *od0, *id0;  // n or null
   double* out;  // M*k
+  // networks whose rows (and columns) do not fit the LDS: M times
+  // net_stats_scratch_words u64 of device memory for them, else null
+  uint64_t* scratch;
 };
 int launch_net_stats(const StatParams& P, void* stream);
 size_t net_stats_lds(int n, int W64, bool need_cols);
+// u64 words of device scratch one network needs (0: LDS-resident).
+size_t net_stats_scratch_words(int n, int W64, bool need_cols);
+bool stats_need_cols(const int* type_ids, int k);
 
 struct BehStatParams {
   int n, M, kb, W64;
@@ -103,8 +110,10 @@ struct BehStatParams {
   const int32_t* bkdist;   // n or null
   double zbar, brange, bsim;
   double* out;             // M*kb
+  double* scratch;         // M*beh_stats_scratch_doubles(n) when above the LDS, else null
 };
 int launch_beh_stats(const BehStatParams& P, void* stream);
+size_t beh_stats_scratch_doubles(int n);
 
 // Expand packed end rows (M*n*W64) to dense uint8 (M*n*n).
 int launch_expand_dense(const uint64_t* rows, int n, int W64, int M, uint8_t* dense,

@@ -12,6 +12,9 @@ SAOM_DECL(2)
 SAOM_DECL(4)
 SAOM_DECL(8)
 SAOM_DECL(16)
+SAOM_DECL(32)
+SAOM_DECL(64)
+SAOM_DECL(128)
 #undef SAOM_DECL
 size_t sim_lds_bytes_w1(const SimParams& P, int nw, int wmax);
 
@@ -40,6 +43,9 @@ int launch_period_ex(const PeriodLaunch& L, LaunchShape s, bool replay, bool sim
     case 4: return launch_period_w4(L, s.nw, replay, simple, stream);
     case 8: return launch_period_w8(L, s.nw, replay, simple, stream);
     case 16: return launch_period_w16(L, s.nw, replay, simple, stream);
+    case 32: return launch_period_w32(L, s.nw, replay, simple, stream);
+    case 64: return launch_period_w64(L, s.nw, replay, simple, stream);
+    case 128: return launch_period_w128(L, s.nw, replay, simple, stream);
     default: return (int)hipErrorInvalidValue;
   }
 }

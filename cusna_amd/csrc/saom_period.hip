@@ -166,14 +166,32 @@ struct Alt {
   __device__ __forceinline__ double dl(int r) const { return 1.0 - 2.0 * (double)bij(r); }
 };
 
-// Ego-level state shared by all alternatives of the ministep.
-template <int WMAX>
+// Ego-level state shared by all alternatives of the ministep. Row and column
+// of i sit in registers when the network is LDS-resident; a streamed network
+// (rows in device memory, W64 up to 128 words) keeps them in LDS instead.
+template <int WMAX, bool STREAM>
 struct Ego {
+  static constexpr int WR = STREAM ? 1 : WMAX;
   int i;
   int outdeg_i, indeg_i, rdeg_i;
   double zvi;
-  uint64_t rowi[WMAX];  // out-ties of i
-  uint64_t coli[WMAX];  // in-ties of i
+  uint64_t rowi_r[WR];  // out-ties of i
+  uint64_t coli_r[WR];  // in-ties of i
+  const uint64_t* rowi_s;
+  const uint64_t* coli_s;
+  int W;  // words looped over: WMAX (zero-padded rows) or W64 (streamed)
+  __device__ __forceinline__ uint64_t rowi(int w) const {
+    if constexpr (STREAM) return rowi_s[w];
+    else return rowi_r[w];
+  }
+  __device__ __forceinline__ uint64_t coli(int w) const {
+    if constexpr (STREAM) return coli_s[w];
+    else return coli_r[w];
+  }
+  __device__ __forceinline__ int words() const {
+    if constexpr (STREAM) return W;
+    else return WMAX;
+  }
 };
 
 // LDS views of the chain state.
@@ -188,9 +206,10 @@ struct ChainLds {
 // alternatives (libcusna_saom_cpu.cpp:49-101), creation/endowment masking
 // applied (:258-259). Covariate values: vmat row e, or the current behaviour
 // minus zbar for behaviour-dynamic effects (:233, 244, 252-253).
-template <int R, int WMAX, bool SIMPLE>
+template <int R, int WMAX, bool STREAM, bool SIMPLE>
 __device__ __forceinline__ void eff_base(const SimParams& P, int e, const Alt<R>& A,
-                                         const Ego<WMAX>& U, const ChainLds& S, double (&d)[R]) {
+                                         const Ego<WMAX, STREAM>& U, const ChainLds& S,
+                                         double (&d)[R]) {
   const uint64_t* rows = S.rows;
   const int RS = S.RS;
   const int* indeg = S.indeg;
@@ -321,8 +340,8 @@ __device__ __forceinline__ void eff_base(const SimParams& P, int e, const Alt<R>
           const uint64_t* tm = A.bij(r) ? S.tpm1 : S.tpm0;
           int c = 0;
 #pragma unroll
-          for (int wd = 0; wd < WMAX; ++wd)
-            c += popc64(U.rowi[wd] & rows[(size_t)(wd) * RS + A.j[r]] & tm[wd]);
+          for (int wd = 0; wd < U.words(); ++wd)
+            c += popc64(U.rowi(wd) & rows[(size_t)(wd) * RS + A.j[r]] & tm[wd]);
           d[r] = A.dl(r) * (double)(c + (A.tp[r] > 0 ? 1 : 0));
         }
         break;
@@ -331,8 +350,8 @@ __device__ __forceinline__ void eff_base(const SimParams& P, int e, const Alt<R>
         for (int r = 0; r < R; ++r) {
           int c = 0;
 #pragma unroll
-          for (int wd = 0; wd < WMAX; ++wd)
-            c += popc64(U.rowi[wd] & U.coli[wd] & rows[(size_t)(wd) * RS + A.j[r]]);
+          for (int wd = 0; wd < U.words(); ++wd)
+            c += popc64(U.rowi(wd) & U.coli(wd) & rows[(size_t)(wd) * RS + A.j[r]]);
           d[r] = A.dl(r) * (double)((A.bji(r) ? A.tp[r] : 0) + c);
         }
         break;
@@ -359,15 +378,15 @@ __device__ __forceinline__ void eff_base(const SimParams& P, int e, const Alt<R>
 #pragma unroll
         for (int r = 0; r < R; ++r) c[r] = 0;
 #pragma unroll
-        for (int wd = 0; wd < WMAX; ++wd) {
-          uint64_t m = U.coli[wd];
+        for (int wd = 0; wd < U.words(); ++wd) {
+          uint64_t m = U.coli(wd);
           while (m) {
             const int h = wd * 64 + __ffsll((long long)m) - 1;
             m &= m - 1;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
               const int word = S.NW * r + S.wv;
-              const uint64_t x = rows[(size_t)(word) * RS + h];
+              const uint64_t x = (!STREAM || word < S.W64) ? rows[(size_t)(word) * RS + h] : 0ull;
               c[r] += (int)((x >> S.lane) & 1);
             }
           }
@@ -432,11 +451,12 @@ __device__ __forceinline__ void eff_base(const SimParams& P, int e, const Alt<R>
 // Full change statistic of effect e, including interactions (type 32):
 // d = (c2 < 0 ? delta : 1) * (d_c0 * d_c1) [* d_c2] (:246-251). Alternatives
 // j == i get 0 (:238).
-template <int R, int WMAX, bool SIMPLE>
+template <int R, int WMAX, bool STREAM, bool SIMPLE>
 __device__ __forceinline__ void eff_full(const SimParams& P, int e, const Alt<R>& A,
-                                         const Ego<WMAX>& U, const ChainLds& S, double (&d)[R]) {
+                                         const Ego<WMAX, STREAM>& U, const ChainLds& S,
+                                         double (&d)[R]) {
   if constexpr (SIMPLE) {
-    eff_base<R, WMAX, true>(P, e, A, U, S, d);
+    eff_base<R, WMAX, STREAM, true>(P, e, A, U, S, d);
   } else {
     // One eff_base call site: a plain effect runs the loop once; an
     // interaction runs it over its two or three components.
@@ -447,7 +467,7 @@ __device__ __forceinline__ void eff_full(const SimParams& P, int e, const Alt<R>
       const int ce = !inter ? (q == 0 ? e : -1) : (q == 0 ? P.c0[e] : (q == 1 ? P.c1[e] : c2));
       if (ce < 0) break;
       double tmp[R];
-      eff_base<R, WMAX, false>(P, ce, A, U, S, tmp);
+      eff_base<R, WMAX, STREAM, false>(P, ce, A, U, S, tmp);
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         if (!inter) d[r] = tmp[r];
@@ -485,20 +505,25 @@ __device__ __forceinline__ double beh_d(int t, double dz, double zic, long long 
 }
 
 struct LdsLayout {
-  size_t rows, indeg, outdeg, z, tpall, soall, coli, tpm0, tpm1, red, tsl, dstar, sc, bsc, cnt,
-      total;
+  size_t rows, indeg, outdeg, z, tpall, soall, coli, rowi, tpm0, tpm1, red, tsl, dstar, sc, bsc,
+      cnt, total;
 };
+
+// Word bounds above this stream the network from device memory.
+constexpr int kLdsWordBound = 16;
 
 __host__ __device__ inline size_t al8(size_t x) { return (x + 7) & ~(size_t)7; }
 
 // Rows are padded to wmax words (zero above W64) so the unrolled word loops of
-// the ministep need no bounds tests.
+// the ministep need no bounds tests. Streamed networks (wmax above
+// kLdsWordBound) keep their rows in device memory and row i in LDS.
 __host__ __device__ inline LdsLayout lds_layout(int n, int k, int kb, int wmax, int RS, int nw,
                                                 int flags) {
   const int W64 = wmax;
+  const bool stream = wmax > kLdsWordBound;
   LdsLayout L;
   size_t o = 0;
-  L.rows = o; o += (size_t)W64 * RS * 8;
+  L.rows = o; if (!stream) o += (size_t)W64 * RS * 8;
   L.indeg = o; o += (size_t)n * 4;
   L.outdeg = o; o += (size_t)n * 4;
   L.z = o; if (flags & F_Z) o += (size_t)n * 4;
@@ -506,6 +531,7 @@ __host__ __device__ inline LdsLayout lds_layout(int n, int k, int kb, int wmax, 
   L.soall = o; if (flags & (F_TPALL | F_SOALL)) o += (size_t)n * 4;
   o = al8(o);
   L.coli = o; o += (size_t)W64 * 8;
+  L.rowi = o; if (stream) o += (size_t)W64 * 8;
   L.tpm0 = o; if (flags & (F_TPALL | F_SOALL)) o += (size_t)W64 * 8;
   L.tpm1 = o; if (flags & (F_TPALL | F_SOALL)) o += (size_t)W64 * 8;
   L.red = o; o += (size_t)nw * 4 * 8;
@@ -543,7 +569,13 @@ __global__ void __launch_bounds__(64 * NW)
   const int n = P.n, k = P.k, kb = P.kb, W64 = P.W64, RS = P.RS;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const LdsLayout L = lds_layout(n, k, kb, WMAX, RS, NW, P.flags);
-  uint64_t* rows = (uint64_t*)(smem + L.rows);
+  constexpr bool STREAM = WMAX > kLdsWordBound;
+  // streamed: this chain's rows in device memory, same word-major layout
+  // with W64 words (no padding)
+  uint64_t* rows;
+  if constexpr (STREAM) rows = P.work + (size_t)chain * W64 * RS;
+  else rows = (uint64_t*)(smem + L.rows);
+  uint64_t* rowi_s = (uint64_t*)(smem + L.rowi);
   int* indeg = (int*)(smem + L.indeg);
   int* outdeg = (int*)(smem + L.outdeg);
   int* z = (int*)(smem + L.z);
@@ -562,8 +594,10 @@ __global__ void __launch_bounds__(64 * NW)
   const ChainLds S{rows, RS, W64, NW, wv, lane, indeg, outdeg, z, tpall, soall, tpm0, tpm1};
 
   // ---- chain set-up (libcusna_saom_cpu.cpp:146-160)
-  for (int q = tid; q < n * WMAX; q += NT) {
-    const int h = q / WMAX, wd = q - h * WMAX;
+  constexpr int WI = STREAM ? 1 : WMAX;  // words per row stored (x W64 when streamed)
+  const int WS = STREAM ? W64 : WI;
+  for (int q = tid; q < n * WS; q += NT) {
+    const int h = q / WS, wd = q - h * WS;
     rows[(size_t)wd * RS + h] = wd < W64 ? P.xs[(size_t)h * W64 + wd] : 0ull;
   }
   for (int q = tid; q < WMAX; q += NT) coli_s[q] = 0ull;
@@ -707,10 +741,17 @@ __global__ void __launch_bounds__(64 * NW)
 
     // ---- network ministep (:229-288)
     // 2. row i (broadcast) and col i (ballot of bit i over the rows)
-    Ego<WMAX> U;
+    Ego<WMAX, STREAM> U;
     U.i = i;
+    if constexpr (STREAM) {
+      for (int wd = tid; wd < W64; wd += NT) rowi_s[wd] = ROWI(wd);  // read after the barrier
+      U.rowi_s = rowi_s;
+      U.coli_s = coli_s;
+      U.W = W64;
+    } else {
 #pragma unroll
-    for (int wd = 0; wd < WMAX; ++wd) U.rowi[wd] = ROWI(wd);
+      for (int wd = 0; wd < WMAX; ++wd) U.rowi_r[wd] = ROWI(wd);
+    }
     Alt<R> A;
     A.in = A.ok = A.xij = A.xji = 0u;
     const int iw = i >> 6, ib = i & 63;
@@ -723,22 +764,24 @@ __global__ void __launch_bounds__(64 * NW)
       const int xji = in ? (int)((rows[(size_t)iw * RS + A.j[r]] >> ib) & 1) : 0;
       const uint64_t bal = __ballot(xji);
       if (lane == 0) coli_s[word] = bal;
-      const uint64_t rw = ROWI(word);
+      const uint64_t rw = (!STREAM || word < W64) ? ROWI(word) : 0ull;
       const int xij = (int)((rw >> lane) & 1);
       A.in |= (uint32_t)in << r;
       A.xji |= (uint32_t)xji << r;
       A.xij |= (uint32_t)xij << r;
     }
     __syncthreads();
+    if constexpr (!STREAM) {
 #pragma unroll
-    for (int wd = 0; wd < WMAX; ++wd) U.coli[wd] = coli_s[wd];
+      for (int wd = 0; wd < WMAX; ++wd) U.coli_r[wd] = coli_s[wd];
+    }
     U.outdeg_i = outdeg[i];
     U.indeg_i = indeg[i];
     U.zvi = (flags & F_Z) ? (double)z[i] - P.zbar : -P.zbar;
     U.rdeg_i = 0;
     if (flags & F_RDEG) {
 #pragma unroll
-      for (int wd = 0; wd < WMAX; ++wd) U.rdeg_i += popc64(U.rowi[wd] & U.coli[wd]);
+      for (int wd = 0; wd < U.words(); ++wd) U.rdeg_i += popc64(U.rowi(wd) & U.coli(wd));
     }
 
     // 3. features: tp_j (two-paths i->h->j), so_j = |row_i & row_j|,
@@ -747,15 +790,15 @@ __global__ void __launch_bounds__(64 * NW)
     for (int r = 0; r < R; ++r) A.tp[r] = 0;
     if (flags & F_TP) {
 #pragma unroll
-      for (int wd = 0; wd < WMAX; ++wd) {
-        uint64_t m = U.rowi[wd];
+      for (int wd = 0; wd < U.words(); ++wd) {
+        uint64_t m = U.rowi(wd);
         while (m) {
           const int h = wd * 64 + __ffsll((long long)m) - 1;
           m &= m - 1;
 #pragma unroll
           for (int r = 0; r < R; ++r) {
             const int word = NW * r + wv;
-            const uint64_t x = rows[(size_t)(word) * RS + h];
+            const uint64_t x = (!STREAM || word < W64) ? rows[(size_t)(word) * RS + h] : 0ull;
             A.tp[r] += (int)((x >> lane) & 1);
           }
         }
@@ -765,12 +808,10 @@ __global__ void __launch_bounds__(64 * NW)
     for (int r = 0; r < R; ++r) {
       int so = 0, cy = 0;
 #pragma unroll
-      for (int wd = 0; wd < WMAX; ++wd) {
-        {
-          const uint64_t x = rows[(size_t)(wd) * RS + A.j[r]];
-          so += popc64(U.rowi[wd] & x);
-          cy += popc64(x & U.coli[wd]);
-        }
+      for (int wd = 0; wd < U.words(); ++wd) {
+        const uint64_t x = rows[(size_t)(wd) * RS + A.j[r]];
+        so += popc64(U.rowi(wd) & x);
+        cy += popc64(x & U.coli(wd));
       }
       A.so[r] = so;
       A.cy[r] = cy;
@@ -814,7 +855,7 @@ __global__ void __launch_bounds__(64 * NW)
     for (int r = 0; r < R; ++r) f[r] = 0.0;
     for (int e = 0; e < k; ++e) {
       double d[R];
-      eff_full<R, WMAX, SIMPLE>(P, e, A, U, S, d);
+      eff_full<R, WMAX, STREAM, SIMPLE>(P, e, A, U, S, d);
       const double te = th[e];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
@@ -898,7 +939,7 @@ __global__ void __launch_bounds__(64 * NW)
     if (lane == 0) tsl[wv * (k + 1) + k] = Sw;
     for (int e = 0; e < k; ++e) {
       double d[R];
-      eff_full<R, WMAX, SIMPLE>(P, e, A, U, S, d);
+      eff_full<R, WMAX, STREAM, SIMPLE>(P, e, A, U, S, d);
       double T = 0.0;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
@@ -1023,7 +1064,7 @@ hipError_t launch_variant(const PeriodLaunch& L, bool replay, bool simple, hipSt
 // This file is compiled once per word bound SAOM_WMAX (1, 2, 4, 8, 16) so the
 // instantiations build in parallel; launch_period (saom_launch.hip) dispatches.
 #ifndef SAOM_WMAX
-#error "compile with -DSAOM_WMAX=<1|2|4|8|16>"
+#error "compile with -DSAOM_WMAX=<1|2|4|8|16|32|64|128>"
 #endif
 #define SAOM_CAT2(a, b) a##b
 #define SAOM_CAT(a, b) SAOM_CAT2(a, b)
@@ -1033,18 +1074,21 @@ int SAOM_CAT(launch_period_w, SAOM_WMAX)(const PeriodLaunch& L, int nw, bool rep
   hipStream_t st = (hipStream_t)stream;
   constexpr int WM = SAOM_WMAX;
   hipError_t e = hipErrorInvalidValue;
-  if constexpr (WM <= 8) {
-    if (nw == 1) e = launch_variant<1, WM>(L, replay, simple, st);
+  if constexpr (WM <= kLdsWordBound) {
+    if constexpr (WM <= 8) {
+      if (nw == 1) e = launch_variant<1, WM>(L, replay, simple, st);
+    }
+    if constexpr (WM >= 2) {
+      if (nw == 2) e = launch_variant<2, WM>(L, replay, simple, st);
+    }
+    if constexpr (WM >= 4) {
+      if (nw == 4) e = launch_variant<4, WM>(L, replay, simple, st);
+    }
+    if constexpr (WM >= 8) {
+      if (nw == 8) e = launch_variant<8, WM>(L, replay, simple, st);
+    }
   }
-  if constexpr (WM >= 2) {
-    if (nw == 2) e = launch_variant<2, WM>(L, replay, simple, st);
-  }
-  if constexpr (WM >= 4) {
-    if (nw == 4) e = launch_variant<4, WM>(L, replay, simple, st);
-  }
-  if constexpr (WM >= 8) {
-    if (nw == 8) e = launch_variant<8, WM>(L, replay, simple, st);
-  }
+  // streamed networks run one shape: 16 waves, WM / 16 words per lane
   if constexpr (WM >= 16) {
     if (nw == 16) e = launch_variant<16, WM>(L, replay, simple, st);
   }

@@ -51,19 +51,29 @@ enum : int {
   I_E = 0, I_RECIP, I_TT, I_C3, I_TTIES, I_TRT, I_INPOP, I_OUTACT, I_RECIACT, I_N
 };
 
+// GROWS: the masked rows (and columns) live in device scratch instead of LDS,
+// for networks above the LDS capacity; the arithmetic is the same.
+template <bool GROWS>
 __global__ void __launch_bounds__(kStatThreads) net_stats_kernel(const StatParams P) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int n = P.n, W64 = P.W64, k = P.k;
   const int tid = threadIdx.x, NT = blockDim.x;
   const int net = blockIdx.x;
-  size_t o = 0;
-  uint64_t* rows = (uint64_t*)(smem + o);
-  o += (size_t)n * W64 * 8;
   bool need_cols = false;
   for (int e = 0; e < k; ++e)
     if (P.tid[e] == 27 || P.tid[e] == 28 || P.tid[e] == 30) need_cols = true;
-  uint64_t* cols = (uint64_t*)(smem + o);
-  if (need_cols) o += (size_t)n * W64 * 8;
+  size_t o = 0;
+  uint64_t* rows;
+  uint64_t* cols;
+  if constexpr (GROWS) {
+    rows = P.scratch + (size_t)net * n * W64 * (need_cols ? 2 : 1);
+    cols = rows + (size_t)n * W64;
+  } else {
+    rows = (uint64_t*)(smem + o);
+    o += (size_t)n * W64 * 8;
+    cols = (uint64_t*)(smem + o);
+    if (need_cols) o += (size_t)n * W64 * 8;
+  }
   int* indeg = (int*)(smem + o);
   o += (size_t)n * 4;
   int* outdeg = (int*)(smem + o);
@@ -349,12 +359,15 @@ __global__ void __launch_bounds__(kStatThreads) net_stats_kernel(const StatParam
 }
 
 // Behaviour statistics (:767-817), one workgroup per behaviour vector.
+template <bool GMEM>
 __global__ void __launch_bounds__(kStatThreads) beh_stats_kernel(const BehStatParams P) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int n = P.n, W64 = P.W64;
   const int tid = threadIdx.x, NT = blockDim.x;
   const int c = blockIdx.x;
-  double* u = (double*)smem;
+  double* u;
+  if constexpr (GMEM) u = P.scratch + (size_t)c * 5 * n;
+  else u = (double*)smem;
   double* term = u + n;
   double* okd = term + n;
   double* od = okd + n;
@@ -484,16 +497,34 @@ int launch_gumbel_check(uint32_t x0, uint32_t stride, uint32_t count,
   return (int)hipGetLastError();
 }
 
+bool stats_need_cols(const int* t, int k) {
+  for (int e = 0; e < k; ++e)
+    if (t[e] == 27 || t[e] == 28 || t[e] == 30) return true;
+  return false;
+}
+
 int launch_net_stats(const StatParams& P, void* stream) {
-  bool need_cols = false;
-  for (int e = 0; e < P.k; ++e)
-    if (P.tid[e] == 27 || P.tid[e] == 28 || P.tid[e] == 30) need_cols = true;
-  const size_t lds = net_stats_lds(P.n, P.W64, need_cols);
+  if (P.M < 1) return 0;
+  const bool need_cols = stats_need_cols(P.tid, P.k);
+  const bool grows = net_stats_scratch_words(P.n, P.W64, need_cols) > 0;
+  if (grows && !P.scratch) return (int)hipErrorInvalidValue;
   static const hipError_t attr = hipFuncSetAttribute(
-      (const void*)net_stats_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+      (const void*)net_stats_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+      160 * 1024);
   if (attr != hipSuccess) return (int)attr;
-  hipLaunchKernelGGL(net_stats_kernel, dim3(P.M), dim3(kStatThreads), lds, (hipStream_t)stream,
-                     P);
+  static const hipError_t attr_g = hipFuncSetAttribute(
+      (const void*)net_stats_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+      160 * 1024);
+  if (attr_g != hipSuccess) return (int)attr_g;
+  if (grows) {
+    const size_t lds = (size_t)P.n * 8 + 8 + 64 * 8;
+    hipLaunchKernelGGL(net_stats_kernel<true>, dim3(P.M), dim3(kStatThreads), lds,
+                       (hipStream_t)stream, P);
+  } else {
+    const size_t lds = net_stats_lds(P.n, P.W64, need_cols);
+    hipLaunchKernelGGL(net_stats_kernel<false>, dim3(P.M), dim3(kStatThreads), lds,
+                       (hipStream_t)stream, P);
+  }
   return (int)hipGetLastError();
 }
 
@@ -501,13 +532,29 @@ size_t net_stats_lds(int n, int W64, bool need_cols) {
   return (size_t)n * W64 * 8 * (need_cols ? 2 : 1) + (size_t)n * 8 + 8 + 64 * 8;
 }
 
+size_t net_stats_scratch_words(int n, int W64, bool need_cols) {
+  if (net_stats_lds(n, W64, need_cols) <= 160 * 1024) return 0;
+  return (size_t)n * W64 * (need_cols ? 2 : 1);
+}
+
+size_t beh_stats_scratch_doubles(int n) {
+  return (size_t)n * 5 * 8 > 160 * 1024 ? (size_t)n * 5 : 0;
+}
+
 int launch_beh_stats(const BehStatParams& P, void* stream) {
-  const size_t lds = (size_t)P.n * 5 * 8;
+  if (P.M < 1) return 0;
   static const hipError_t attr = hipFuncSetAttribute(
-      (const void*)beh_stats_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+      (const void*)beh_stats_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+      160 * 1024);
   if (attr != hipSuccess) return (int)attr;
-  hipLaunchKernelGGL(beh_stats_kernel, dim3(P.M), dim3(kStatThreads), lds, (hipStream_t)stream,
-                     P);
+  if (beh_stats_scratch_doubles(P.n) > 0) {
+    if (!P.scratch) return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(beh_stats_kernel<true>, dim3(P.M), dim3(kStatThreads), 0,
+                       (hipStream_t)stream, P);
+  } else {
+    hipLaunchKernelGGL(beh_stats_kernel<false>, dim3(P.M), dim3(kStatThreads),
+                       (size_t)P.n * 5 * 8, (hipStream_t)stream, P);
+  }
   return (int)hipGetLastError();
 }
 
