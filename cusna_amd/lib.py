@@ -26,7 +26,9 @@ EXPORTS = (
     "cusna_period_plan_run", "cusna_period_plan_sync", "cusna_period_plan_fetch",
     "cusna_period_plan_kernel_ms", "cusna_period_plan_shape", "cusna_period_plan_destroy",
     "cusna_gumbel_bound_check", "cusna_mom_estimate", "cusna_period_plan_autotune",
-    "cusna_saom_simulate_groups",
+    "cusna_saom_simulate_groups", "cusna_mlp_create", "cusna_mlp_param_count",
+    "cusna_mlp_layer_shape", "cusna_mlp_get_params", "cusna_mlp_set_params",
+    "cusna_mlp_forward_backward", "cusna_mlp_train", "cusna_mlp_predict", "cusna_mlp_destroy",
 )
 
 _lib = None
@@ -85,6 +87,26 @@ def load(path: str | None = None) -> C.CDLL:
     L.cusna_period_plan_autotune.argtypes = [_vp, C.c_int32, C.POINTER(C.c_int32)]
     L.cusna_saom_simulate_groups.restype = C.c_int
     L.cusna_saom_simulate_groups.argtypes = [_vp, C.c_int32, _vp]
+    L.cusna_mlp_create.restype = C.c_int
+    L.cusna_mlp_create.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_uint64,
+                                   C.POINTER(_vp)]
+    L.cusna_mlp_param_count.restype = C.c_int64
+    L.cusna_mlp_param_count.argtypes = [_vp]
+    L.cusna_mlp_layer_shape.restype = C.c_int
+    L.cusna_mlp_layer_shape.argtypes = [_vp, C.c_int32, C.POINTER(C.c_int32),
+                                        C.POINTER(C.c_int32)]
+    for f in ("cusna_mlp_get_params", "cusna_mlp_set_params"):
+        getattr(L, f).restype = C.c_int
+        getattr(L, f).argtypes = [_vp, _vp]
+    L.cusna_mlp_forward_backward.restype = C.c_int
+    L.cusna_mlp_forward_backward.argtypes = [_vp, _vp, _vp, _vp, _vp, C.POINTER(C.c_double)]
+    L.cusna_mlp_train.restype = C.c_int
+    L.cusna_mlp_train.argtypes = [_vp, _vp, _vp, C.c_int64, C.c_int32, C.c_float, C.c_uint64,
+                                  C.POINTER(C.c_double)]
+    L.cusna_mlp_predict.restype = C.c_int
+    L.cusna_mlp_predict.argtypes = [_vp, _vp, C.c_int64, _vp]
+    L.cusna_mlp_destroy.restype = None
+    L.cusna_mlp_destroy.argtypes = [_vp]
     L.cusna_mom_estimate.restype = C.c_int
     L.cusna_mom_estimate.argtypes = [_vp, _vp, _vp, _vp, _vp, C.POINTER(cusna_mom_options),
                                      C.POINTER(cusna_mom_result)]
@@ -194,6 +216,73 @@ def behavior_stats(z, bty, xnet, bkdist, zbar, brange, bsimmean) -> np.ndarray:
                                      float(zbar), float(brange), float(bsimmean), _p(out))
     _check("cusna_behavior_stats", st)
     return out
+
+
+class Mlp:
+    """Surrogate perceptron on the BF16 matrix cores (cusna_mlp_*)."""
+
+    def __init__(self, p_in: int, p_out: int, width: int = 256, batch: int = 1024,
+                 seed: int = 1):
+        self.L = load()
+        self.h = _vp()
+        _check("cusna_mlp_create", self.L.cusna_mlp_create(p_in, p_out, width, batch,
+                                                            int(seed), C.byref(self.h)))
+        self.p_in, self.p_out, self.width, self.batch = p_in, p_out, width, batch
+        self.n_params = int(self.L.cusna_mlp_param_count(self.h))
+        K, N = C.c_int32(), C.c_int32()
+        self.layer_shapes = []
+        for layer in range(4):
+            _check("cusna_mlp_layer_shape",
+                   self.L.cusna_mlp_layer_shape(self.h, layer, C.byref(K), C.byref(N)))
+            self.layer_shapes.append((K.value, N.value))
+
+    def get_params(self) -> np.ndarray:
+        out = np.zeros(self.n_params, np.float32)
+        _check("cusna_mlp_get_params", self.L.cusna_mlp_get_params(self.h, _p(out)))
+        return out
+
+    def set_params(self, flat) -> None:
+        f = np.ascontiguousarray(flat, np.float32)
+        assert f.shape == (self.n_params,)
+        _check("cusna_mlp_set_params", self.L.cusna_mlp_set_params(self.h, _p(f)))
+
+    def forward_backward(self, x, y) -> dict:
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        assert x.shape == (self.batch, self.p_in) and y.shape == (self.batch, self.p_out)
+        out = np.zeros((self.batch, self.layer_shapes[3][1]), np.float32)
+        g = np.zeros(self.n_params, np.float32)
+        loss = C.c_double()
+        _check("cusna_mlp_forward_backward",
+               self.L.cusna_mlp_forward_backward(self.h, _p(x), _p(y), _p(out), _p(g),
+                                                 C.byref(loss)))
+        return {"out": out, "grads": g, "loss": loss.value}
+
+    def train(self, x_all, y_all, steps: int, lr: float = 1e-3, seed: int = 7) -> float:
+        x = np.ascontiguousarray(x_all, np.float32)
+        y = np.ascontiguousarray(y_all, np.float32)
+        loss = C.c_double()
+        _check("cusna_mlp_train", self.L.cusna_mlp_train(self.h, _p(x), _p(y), x.shape[0],
+                                                         int(steps), float(lr), int(seed),
+                                                         C.byref(loss)))
+        return loss.value
+
+    def predict(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32)
+        out = np.zeros((x.shape[0], 2 * self.p_out), np.float32)
+        _check("cusna_mlp_predict", self.L.cusna_mlp_predict(self.h, _p(x), x.shape[0], _p(out)))
+        return out
+
+    def close(self) -> None:
+        if self.h:
+            self.L.cusna_mlp_destroy(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def simulate_groups(problems: list, want_end_nets: bool = False) -> list:

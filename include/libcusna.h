@@ -355,6 +355,40 @@ typedef struct cusna_mom_result {
   double wall_seconds;
 } cusna_mom_result;
 
+/*
+ * Surrogate perceptron (no counterpart in the reference). Four layers
+ * x (p_in) -> width -> width -> width -> [mu (p_out), log-variance (p_out)],
+ * SiLU activations, trained by Adam on the Gaussian negative log-likelihood
+ * mean_b sum_a 0.5 [(y - mu)^2 exp(-lv) + lv]; products on the BF16 matrix
+ * cores with float32 accumulation (weights and activations rounded to
+ * bfloat16 as they enter a product), float32 master weights.
+ *
+ *   create        p_in, p_out in 1..64; width, batch multiples of 64
+ *   param_count   floats in the flat parameter vector: for each layer l,
+ *                 W_l [K_l][N_l] row-major then b_l [N_l] (layer_shape);
+ *                 K_0 = p_in and N_3 = 2 p_out padded to multiples of 64
+ *   forward_backward  one batch (batch x p_in, batch x p_out, host float32):
+ *                 outputs (batch x N_3), gradients (flat), mean loss
+ *   train         `steps` Adam steps on minibatches drawn with replacement
+ *                 from n host samples; last minibatch's mean loss
+ *   predict       outputs [mu, lv] (n x 2 p_out) for n inputs
+ */
+typedef struct cusna_mlp cusna_mlp;
+CUSNA_API cusna_status cusna_mlp_create(int32_t p_in, int32_t p_out, int32_t width, int32_t batch,
+                                        uint64_t seed, cusna_mlp **mlp);
+CUSNA_API int64_t cusna_mlp_param_count(const cusna_mlp *mlp);
+CUSNA_API cusna_status cusna_mlp_layer_shape(const cusna_mlp *mlp, int32_t layer, int32_t *K,
+                                             int32_t *N);
+CUSNA_API cusna_status cusna_mlp_get_params(cusna_mlp *mlp, float *flat);
+CUSNA_API cusna_status cusna_mlp_set_params(cusna_mlp *mlp, const float *flat);
+CUSNA_API cusna_status cusna_mlp_forward_backward(cusna_mlp *mlp, const float *x, const float *y,
+                                                  float *out, float *grads, double *loss);
+CUSNA_API cusna_status cusna_mlp_train(cusna_mlp *mlp, const float *x_all, const float *y_all,
+                                       int64_t n, int32_t steps, float lr, uint64_t seed,
+                                       double *last_loss);
+CUSNA_API cusna_status cusna_mlp_predict(cusna_mlp *mlp, const float *x, int64_t n, float *out);
+CUSNA_API void cusna_mlp_destroy(cusna_mlp *mlp);
+
 CUSNA_API cusna_status cusna_mom_estimate(const cusna_saom_problem *const *periods,
                                           const uint8_t *const *xnet, const double *targets,
                                           const double *rate_consts, const double *theta0,

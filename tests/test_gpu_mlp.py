@@ -1,0 +1,80 @@
+"""GPU: the surrogate perceptron's BF16 MFMA forward/backward against the
+float64 oracle (with the device's bfloat16 rounding points), and training.
+
+Tolerances: the device accumulates in float32 (the oracle in float64), and a
+float32-vs-float64 difference can move a bfloat16 rounding by one unit
+(2^-8 relative) for an occasional element; the comparisons are relative to
+each array's scale: outputs and loss 1e-3, gradients 2e-2 per element of the
+largest magnitude and 5e-3 in norm.
+"""
+import numpy as np
+import pytest
+
+import mlp_oracle as MO
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("p_in,p_out", [(3, 3), (13, 13), (40, 7)])
+def test_forward_backward_matches_oracle(gpu_lib, p_in, p_out):
+    rng = np.random.default_rng(p_in)
+    B, width = 128, 256
+    m = gpu_lib.Mlp(p_in, p_out, width=width, batch=B, seed=3)
+    try:
+        flat = MO.init_flat(p_in, p_out, width, rng).astype(np.float32)
+        m.set_params(flat)
+        x = rng.uniform(-1, 1, (B, p_in)).astype(np.float32)
+        y = rng.normal(size=(B, p_out)).astype(np.float32)
+        got = m.forward_backward(x, y)
+    finally:
+        m.close()
+    ref = MO.forward_backward(flat.astype(np.float64), x.astype(np.float64),
+                              y.astype(np.float64), p_in, p_out, width, bf16=True)
+    N3 = MO.pad64(2 * p_out)
+    out, rout = got["out"].astype(np.float64), ref["out"]
+    assert out.shape == (B, N3)
+    assert np.max(np.abs(out - rout)) <= 1e-3 * np.max(np.abs(rout))
+    assert abs(got["loss"] - ref["loss"]) <= 1e-3 * abs(ref["loss"])
+    g, rg = got["grads"].astype(np.float64), ref["grads"]
+    o = 0
+    for K, N in MO.shapes(p_in, p_out, width):
+        for size in (K * N, N):
+            a, b = g[o:o + size], rg[o:o + size]
+            scale = np.max(np.abs(b)) + 1e-30
+            assert np.max(np.abs(a - b)) <= 2e-2 * scale, (o, np.max(np.abs(a - b)), scale)
+            assert np.linalg.norm(a - b) <= 5e-3 * np.linalg.norm(b) + 1e-30
+            o += size
+    # padding rows of the first layer and padding output columns stay at zero gradient
+    K0 = MO.pad64(p_in)
+    assert np.all(g[:K0 * width].reshape(K0, width)[p_in:] == 0.0)
+
+
+def test_training_fits_a_smooth_map(gpu_lib):
+    """Heteroscedastic regression y = f(x) + s(x) eps: the trained mean is
+    within a small fraction of the noise of f on fresh inputs, and the
+    predicted log-variance tracks log s^2."""
+    rng = np.random.default_rng(5)
+    p_in, p_out, n = 3, 2, 1 << 16
+    A = rng.normal(size=(p_in, p_out))
+
+    def f(x):
+        return np.tanh(x @ A) + 0.3 * x[:, :p_out] ** 2
+
+    def sd(x):
+        return 0.2 + 0.1 * np.abs(x[:, :1])
+
+    x = rng.uniform(-1, 1, (n, p_in))
+    y = f(x) + sd(x) * rng.normal(size=(n, p_out))
+    m = gpu_lib.Mlp(p_in, p_out, width=256, batch=1024, seed=11)
+    try:
+        first = m.forward_backward(x[:1024], y[:1024])["loss"]
+        last = m.train(x, y, steps=800, lr=2e-3, seed=3)
+        xt = rng.uniform(-0.9, 0.9, (4096, p_in))
+        pred = m.predict(xt).astype(np.float64)
+    finally:
+        m.close()
+    assert last < first
+    err = pred[:, :p_out] - f(xt)
+    assert np.sqrt(np.mean(err ** 2)) < 0.05, np.sqrt(np.mean(err ** 2))
+    lv = pred[:, p_out:]
+    assert np.corrcoef(lv[:, 0], np.log(sd(xt)[:, 0] ** 2))[0, 1] > 0.8
