@@ -87,6 +87,9 @@ class cusna_mom_options(C.Structure):
         ("firstg", C.c_double), ("diagonalize", C.c_double), ("seed", C.c_uint64),
         ("use_graph", C.c_int32), ("max_repeats", C.c_int32), ("conv_tol", C.c_double),
         ("beh_zbar", C.c_double), ("beh_brange", C.c_double), ("beh_bsimmean", C.c_double),
+        ("surrogate", C.c_int32), ("surrogate_pairs_log2", C.c_int32),
+        ("surrogate_steps", C.c_int32), ("surrogate_rounds", C.c_int32),
+        ("surrogate_corrections", C.c_int32), ("reserved2", C.c_int32),
     ]
 
 

@@ -24,6 +24,8 @@ class Mlp {
  public:
   // p_in, p_out <= 64; width a multiple of 64; batch a multiple of 64.
   cusna_status init(int p_in, int p_out, int width, int batch, uint64_t seed, hipStream_t s);
+  // Fresh Glorot-uniform weights, zero biases and optimizer state.
+  cusna_status reset(uint64_t seed);
 
   int p_in() const { return p_in_; }
   int p_out() const { return p_out_; }
@@ -48,7 +50,8 @@ class Mlp {
   // Adam step over all parameters with the current gradients.
   cusna_status adam_step(float lr);
   // Train on n samples (device, float32, x n x p_in, y n x p_out): `steps`
-  // Adam steps on minibatches drawn with replacement by a counter hash.
+  // Adam steps on minibatches drawn with replacement by a counter hash; the
+  // learning rate decays linearly from lr to lr / 10.
   cusna_status train(const float* x_all, const float* y_all, int64_t n, int steps, float lr,
                      uint64_t seed);
   // Mean loss of the last forward_backward (device -> host, synchronises).

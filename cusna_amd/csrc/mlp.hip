@@ -258,8 +258,13 @@ cusna_status Mlp::init(int p_in, int p_out, int width, int batch, uint64_t seed,
   Yb_ = D_.at<float>(o_yb);
   Xb_ = D_.at<float>(o_xb);
 
+  return reset(seed);
+}
+
+cusna_status Mlp::reset(uint64_t seed) {
   // Glorot-uniform weights over the real (unpadded) fan-in / fan-out, zero
   // biases and padding, from a splitmix64 stream
+  const int p_in = p_in_, p_out = p_out_, width = width_;
   std::vector<float> h(n_params_, 0.0f);
   uint64_t st = seed;
   auto next = [&]() {
@@ -391,7 +396,8 @@ cusna_status Mlp::train(const float* x_all, const float* y_all, int64_t n, int s
                        (long long)n, p_in_, p_out_, B_, seed, (long long)sidx, Xb_, Yb_);
     cusna_status st = forward_backward(Xb_, Yb_);
     if (st != CUSNA_OK) return st;
-    if ((st = adam_step(lr)) != CUSNA_OK) return st;
+    const float lr_s = lr * (1.0f - 0.9f * (float)sidx / (float)(steps > 1 ? steps - 1 : 1));
+    if ((st = adam_step(lr_s)) != CUSNA_OK) return st;
   }
   return CUSNA_OK;
 }

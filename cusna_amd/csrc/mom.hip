@@ -18,14 +18,25 @@
 // round trip. The host only inverts the p x p matrices between sub-phases.
 // theta is rounded to float32 before the kernels read it, as the R glue
 // rounds it for every period call (cusna_saom_glue.cpp:75-78).
+//
+// Surrogate phase 2 (no counterpart in the reference): instead of the
+// Robbins-Monro iterations, simulate 2^q chains with per-chain theta drawn
+// uniformly from a box around the phase-1 Newton point, fit the BF16 MFMA
+// perceptron (theta -> mean and log-variance of the moments) to them, and
+// solve E s(theta) = target on the fitted mean by damped Newton with its
+// exact Jacobian; then correct the solution by a few Newton steps whose
+// residual and derivative come from nD plain chains. Phase 3 then runs on
+// the plain simulator, as in the reference.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstring>
 #include <memory>
 #include <vector>
 
+#include "mlp.h"
 #include "plan.h"
 
 namespace {
@@ -131,6 +142,52 @@ __global__ void mom_mean_kernel(const double* X, int M, int p, double* mean) {
   for (int c = threadIdx.x; c < M; c += blockDim.x) v += X[(size_t)c * p + a];
   v = block_sum(v, sh);
   if (threadIdx.x == 0) mean[a] = v / (double)M;
+}
+
+// Column variances (divisor M - 1) of an M x p matrix, one block per column.
+__global__ void mom_var_kernel(const double* X, int M, int p, const double* mean, double* var) {
+  __shared__ double sh[kMomThreads];
+  const int a = blockIdx.x;
+  double v = 0.0;
+  for (int c = threadIdx.x; c < M; c += blockDim.x) {
+    const double d = X[(size_t)c * p + a] - mean[a];
+    v += d * d;
+  }
+  v = block_sum(v, sh);
+  if (threadIdx.x == 0) var[a] = v / (double)(M - 1);
+}
+
+// Surrogate training inputs: chain i gets theta_a = c_a + h_a u (u uniform
+// on (-1, 1) from a counter hash of (seed, round, i, a)), rates kept at or
+// above a tenth of the box centre; the kernels read the float32 rounding
+// (row i of th_sim, slot p = internal rate 1), the perceptron the
+// normalised x = (theta - c) / h of that rounding.
+__global__ void sample_theta_kernel(const double* c, const double* h, int p, int M,
+                                    uint64_t seed, unsigned round, int n_rates, float* x,
+                                    double* th_sim) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  for (int a = 0; a < p; ++a) {
+    const uint64_t r = splitmix64(seed ^ splitmix64(((uint64_t)round << 40) ^
+                                                    ((uint64_t)i << 7) ^ (uint64_t)a));
+    const double u = 2.0 * ((double)(r >> 11) * (1.0 / 9007199254740992.0)) - 1.0;
+    double th = c[a] + h[a] * u;
+    if (a < n_rates) th = fmax(th, 0.1 * fabs(c[a]));
+    const float t32 = (float)th;
+    x[(size_t)i * p + a] = (float)(((double)t32 - c[a]) / h[a]);
+    th_sim[(size_t)i * (p + 1) + a] = (double)t32;
+  }
+  th_sim[(size_t)i * (p + 1) + p] = 1.0;
+}
+
+// y = (S - mean) / sd per column, float32 (perceptron targets).
+__global__ void normalize_kernel(const double* S, int M, int p, const double* mean,
+                                 const double* var, float* y) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)M * p) return;
+  const int a = (int)(i % p);
+  const double sd = var[a] > 0.0 ? sqrt(var[a]) : 1.0;
+  y[i] = (float)((S[i] - mean[a]) / sd);
 }
 
 // out[a][b] (+)= sum_c (A[c][a] - mA[a]) (B[c][b] - mB[b]) / div, one block
@@ -294,27 +351,158 @@ Mat safe_solve(const Mat& A, int n) {
   return pinv(A, n);
 }
 
+// ------------------------------------------------- surrogate on the host
+
+// bfloat16 rounding (nearest even) of a float32, as the device rounds
+// weights entering its products.
+double bf16_round(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  u = ((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16) << 16;
+  float r;
+  std::memcpy(&r, &u, 4);
+  return (double)r;
+}
+
+// The trained perceptron in float64 with its bfloat16 weights: mean and
+// log-variance of the normalised moments at normalised x, and the exact
+// Jacobian d mean / d x by forward-mode differentiation.
+struct HostNet {
+  int p = 0, width = 0;
+  int K[kMlpLayers], N[kMlpLayers];
+  std::vector<double> W[kMlpLayers], b[kMlpLayers];
+
+  void load(const Mlp& net, const std::vector<float>& flat, int p_, int width_) {
+    p = p_;
+    width = width_;
+    for (int l = 0; l < kMlpLayers; ++l) {
+      K[l] = net.K(l);
+      N[l] = net.N(l);
+      W[l].resize((size_t)K[l] * N[l]);
+      b[l].resize(N[l]);
+      for (size_t q = 0; q < W[l].size(); ++q) W[l][q] = bf16_round(flat[net.w_offset(l) + q]);
+      for (int q = 0; q < N[l]; ++q) b[l][q] = flat[net.b_offset(l) + q];
+    }
+  }
+
+  void eval(const double* x, double* mu, double* lv, double* J) const {
+    std::vector<double> h(x, x + p), T((size_t)p * p, 0.0);  // T[i][d] = dh_i/dx_d
+    for (int i = 0; i < p; ++i) T[(size_t)i * p + i] = 1.0;
+    int hin = p;
+    for (int l = 0; l < kMlpLayers; ++l) {
+      const int Nl = (l == kMlpLayers - 1) ? 2 * p : width;
+      std::vector<double> z(b[l].begin(), b[l].begin() + Nl), Tz((size_t)Nl * p, 0.0);
+      for (int i = 0; i < hin; ++i) {
+        const double* wr = &W[l][(size_t)i * N[l]];
+        const double hi = h[i];
+        const double* ti = &T[(size_t)i * p];
+        for (int nn = 0; nn < Nl; ++nn) {
+          const double w = wr[nn];
+          if (w == 0.0) continue;
+          z[nn] += hi * w;
+          double* tz = &Tz[(size_t)nn * p];
+          for (int d = 0; d < p; ++d) tz[d] += ti[d] * w;
+        }
+      }
+      if (l < kMlpLayers - 1) {
+        h.assign(Nl, 0.0);
+        T.assign((size_t)Nl * p, 0.0);
+        for (int nn = 0; nn < Nl; ++nn) {
+          const double sg = 1.0 / (1.0 + std::exp(-z[nn]));
+          h[nn] = z[nn] * sg;
+          const double ds = sg * (1.0 + z[nn] * (1.0 - sg));
+          for (int d = 0; d < p; ++d) T[(size_t)nn * p + d] = Tz[(size_t)nn * p + d] * ds;
+        }
+        hin = Nl;
+      } else {
+        for (int a = 0; a < p; ++a) {
+          mu[a] = z[a];
+          lv[a] = z[p + a];
+          for (int d = 0; d < p; ++d) J[(size_t)a * p + d] = Tz[(size_t)a * p + d];
+        }
+      }
+    }
+  }
+};
+
+// Damped Newton for mu(x) = t inside the box |x_a| <= 1: full steps from
+// x = 0, halved until the residual norm falls; stops when the step is tiny
+// or no halving helps. Returns the final residual norm.
+double surrogate_newton(const HostNet& H, const std::vector<double>& t, std::vector<double>& x,
+                        std::vector<double>& J, std::vector<double>& lv) {
+  const int p = H.p;
+  std::vector<double> mu(p), r(p), xn(p), mun(p), lvn(p), Jn((size_t)p * p);
+  x.assign(p, 0.0);
+  J.assign((size_t)p * p, 0.0);
+  lv.assign(p, 0.0);
+  H.eval(x.data(), mu.data(), lv.data(), J.data());
+  auto norm = [&](const std::vector<double>& m) {
+    double s2 = 0.0;
+    for (int a = 0; a < p; ++a) s2 += (m[a] - t[a]) * (m[a] - t[a]);
+    return std::sqrt(s2);
+  };
+  double nr = norm(mu);
+  for (int it = 0; it < 60 && nr > 1e-9; ++it) {
+    for (int a = 0; a < p; ++a) r[a] = mu[a] - t[a];
+    const Mat Ji = safe_solve(J, p);
+    std::vector<double> dx(p, 0.0);
+    for (int a = 0; a < p; ++a)
+      for (int d = 0; d < p; ++d) dx[a] += Ji[(size_t)a * p + d] * r[d];
+    double alpha = 1.0, nn = nr;
+    bool moved = false;
+    for (int half = 0; half < 12; ++half, alpha *= 0.5) {
+      for (int a = 0; a < p; ++a) xn[a] = std::min(1.0, std::max(-1.0, x[a] - alpha * dx[a]));
+      H.eval(xn.data(), mun.data(), lvn.data(), Jn.data());
+      nn = norm(mun);
+      if (nn < nr) {
+        moved = true;
+        break;
+      }
+    }
+    if (!moved) break;
+    double step = 0.0;
+    for (int a = 0; a < p; ++a) step = std::max(step, std::fabs(xn[a] - x[a]));
+    x = xn;
+    mu = mun;
+    lv = lvn;
+    J = Jn;
+    nr = nn;
+    if (step < 1e-10) break;
+  }
+  return nr;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ engine
-struct cusna_mom_engine {
-  int P = 0, k = 0, kb = 0, p = 0, mode = 0, n = 0, Mmax = 0, n_rates = 0;
+
+// One set of period plans simulated together: all periods of a round in one
+// launch (item b = period b % P, chain b / P), then statistics, moment rows
+// and their column means.
+struct SimSet {
   std::vector<std::unique_ptr<cusna_period_plan>> plans;
+  DeviceBlock D;
+  int M = 0;  // chains allocated
+  saom::SimParams* groups = nullptr;
+  uint32_t* items = nullptr;
+  PeriodOut* per = nullptr;
+  double *S = nullptr, *SP = nullptr, *SC = nullptr, *bstats = nullptr, *mean = nullptr;
+};
+
+struct cusna_mom_engine {
+  int P = 0, k = 0, kb = 0, p = 0, mode = 0, n = 0, Mmax = 0, n_rates = 0, W64 = 0;
+  SimSet main;
   DeviceBlock D;
   hipStream_t s = nullptr;
   double *theta = nullptr, *theta_sim = nullptr, *thav = nullptr, *dinv = nullptr;
-  double *targets = nullptr, *rate_consts = nullptr, *mean = nullptr, *S = nullptr;
-  double *SP = nullptr, *SC = nullptr, *mSP = nullptr, *mSC = nullptr, *Dm = nullptr,
-         *msf = nullptr;
-  double* bstats = nullptr;
+  double *targets = nullptr, *rate_consts = nullptr;
+  double *mSP = nullptr, *mSC = nullptr, *Dm = nullptr, *msf = nullptr, *var = nullptr;
   double* bscratch = nullptr;  // behaviour statistics above the LDS (else null)
+  size_t bscratch_doubles = 0;
   uint64_t* xnet = nullptr;
   int32_t* bkdist = nullptr;
   uint32_t* keys = nullptr;
   unsigned long long* counter = nullptr;
-  saom::SimParams* groups = nullptr;         // device parameters of each period
-  uint32_t* items = nullptr;                 // P * Mmax launch items
-  PeriodOut* per = nullptr;
   uint64_t seed = 0;
   long long n_sims = 0;
   double beh_zbar = 0.0, beh_brange = 1.0, beh_bsim = 0.0;
@@ -325,17 +513,97 @@ struct cusna_mom_engine {
 
 namespace {
 
-cusna_status engine_simulate(cusna_mom_engine* E, int M) {
+// Plans of every period for M chains. theta_base: the float32-rounded
+// parameter rows the kernels read (one shared row when row_stride is 0, else
+// row c at c * row_stride), laid out as the theta vector plus the constant
+// internal rate 1 in slot p.
+cusna_status build_simset(cusna_mom_engine* E, SimSet& X, const cusna_saom_problem* const* periods,
+                          int M, double* theta_base, int row_stride) {
+  const int P = E->P, p = E->p, k = E->k, kb = E->kb, mode = E->mode;
+  hipStream_t s = E->s;
+  X.M = M;
+  DeviceBlock& D = X.D;
+  const size_t o_S = D.add((size_t)M * p * 8), o_SP = D.add((size_t)P * M * p * 8);
+  const size_t o_SC = D.add((size_t)P * M * p * 8), o_mean = D.add(p * 8);
+  const size_t o_bst = D.add((size_t)P * M * (kb > 0 ? kb : 1) * 8);
+  const size_t o_per = D.add(P * sizeof(PeriodOut));
+  const size_t o_grp = D.add(P * sizeof(saom::SimParams)), o_items = D.add((size_t)P * M * 4);
+  HIP_TRY(D.alloc());
+  X.S = D.at<double>(o_S);
+  X.SP = D.at<double>(o_SP);
+  X.SC = D.at<double>(o_SC);
+  X.mean = D.at<double>(o_mean);
+  X.bstats = D.at<double>(o_bst);
+  X.per = D.at<PeriodOut>(o_per);
+  X.groups = D.at<saom::SimParams>(o_grp);
+  X.items = D.at<uint32_t>(o_items);
+  std::vector<PeriodOut> per(P);
+  std::vector<saom::SimParams> grp(P);
+  for (int pp = 0; pp < P; ++pp) {
+    const cusna_saom_problem* pb = periods[pp];
+    cusna_saom_problem q = *pb;
+    q.M = M;
+    std::vector<float> thf((size_t)M * k, 0.0f), lmf(M, 1.0f),
+        thbf((size_t)M * (kb > 0 ? kb : 1), 0.0f), lmbf(M, 1.0f);
+    q.theta = thf.data();
+    q.lam = lmf.data();
+    if (kb > 0) {
+      q.thetab = thbf.data();
+      q.lamb = lmbf.data();
+    }
+    q.conditional = mode == 1 ? 1 : 0;
+    cusna_period_plan* raw = nullptr;
+    cusna_status st = plan_create(&q, false, nullptr, nullptr, 0, s, &raw);
+    if (st != CUSNA_OK) return st;
+    X.plans.emplace_back(raw);
+    saom::SimParams& SPm = raw->P;
+    const int off = mode == 2 ? 2 * P : (mode == 1 ? 0 : P);
+    SPm.theta = theta_base + off;
+    SPm.th_stride = row_stride;
+    SPm.lam = mode == 1 ? theta_base + p : theta_base + pp;
+    SPm.lam_stride = row_stride;
+    if (mode == 2) {
+      SPm.lamb = theta_base + P + pp;
+      SPm.thetab = theta_base + 2 * P + k;
+      SPm.thb_stride = row_stride;
+    }
+    SPm.keys = E->keys + 2 * pp;
+    HIP_TRY(hipMemcpyAsync(raw->D.at<char>(raw->o_params), &SPm, sizeof(SPm),
+                           hipMemcpyHostToDevice, s));
+    per[pp].dist_stat = SPm.dist_stat;
+    per[pp].scores = SPm.scores;
+    per[pp].stats = raw->D.at<double>(raw->o_stats);
+    per[pp].bdist = SPm.bdist;
+    per[pp].bscores = SPm.bscores;
+    per[pp].bstats = X.bstats + (size_t)pp * M * (kb > 0 ? kb : 1);
+    if (raw->nw != X.plans[0]->nw || raw->wmax != X.plans[0]->wmax) {
+      set_error("cusna_mom_estimate: periods need the same launch shape");
+      return CUSNA_ERR_SHAPE;
+    }
+    grp[pp] = SPm;
+  }
+  std::vector<uint32_t> items((size_t)P * M);
+  for (size_t b = 0; b < items.size(); ++b)
+    items[b] = ((uint32_t)(b % P) << 20) | (uint32_t)(b / P);
+  HIP_TRY(hipMemcpyAsync(X.per, per.data(), P * sizeof(PeriodOut), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(X.groups, grp.data(), P * sizeof(saom::SimParams),
+                         hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(X.items, items.data(), items.size() * 4, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return CUSNA_OK;
+}
+
+// One simulation round of M chains over every period of X: fresh keys,
+// one period launch, statistics, moment rows (X.S, X.SP, X.SC), means.
+cusna_status engine_simulate(cusna_mom_engine* E, SimSet& X, int M) {
   hipStream_t s = E->s;
   hipLaunchKernelGGL(mom_seed_kernel, dim3(1), dim3(64), 0, s, E->keys, E->counter, E->seed,
                      E->P);
-  // all periods in one launch: item b = (period b % P, chain b / P), so the
-  // first P * M items are chains 0..M-1 of every period
   {
-    cusna_period_plan* p0 = E->plans[0].get();
+    cusna_period_plan* p0 = X.plans[0].get();
     saom::PeriodLaunch L;
-    L.Pd = E->groups;
-    L.items = E->items;
+    L.Pd = X.groups;
+    L.items = X.items;
     L.grid = E->P * M;
     L.lds = saom::sim_lds_bytes(p0->P, p0->nw, p0->wmax);
     L.key0 = L.key1 = 0u;
@@ -347,7 +615,7 @@ cusna_status engine_simulate(cusna_mom_engine* E, int M) {
     }
   }
   for (int pp = 0; pp < E->P; ++pp) {
-    cusna_period_plan* pl = E->plans[pp].get();
+    cusna_period_plan* pl = X.plans[pp].get();
     cusna_status st = plan_stats(pl, M, s);
     if (st != CUSNA_OK) return st;
     if (E->mode == 2) {
@@ -356,34 +624,37 @@ cusna_status engine_simulate(cusna_mom_engine* E, int M) {
       BP.n = E->n;
       BP.M = M;
       BP.kb = E->kb;
-      BP.W64 = (E->n + 63) / 64;
+      BP.W64 = E->W64;
       for (int e = 0; e < E->kb; ++e) BP.bty[e] = pl->P.bty[e];
       BP.z = pl->P.z_end;
-      BP.xnet = E->xnet + (size_t)pp * E->n * BP.W64;
+      BP.xnet = E->xnet + (size_t)pp * E->n * E->W64;
       BP.bkdist = E->bkdist + (size_t)pp * E->n;
       BP.zbar = E->beh_zbar;
       BP.brange = E->beh_brange;
       BP.bsim = E->beh_bsim;
-      BP.out = E->bstats + (size_t)pp * E->Mmax * E->kb;
+      BP.out = X.bstats + (size_t)pp * X.M * E->kb;
       BP.scratch = E->bscratch;
+      if (E->bscratch_doubles > 0 && M > E->Mmax) {
+        set_error("cusna_mom_estimate: behaviour scratch smaller than the round");
+        return CUSNA_ERR_INTERNAL;
+      }
       if (saom::launch_beh_stats(BP, s)) return CUSNA_ERR_CUDA;
     }
   }
-  GatherArgs G{M, E->P, E->p, E->k, E->kb, E->mode, E->per, E->rate_consts, E->S, E->SP, E->SC};
+  GatherArgs G{M, E->P, E->p, E->k, E->kb, E->mode, X.per, E->rate_consts, X.S, X.SP, X.SC};
   hipLaunchKernelGGL(mom_gather_kernel, dim3((M + 127) / 128), dim3(128), 0, s, G);
-  hipLaunchKernelGGL(mom_mean_kernel, dim3(E->p), dim3(kMomThreads), 0, s, E->S, M, E->p,
-                     E->mean);
+  hipLaunchKernelGGL(mom_mean_kernel, dim3(E->p), dim3(kMomThreads), 0, s, X.S, M, E->p, X.mean);
   E->n_sims += M;
   return hipGetLastError() == hipSuccess ? CUSNA_OK : CUSNA_ERR_CUDA;
 }
 
 // D = sum over periods of crossprod(centred sp, centred scp) / M (mom.R:173-182)
-void engine_derivative(cusna_mom_engine* E, int M) {
+void engine_derivative(cusna_mom_engine* E, SimSet& X, int M) {
   hipStream_t s = E->s;
   const int p = E->p;
   for (int pp = 0; pp < E->P; ++pp) {
-    const double* sp = E->SP + (size_t)pp * M * p;  // gather's layout: [period][chain][a]
-    const double* sc = E->SC + (size_t)pp * M * p;
+    const double* sp = X.SP + (size_t)pp * M * p;  // gather's layout: [period][chain][a]
+    const double* sc = X.SC + (size_t)pp * M * p;
     hipLaunchKernelGGL(mom_mean_kernel, dim3(p), dim3(kMomThreads), 0, s, sp, M, p, E->mSP);
     hipLaunchKernelGGL(mom_mean_kernel, dim3(p), dim3(kMomThreads), 0, s, sc, M, p, E->mSC);
     hipLaunchKernelGGL(mom_cov_kernel, dim3(p * p), dim3(kMomThreads), 0, s, sp, E->mSP, sc,
@@ -410,8 +681,147 @@ Mat dinv_of(const Mat& Dm, int p, double diag) {
   Mat mix(Dm);
   for (int a = 0; a < p; ++a)
     for (int b = 0; b < p; ++b)
-      mix[(size_t)a * p + b] = (1.0 - diag) * Dm[(size_t)a * p + b] + (a == b ? diag * Dm[(size_t)a * p + b] : 0.0);
+      mix[(size_t)a * p + b] =
+          (1.0 - diag) * Dm[(size_t)a * p + b] + (a == b ? diag * Dm[(size_t)a * p + b] : 0.0);
   return safe_solve(mix, p);
+}
+
+// sqrt(|diag(A^-1 S A^-T)|)
+std::vector<double> sandwich_se(const Mat& A, const Mat& S, int p) {
+  const Mat Ai = safe_solve(A, p);
+  std::vector<double> se(p);
+  for (int a = 0; a < p; ++a) {
+    double v = 0.0;
+    for (int b = 0; b < p; ++b)
+      for (int c = 0; c < p; ++c)
+        v += Ai[(size_t)a * p + b] * S[(size_t)b * p + c] * Ai[(size_t)a * p + c];
+    se[a] = std::sqrt(std::fabs(v));
+  }
+  return se;
+}
+
+// Surrogate phase 2. Box centre c and half-widths h on the host. Each
+// training round simulates the box, trains the perceptron and solves
+// E s(theta) = target on its mean (the next round's box re-centres there and,
+// when the solution is inside, shrinks to two standard errors). Then up to
+// `corrections` simulation-checked Newton steps: nD chains at theta give the
+// residual mean(s) - target and the score-function derivative, the step is
+// guarded as the Robbins-Monro steps are; they stop once every |residual| is
+// below conv_tol moment standard deviations. Leaves theta in the engine.
+cusna_status surrogate_phase(cusna_mom_engine* E, const cusna_saom_problem* const* periods,
+                             const cusna_mom_options* opt, std::vector<double> c,
+                             std::vector<double> h, const std::vector<double>& tgt) {
+  const int p = E->p;
+  hipStream_t s = E->s;
+  const int q = opt->surrogate_pairs_log2 > 0 ? opt->surrogate_pairs_log2 : 14;
+  if (q > 20 || ((long long)E->P << q) > (1ll << 30)) {
+    set_error("cusna_mom_estimate: surrogate_pairs_log2 too large");
+    return CUSNA_ERR_SHAPE;
+  }
+  const int Mtr = 1 << q;
+  const int steps = opt->surrogate_steps > 0 ? opt->surrogate_steps : 400;
+  const int rounds = opt->surrogate_rounds > 0 ? opt->surrogate_rounds : 1;
+  const int corrections = opt->surrogate_corrections >= 0 ? opt->surrogate_corrections : 4;
+  if (E->bscratch_doubles > 0 && Mtr > E->Mmax) {
+    set_error("cusna_mom_estimate: surrogate rounds above the behaviour scratch");
+    return CUSNA_ERR_UNSUPPORTED;
+  }
+  DeviceBlock D;
+  const size_t o_th = D.add((size_t)Mtr * (p + 1) * 8), o_x = D.add((size_t)Mtr * p * 4);
+  const size_t o_y = D.add((size_t)Mtr * p * 4), o_c = D.add(p * 8), o_h = D.add(p * 8);
+  HIP_TRY(D.alloc());
+  double* th_tr = D.at<double>(o_th);
+  float* x_tr = D.at<float>(o_x);
+  float* y_tr = D.at<float>(o_y);
+  SimSet T;
+  cusna_status st = build_simset(E, T, periods, Mtr, th_tr, p + 1);
+  if (st != CUSNA_OK) return st;
+  Mlp net;
+  if ((st = net.init(p, p, 256, 1024, opt->seed ^ 0x5u, s)) != CUSNA_OK) return st;
+  std::vector<float> flat(net.param_count());
+  HostNet H;
+  std::vector<double> mean(p), var(p), sd(p), tn(p), x, Jn, lv;
+  for (int r = 0; r < rounds; ++r) {
+    HIP_TRY(hipMemcpyAsync(D.at<char>(o_c), c.data(), p * 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(D.at<char>(o_h), h.data(), p * 8, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(sample_theta_kernel, dim3((Mtr + 255) / 256), dim3(256), 0, s,
+                       D.at<double>(o_c), D.at<double>(o_h), p, Mtr, opt->seed, (unsigned)r,
+                       E->n_rates, x_tr, th_tr);
+    if ((st = engine_simulate(E, T, Mtr)) != CUSNA_OK) return st;
+    hipLaunchKernelGGL(mom_var_kernel, dim3(p), dim3(kMomThreads), 0, s, T.S, Mtr, p, T.mean,
+                       E->var);
+    hipLaunchKernelGGL(normalize_kernel, dim3((unsigned)(((size_t)Mtr * p + 255) / 256)),
+                       dim3(256), 0, s, T.S, Mtr, p, T.mean, E->var, y_tr);
+    if ((st = fetch(E, mean.data(), T.mean, p)) != CUSNA_OK) return st;
+    if ((st = fetch(E, var.data(), E->var, p)) != CUSNA_OK) return st;
+    if ((st = net.reset(opt->seed + 17u * (uint64_t)r)) != CUSNA_OK) return st;
+    if ((st = net.train(x_tr, y_tr, Mtr, steps, 2e-3f, opt->seed + 1009u * (uint64_t)r)) !=
+        CUSNA_OK)
+      return st;
+    if ((st = net.get_params(flat.data())) != CUSNA_OK) return st;
+    H.load(net, flat, p, 256);
+    for (int a = 0; a < p; ++a) {
+      sd[a] = var[a] > 0.0 ? std::sqrt(var[a]) : 1.0;
+      tn[a] = (tgt[a] - mean[a]) / sd[a];
+    }
+    surrogate_newton(H, tn, x, Jn, lv);
+    bool interior = true;
+    for (int a = 0; a < p; ++a) {
+      c[a] = c[a] + h[a] * x[a];
+      if (std::fabs(x[a]) > 0.9) interior = false;
+    }
+    if (interior && r + 1 < rounds) {
+      // J in theta units and the predicted moment variances (diagonal)
+      Mat J((size_t)p * p), Sg((size_t)p * p, 0.0);
+      for (int a = 0; a < p; ++a) {
+        for (int d = 0; d < p; ++d) J[(size_t)a * p + d] = sd[a] * Jn[(size_t)a * p + d] / h[d];
+        Sg[(size_t)a * p + a] = std::exp(std::min(12.0, std::max(-12.0, lv[a]))) * var[a];
+      }
+      const std::vector<double> se = sandwich_se(J, Sg, p);
+      for (int a = 0; a < p; ++a)
+        h[a] = std::min(h[a], std::max(2.0 * se[a], 1e-3 * (1.0 + std::fabs(c[a]))));
+    }
+  }
+  // simulation-checked Newton corrections: residual and score-function
+  // derivative from nD plain chains at theta
+  std::vector<double> theta = c, sbar(p), msf((size_t)p * p), Dm((size_t)p * p);
+  auto put_theta = [&]() -> cusna_status {
+    std::vector<double> ts(p + 1);
+    for (int a = 0; a < p; ++a) ts[a] = (double)(float)theta[a];
+    ts[p] = 1.0;
+    HIP_TRY(hipMemcpyAsync(E->theta, theta.data(), p * 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(E->theta_sim, ts.data(), (p + 1) * 8, hipMemcpyHostToDevice, s));
+    return CUSNA_OK;
+  };
+  if ((st = put_theta()) != CUSNA_OK) return st;
+  SimSet& X = E->main;
+  for (int it = 0; it < corrections; ++it) {
+    if ((st = engine_simulate(E, X, opt->nD)) != CUSNA_OK) return st;
+    engine_derivative(E, X, opt->nD);
+    hipLaunchKernelGGL(mom_cov_kernel, dim3(p * p), dim3(kMomThreads), 0, s, X.S, X.mean, X.S,
+                       X.mean, opt->nD, p, (double)(opt->nD - 1), 0, E->msf);
+    if ((st = fetch(E, sbar.data(), X.mean, p)) != CUSNA_OK) return st;
+    if ((st = fetch(E, msf.data(), E->msf, (size_t)p * p)) != CUSNA_OK) return st;
+    if ((st = fetch(E, Dm.data(), E->Dm, (size_t)p * p)) != CUSNA_OK) return st;
+    double worst = 0.0;
+    for (int a = 0; a < p; ++a) {
+      const double v = msf[(size_t)a * p + a];
+      worst = std::max(worst, std::fabs(sbar[a] - tgt[a]) / (v > 0.0 ? std::sqrt(v) : 1.0));
+    }
+    if (worst <= opt->conv_tol) break;
+    const Mat Di = safe_solve(Dm, p);
+    for (int a = 0; a < p; ++a) {
+      double stp = 0.0;
+      for (int b = 0; b < p; ++b) stp += Di[(size_t)a * p + b] * (sbar[b] - tgt[b]);
+      const double cap = std::max(1.0, 0.5 * std::fabs(theta[a]));
+      stp = std::min(cap, std::max(-cap, stp));
+      if (a < E->n_rates && theta[a] - stp < 0.1 * theta[a]) stp = 0.5 * theta[a];
+      theta[a] -= stp;
+    }
+    if ((st = put_theta()) != CUSNA_OK) return st;
+  }
+  HIP_TRY(hipStreamSynchronize(s));
+  return CUSNA_OK;
 }
 
 }  // namespace
@@ -454,6 +864,10 @@ CUSNA_API cusna_status cusna_mom_estimate(const cusna_saom_problem* const* perio
     set_error("cusna_mom_estimate: at most 2^20 chains per simulation round");
     return CUSNA_ERR_SHAPE;
   }
+  for (int pp = 0; pp < P; ++pp) {
+    const cusna_saom_problem* pb = periods[pp];
+    if (!pb || pb->n != n || pb->k != k || (mode == 2 && pb->kb != kb)) return CUSNA_ERR_SHAPE;
+  }
 
   std::unique_ptr<cusna_mom_engine> E(new cusna_mom_engine());
   E->P = P;
@@ -462,6 +876,7 @@ CUSNA_API cusna_status cusna_mom_estimate(const cusna_saom_problem* const* perio
   E->p = p;
   E->mode = mode;
   E->n = n;
+  E->W64 = (n + 63) / 64;
   E->Mmax = Mmax;
   E->n_rates = mode == 1 ? 0 : (mode == 2 ? 2 * P : P);
   E->seed = opt->seed;
@@ -469,20 +884,16 @@ CUSNA_API cusna_status cusna_mom_estimate(const cusna_saom_problem* const* perio
   E->beh_brange = opt->beh_brange;
   E->beh_bsim = opt->beh_bsimmean;
   HIP_TRY(hipStreamCreateWithFlags(&E->s, hipStreamNonBlocking));
-  const int W64 = (n + 63) / 64;
+  const int W64 = E->W64;
   DeviceBlock& D = E->D;
   const size_t o_theta = D.add(p * 8), o_tsim = D.add((p + 1) * 8), o_thav = D.add(p * 8);
   const size_t o_dinv = D.add((size_t)p * p * 8), o_tg = D.add(p * 8), o_rc = D.add(P * 8);
-  const size_t o_mean = D.add(p * 8), o_S = D.add((size_t)Mmax * p * 8);
-  const size_t o_SP = D.add((size_t)P * Mmax * p * 8), o_SC = D.add((size_t)P * Mmax * p * 8);
   const size_t o_mSP = D.add(p * 8), o_mSC = D.add(p * 8), o_Dm = D.add((size_t)p * p * 8);
-  const size_t o_msf = D.add((size_t)p * p * 8);
-  const size_t o_bst = D.add((size_t)P * Mmax * (kb > 0 ? kb : 1) * 8);
+  const size_t o_msf = D.add((size_t)p * p * 8), o_var = D.add(p * 8);
   const size_t o_xn = D.add((size_t)P * n * W64 * 8), o_bk = D.add((size_t)P * n * 4);
-  const size_t o_keys = D.add(P * 8), o_cnt = D.add(8), o_per = D.add(P * sizeof(PeriodOut));
-  const size_t o_grp = D.add(P * sizeof(saom::SimParams)), o_items = D.add((size_t)P * Mmax * 4);
-  const size_t bscr_d = mode == 2 ? saom::beh_stats_scratch_doubles(n) : 0;
-  const size_t o_bscr = D.add((size_t)Mmax * bscr_d * 8);
+  const size_t o_keys = D.add(P * 8), o_cnt = D.add(8);
+  E->bscratch_doubles = mode == 2 ? saom::beh_stats_scratch_doubles(n) : 0;
+  const size_t o_bscr = D.add((size_t)Mmax * E->bscratch_doubles * 8);
   HIP_TRY(D.alloc());
   E->theta = D.at<double>(o_theta);
   E->theta_sim = D.at<double>(o_tsim);
@@ -490,23 +901,16 @@ CUSNA_API cusna_status cusna_mom_estimate(const cusna_saom_problem* const* perio
   E->dinv = D.at<double>(o_dinv);
   E->targets = D.at<double>(o_tg);
   E->rate_consts = D.at<double>(o_rc);
-  E->mean = D.at<double>(o_mean);
-  E->S = D.at<double>(o_S);
-  E->SP = D.at<double>(o_SP);
-  E->SC = D.at<double>(o_SC);
   E->mSP = D.at<double>(o_mSP);
   E->mSC = D.at<double>(o_mSC);
   E->Dm = D.at<double>(o_Dm);
   E->msf = D.at<double>(o_msf);
-  E->bstats = D.at<double>(o_bst);
+  E->var = D.at<double>(o_var);
   E->xnet = D.at<uint64_t>(o_xn);
   E->bkdist = D.at<int32_t>(o_bk);
   E->keys = D.at<uint32_t>(o_keys);
   E->counter = D.at<unsigned long long>(o_cnt);
-  E->per = D.at<PeriodOut>(o_per);
-  E->groups = D.at<saom::SimParams>(o_grp);
-  E->items = D.at<uint32_t>(o_items);
-  E->bscratch = bscr_d ? D.at<double>(o_bscr) : nullptr;
+  E->bscratch = E->bscratch_doubles ? D.at<double>(o_bscr) : nullptr;
   hipStream_t s = E->s;
 
   std::vector<double> rc(P, 0.0);
@@ -523,49 +927,8 @@ CUSNA_API cusna_status cusna_mom_estimate(const cusna_saom_problem* const* perio
   HIP_TRY(hipMemcpyAsync(E->targets, targets, p * 8, hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(E->rate_consts, rc.data(), P * 8, hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(E->counter, &zero, 8, hipMemcpyHostToDevice, s));
-
-  // one plan per period; the kernels read theta/lambda from the engine's
-  // float32-rounded theta (shared row, stride 0) and the keys from memory
-  std::vector<PeriodOut> per(P);
-  for (int pp = 0; pp < P; ++pp) {
-    const cusna_saom_problem* pb = periods[pp];
-    if (!pb || pb->n != n || pb->k != k || (mode == 2 && pb->kb != kb)) return CUSNA_ERR_SHAPE;
-    cusna_saom_problem q = *pb;
-    q.M = Mmax;
-    std::vector<float> thf((size_t)Mmax * k, 0.0f), lmf(Mmax, 1.0f),
-        thbf((size_t)Mmax * (kb > 0 ? kb : 1), 0.0f), lmbf(Mmax, 1.0f);
-    q.theta = thf.data();
-    q.lam = lmf.data();
-    if (kb > 0) {
-      q.thetab = thbf.data();
-      q.lamb = lmbf.data();
-    }
-    q.conditional = mode == 1 ? 1 : 0;
-    cusna_period_plan* raw = nullptr;
-    cusna_status st = plan_create(&q, false, nullptr, nullptr, 0, s, &raw);
-    if (st != CUSNA_OK) return st;
-    E->plans.emplace_back(raw);
-    saom::SimParams& SPm = raw->P;
-    const int off = mode == 2 ? 2 * P : (mode == 1 ? 0 : P);
-    SPm.theta = E->theta_sim + off;
-    SPm.th_stride = 0;
-    SPm.lam = mode == 1 ? E->theta_sim + p : E->theta_sim + pp;
-    SPm.lam_stride = 0;
-    if (mode == 2) {
-      SPm.lamb = E->theta_sim + P + pp;
-      SPm.thetab = E->theta_sim + 2 * P + k;
-      SPm.thb_stride = 0;
-    }
-    SPm.keys = E->keys + 2 * pp;
-    HIP_TRY(hipMemcpyAsync(raw->D.at<char>(raw->o_params), &SPm, sizeof(SPm),
-                           hipMemcpyHostToDevice, s));
-    per[pp].dist_stat = SPm.dist_stat;
-    per[pp].scores = SPm.scores;
-    per[pp].stats = raw->D.at<double>(raw->o_stats);
-    per[pp].bdist = SPm.bdist;
-    per[pp].bscores = SPm.bscores;
-    per[pp].bstats = E->bstats + (size_t)pp * Mmax * (kb > 0 ? kb : 1);
-    if (mode == 2) {
+  if (mode == 2) {
+    for (int pp = 0; pp < P; ++pp) {
       std::vector<uint64_t> xr((size_t)n * W64, 0ull);
       for (int i = 0; i < n; ++i)
         for (int j = 0; j < n; ++j)
@@ -573,32 +936,20 @@ CUSNA_API cusna_status cusna_mom_estimate(const cusna_saom_problem* const* perio
       HIP_TRY(hipMemcpyAsync(E->xnet + (size_t)pp * n * W64, xr.data(), xr.size() * 8,
                              hipMemcpyHostToDevice, s));
       std::vector<int32_t> bk(n, 1);
-      if (pb->bkdist)
-        for (int i = 0; i < n; ++i) bk[i] = pb->bkdist[i];
+      if (periods[pp]->bkdist)
+        for (int i = 0; i < n; ++i) bk[i] = periods[pp]->bkdist[i];
       HIP_TRY(hipMemcpyAsync(E->bkdist + (size_t)pp * n, bk.data(), n * 4, hipMemcpyHostToDevice,
                              s));
       HIP_TRY(hipStreamSynchronize(s));
     }
   }
-  HIP_TRY(hipMemcpyAsync(E->per, per.data(), P * sizeof(PeriodOut), hipMemcpyHostToDevice, s));
-  std::vector<saom::SimParams> grp(P);
-  for (int pp = 0; pp < P; ++pp) {
-    cusna_period_plan* pl = E->plans[pp].get();
-    if (pl->nw != E->plans[0]->nw || pl->wmax != E->plans[0]->wmax) {
-      set_error("cusna_mom_estimate: periods need the same launch shape");
-      return CUSNA_ERR_SHAPE;
-    }
-    grp[pp] = pl->P;
-  }
-  std::vector<uint32_t> items((size_t)P * Mmax);
-  for (size_t b = 0; b < items.size(); ++b)
-    items[b] = ((uint32_t)(b % P) << 20) | (uint32_t)(b / P);
-  HIP_TRY(hipMemcpyAsync(E->groups, grp.data(), P * sizeof(saom::SimParams),
-                         hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(E->items, items.data(), items.size() * 4, hipMemcpyHostToDevice, s));
-  HIP_TRY(hipStreamSynchronize(s));
+  // the estimation rounds read the engine's float32-rounded theta (one shared
+  // row) and the Philox keys from device memory
+  cusna_status st = build_simset(E.get(), E->main, periods, Mmax, E->theta_sim, 0);
+  if (st != CUSNA_OK) return st;
+  SimSet& X = E->main;
 
-  Mat Dm((size_t)p * p), mean(p);
+  Mat Dm((size_t)p * p);
   std::vector<double> tgt(targets, targets + p);
   auto upload_dinv = [&](const Mat& di) -> cusna_status {
     HIP_TRY(hipMemcpyAsync(E->dinv, di.data(), (size_t)p * p * 8, hipMemcpyHostToDevice, s));
@@ -607,31 +958,30 @@ CUSNA_API cusna_status cusna_mom_estimate(const cusna_saom_problem* const* perio
   };
   auto t_phase = clk::now();
   double phase_t[3] = {0, 0, 0};
-  cusna_status st;
 
   // ---- phase 1 (mom.R:224-229)
-  if ((st = engine_simulate(E.get(), opt->n1)) != CUSNA_OK) return st;
-  engine_derivative(E.get(), opt->n1);
+  if ((st = engine_simulate(E.get(), X, opt->n1)) != CUSNA_OK) return st;
+  engine_derivative(E.get(), X, opt->n1);
   if ((st = fetch(E.get(), Dm.data(), E->Dm, (size_t)p * p)) != CUSNA_OK) return st;
   if ((st = upload_dinv(dinv_of(Dm, p, opt->diagonalize))) != CUSNA_OK) return st;
   hipLaunchKernelGGL(mom_update_kernel, dim3(1), dim3(64), 0, s, E->theta, E->theta_sim,
-                     E->thav, E->dinv, E->mean, E->targets, p, 0.5 * opt->firstg, E->n_rates, 0);
+                     E->thav, E->dinv, X.mean, E->targets, p, 0.5 * opt->firstg, E->n_rates, 0);
   phase_t[0] = std::chrono::duration<double>(clk::now() - t_phase).count();
   t_phase = clk::now();
 
   // ---- phase 2 (mom.R:232-248), sub-phases replayed from a captured graph
   auto subphase = [&](double gain, int n2) -> cusna_status {
-    cusna_status st2 = engine_simulate(E.get(), opt->nD);
+    cusna_status st2 = engine_simulate(E.get(), X, opt->nD);
     if (st2 != CUSNA_OK) return st2;
-    engine_derivative(E.get(), opt->nD);
+    engine_derivative(E.get(), X, opt->nD);
     if ((st2 = fetch(E.get(), Dm.data(), E->Dm, (size_t)p * p)) != CUSNA_OK) return st2;
     if ((st2 = upload_dinv(dinv_of(Dm, p, opt->diagonalize))) != CUSNA_OK) return st2;
     if (n2 < 1) return CUSNA_OK;
     auto iteration = [&]() -> cusna_status {
-      cusna_status st3 = engine_simulate(E.get(), opt->batch2);
+      cusna_status st3 = engine_simulate(E.get(), X, opt->batch2);
       if (st3 != CUSNA_OK) return st3;
       hipLaunchKernelGGL(mom_update_kernel, dim3(1), dim3(64), 0, s, E->theta, E->theta_sim,
-                         E->thav, E->dinv, E->mean, E->targets, p, gain, E->n_rates, 1);
+                         E->thav, E->dinv, X.mean, E->targets, p, gain, E->n_rates, 1);
       return hipGetLastError() == hipSuccess ? CUSNA_OK : CUSNA_ERR_CUDA;
     };
     if (opt->use_graph) {
@@ -662,8 +1012,32 @@ CUSNA_API cusna_status cusna_mom_estimate(const cusna_saom_problem* const* perio
                        E->thav, p, (double)n2);
     return CUSNA_OK;
   };
-  for (int sub = 0; sub < opt->nsub; ++sub) {
-    if ((st = subphase(opt->firstg * std::pow(0.5, sub), opt->n2[sub])) != CUSNA_OK) return st;
+  if (opt->surrogate) {
+    // box from phase 1: Newton point of the phase-1 moments and derivative,
+    // half-widths covering the step and three standard errors
+    std::vector<double> m1(p), msf1((size_t)p * p);
+    hipLaunchKernelGGL(mom_cov_kernel, dim3(p * p), dim3(kMomThreads), 0, s, X.S, X.mean, X.S,
+                       X.mean, opt->n1, p, (double)(opt->n1 - 1), 0, E->msf);
+    if ((st = fetch(E.get(), m1.data(), X.mean, p)) != CUSNA_OK) return st;
+    if ((st = fetch(E.get(), msf1.data(), E->msf, (size_t)p * p)) != CUSNA_OK) return st;
+    const Mat D1i = safe_solve(Dm, p);
+    std::vector<double> c(p), h(p);
+    const std::vector<double> se1 = sandwich_se(Dm, msf1, p);
+    for (int a = 0; a < p; ++a) {
+      double step = 0.0;
+      for (int b = 0; b < p; ++b) step += D1i[(size_t)a * p + b] * (m1[b] - tgt[b]);
+      const double cap = 2.0 * std::max(1.0, std::fabs(th0[a]));
+      step = std::min(cap, std::max(-cap, step));
+      c[a] = th0[a] - step;
+      if (a < E->n_rates && c[a] < 0.2 * th0[a]) c[a] = 0.2 * th0[a];
+      h[a] = std::max(std::max(2.0 * std::fabs(c[a] - th0[a]), 3.0 * se1[a]),
+                      1e-3 * (1.0 + std::fabs(c[a])));
+    }
+    if ((st = surrogate_phase(E.get(), periods, opt, c, h, tgt)) != CUSNA_OK) return st;
+  } else {
+    for (int sub = 0; sub < opt->nsub; ++sub) {
+      if ((st = subphase(opt->firstg * std::pow(0.5, sub), opt->n2[sub])) != CUSNA_OK) return st;
+    }
   }
   phase_t[1] = std::chrono::duration<double>(clk::now() - t_phase).count();
 
@@ -675,15 +1049,14 @@ CUSNA_API cusna_status cusna_mom_estimate(const cusna_saom_problem* const* perio
   std::vector<double> theta_h(p);
   for (;;) {
     t_phase = clk::now();
-    if ((st = engine_simulate(E.get(), opt->n3)) != CUSNA_OK) return st;
-    engine_derivative(E.get(), opt->n3);
-    hipLaunchKernelGGL(mom_cov_kernel, dim3(p * p), dim3(kMomThreads), 0, s, E->S, E->mean, E->S,
-                       E->mean, opt->n3, p, (double)(opt->n3 - 1), 0, E->msf);
+    if ((st = engine_simulate(E.get(), X, opt->n3)) != CUSNA_OK) return st;
+    engine_derivative(E.get(), X, opt->n3);
+    hipLaunchKernelGGL(mom_cov_kernel, dim3(p * p), dim3(kMomThreads), 0, s, X.S, X.mean, X.S,
+                       X.mean, opt->n3, p, (double)(opt->n3 - 1), 0, E->msf);
     if ((st = fetch(E.get(), Dm.data(), E->Dm, (size_t)p * p)) != CUSNA_OK) return st;
     if ((st = fetch(E.get(), msf.data(), E->msf, (size_t)p * p)) != CUSNA_OK) return st;
-    if ((st = fetch(E.get(), sim_means.data(), E->mean, p)) != CUSNA_OK) return st;
+    if ((st = fetch(E.get(), sim_means.data(), X.mean, p)) != CUSNA_OK) return st;
     if ((st = fetch(E.get(), theta_h.data(), E->theta, p)) != CUSNA_OK) return st;
-    const Mat dinv = safe_solve(Dm, p);
     std::vector<double> dev(p);
     max_abs_t = 0.0;
     for (int a = 0; a < p; ++a) {
@@ -696,13 +1069,7 @@ CUSNA_API cusna_status cusna_mom_estimate(const cusna_saom_problem* const* perio
     for (int a = 0; a < p; ++a)
       for (int b = 0; b < p; ++b) q += dev[a] * msfi[(size_t)a * p + b] * dev[b];
     tconv_max = std::sqrt(q);
-    for (int a = 0; a < p; ++a) {
-      double v = 0.0;  // diag(dinv msf dinv^T)
-      for (int b = 0; b < p; ++b)
-        for (int c = 0; c < p; ++c)
-          v += dinv[(size_t)a * p + b] * msf[(size_t)b * p + c] * dinv[(size_t)a * p + c];
-      se[a] = std::sqrt(std::fabs(v));
-    }
+    se = sandwich_se(Dm, msf, p);  // diag(D^-1 msf D^-T)
     phase_t[2] += std::chrono::duration<double>(clk::now() - t_phase).count();
     if (max_abs_t <= opt->conv_tol || repeats >= opt->max_repeats) break;
     ++repeats;
@@ -717,7 +1084,7 @@ CUSNA_API cusna_status cusna_mom_estimate(const cusna_saom_problem* const* perio
   if (mode == 1 && (res->rate_est || res->rate_sd)) {
     for (int pp = 0; pp < P; ++pp) {
       std::vector<double> nt(opt->n3);
-      if ((st = fetch(E.get(), nt.data(), E->plans[pp]->P.ntim, opt->n3)) != CUSNA_OK) return st;
+      if ((st = fetch(E.get(), nt.data(), X.plans[pp]->P.ntim, opt->n3)) != CUSNA_OK) return st;
       double m = 0.0;
       for (double x : nt) m += x;
       m /= opt->n3;
@@ -728,14 +1095,14 @@ CUSNA_API cusna_status cusna_mom_estimate(const cusna_saom_problem* const* perio
     }
   }
   if (res->phase3_stats) {
-    if ((st = fetch(E.get(), res->phase3_stats, E->S, (size_t)opt->n3 * p)) != CUSNA_OK) return st;
+    if ((st = fetch(E.get(), res->phase3_stats, X.S, (size_t)opt->n3 * p)) != CUSNA_OK) return st;
   }
   if (res->phase3_sp) {
-    if ((st = fetch(E.get(), res->phase3_sp, E->SP, (size_t)P * opt->n3 * p)) != CUSNA_OK)
+    if ((st = fetch(E.get(), res->phase3_sp, X.SP, (size_t)P * opt->n3 * p)) != CUSNA_OK)
       return st;
   }
   if (res->phase3_scp) {
-    if ((st = fetch(E.get(), res->phase3_scp, E->SC, (size_t)P * opt->n3 * p)) != CUSNA_OK)
+    if ((st = fetch(E.get(), res->phase3_scp, X.SC, (size_t)P * opt->n3 * p)) != CUSNA_OK)
       return st;
   }
   res->p = p;

@@ -300,7 +300,10 @@ def simulate_groups(problems: list, want_end_nets: bool = False) -> list:
 def mom_estimate(problems: list, targets, theta0, mode: int, n1: int, nD: int, batch2: int,
                  n3: int, n2: list, firstg: float = 0.4, diagonalize: float = 0.2,
                  seed: int = 1234, xnets: list | None = None, rate_consts=None,
-                 use_graph: bool = True, max_repeats: int = 0, conv_tol: float = 0.1) -> dict:
+                 use_graph: bool = True, max_repeats: int = 0, conv_tol: float = 0.1,
+                 surrogate: bool = False, surrogate_pairs_log2: int = 0,
+                 surrogate_steps: int = 0, surrogate_rounds: int = 0,
+                 surrogate_corrections: int = -1) -> dict:
     """Method-of-moments estimate on the device (cusna_mom_estimate).
 
     ``problems``: one SaomProblem per period (inputs only); ``xnets``: the
@@ -332,6 +335,9 @@ def mom_estimate(problems: list, targets, theta0, mode: int, n1: int, nD: int, b
     o.firstg, o.diagonalize = float(firstg), float(diagonalize)
     o.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
     o.use_graph, o.max_repeats, o.conv_tol = int(use_graph), int(max_repeats), float(conv_tol)
+    o.surrogate, o.surrogate_pairs_log2 = int(surrogate), int(surrogate_pairs_log2)
+    o.surrogate_steps, o.surrogate_rounds = int(surrogate_steps), int(surrogate_rounds)
+    o.surrogate_corrections = int(surrogate_corrections)
     if mode == 2:
         pr0 = problems[0]
         o.beh_zbar, o.beh_brange, o.beh_bsimmean = (float(pr0.zbar), float(pr0.brange),

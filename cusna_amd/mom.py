@@ -124,13 +124,20 @@ class MomEstimator:
         return np.concatenate([lam0, ev0])
 
     def run(self, theta0=None, use_graph: bool = True, max_repeats: int = 0,
-            conv_tol: float = 0.1) -> MomFit:
+            conv_tol: float = 0.1, surrogate: bool = False, surrogate_pairs_log2: int = 0,
+            surrogate_steps: int = 0, surrogate_rounds: int = 0,
+            surrogate_corrections: int = -1) -> MomFit:
+        """Estimate. ``surrogate`` replaces the Robbins-Monro sub-phases by the
+        perceptron surrogate (see cusna_mom_options in include/libcusna.h)."""
         c = self.control
         th0 = self.default_theta0() if theta0 is None else np.asarray(theta0, float)
         r = lib.mom_estimate(self.problems, self.targets, th0, self.mode, c.n1, c.nD, c.batch2,
                              c.n3, list(c.n2), c.firstg, c.diagonalize, c.seed,
                              xnets=self.xnets, rate_consts=self.rate_consts,
-                             use_graph=use_graph, max_repeats=max_repeats, conv_tol=conv_tol)
+                             use_graph=use_graph, max_repeats=max_repeats, conv_tol=conv_tol,
+                             surrogate=surrogate, surrogate_pairs_log2=surrogate_pairs_log2,
+                             surrogate_steps=surrogate_steps, surrogate_rounds=surrogate_rounds,
+                             surrogate_corrections=surrogate_corrections)
         return MomFit(theta=r["theta"], se=r["se"], tconv=r["tconv"],
                       tconv_max=r["tconv_max"], max_abs_tconv=r["max_abs_tconv"],
                       param_names=self.param_names, targets=self.targets,

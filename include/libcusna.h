@@ -335,6 +335,22 @@ typedef struct cusna_mom_options {
    * behaviour statistics take (mom.R:114-115); the period problems carry
    * their float32 roundings for the simulator */
   double beh_zbar, beh_brange, beh_bsimmean;
+  /* Surrogate phase 2 (no counterpart in the reference) in place of the
+   * Robbins-Monro sub-phases: per round, 2^surrogate_pairs_log2 chains (0:
+   * 2^14) with per-chain theta drawn uniformly from a box around the phase-1
+   * Newton point, the surrogate perceptron trained on them for
+   * surrogate_steps Adam steps (0: 400), E s(theta) = target solved on its
+   * mean; the box re-centres on the solution and shrinks to two standard
+   * errors over surrogate_rounds rounds (0: 1). Then up to
+   * surrogate_corrections (negative: 4) Newton steps with residual and
+   * score-function derivative from nD plain chains, stopping once every
+   * residual is below conv_tol moment standard deviations. Phase 3 as usual. */
+  int32_t surrogate;
+  int32_t surrogate_pairs_log2;
+  int32_t surrogate_steps;
+  int32_t surrogate_rounds;
+  int32_t surrogate_corrections;
+  int32_t reserved2;
 } cusna_mom_options;
 
 typedef struct cusna_mom_result {
@@ -370,7 +386,8 @@ typedef struct cusna_mom_result {
  *   forward_backward  one batch (batch x p_in, batch x p_out, host float32):
  *                 outputs (batch x N_3), gradients (flat), mean loss
  *   train         `steps` Adam steps on minibatches drawn with replacement
- *                 from n host samples; last minibatch's mean loss
+ *                 from n host samples, learning rate decaying linearly from lr
+ *                 to lr / 10; last minibatch's mean loss
  *   predict       outputs [mu, lv] (n x 2 p_out) for n inputs
  */
 typedef struct cusna_mlp cusna_mlp;

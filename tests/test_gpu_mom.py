@@ -94,3 +94,32 @@ def test_rejects_bad_schedule(gpu_lib):
     est, _ = _est("baseline", n3=1)
     with pytest.raises(CusnaError):
         est.run()
+
+
+def test_surrogate_ablation(gpu_lib):
+    """Surrogate phase 2 (perceptron root + simulation-checked Newton
+    corrections) against the plain Robbins-Monro phase 2 on the config-1 fit
+    (50 actors, conditional, reference default schedule, n3 = 1000): both
+    reach max|t| <= 0.1 (repeats allowed), the estimates agree within one
+    standard error of the plain fit, and the surrogate uses fewer simulated
+    chains and less wall time."""
+    import time
+
+    from cusna_amd.mom import MomControl, MomEstimator
+    from cusna_amd.workloads import c1_estimation
+
+    data, eff = c1_estimation(lambda p: gpu_lib.simulate_period(p, want_end_nets=True)["a_ends"])
+    est = MomEstimator(data, eff, None, True, MomControl(n3=1000))
+    est.run(max_repeats=0)  # code objects loaded
+    t0 = time.perf_counter()
+    plain = est.run(max_repeats=10, conv_tol=0.1)
+    t_plain = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    sur = est.run(max_repeats=10, conv_tol=0.1, surrogate=True)
+    t_sur = time.perf_counter() - t0
+    assert plain.max_abs_tconv <= 0.1 and sur.max_abs_tconv <= 0.1
+    assert np.all(np.abs(sur.theta - plain.theta) <= plain.se), (sur.theta, plain.theta)
+    assert sur.n_sims < plain.n_sims
+    assert t_sur < t_plain, (t_sur, t_plain)
+    print(f"plain {t_plain:.3f} s, {plain.n_sims} chains; surrogate {t_sur:.3f} s, "
+          f"{sur.n_sims} chains")
