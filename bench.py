@@ -204,7 +204,8 @@ def main() -> int:
         "roofline": {"bound": "valu", "achieved": round(achieved / 1e12, 3),
                      "peak": round(VALU_PEAK_OPS / 1e12, 3), "unit": "Tops/s",
                      "frac": round(achieved / VALU_PEAK_OPS, 4), "traffic": None,
-                     "kernel": "saom_period_kernel", "kernel_ms": round(kms, 3),
+                     "kernel": "saom_period_rtc" if plans[0].specialized
+                               else "saom_period_kernel", "kernel_ms": round(kms, 3),
                      "alternatives_per_launch": float(np.mean(units)),
                      "ops_per_alternative": ops_unit},
     }

@@ -28,7 +28,7 @@ HIPCC_FLAGS = [
 
 # (source, extra defines, object name)
 UNITS = [("capi.hip", [], "capi.o"), ("saom_stats.hip", [], "saom_stats.o"),
-         ("saom_launch.hip", [], "saom_launch.o"), ("mom.hip", [], "mom.o"), ("mlp.hip", [], "mlp.o")] + [
+         ("saom_launch.hip", [], "saom_launch.o"), ("mom.hip", [], "mom.o"), ("mlp.hip", [], "mlp.o"), ("rtc.hip", [], "rtc.o")] + [
     ("saom_period.hip", [f"-DSAOM_WMAX={w}"], f"saom_period_w{w}.o") for w in (1, 2, 4, 8, 16, 32, 64, 128)]
 
 
@@ -48,8 +48,50 @@ def _compile(src: str, defs: list, obj: str, verbose: bool) -> None:
         raise RuntimeError(f"hipcc failed on {src}:\n{r.stderr[-4000:]}")
 
 
+# Sources of the period kernel embedded in the library for runtime
+# specialisation (rtc.hip): headers inlined in dependency order, local
+# includes dropped, a preamble supplying what hipRTC's implicit runtime header
+# names differently.
+RTC_SOURCES = ["saom_common.h", "saom_kernels.h", "saom_device.h", "saom_period.hip"]
+RTC_PREAMBLE = """#define SAOM_RTC 1
+typedef __hip_internal::uint8_t uint8_t;
+typedef __hip_internal::uint32_t uint32_t;
+typedef __hip_internal::uint64_t uint64_t;
+typedef __hip_internal::int32_t int32_t;
+typedef __hip_internal::int64_t int64_t;
+#ifndef INFINITY
+#define INFINITY __builtin_inff()
+#endif
+"""
+
+
+def write_rtc_source() -> str:
+    """Generate csrc/rtc_source.inc (a raw string literal of the kernel source)."""
+    import re
+
+    parts = [RTC_PREAMBLE]
+    for f in RTC_SOURCES:
+        with open(os.path.join(CSRC, f)) as fh:
+            src = fh.read()
+        src = re.sub(r"^#pragma once\s*$", "", src, flags=re.M)
+        src = re.sub(r"^#include .*$", "", src, flags=re.M)
+        parts.append(f"// ---- {f}\n{src}")
+    body = "\n".join(parts)
+    if ")CUSNASRC" in body:
+        raise RuntimeError("kernel source contains the raw-string delimiter")
+    out = os.path.join(CSRC, "rtc_source.inc")
+    text = ("// generated by cusna_amd/build.py from " + ", ".join(RTC_SOURCES) + "\n"
+            'static const char kRtcPeriodSource[] = R"CUSNASRC(' + body + ')CUSNASRC";\n')
+    old = open(out).read() if os.path.exists(out) else None
+    if old != text:
+        with open(out, "w") as fh:
+            fh.write(text)
+    return out
+
+
 def build(verbose: bool = False, jobs: int | None = None) -> str:
     """Compile every HIP unit for gfx950 and link the shared library."""
+    write_rtc_source()
     os.makedirs(OUT_DIR, exist_ok=True)
     objdir = os.path.join(OUT_DIR, "obj")
     os.makedirs(objdir, exist_ok=True)
@@ -60,7 +102,7 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
             f.result()
     tmp = LIB + ".tmp"
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp,
-           *[os.path.join(objdir, o) for _, _, o in UNITS]]
+           *[os.path.join(objdir, o) for _, _, o in UNITS], "-lhiprtc"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")

@@ -257,6 +257,7 @@ cusna_status plan_create(const cusna_saom_problem* pb, bool want_dense, const do
     if (P.dci[e] + 1 > nd) nd = P.dci[e] + 1;
   }
   for (int e = 0; e < kb; ++e) P.bty[e] = pb->bty[e];
+  pl->specialize = !replay && saom::specialize_policy((long long)M * n);
   pl->nw = pick_nw(wmax, pb->threads);
   if (pb->threads <= 0) {
     const int tuned = cached_nw(ShapeKey{wmax, W64, k, P.flags});
@@ -444,7 +445,7 @@ cusna_status plan_run(cusna_period_plan* pl, uint64_t seed, bool want_stats, int
   if (timed) HIP_TRY(hipEventRecord(pl->ev0, s));
   int le = saom::launch_period(P, pl->D.at<saom::SimParams>(pl->o_params),
                                saom::LaunchShape{pl->nw, pl->wmax}, pl->replay, P.key0, P.key1,
-                               s);
+                               s, pl->specialize, &pl->last_rtc);
   if (le != 0) {
     set_error(std::string("period kernel launch: ") + hipGetErrorString((hipError_t)le));
     return CUSNA_ERR_CUDA;
@@ -627,6 +628,14 @@ cusna_status simulate_groups(const cusna_saom_problem* const* pbs, int G,
     L.grid = (int)(items.size() - first);
     L.lds = lds;
     L.key0 = L.key1 = 0u;
+    bool uniform = true;
+    long long work = 0;
+    for (int g : b.groups) {
+      uniform = uniform && saom::same_effects(plans[g]->P, plans[b.groups[0]]->P);
+      work += (long long)plans[g]->M * plans[g]->n;
+    }
+    L.Ph = &plans[b.groups[0]]->P;
+    L.specialize = uniform && saom::specialize_policy(work);
     HIP_TRY(hipMemcpyAsync(D.at<uint32_t>(o_items) + first, items.data() + first,
                            (items.size() - first) * 4, hipMemcpyHostToDevice, s));
     const int le =
@@ -788,6 +797,20 @@ CUSNA_API cusna_status cusna_saom_simulate_groups(const cusna_saom_problem* cons
   if (ngroups < 1 || ngroups > 4096) return CUSNA_ERR_SHAPE;
   if (device_count() < 1) return CUSNA_ERR_NO_CUDA;
   return simulate_groups(problems, ngroups, results);
+}
+
+CUSNA_API cusna_status cusna_period_plan_specialize(cusna_period_plan* plan, int32_t mode) {
+  if (!plan) return CUSNA_ERR_NULL_ARG;
+  if (plan->replay && mode > 0) return CUSNA_ERR_UNSUPPORTED;
+  plan->specialize = mode > 0 ? true
+                     : mode == 0 ? false
+                                 : (!plan->replay &&
+                                    saom::specialize_policy((long long)plan->M * plan->n));
+  return CUSNA_OK;
+}
+
+CUSNA_API int32_t cusna_period_plan_specialized(const cusna_period_plan* plan) {
+  return plan && plan->last_rtc ? 1 : 0;
 }
 
 CUSNA_API cusna_status cusna_period_plan_autotune(cusna_period_plan* plan, int32_t probe_chains,

@@ -487,6 +487,7 @@ struct SimSet {
   uint32_t* items = nullptr;
   PeriodOut* per = nullptr;
   double *S = nullptr, *SP = nullptr, *SC = nullptr, *bstats = nullptr, *mean = nullptr;
+  bool specialize = false;  // runtime-specialised period kernel (same effects in every period)
 };
 
 struct cusna_mom_engine {
@@ -505,6 +506,7 @@ struct cusna_mom_engine {
   unsigned long long* counter = nullptr;
   uint64_t seed = 0;
   long long n_sims = 0;
+  int specialize = -1;  // 1 on, 0 off, -1 by the launch-size policy
   double beh_zbar = 0.0, beh_brange = 1.0, beh_bsim = 0.0;
   ~cusna_mom_engine() {
     if (s) (void)hipStreamDestroy(s);
@@ -590,6 +592,11 @@ cusna_status build_simset(cusna_mom_engine* E, SimSet& X, const cusna_saom_probl
                          hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(X.items, items.data(), items.size() * 4, hipMemcpyHostToDevice, s));
   HIP_TRY(hipStreamSynchronize(s));
+  bool uniform = true;
+  for (int pp = 1; pp < P; ++pp) uniform = uniform && saom::same_effects(grp[pp], grp[0]);
+  X.specialize = uniform && (E->specialize > 0 ||
+                             (E->specialize < 0 &&
+                              saom::specialize_policy((long long)P * M * E->n)));
   return CUSNA_OK;
 }
 
@@ -607,6 +614,8 @@ cusna_status engine_simulate(cusna_mom_engine* E, SimSet& X, int M) {
     L.grid = E->P * M;
     L.lds = saom::sim_lds_bytes(p0->P, p0->nw, p0->wmax);
     L.key0 = L.key1 = 0u;
+    L.Ph = &p0->P;
+    L.specialize = X.specialize;
     const int le = saom::launch_period_ex(L, saom::LaunchShape{p0->nw, p0->wmax}, false,
                                           saom::simple_effects(p0->P.tid, p0->P.k), s);
     if (le != 0) {
@@ -880,6 +889,7 @@ CUSNA_API cusna_status cusna_mom_estimate(const cusna_saom_problem* const* perio
   E->Mmax = Mmax;
   E->n_rates = mode == 1 ? 0 : (mode == 2 ? 2 * P : P);
   E->seed = opt->seed;
+  E->specialize = opt->specialize > 0 ? 1 : (opt->specialize < 0 ? 0 : -1);
   E->beh_zbar = opt->beh_zbar;
   E->beh_brange = opt->beh_brange;
   E->beh_bsim = opt->beh_bsimmean;

@@ -69,6 +69,8 @@ struct cusna_period_plan {
   size_t o_dense = 0, o_th = 0, o_lam = 0, o_thb = 0, o_lamb = 0, o_params = 0;
   size_t o_sscr = 0;       // statistics scratch (networks above the LDS)
   bool has_sscr = false;
+  bool specialize = false;  // launch the runtime-specialised period kernel
+  bool last_rtc = false;    // the last run did
   bool stats_valid = false;
   ~cusna_period_plan() {
     if (ev0) (void)hipEventDestroy(ev0);

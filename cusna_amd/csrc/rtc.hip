@@ -1,0 +1,181 @@
+// rtc.hip -- runtime specialisation of the period kernel with hipRTC.
+//
+// The library embeds the period kernel's source (rtc_source.inc, generated by
+// the build). For a launch it can compile a kernel specialised to the
+// problem's effect list (types, creation/endowment types, behaviour-dynamic
+// flags), word bound, block shape and variant: the effect loops unroll over
+// compile-time types and, when k x alternatives-per-lane is small, the change
+// statistics stay in registers between the choice and score passes. The
+// arithmetic and its order are those of the generic kernel, so results are
+// identical. Compiled code objects are cached in memory per device and on
+// disk (CUSNA_RTC_CACHE, else $XDG_CACHE_HOME/cusna_amd, else
+// $HOME/.cache/cusna_amd); any failure falls back to the generic kernel.
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "saom_kernels.h"
+
+#include "rtc_source.inc"
+
+namespace saom {
+namespace {
+
+struct RtcEntry {
+  hipModule_t mod = nullptr;
+  hipFunction_t fn = nullptr;
+  bool failed = false;
+};
+
+std::mutex g_rtc_mu;
+std::map<std::string, RtcEntry> g_rtc;  // key: device + options
+
+uint64_t fnv1a(const std::string& a, uint64_t h = 1469598103934665603ull) {
+  for (unsigned char c : a) {
+    h ^= c;
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+
+std::string cache_dir() {
+  const char* e = std::getenv("CUSNA_RTC_CACHE");
+  std::string d;
+  if (e && *e) {
+    d = e;
+  } else if ((e = std::getenv("XDG_CACHE_HOME")) && *e) {
+    d = std::string(e) + "/cusna_amd";
+  } else if ((e = std::getenv("HOME")) && *e) {
+    d = std::string(e) + "/.cache/cusna_amd";
+  } else {
+    return std::string();
+  }
+  // create each component (mkdir -p); failures just disable the disk cache
+  for (size_t pos = 1; pos <= d.size(); ++pos)
+    if (pos == d.size() || d[pos] == '/') (void)mkdir(d.substr(0, pos).c_str(), 0755);
+  return access(d.c_str(), W_OK) == 0 ? d : std::string();
+}
+
+bool read_file(const std::string& path, std::vector<char>& out) {
+  FILE* f = std::fopen(path.c_str(), "rb");
+  if (!f) return false;
+  std::fseek(f, 0, SEEK_END);
+  const long sz = std::ftell(f);
+  std::fseek(f, 0, SEEK_SET);
+  out.resize(sz > 0 ? (size_t)sz : 0);
+  const bool ok = sz > 0 && std::fread(out.data(), 1, out.size(), f) == out.size();
+  std::fclose(f);
+  return ok;
+}
+
+void write_file(const std::string& path, const std::vector<char>& data) {
+  const std::string tmp = path + ".tmp" + std::to_string((long)getpid());
+  FILE* f = std::fopen(tmp.c_str(), "wb");
+  if (!f) return;
+  const bool ok = std::fwrite(data.data(), 1, data.size(), f) == data.size();
+  std::fclose(f);
+  if (ok) (void)std::rename(tmp.c_str(), path.c_str());
+  else (void)std::remove(tmp.c_str());
+}
+
+std::string list(const int* v, int k) {
+  std::string s;
+  for (int e = 0; e < k; ++e) s += (e ? "," : "") + std::to_string(v[e]);
+  return s;
+}
+
+bool compile(const std::vector<std::string>& opts, std::vector<char>& code) {
+  hiprtcProgram prog;
+  if (hiprtcCreateProgram(&prog, kRtcPeriodSource, "saom_period_rtc.hip", 0, nullptr,
+                          nullptr) != HIPRTC_SUCCESS)
+    return false;
+  std::vector<const char*> o;
+  for (const auto& x : opts) o.push_back(x.c_str());
+  bool ok = hiprtcCompileProgram(prog, (int)o.size(), o.data()) == HIPRTC_SUCCESS;
+  size_t cs = 0;
+  if (ok) ok = hiprtcGetCodeSize(prog, &cs) == HIPRTC_SUCCESS && cs > 0;
+  if (ok) {
+    code.resize(cs);
+    ok = hiprtcGetCode(prog, code.data()) == HIPRTC_SUCCESS;
+  }
+  (void)hiprtcDestroyProgram(&prog);
+  return ok;
+}
+
+}  // namespace
+
+void* rtc_period_kernel(const SimParams& P, LaunchShape s, bool replay, bool simple) {
+  const int R = s.wmax / s.nw;
+  std::vector<std::string> opts = {
+      "--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
+      "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+      "-DSAOM_WMAX=" + std::to_string(s.wmax), "-DSAOM_NW=" + std::to_string(s.nw),
+      std::string("-DSAOM_REPLAY=") + (replay ? "1" : "0"),
+      std::string("-DSAOM_SIMPLE=") + (simple ? "1" : "0"),
+      "-DSAOM_FIXED_K=" + std::to_string(P.k),
+      "-DSAOM_FIXED_TYPES=" + list(P.tid, P.k), "-DSAOM_FIXED_ETYPES=" + list(P.ety, P.k),
+      "-DSAOM_FIXED_DYN=" + list(P.dyn, P.k),
+      std::string("-DSAOM_CACHE_D=") + (P.k * R <= 32 ? "1" : "0")};
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::string key = std::to_string(dev);
+  for (const auto& x : opts) key += " " + x;
+  std::lock_guard<std::mutex> lk(g_rtc_mu);
+  RtcEntry& ent = g_rtc[key];
+  if (ent.fn || ent.failed) return (void*)ent.fn;
+  ent.failed = true;  // until loaded
+  std::vector<char> code;
+  std::string path;
+  const std::string dir = cache_dir();
+  if (!dir.empty()) {
+    char name[64];
+    std::snprintf(name, sizeof(name), "/period_%016llx.co",
+                  (unsigned long long)fnv1a(key.substr(key.find(' ')), fnv1a(kRtcPeriodSource)));
+    path = dir + name;
+  }
+  if (path.empty() || !read_file(path, code)) {
+    if (!compile(opts, code)) return nullptr;
+    if (!path.empty()) write_file(path, code);
+  }
+  if (hipModuleLoadData(&ent.mod, code.data()) != hipSuccess) return nullptr;
+  if (hipModuleGetFunction(&ent.fn, ent.mod, "saom_period_rtc") != hipSuccess) {
+    ent.fn = nullptr;
+    return nullptr;
+  }
+  ent.failed = false;
+  return (void*)ent.fn;
+}
+
+bool specialize_policy(long long chains_x_actors) {
+  const char* e = std::getenv("CUSNA_SPECIALIZE");
+  if (e && e[0] == '0') return false;
+  if (e && e[0] == '1') return true;
+  return chains_x_actors >= kSpecializeWork;
+}
+
+bool same_effects(const SimParams& a, const SimParams& b) {
+  if (a.k != b.k) return false;
+  for (int e = 0; e < a.k; ++e)
+    if (a.tid[e] != b.tid[e] || a.ety[e] != b.ety[e] || a.dyn[e] != b.dyn[e]) return false;
+  return true;
+}
+
+int launch_period_rtc(void* fn, const PeriodLaunch& L, int nw, void* stream) {
+  const SimParams* pd = L.Pd;
+  const uint32_t* items = L.items;
+  uint32_t k0 = L.key0, k1 = L.key1;
+  void* args[] = {&pd, &items, &k0, &k1};
+  return (int)hipModuleLaunchKernel((hipFunction_t)fn, (unsigned)L.grid, 1, 1, 64u * (unsigned)nw, 1, 1,
+                                    (unsigned)L.lds, (hipStream_t)stream, args, nullptr);
+}
+
+}  // namespace saom

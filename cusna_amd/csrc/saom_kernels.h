@@ -64,22 +64,41 @@ bool simple_effects(const int* type_ids, int k);
 // Launch the period kernel. P is the host copy (shape, LDS size); Pd the same
 // struct in device memory, read by the kernel. Returns 0 or a hipError_t.
 int launch_period(const SimParams& P, const SimParams* Pd, LaunchShape s, bool replay,
-                  uint32_t key0, uint32_t key1, void* stream);
+                  uint32_t key0, uint32_t key1, void* stream, bool specialize = false,
+                  bool* used = nullptr);
 
 // General form. One group: Pd, grid = chains, key0/key1 the Philox key.
 // Several groups: Pd[g] are group g's device parameters (its Philox key in
 // their key0/key1 or keys), items[b] = (g << 20) | chain names block b's
 // work; grid = number of items; lds = the largest of the groups' LDS sizes.
 // All groups of a launch share the word bound and the block shape.
+// specialize: use the runtime-specialised kernel for the effect list of Ph
+// (host copy of the parameters; every group of the launch must share it)
+// when it compiles and fits, else the generic kernel. *used (optional) tells
+// which ran.
 struct PeriodLaunch {
   const SimParams* Pd;
   const uint32_t* items;
   int grid;
   size_t lds;
   uint32_t key0, key1;
+  const SimParams* Ph = nullptr;
+  bool specialize = false;
+  bool* used = nullptr;
 };
 int launch_period_ex(const PeriodLaunch& L, LaunchShape s, bool replay, bool simple,
                      void* stream);
+
+// Runtime specialisation (rtc.hip): the compiled kernel for P's effect list
+// and this launch shape (null when unavailable), and its launch.
+void* rtc_period_kernel(const SimParams& P, LaunchShape s, bool replay, bool simple);
+int launch_period_rtc(void* fn, const PeriodLaunch& L, int nw, void* stream);
+// Launch policy: CUSNA_SPECIALIZE=0 never, =1 always, unset: when the launch
+// has at least this many chains x actors.
+constexpr long long kSpecializeWork = 1ll << 18;
+bool specialize_policy(long long chains_x_actors);
+// Same effect list (what the specialised kernel bakes in).
+bool same_effects(const SimParams& a, const SimParams& b);
 
 // Statistics of M packed networks (rows u64, n*W64 per network).
 struct StatParams {

@@ -37,6 +37,17 @@ bool simple_effects(const int* t, int k) {
 int launch_period_ex(const PeriodLaunch& L, LaunchShape s, bool replay, bool simple,
                      void* stream) {
   if (L.grid < 1) return 0;
+  if (L.used) *L.used = false;
+  // specialised kernels launch through the module API, which this build keeps
+  // within the default 64 KB of dynamic LDS
+  if (L.specialize && L.Ph && L.lds <= 64 * 1024) {
+    void* fn = rtc_period_kernel(*L.Ph, s, replay, simple);
+    if (fn) {
+      const int e = launch_period_rtc(fn, L, s.nw, stream);
+      if (e == 0 && L.used) *L.used = true;
+      return e;
+    }
+  }
   switch (s.wmax) {
     case 1: return launch_period_w1(L, s.nw, replay, simple, stream);
     case 2: return launch_period_w2(L, s.nw, replay, simple, stream);
@@ -51,7 +62,7 @@ int launch_period_ex(const PeriodLaunch& L, LaunchShape s, bool replay, bool sim
 }
 
 int launch_period(const SimParams& P, const SimParams* Pd, LaunchShape s, bool replay,
-                  uint32_t k0, uint32_t k1, void* stream) {
+                  uint32_t k0, uint32_t k1, void* stream, bool specialize, bool* used) {
   PeriodLaunch L;
   L.Pd = Pd;
   L.items = nullptr;
@@ -59,6 +70,9 @@ int launch_period(const SimParams& P, const SimParams* Pd, LaunchShape s, bool r
   L.lds = sim_lds_bytes(P, s.nw, s.wmax);
   L.key0 = k0;
   L.key1 = k1;
+  L.Ph = &P;
+  L.specialize = specialize;
+  L.used = used;
   return launch_period_ex(L, s, replay, simple_effects(P.tid, P.k), stream);
 }
 

@@ -26,13 +26,36 @@
 //      score_e += d_e(j*) - T_e / S.
 //   7. Toggle i->j* (and j*->i when symmetric), degrees, distance.
 // Behaviour ministeps follow libcusna_saom_cpu.cpp:202-227.
+//
+// Specialised builds: runtime compilation (rtc.hip) defines SAOM_FIXED_K and
+// the effect types / creation-endowment types / behaviour-dynamic flags of
+// one effect list, and the word bound, block shape and variant of one
+// launch. Effect loops then unroll over compile-time effect types (no
+// per-effect dispatch) and, when they fit, the change statistics of the
+// choice pass stay in registers for the score pass. Same arithmetic, same
+// order: results are identical to the generic kernel's.
+#ifndef SAOM_RTC
 #include <hip/hip_runtime.h>
 
 #include "saom_device.h"
 #include "saom_kernels.h"
+#endif
 
 namespace saom {
 namespace {
+
+#ifdef SAOM_FIXED_K
+constexpr int kFixedT[] = {SAOM_FIXED_TYPES};
+constexpr int kFixedE[] = {SAOM_FIXED_ETYPES};
+constexpr int kFixedD[] = {SAOM_FIXED_DYN};
+__device__ __forceinline__ int eff_type(const SimParams&, int e) { return kFixedT[e]; }
+__device__ __forceinline__ int eff_ety(const SimParams&, int e) { return kFixedE[e]; }
+__device__ __forceinline__ int eff_dyn(const SimParams&, int e) { return kFixedD[e]; }
+#else
+__device__ __forceinline__ int eff_type(const SimParams& P, int e) { return P.tid[e]; }
+__device__ __forceinline__ int eff_ety(const SimParams& P, int e) { return P.ety[e]; }
+__device__ __forceinline__ int eff_dyn(const SimParams& P, int e) { return P.dyn[e]; }
+#endif
 
 __device__ __forceinline__ int popc64(uint64_t x) { return __popcll(x); }
 
@@ -131,7 +154,7 @@ __device__ __forceinline__ void philox_alternatives(uint32_t c0, uint32_t c1, co
     d[r] = (uint32_t)j[r];
   }
 #pragma unroll 1
-  for (int round = 0; round < 10; ++round) {
+  for (int round = 0; round < 9; ++round) {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const uint64_t m0 = (uint64_t)0xD2511F53u * a[r];
@@ -146,8 +169,12 @@ __device__ __forceinline__ void philox_alternatives(uint32_t c0, uint32_t c1, co
     k0 += 0x9E3779B9u;
     k1 += 0xBB67AE85u;
   }
+  // tenth round: only output word 0 is used, which needs one of the two products
 #pragma unroll
-  for (int r = 0; r < R; ++r) u[r] = a[r];
+  for (int r = 0; r < R; ++r) {
+    const uint64_t m1 = (uint64_t)0xCD9E8D57u * c[r];
+    u[r] = (uint32_t)(m1 >> 32) ^ b[r] ^ k0;
+  }
 }
 
 // Per-alternative features of one lane (R alternatives). Alternative r of
@@ -217,9 +244,9 @@ __device__ __forceinline__ void eff_base(const SimParams& P, int e, const Alt<R>
   const int* z = S.z;
   const int* tpall = S.tpall;
   const int* soall = S.soall;
-  const int t = P.tid[e];
+  const int t = eff_type(P, e);
   const int n = P.n;
-  const bool dyn = P.dyn[e] != 0;
+  const bool dyn = eff_dyn(P, e) != 0;
   const double* vrow = P.vmat + (size_t)e * n;
   const double* w = P.wtab + (size_t)e * (n + 1);
   const double p1 = P.p1[e], p2 = P.p2[e];
@@ -440,7 +467,7 @@ __device__ __forceinline__ void eff_base(const SimParams& P, int e, const Alt<R>
       }
       break;
   }
-  const int et = P.ety[e];
+  const int et = eff_ety(P, e);
   if (et != 0) {
 #pragma unroll
     for (int r = 0; r < R; ++r)
@@ -460,7 +487,7 @@ __device__ __forceinline__ void eff_full(const SimParams& P, int e, const Alt<R>
   } else {
     // One eff_base call site: a plain effect runs the loop once; an
     // interaction runs it over its two or three components.
-    const bool inter = P.tid[e] == 32;
+    const bool inter = eff_type(P, e) == 32;
     const int c2 = P.c2[e];
     double prod[R];
     for (int q = 0; q < 3; ++q) {
@@ -544,13 +571,11 @@ __host__ __device__ inline LdsLayout lds_layout(int n, int k, int kb, int wmax, 
   return L;
 }
 
-// Register budget: two waves per SIMD for the two-wave shape, three for the
-// four-wave shape (LDS admits four chains of n = 500 per CU).
+// The period of one chain (the kernel body; see the file comment).
 template <int NW, int R, int WMAX, bool REPLAY, bool SIMPLE>
-__global__ void __launch_bounds__(64 * NW)
-    __attribute__((amdgpu_waves_per_eu(NW == 1 ? 1 : (NW == 2 ? 2 : 3))))
-    saom_period_kernel(const SimParams* __restrict__ Pp, const uint32_t* __restrict__ items,
-                       uint32_t key0_arg, uint32_t key1_arg) {
+__device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
+                                            const uint32_t* __restrict__ items,
+                                            uint32_t key0_arg, uint32_t key1_arg) {
   // One group: block b runs chain b of Pp[0]. Several groups in one launch:
   // item b names the group g (high 12 bits, parameters Pp[g]) and the chain
   // (low 20 bits).
@@ -853,6 +878,22 @@ __global__ void __launch_bounds__(64 * NW)
     double f[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) f[r] = 0.0;
+#ifdef SAOM_FIXED_K
+    constexpr int KC = SAOM_CACHE_D ? SAOM_FIXED_K : 1;
+    double dc[KC][R];
+#pragma unroll
+    for (int e = 0; e < SAOM_FIXED_K; ++e) {
+      double d[R];
+      eff_full<R, WMAX, STREAM, SIMPLE>(P, e, A, U, S, d);
+      const double te = th[e];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const double prod = te * d[r];
+        f[r] = f[r] + prod;
+        if (SAOM_CACHE_D) dc[SAOM_CACHE_D ? e : 0][r] = d[r];
+      }
+    }
+#else
     for (int e = 0; e < k; ++e) {
       double d[R];
       eff_full<R, WMAX, STREAM, SIMPLE>(P, e, A, U, S, d);
@@ -863,6 +904,7 @@ __global__ void __launch_bounds__(64 * NW)
         f[r] = f[r] + prod;
       }
     }
+#endif
     // 5. Gumbel-max choice (:265-268) and lmax
     double gb = -1e30, lm = -1e30;
     int jb = i;
@@ -937,9 +979,21 @@ __global__ void __launch_bounds__(64 * NW)
     }
     Sw = wave_sum(Sw);
     if (lane == 0) tsl[wv * (k + 1) + k] = Sw;
+#ifdef SAOM_FIXED_K
+#pragma unroll
+    for (int e = 0; e < SAOM_FIXED_K; ++e) {
+      double d[R];
+      if (SAOM_CACHE_D) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) d[r] = dc[SAOM_CACHE_D ? e : 0][r];
+      } else {
+        eff_full<R, WMAX, STREAM, SIMPLE>(P, e, A, U, S, d);
+      }
+#else
     for (int e = 0; e < k; ++e) {
       double d[R];
       eff_full<R, WMAX, STREAM, SIMPLE>(P, e, A, U, S, d);
+#endif
       double T = 0.0;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
@@ -1038,6 +1092,17 @@ __global__ void __launch_bounds__(64 * NW)
   }
 }
 
+#ifndef SAOM_RTC
+// Register budget: two waves per SIMD for the two-wave shape, three for the
+// four-wave shape (LDS admits four chains of n = 500 per CU).
+template <int NW, int R, int WMAX, bool REPLAY, bool SIMPLE>
+__global__ void __launch_bounds__(64 * NW)
+    __attribute__((amdgpu_waves_per_eu(NW == 1 ? 1 : (NW == 2 ? 2 : 3))))
+    saom_period_kernel(const SimParams* __restrict__ Pp, const uint32_t* __restrict__ items,
+                       uint32_t key0_arg, uint32_t key1_arg) {
+  period_body<NW, R, WMAX, REPLAY, SIMPLE>(Pp, items, key0_arg, key1_arg);
+}
+
 template <int NW, int WMAX, bool REPLAY, bool SIMPLE>
 hipError_t launch_shape(const PeriodLaunch& L, hipStream_t st) {
   constexpr int R = WMAX / NW;
@@ -1100,3 +1165,16 @@ size_t SAOM_CAT(sim_lds_bytes_w, SAOM_WMAX)(const SimParams& P, int nw, int wmax
 }
 
 }  // namespace saom
+#else   // SAOM_RTC: one specialised kernel, shape and effect list from macros
+}  // namespace
+
+extern "C" __global__ void __launch_bounds__(64 * SAOM_NW)
+    __attribute__((amdgpu_waves_per_eu(SAOM_NW == 1 ? 1 : (SAOM_NW == 2 ? 2 : 3))))
+    saom_period_rtc(const SimParams* __restrict__ Pp, const uint32_t* __restrict__ items,
+                    uint32_t key0_arg, uint32_t key1_arg) {
+  period_body<SAOM_NW, SAOM_WMAX / SAOM_NW, SAOM_WMAX, SAOM_REPLAY != 0, SAOM_SIMPLE != 0>(
+      Pp, items, key0_arg, key1_arg);
+}
+
+}  // namespace saom
+#endif

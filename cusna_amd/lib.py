@@ -29,6 +29,7 @@ EXPORTS = (
     "cusna_saom_simulate_groups", "cusna_mlp_create", "cusna_mlp_param_count",
     "cusna_mlp_layer_shape", "cusna_mlp_get_params", "cusna_mlp_set_params",
     "cusna_mlp_forward_backward", "cusna_mlp_train", "cusna_mlp_predict", "cusna_mlp_destroy",
+    "cusna_period_plan_specialize", "cusna_period_plan_specialized",
 )
 
 _lib = None
@@ -87,6 +88,10 @@ def load(path: str | None = None) -> C.CDLL:
     L.cusna_period_plan_autotune.argtypes = [_vp, C.c_int32, C.POINTER(C.c_int32)]
     L.cusna_saom_simulate_groups.restype = C.c_int
     L.cusna_saom_simulate_groups.argtypes = [_vp, C.c_int32, _vp]
+    L.cusna_period_plan_specialize.restype = C.c_int
+    L.cusna_period_plan_specialize.argtypes = [_vp, C.c_int32]
+    L.cusna_period_plan_specialized.restype = C.c_int32
+    L.cusna_period_plan_specialized.argtypes = [_vp]
     L.cusna_mlp_create.restype = C.c_int
     L.cusna_mlp_create.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_uint64,
                                    C.POINTER(_vp)]
@@ -303,7 +308,7 @@ def mom_estimate(problems: list, targets, theta0, mode: int, n1: int, nD: int, b
                  use_graph: bool = True, max_repeats: int = 0, conv_tol: float = 0.1,
                  surrogate: bool = False, surrogate_pairs_log2: int = 0,
                  surrogate_steps: int = 0, surrogate_rounds: int = 0,
-                 surrogate_corrections: int = -1) -> dict:
+                 surrogate_corrections: int = -1, specialize: int = 0) -> dict:
     """Method-of-moments estimate on the device (cusna_mom_estimate).
 
     ``problems``: one SaomProblem per period (inputs only); ``xnets``: the
@@ -338,6 +343,7 @@ def mom_estimate(problems: list, targets, theta0, mode: int, n1: int, nD: int, b
     o.surrogate, o.surrogate_pairs_log2 = int(surrogate), int(surrogate_pairs_log2)
     o.surrogate_steps, o.surrogate_rounds = int(surrogate_steps), int(surrogate_rounds)
     o.surrogate_corrections = int(surrogate_corrections)
+    o.specialize = int(specialize)
     if mode == 2:
         pr0 = problems[0]
         o.beh_zbar, o.beh_brange, o.beh_bsimmean = (float(pr0.zbar), float(pr0.brange),
@@ -420,6 +426,16 @@ class PeriodPlan:
         res = SaomResult(self.prob.n, self.prob.k, self.prob.M, self.prob.kb, want_end_nets)
         _check("cusna_period_plan_fetch", self.L.cusna_period_plan_fetch(self.h, C.byref(res.c)))
         return res.as_dict()
+
+    def specialize(self, mode: int = 1) -> None:
+        """Runtime-specialised period kernel: 1 on, 0 off, -1 by launch size."""
+        _check("cusna_period_plan_specialize",
+               self.L.cusna_period_plan_specialize(self.h, int(mode)))
+
+    @property
+    def specialized(self) -> bool:
+        """Whether the last run launched the specialised kernel."""
+        return bool(self.L.cusna_period_plan_specialized(self.h))
 
     def autotune(self, probe_chains: int = 0) -> int:
         """Pick the fastest block shape (cusna_period_plan_autotune); returns

@@ -126,7 +126,7 @@ class MomEstimator:
     def run(self, theta0=None, use_graph: bool = True, max_repeats: int = 0,
             conv_tol: float = 0.1, surrogate: bool = False, surrogate_pairs_log2: int = 0,
             surrogate_steps: int = 0, surrogate_rounds: int = 0,
-            surrogate_corrections: int = -1) -> MomFit:
+            surrogate_corrections: int = -1, specialize: int = 0) -> MomFit:
         """Estimate. ``surrogate`` replaces the Robbins-Monro sub-phases by the
         perceptron surrogate (see cusna_mom_options in include/libcusna.h)."""
         c = self.control
@@ -137,7 +137,8 @@ class MomEstimator:
                              use_graph=use_graph, max_repeats=max_repeats, conv_tol=conv_tol,
                              surrogate=surrogate, surrogate_pairs_log2=surrogate_pairs_log2,
                              surrogate_steps=surrogate_steps, surrogate_rounds=surrogate_rounds,
-                             surrogate_corrections=surrogate_corrections)
+                             surrogate_corrections=surrogate_corrections,
+                             specialize=specialize)
         return MomFit(theta=r["theta"], se=r["se"], tconv=r["tconv"],
                       tconv_max=r["tconv_max"], max_abs_tconv=r["max_abs_tconv"],
                       param_names=self.param_names, targets=self.targets,

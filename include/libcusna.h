@@ -285,6 +285,21 @@ CUSNA_API void cusna_period_plan_destroy(cusna_period_plan *plan);
  * statistics and end networks do not depend on the shape; scores can differ
  * in the last float64 bits (their sums run in a shape-dependent order).
  */
+/*
+ * Runtime specialisation of the period kernel (no counterpart in the
+ * reference). The library embeds the kernel source and can compile, with
+ * hipRTC, a kernel for one effect list and launch shape: effect dispatch
+ * resolved at compile time, change statistics kept in registers between the
+ * choice and score passes. Results are identical to the generic kernel's.
+ * Code objects are cached in memory and on disk (CUSNA_RTC_CACHE, else
+ * $XDG_CACHE_HOME/cusna_amd, else $HOME/.cache/cusna_amd). Plans use it when
+ * M x n >= 2^18 unless CUSNA_SPECIALIZE=0 (never) or =1 (always) is set.
+ *   specialize    mode 1 on, 0 off, -1 by that policy (replay plans: off)
+ *   specialized   1 when the plan's last run launched the specialised kernel
+ */
+CUSNA_API cusna_status cusna_period_plan_specialize(cusna_period_plan *plan, int32_t mode);
+CUSNA_API int32_t cusna_period_plan_specialized(const cusna_period_plan *plan);
+
 CUSNA_API cusna_status cusna_period_plan_autotune(cusna_period_plan *plan, int32_t probe_chains,
                                                   int32_t *best_threads);
 
@@ -350,7 +365,9 @@ typedef struct cusna_mom_options {
   int32_t surrogate_steps;
   int32_t surrogate_rounds;
   int32_t surrogate_corrections;
-  int32_t reserved2;
+  /* period kernel specialised at run time to the effect list (hipRTC): 1 on,
+   * -1 off, 0 when a simulation round is large (chains x actors >= 2^18) */
+  int32_t specialize;
 } cusna_mom_options;
 
 typedef struct cusna_mom_result {

@@ -1,0 +1,51 @@
+"""GPU: the runtime-specialised period kernel (hipRTC) against the generic
+kernel and the reference engine's outputs: identical trajectories, end
+networks, statistics and scores (same arithmetic in the same order)."""
+import numpy as np
+import pytest
+
+from conftest import load_golden
+from test_gpu_parity import assert_period_equal
+
+pytestmark = pytest.mark.gpu
+
+
+def _both(gpu_lib, prob, seed=7):
+    plan = gpu_lib.PeriodPlan(prob, want_end_nets=True)
+    try:
+        plan.specialize(0)
+        plan.run(seed)
+        a = plan.fetch(want_end_nets=True)
+        assert not plan.specialized
+        plan.specialize(1)
+        plan.run(seed)
+        b = plan.fetch(want_end_nets=True)
+        assert plan.specialized, "specialised kernel did not run"
+    finally:
+        plan.close()
+    for key in a:
+        np.testing.assert_array_equal(np.asarray(b[key]), np.asarray(a[key]), err_msg=key)
+    return b
+
+
+REF_IDX = list(range(0, 38, 3)) + [33, 34, 35, 36, 37]
+
+
+@pytest.mark.parametrize("idx", sorted(set(REF_IDX)))
+def test_specialised_matches_generic_and_reference(gpu_lib, idx):
+    from ref_cases_loader import spec_problem
+
+    case = load_golden("ref_cases.json.gz")["cases"][idx]
+    prob = spec_problem(case["spec"])
+    print(f"case {case['spec']['name']} n={prob.n} k={prob.k}", flush=True)
+    got = _both(gpu_lib, prob, seed=case["spec"]["seed"])
+    assert_period_equal(got, case["out"], label=case["spec"]["name"])
+
+
+def test_specialised_c2_and_streamed(gpu_lib):
+    from cusna_amd.workloads import c2_problem
+    from test_gpu_stream import CASES, _problem
+
+    _both(gpu_lib, c2_problem(M=64, n=500), seed=11)
+    n, names, theta, lam, M = CASES["n1100_simple"]
+    _both(gpu_lib, _problem(n, names, theta, lam, M, 31), seed=31)
