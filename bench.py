@@ -47,6 +47,19 @@ def algorithmic_ops_per_alternative(n: int, k: int) -> float:
     return 100 + 8 * w64 + 4 * k + 3 + 4
 
 
+def measured_traffic(n: int, chains: int):
+    """HBM bytes per period launch of the config-2 workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a plain
+    copy); None for other shapes."""
+    path = os.path.join(ROOT, "profiles", "period_kernel_rtc_c2_n500_traffic.md")
+    if n != 500 or chains != 4096 or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
+
+
 def cpu_baseline(prob_factory, threads: int, chains: int) -> dict:
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -146,7 +159,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has one GPU, and the "
-              "path shards over chains without cross-GPU traffic (see DESIGN.md).",
+              "path shards over chains without cross-GPU traffic.",
               file=sys.stderr)
         return 2
 
@@ -203,7 +216,8 @@ def main() -> int:
                    "block_threads": plans[0].block_threads},
         "roofline": {"bound": "valu", "achieved": round(achieved / 1e12, 3),
                      "peak": round(VALU_PEAK_OPS / 1e12, 3), "unit": "Tops/s",
-                     "frac": round(achieved / VALU_PEAK_OPS, 4), "traffic": None,
+                     "frac": round(achieved / VALU_PEAK_OPS, 4),
+                     "traffic": measured_traffic(n, M) if plans[0].specialized else None,
                      "kernel": "saom_period_rtc" if plans[0].specialized
                                else "saom_period_kernel", "kernel_ms": round(kms, 3),
                      "alternatives_per_launch": float(np.mean(units)),
