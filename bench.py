@@ -15,7 +15,8 @@ Also reported: the period kernel against its VALU roof (HIP events on the
 plan's stream), the reference CPU engine (compiled from the reference's own
 source under oracle/_ref) timed on this host's cores on a bounded sample, and
 ("mom") the wall time of a method-of-moments fit to max|t| <= 0.1 on the
-device next to one pass of the same schedule through the reference engine.
+device next to one pass of the same schedule through the reference engine,
+and ("groups") the config-3 batch of 256 networks of 30..1200 actors.
 """
 from __future__ import annotations
 
@@ -83,6 +84,34 @@ def cpu_baseline(prob_factory, threads: int, chains: int) -> dict:
             "sample": (f"{chains} chains of period 1 of the same workload (n=500, rate 8, 7 "
                        f"effects), OpenMP static schedule over chains, {threads} threads, "
                        f"{wall:.1f} s wall, mean ministeps {float(r['steps'].mean()):.0f}")}
+
+
+def groups_measure(lib) -> dict:
+    """Config-3 batch: 256 independent networks (n uniform on 30..1200, 500
+    chains each) through one cusna_saom_simulate_groups call per run: one
+    period launch per launch shape, items longest-first. Timed: the whole
+    call (plan set-up and uploads, launches, statistics, copies back), after
+    a 4-chain warm-up call that loads the code objects."""
+    import numpy as np
+
+    from cusna_amd.workloads import c3_problems
+
+    small = c3_problems(chains=4)
+    lib.simulate_groups(small)
+    probs = c3_problems()
+    t0 = time.perf_counter()
+    outs = lib.simulate_groups(probs)
+    wall = time.perf_counter() - t0
+    cp = sum(p.M for p in probs)
+    alts = sum(float(o["steps"].astype(np.int64).sum()) * p.n for p, o in zip(probs, outs))
+    ns = [p.n for p in probs]
+    return {"workload": "config-3 groups: 256 networks, n uniform 30..1200 (%d..%d here), "
+                        "500 chains each, rate 4, density/recip/transTrip/inPop/egoX/altX"
+                        % (min(ns), max(ns)),
+            "chain_periods": cp, "wall_s": round(wall, 3),
+            "chain_periods_per_s": round(cp / wall, 1),
+            "alternatives_per_s": float("%.4g" % (alts / wall)),
+            "streamed_groups": sum(1 for v in ns if v > 1024)}
 
 
 def mom_measure(lib, cpu: bool, threads: int) -> dict:
@@ -156,6 +185,7 @@ def main() -> int:
     ap.add_argument("--cpu-chains", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mom", action="store_true")
+    ap.add_argument("--no-groups", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has one GPU, and the "
@@ -230,6 +260,8 @@ def main() -> int:
             lambda m: c2_problem(M=m, n=n, x0=w1, period_seed=11), threads, chains)
     for pl in plans:
         pl.close()
+    if not args.no_groups:
+        line["groups"] = groups_measure(lib)
     if not args.no_mom:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
         line["mom"] = mom_measure(lib, not args.no_cpu, threads)

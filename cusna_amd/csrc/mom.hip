@@ -507,6 +507,7 @@ struct cusna_mom_engine {
   uint64_t seed = 0;
   long long n_sims = 0;
   int specialize = -1;  // 1 on, 0 off, -1 by the launch-size policy
+  long long work = 0;   // chains x periods x actors of the whole schedule
   double beh_zbar = 0.0, beh_brange = 1.0, beh_bsim = 0.0;
   ~cusna_mom_engine() {
     if (s) (void)hipStreamDestroy(s);
@@ -596,7 +597,8 @@ cusna_status build_simset(cusna_mom_engine* E, SimSet& X, const cusna_saom_probl
   for (int pp = 1; pp < P; ++pp) uniform = uniform && saom::same_effects(grp[pp], grp[0]);
   X.specialize = uniform && (E->specialize > 0 ||
                              (E->specialize < 0 &&
-                              saom::specialize_policy((long long)P * M * E->n)));
+                              saom::specialize_policy(std::max((long long)P * M * E->n,
+                                                               E->work / 16))));
   return CUSNA_OK;
 }
 
@@ -890,6 +892,12 @@ CUSNA_API cusna_status cusna_mom_estimate(const cusna_saom_problem* const* perio
   E->n_rates = mode == 1 ? 0 : (mode == 2 ? 2 * P : P);
   E->seed = opt->seed;
   E->specialize = opt->specialize > 0 ? 1 : (opt->specialize < 0 ? 0 : -1);
+  {
+    long long chains = (long long)opt->n1 + opt->n3;
+    for (int sub = 0; sub < opt->nsub; ++sub)
+      chains += opt->nD + (long long)opt->n2[sub] * opt->batch2;
+    E->work = chains * P * n;
+  }
   E->beh_zbar = opt->beh_zbar;
   E->beh_brange = opt->beh_brange;
   E->beh_bsim = opt->beh_bsimmean;

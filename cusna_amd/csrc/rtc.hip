@@ -170,6 +170,17 @@ bool same_effects(const SimParams& a, const SimParams& b) {
 }
 
 int launch_period_rtc(void* fn, const PeriodLaunch& L, int nw, void* stream) {
+  // raise the function's dynamic LDS limit to the CU's 160 KB once
+  static std::mutex mu;
+  static std::map<void*, bool> raised;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    if (!raised[fn]) {
+      raised[fn] = true;
+      (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+      (void)hipGetLastError();
+    }
+  }
   const SimParams* pd = L.Pd;
   const uint32_t* items = L.items;
   uint32_t k0 = L.key0, k1 = L.key1;

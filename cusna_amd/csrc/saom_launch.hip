@@ -38,14 +38,17 @@ int launch_period_ex(const PeriodLaunch& L, LaunchShape s, bool replay, bool sim
                      void* stream) {
   if (L.grid < 1) return 0;
   if (L.used) *L.used = false;
-  // specialised kernels launch through the module API, which this build keeps
-  // within the default 64 KB of dynamic LDS
-  if (L.specialize && L.Ph && L.lds <= 64 * 1024) {
+  // the specialised kernel (module API); a launch the runtime refuses (for
+  // instance for its dynamic LDS size) falls back to the generic kernel
+  if (L.specialize && L.Ph) {
     void* fn = rtc_period_kernel(*L.Ph, s, replay, simple);
     if (fn) {
       const int e = launch_period_rtc(fn, L, s.nw, stream);
-      if (e == 0 && L.used) *L.used = true;
-      return e;
+      if (e == 0) {
+        if (L.used) *L.used = true;
+        return 0;
+      }
+      (void)hipGetLastError();
     }
   }
   switch (s.wmax) {

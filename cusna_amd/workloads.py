@@ -9,6 +9,7 @@ NULL masks, float32 theta/lambda):
   c2_problem       500 actors, directed, rate 8 per period, mean outdegree 5;
                    density -2.8, recip 1.8, transTrip 0.3, cycle3 -0.1,
                    egoX/altX/simX 0.1/0.1/0.6 on a covariate uniform on 1..5.
+  c3_problems      256 networks of 30..1200 actors, 500 chains each (groups).
   c4_problem       300 actors with an ordinal behaviour 1..5 = Binomial(4, 0.4)+1;
                    network density/recip/transTrip/simX(behaviour), behaviour
                    rate 4, linear -0.2, quad -0.3, avAlt 0.8.
@@ -124,6 +125,23 @@ def c2_waves(simulate, n: int = 500, seed: int = 2) -> list:
     prob = c2_problem(M=1, n=n, seed=seed, x0=w1, period_seed=seed + 7)
     w2 = simulate(prob)[0]
     return [w1, np.asarray(w2, np.uint8)]
+
+
+def c3_problems(groups: int = 256, chains: int = 500, seed: int = 3) -> list:
+    """Config 3: `groups` independent networks, n uniform on 30..1200, mean
+    outdegree 4, density/recip/transTrip/inPop/egoX/altX on a Bernoulli(0.5)
+    covariate, rate 4, `chains` chains each (one period)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for g in range(groups):
+        n = int(rng.integers(30, 1201))
+        x0 = bernoulli_network(n, 4.0, rng)
+        cov = (rng.random(n) < 0.5).astype(float)
+        eff = [Effect("density"), Effect("recip"), Effect("transTrip"), Effect("inPop"),
+               Effect("egoX", cov), Effect("altX", cov)]
+        out.append(effect_problem(x0, eff, [-2.8, 1.8, 0.3, 0.02, 0.1, 0.1], 4.0, chains,
+                                  seed * 100000 + g))
+    return out
 
 
 def c4_problem(M: int = 2048, n: int = 300, seed: int = 4, period_seed: int = 13) -> SaomProblem:

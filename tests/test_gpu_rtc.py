@@ -49,3 +49,26 @@ def test_specialised_c2_and_streamed(gpu_lib):
     _both(gpu_lib, c2_problem(M=64, n=500), seed=11)
     n, names, theta, lam, M = CASES["n1100_simple"]
     _both(gpu_lib, _problem(n, names, theta, lam, M, 31), seed=31)
+
+
+def test_specialised_large_chain_state(gpu_lib):
+    """n = 1000 (16 words per row, 136 KB of chain state in LDS): the
+    specialised kernel, or the generic one where the runtime refuses the
+    module launch, gives the generic kernel's results."""
+    from test_gpu_stream import _problem
+
+    pr = _problem(1000, ["density", "recip", "transTrip", "egoX"], [-3.0, 1.8, 0.3, 0.1],
+                  1.0, 6, 17)
+    plan = gpu_lib.PeriodPlan(pr)
+    try:
+        plan.specialize(0)
+        plan.run(5)
+        a = plan.fetch()
+        plan.specialize(1)
+        plan.run(5)
+        b = plan.fetch()
+        print("specialised kernel used:", plan.specialized, flush=True)
+    finally:
+        plan.close()
+    for key in a:
+        np.testing.assert_array_equal(np.asarray(b[key]), np.asarray(a[key]), err_msg=key)
