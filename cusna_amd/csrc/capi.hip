@@ -769,9 +769,94 @@ cusna_status net_stats_packed(const uint64_t* rows64_host, int n, int M, int k,
   return CUSNA_OK;
 }
 
+// GoF distributions of M packed networks already in device memory (rows)
+// or uploaded from host bit rows; masks optional (device).
+cusna_status gof_device(const uint64_t* rows_d, int n, int M, const uint64_t* keep_d,
+                        const uint64_t* force_d, int kind, const int32_t* levls, int nlev,
+                        double* out, hipStream_t s) {
+  if (kind < 0 || kind > 2) return CUSNA_ERR_UNSUPPORTED;
+  const int W64 = (n + 63) / 64;
+  int kmax = 0;
+  if (kind == 2) {
+    for (int L = 0; L < nlev; ++L) {
+      if (levls[L] < 0) {
+        set_error("cusna_gof_distribution: negative geodesic level");
+        return CUSNA_ERR_SHAPE;
+      }
+      kmax = std::max(kmax, levls[L]);
+    }
+    kmax = std::min(kmax, n);  // no shortest path is longer than n - 1
+    if (saom::gof_geodesic_lds(n, W64, kmax) == 0) {
+      set_error("geodesic distribution above 1024 actors is not supported on the device");
+      return CUSNA_ERR_UNSUPPORTED;
+    }
+  }
+  const int cols = nlev + (kind == 2 ? 1 : 0);
+  DeviceBlock D;
+  const size_t o_lv = D.add((size_t)nlev * 4), o_out = D.add((size_t)M * cols * 8);
+  HIP_TRY(D.alloc());
+  HIP_TRY(hipMemcpyAsync(D.at<char>(o_lv), levls, (size_t)nlev * 4, hipMemcpyHostToDevice, s));
+  saom::GofParams G;
+  G.n = n;
+  G.M = M;
+  G.W64 = W64;
+  G.kind = kind;
+  G.nlev = nlev;
+  G.kmax = kmax;
+  G.rows = rows_d;
+  G.keep = keep_d;
+  G.force = force_d;
+  G.levls = D.at<int32_t>(o_lv);
+  G.out = D.at<double>(o_out);
+  const int le = saom::launch_gof(G, s);
+  if (le != 0) {
+    set_error(std::string("gof kernel launch: ") + hipGetErrorString((hipError_t)le));
+    return CUSNA_ERR_CUDA;
+  }
+  HIP_TRY(hipMemcpyAsync(out, D.at<char>(o_out), (size_t)M * cols * 8, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return CUSNA_OK;
+}
+
 }  // namespace cusna_impl
 
 extern "C" {
+
+CUSNA_API cusna_status cusna_gof_distribution(const uint8_t* a_batch, int32_t n, int32_t M,
+                                              int32_t kind, const int32_t* levls, int32_t nlev,
+                                              double* out) {
+  g_last_error.clear();
+  // argument checks and codes as libcusna_saom_cpu.cpp:719
+  if (!a_batch || !levls || !out || n < 1 || M < 1 || nlev < 1) return CUSNA_ERR_NULL_ARG;
+  if (kind < 0 || kind > 2) return CUSNA_ERR_UNSUPPORTED;
+  if (n > CUSNA_MAX_N) return CUSNA_ERR_SHAPE;
+  if (device_count() < 1) return CUSNA_ERR_NO_CUDA;
+  const int W64 = (n + 63) / 64;
+  std::vector<uint64_t> rows((size_t)M * n * W64, 0ull);
+  for (int c = 0; c < M; ++c) {
+    const std::vector<uint64_t> r = dense_rows64(a_batch + (size_t)c * n * n, n, W64);
+    std::copy(r.begin(), r.end(), rows.begin() + (size_t)c * n * W64);
+  }
+  hipStream_t s = thread_stream();
+  DeviceBlock D;
+  const size_t o_rows = D.add(rows.size() * 8);
+  HIP_TRY(D.alloc());
+  HIP_TRY(hipMemcpyAsync(D.at<char>(o_rows), rows.data(), rows.size() * 8, hipMemcpyHostToDevice,
+                         s));
+  return gof_device(D.at<uint64_t>(o_rows), n, M, nullptr, nullptr, kind, levls, nlev, out, s);
+}
+
+CUSNA_API cusna_status cusna_period_plan_gof(cusna_period_plan* plan, int32_t masked, int32_t kind,
+                                             const int32_t* levls, int32_t nlev, double* out) {
+  g_last_error.clear();
+  if (!plan || !levls || !out) return CUSNA_ERR_NULL_ARG;
+  if (nlev < 1) return CUSNA_ERR_NULL_ARG;
+  HIP_TRY(hipStreamSynchronize(plan->s));
+  return gof_device(plan->P.end_rows, plan->n, plan->M,
+                    masked ? plan->D.at<uint64_t>(plan->o_keep) : nullptr,
+                    masked ? plan->D.at<uint64_t>(plan->o_force) : nullptr, kind, levls, nlev,
+                    out, plan->s);
+}
 
 CUSNA_API int32_t cusna_abi_version(void) { return CUSNA_ABI_VERSION; }
 

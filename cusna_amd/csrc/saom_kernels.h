@@ -134,6 +134,23 @@ struct BehStatParams {
 int launch_beh_stats(const BehStatParams& P, void* stream);
 size_t beh_stats_scratch_doubles(int n);
 
+// sienaGOF auxiliary distributions of M packed networks
+// (libcusna_saom_cpu.cpp:716-765): kind 0 indegree / 1 outdegree counts of
+// actors with degree <= levls[L]; kind 2 geodesic counts of ordered pairs within
+// distance levls[L] plus the unreachable pairs. Networks are (rows & keep) |
+// force when the masks are given.
+struct GofParams {
+  int n, M, W64, kind, nlev;
+  int kmax;                      // geodesic: largest level, capped at n
+  const uint64_t* rows;          // M*n*W64
+  const uint64_t *keep, *force;  // n*W64 or null
+  const int32_t* levls;          // nlev (device)
+  double* out;                   // M*(nlev + (kind == 2))
+};
+int launch_gof(const GofParams& P, void* stream);
+// LDS bytes of the geodesic kernel (0: the network is too large for it).
+size_t gof_geodesic_lds(int n, int W64, int kmax);
+
 // Expand packed end rows (M*n*W64) to dense uint8 (M*n*n).
 int launch_expand_dense(const uint64_t* rows, int n, int W64, int M, uint8_t* dense,
                         void* stream);

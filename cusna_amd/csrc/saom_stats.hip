@@ -558,6 +558,166 @@ int launch_beh_stats(const BehStatParams& P, void* stream) {
   return (int)hipGetLastError();
 }
 
+// ------------------------------------------------------------------ GoF
+
+__device__ __forceinline__ uint64_t gof_word(const GofParams& P, const uint64_t* src, int i,
+                                             int w) {
+  uint64_t v = src[(size_t)i * P.W64 + w];
+  if (P.keep) v &= P.keep[(size_t)i * P.W64 + w];
+  if (P.force) v |= P.force[(size_t)i * P.W64 + w];
+  // no bits beyond actor n - 1 (every column index below stays in range)
+  if (w == P.W64 - 1 && (P.n & 63)) v &= (1ull << (P.n & 63)) - 1ull;
+  return v;
+}
+
+// Degree distributions: one workgroup per network, degrees in LDS.
+__global__ void __launch_bounds__(kStatThreads) gof_degree_kernel(const GofParams P) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  int* deg = (int*)smem;
+  const int n = P.n, net = blockIdx.x;
+  const uint64_t* src = P.rows + (size_t)net * n * P.W64;
+  for (int v = threadIdx.x; v < n; v += blockDim.x) deg[v] = 0;
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    int od = 0;
+    for (int w = 0; w < P.W64; ++w) {
+      uint64_t m = gof_word(P, src, i, w);
+      od += __popcll(m);
+      if (P.kind == 0)
+        while (m) {
+          const int j = w * 64 + __ffsll((long long)m) - 1;
+          m &= m - 1;
+          atomicAdd(&deg[j], 1);
+        }
+    }
+    if (P.kind == 1) deg[i] = od;
+  }
+  __syncthreads();
+  for (int L = 0; L < P.nlev; ++L) {
+    const int lev = P.levls[L];
+    int c = 0;
+    for (int v = threadIdx.x; v < n; v += blockDim.x) c += deg[v] <= lev ? 1 : 0;
+    for (int o = 32; o >= 1; o >>= 1) c += __shfl_xor(c, o, 64);
+    __shared__ int part[kStatThreads / 64];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int t = 0;
+      for (int q = 0; q < (int)(blockDim.x >> 6); ++q) t += part[q];
+      P.out[(size_t)net * P.nlev + L] = (double)t;
+    }
+    __syncthreads();
+  }
+}
+
+// Geodesic distribution: one workgroup per network, masked rows in LDS; each
+// thread runs a breadth-first expansion from its source actors with the
+// reached set and frontier as bit rows in registers (word bound WMAX), adding
+// the number of newly reached actors at each distance into LDS counters.
+// reached_k = (reached_{k-1} | OR of the rows of the frontier) without the
+// source, which equals the reference's boolean matrix powers with the diagonal
+// cleared after each step.
+template <int WMAX>
+__global__ void __launch_bounds__(kStatThreads) gof_geodesic_kernel(const GofParams P) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int n = P.n, W64 = P.W64, net = blockIdx.x, kmax = P.kmax;
+  uint64_t* rows = (uint64_t*)smem;
+  unsigned long long* delta = (unsigned long long*)(rows + (size_t)n * W64);
+  const uint64_t* src = P.rows + (size_t)net * n * W64;
+  for (int q = threadIdx.x; q < n * W64; q += blockDim.x)
+    rows[q] = gof_word(P, src, q / W64, q % W64);
+  for (int k = threadIdx.x; k <= kmax; k += blockDim.x) delta[k] = 0ull;
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    uint64_t R[WMAX], F[WMAX];
+    const int iw = i >> 6;
+    const uint64_t ib = 1ull << (i & 63);
+    int cnt = 0;
+#pragma unroll
+    for (int w = 0; w < WMAX; ++w) {
+      R[w] = w < W64 ? rows[(size_t)i * W64 + w] : 0ull;
+      F[w] = R[w];
+      cnt += __popcll(w == iw ? (R[w] & ~ib) : R[w]);
+    }
+    if (kmax >= 1 && cnt) atomicAdd(&delta[1], (unsigned long long)cnt);
+    for (int k = 2; k <= kmax; ++k) {
+      uint64_t N[WMAX];
+#pragma unroll
+      for (int w = 0; w < WMAX; ++w) N[w] = 0ull;
+#pragma unroll
+      for (int fw = 0; fw < WMAX; ++fw) {
+        uint64_t m = F[fw];
+        while (m) {
+          const int h = fw * 64 + __ffsll((long long)m) - 1;
+          m &= m - 1;
+#pragma unroll
+          for (int w = 0; w < WMAX; ++w)
+            if (w < W64) N[w] |= rows[(size_t)h * W64 + w];
+        }
+      }
+      int grow = 0, any = 0;
+#pragma unroll
+      for (int w = 0; w < WMAX; ++w) {
+        uint64_t rn = R[w] | N[w];
+        if (w == iw) rn &= ~ib;
+        const uint64_t old = w == iw ? (R[w] & ~ib) : R[w];
+        grow += __popcll(rn) - __popcll(old);
+        F[w] = rn & ~R[w];
+        any |= F[w] != 0ull;
+        R[w] = rn;
+      }
+      if (grow) atomicAdd(&delta[k], (unsigned long long)grow);
+      if (!any) break;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    // counts[k] = sum of deltas up to k; levels beyond kmax saturate
+    double* out = P.out + (size_t)net * (P.nlev + 1);
+    for (int k = 2; k <= kmax; ++k) delta[k] += delta[k - 1];
+    for (int L = 0; L < P.nlev; ++L) {
+      const int lev = P.levls[L] < kmax ? P.levls[L] : kmax;
+      out[L] = lev >= 1 ? (double)delta[lev] : 0.0;
+    }
+    const unsigned long long reach = kmax >= 1 ? delta[kmax] : 0ull;
+    out[P.nlev] = (double)((long long)n * (n - 1) - (long long)reach);
+  }
+}
+
+size_t gof_geodesic_lds(int n, int W64, int kmax) {
+  if (W64 > 16) return 0;
+  const size_t b = (size_t)n * W64 * 8 + (size_t)(kmax + 1) * 8;
+  return b <= 160 * 1024 ? b : 0;
+}
+
+int launch_gof(const GofParams& P, void* stream) {
+  if (P.M < 1) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  if (P.kind == 0 || P.kind == 1) {
+    hipLaunchKernelGGL(gof_degree_kernel, dim3(P.M), dim3(kStatThreads), (size_t)P.n * 4, st, P);
+    return (int)hipGetLastError();
+  }
+  const size_t lds = gof_geodesic_lds(P.n, P.W64, P.kmax);
+  if (lds == 0) return (int)hipErrorInvalidValue;
+  int wmax = 1;
+  while (wmax < P.W64) wmax <<= 1;
+  auto go = [&](const void* k, void (*kern)(const GofParams)) -> int {
+    const hipError_t attr =
+        hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    if (attr != hipSuccess) return (int)attr;
+    hipLaunchKernelGGL(kern, dim3(P.M), dim3(kStatThreads), lds, st, P);
+    return (int)hipGetLastError();
+  };
+  switch (wmax) {
+    case 1: return go((const void*)gof_geodesic_kernel<1>, gof_geodesic_kernel<1>);
+    case 2: return go((const void*)gof_geodesic_kernel<2>, gof_geodesic_kernel<2>);
+    case 4: return go((const void*)gof_geodesic_kernel<4>, gof_geodesic_kernel<4>);
+    case 8: return go((const void*)gof_geodesic_kernel<8>, gof_geodesic_kernel<8>);
+    case 16: return go((const void*)gof_geodesic_kernel<16>, gof_geodesic_kernel<16>);
+    default: return (int)hipErrorInvalidValue;
+  }
+}
+
 int launch_expand_dense(const uint64_t* rows, int n, int W64, int M, uint8_t* dense,
                         void* stream) {
   const long long total = (long long)M * n * n;

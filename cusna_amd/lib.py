@@ -29,7 +29,8 @@ EXPORTS = (
     "cusna_saom_simulate_groups", "cusna_mlp_create", "cusna_mlp_param_count",
     "cusna_mlp_layer_shape", "cusna_mlp_get_params", "cusna_mlp_set_params",
     "cusna_mlp_forward_backward", "cusna_mlp_train", "cusna_mlp_predict", "cusna_mlp_destroy",
-    "cusna_period_plan_specialize", "cusna_period_plan_specialized",
+    "cusna_period_plan_specialize", "cusna_period_plan_specialized", "cusna_gof_distribution",
+    "cusna_period_plan_gof",
 )
 
 _lib = None
@@ -88,6 +89,11 @@ def load(path: str | None = None) -> C.CDLL:
     L.cusna_period_plan_autotune.argtypes = [_vp, C.c_int32, C.POINTER(C.c_int32)]
     L.cusna_saom_simulate_groups.restype = C.c_int
     L.cusna_saom_simulate_groups.argtypes = [_vp, C.c_int32, _vp]
+    L.cusna_gof_distribution.restype = C.c_int
+    L.cusna_gof_distribution.argtypes = [_vp, C.c_int32, C.c_int32, C.c_int32, _vp, C.c_int32,
+                                         _vp]
+    L.cusna_period_plan_gof.restype = C.c_int
+    L.cusna_period_plan_gof.argtypes = [_vp, C.c_int32, C.c_int32, _vp, C.c_int32, _vp]
     L.cusna_period_plan_specialize.restype = C.c_int
     L.cusna_period_plan_specialize.argtypes = [_vp, C.c_int32]
     L.cusna_period_plan_specialized.restype = C.c_int32
@@ -290,6 +296,17 @@ class Mlp:
             pass
 
 
+def gof_distribution(a_batch, kind: int, levls) -> np.ndarray:
+    """sienaGOF auxiliary distributions of M dense networks (cusna_gof_distribution)."""
+    a = np.ascontiguousarray(np.asarray(a_batch) != 0, np.uint8)
+    M, n, _ = a.shape
+    lv = np.ascontiguousarray(levls, np.int32)
+    out = np.zeros((M, len(lv) + (1 if kind == 2 else 0)), np.float64)
+    _check("cusna_gof_distribution",
+           load().cusna_gof_distribution(_p(a), n, M, int(kind), _p(lv), len(lv), _p(out)))
+    return out
+
+
 def simulate_groups(problems: list, want_end_nets: bool = False) -> list:
     """Several period problems in one call (cusna_saom_simulate_groups)."""
     L = load()
@@ -426,6 +443,16 @@ class PeriodPlan:
         res = SaomResult(self.prob.n, self.prob.k, self.prob.M, self.prob.kb, want_end_nets)
         _check("cusna_period_plan_fetch", self.L.cusna_period_plan_fetch(self.h, C.byref(res.c)))
         return res.as_dict()
+
+    def gof(self, kind: int, levls, masked: bool = True) -> np.ndarray:
+        """GoF distributions of the end networks of the last run, computed in
+        device memory (cusna_period_plan_gof)."""
+        lv = np.ascontiguousarray(levls, np.int32)
+        out = np.zeros((self.prob.M, len(lv) + (1 if kind == 2 else 0)), np.float64)
+        _check("cusna_period_plan_gof",
+               self.L.cusna_period_plan_gof(self.h, int(masked), int(kind), _p(lv), len(lv),
+                                            _p(out)))
+        return out
 
     def specialize(self, mode: int = 1) -> None:
         """Runtime-specialised period kernel: 1 on, 0 off, -1 by launch size."""

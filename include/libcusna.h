@@ -192,6 +192,16 @@ CUSNA_API cusna_status cusna_behavior_stats(const int32_t *z_batch, int32_t n, i
 
 /* Introspection (reference libcusna.h:527-530). cusna_has_cuda keeps its ABI
  * name and reports whether a HIP device is usable. */
+/* sienaGOF auxiliary distributions over a batch of end networks (reference
+ * libcusna.h:334-343, libcusna_saom_cpu.cpp:716-765). a_batch is M*n*n dense;
+ * kind 0 indegree / 1 outdegree: out[c][L] = actors with degree <= levls[L]
+ * (M*nlev); kind 2 geodesic: ordered pairs i != j at distance <= levls[L],
+ * then the unreachable pairs (M*(nlev+1)); geodesic levels must be >= 0 and
+ * n <= 1024 on the device. Status codes as the reference's. */
+CUSNA_API cusna_status cusna_gof_distribution(const uint8_t *a_batch, int32_t n, int32_t M,
+                                              int32_t kind, const int32_t *levls, int32_t nlev,
+                                              double *out);
+
 CUSNA_API int32_t cusna_abi_version(void);
 CUSNA_API const char *cusna_version_string(void);
 CUSNA_API int32_t cusna_has_cuda(void);
@@ -297,6 +307,13 @@ CUSNA_API void cusna_period_plan_destroy(cusna_period_plan *plan);
  *   specialize    mode 1 on, 0 off, -1 by that policy (replay plans: off)
  *   specialized   1 when the plan's last run launched the specialised kernel
  */
+/* GoF distributions (as cusna_gof_distribution) of a plan's end networks,
+ * read where they are in device memory after the last run (no dense copy);
+ * masked != 0 applies the plan's keep/force masks (the statistic network). */
+CUSNA_API cusna_status cusna_period_plan_gof(cusna_period_plan *plan, int32_t masked,
+                                             int32_t kind, const int32_t *levls, int32_t nlev,
+                                             double *out);
+
 CUSNA_API cusna_status cusna_period_plan_specialize(cusna_period_plan *plan, int32_t mode);
 CUSNA_API int32_t cusna_period_plan_specialized(const cusna_period_plan *plan);
 
@@ -366,7 +383,8 @@ typedef struct cusna_mom_options {
   int32_t surrogate_rounds;
   int32_t surrogate_corrections;
   /* period kernel specialised at run time to the effect list (hipRTC): 1 on,
-   * -1 off, 0 when a simulation round is large (chains x actors >= 2^18) */
+   * -1 off, 0 when a simulation round has >= 2^18 chains x actors or the
+   * whole schedule >= 2^22 (the ~1 s compile, cached on disk, amortised) */
   int32_t specialize;
 } cusna_mom_options;
 

@@ -58,6 +58,9 @@ def oracle() -> C.CDLL:
                                             _vp]
         lib.oracle_set_threads.restype = C.c_int
         lib.oracle_set_threads.argtypes = [C.c_int]
+        lib.oracle_gof_distribution.restype = C.c_int
+        lib.oracle_gof_distribution.argtypes = [_vp, C.c_int32, C.c_int32, C.c_int32, _vp,
+                                                C.c_int32, _vp]
         _cache["o"] = lib
     return _cache["o"]
 
@@ -76,6 +79,9 @@ def reference():
             lib.cusna_behavior_stats.restype = C.c_int
             lib.cusna_behavior_stats.argtypes = [_vp, C.c_int32, C.c_int32, C.c_int32, _vp, _vp,
                                                  _vp, C.c_double, C.c_double, C.c_double, _vp]
+            lib.cusna_gof_distribution.restype = C.c_int
+            lib.cusna_gof_distribution.argtypes = [_vp, C.c_int32, C.c_int32, C.c_int32, _vp,
+                                                   C.c_int32, _vp]
         _cache["r"] = lib
     return _cache["r"]
 
@@ -161,6 +167,22 @@ def behavior_stats(z, bty, xnet, bkdist, zbar, brange, bsimmean, lib=None,
     f = getattr(lib or oracle(), fn)
     st = f(_p(z), n, M, len(bty), _p(bty), _p(xnet), _p(bk), float(zbar), float(brange),
            float(bsimmean), _p(out))
+    if st != 0:
+        raise RuntimeError(f"{fn} status {st}")
+    return out
+
+
+def gof_distribution(a_batch, kind, levls, lib=None, fn="oracle_gof_distribution"):
+    """Degree (kind 0 in, 1 out) or geodesic (kind 2) distributions of M dense
+    networks (M, n, n); the reference's own function with lib=reference(),
+    fn="cusna_gof_distribution"."""
+    a = np.ascontiguousarray(np.asarray(a_batch) != 0, np.uint8)
+    M, n, _ = a.shape
+    lv = np.ascontiguousarray(levls, np.int32)
+    nlev = len(lv)
+    out = np.zeros((M, nlev + (1 if kind == 2 else 0)), np.float64)
+    f = getattr(lib or oracle(), fn)
+    st = f(_p(a), n, M, int(kind), _p(lv), nlev, _p(out))
     if st != 0:
         raise RuntimeError(f"{fn} status {st}")
     return out

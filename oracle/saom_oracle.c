@@ -1064,3 +1064,76 @@ ORACLE_API int oracle_set_threads(int t) {
   return 0;
 #endif
 }
+
+/* sienaGOF auxiliary distributions (libcusna_saom_cpu.cpp:716-765): kind 0
+ * indegree / 1 outdegree -- actors with degree <= levls[L]; kind 2 geodesic --
+ * ordered pairs i != j within distance levls[L] (boolean powers of the
+ * adjacency, diagonal cleared after each step), plus the unreachable pairs.
+ * The argument checks and their status codes are the reference's. */
+ORACLE_API int oracle_gof_distribution(const uint8_t *a_batch, int n, int M, int kind,
+                                       const int32_t *levls, int nlev, double *out) {
+  if (!a_batch || !levls || !out || n < 1 || M < 1 || nlev < 1) return CUSNA_ERR_NULL_ARG;
+  const size_t nn = (size_t)n * n;
+  if (kind == 0 || kind == 1) {
+    int *deg = (int *)malloc(sizeof(int) * n);
+    for (int c = 0; c < M; ++c) {
+      const uint8_t *a = a_batch + (size_t)c * nn;
+      for (int v = 0; v < n; ++v) deg[v] = 0;
+      for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j)
+          if (a[(size_t)i * n + j]) deg[kind == 0 ? j : i]++;
+      for (int L = 0; L < nlev; ++L) {
+        long cnt = 0;
+        for (int v = 0; v < n; ++v)
+          if (deg[v] <= levls[L]) cnt++;
+        out[(size_t)c * nlev + L] = (double)cnt;
+      }
+    }
+    free(deg);
+    return CUSNA_OK;
+  }
+  if (kind == 2) {
+    int maxk = 0;
+    for (int L = 0; L < nlev; ++L) {
+      if (levls[L] < 0) return CUSNA_ERR_SHAPE;
+      if (levls[L] > maxk) maxk = levls[L];
+    }
+    uint8_t *cur = (uint8_t *)malloc(nn), *nxt = (uint8_t *)malloc(nn);
+    long *counts = (long *)calloc((size_t)maxk + 1, sizeof(long));
+    for (int c = 0; c < M; ++c) {
+      const uint8_t *reach = a_batch + (size_t)c * nn;
+      for (size_t q = 0; q < nn; ++q) cur[q] = reach[q] ? 1 : 0;
+      for (int k = 1; k <= maxk; ++k) {
+        long cn = 0;
+        for (int i = 0; i < n; ++i)
+          for (int j = 0; j < n; ++j)
+            if (i != j && cur[(size_t)i * n + j]) cn++;
+        counts[k] = cn;
+        if (k < maxk) {
+          for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) {
+              uint8_t r = cur[(size_t)i * n + j];
+              if (!r)
+                for (int m = 0; m < n; ++m)
+                  if (cur[(size_t)i * n + m] && reach[(size_t)m * n + j]) {
+                    r = 1;
+                    break;
+                  }
+              nxt[(size_t)i * n + j] = r;
+            }
+          for (int i = 0; i < n; ++i) nxt[(size_t)i * n + i] = 0;
+          uint8_t *t = cur;
+          cur = nxt;
+          nxt = t;
+        }
+      }
+      for (int L = 0; L < nlev; ++L) out[(size_t)c * (nlev + 1) + L] = (double)counts[levls[L]];
+      out[(size_t)c * (nlev + 1) + nlev] = (double)((long)n * (n - 1) - counts[maxk]);
+    }
+    free(cur);
+    free(nxt);
+    free(counts);
+    return CUSNA_OK;
+  }
+  return CUSNA_ERR_UNSUPPORTED;
+}
