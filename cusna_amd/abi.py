@@ -345,3 +345,154 @@ def run_period(fn, prob: SaomProblem, want_end_nets: bool = False, name: str = "
     if st != 0:
         raise CusnaError(name, st)
     return res.as_dict()
+
+
+# ------------------------------------------------------- multi-network ABI
+MAXNET = 4
+CROSS_IDS = (33, 34, 35, 36, 37)
+
+
+class cusna_saom_multi_problem(C.Structure):
+    """Reference libcusna.h:238-267."""
+    _fields_ = [
+        ("struct_version", C.c_int32), ("n", C.c_int32), ("W", C.c_int32), ("k", C.c_int32),
+        ("M", C.c_int32), ("D", C.c_int32), ("threads", C.c_int32),
+        ("x0_out", _p), ("x0_in", _p), ("lam", _p), ("theta", _p),
+        ("type_ids", _p), ("etypes", _p), ("netof", _p), ("netref", _p),
+        ("wtab", _p), ("p1", _p), ("p2", _p), ("vmat", _p), ("dyc", _p), ("dyc_idx", _p),
+        ("perm", _p), ("symf", _p), ("seed", C.c_uint64), ("act", _p), ("nact", C.c_int32),
+    ]
+
+
+class cusna_saom_multi_result(C.Structure):
+    """Reference libcusna.h:271-278."""
+    _fields_ = [
+        ("struct_version", C.c_int32), ("rscores", _p), ("scores", _p), ("ntim", _p),
+        ("steps", _p), ("a_ends", _p),
+    ]
+
+
+@dataclass
+class MultiProblem:
+    """One multi-network period (D networks over one actor set) as the R
+    estimator hands it to the native simulator (R/mom.R:352-364,
+    cusna_saom_glue.cpp:229-279). x0 and perm are (D, n, n); lam (D,) or
+    (M, D); theta (k,) or (M, k); descriptors float32 on the wire."""
+    x0: np.ndarray
+    lam: np.ndarray
+    theta: np.ndarray
+    type_ids: np.ndarray
+    netof: np.ndarray
+    M: int
+    seed: int
+    netref: np.ndarray | None = None
+    etypes: np.ndarray | None = None
+    p1: np.ndarray | None = None
+    p2: np.ndarray | None = None
+    vmat: np.ndarray | None = None
+    wtab: np.ndarray | None = None
+    dyc: np.ndarray | None = None       # (nd, n, n)
+    dyc_idx: np.ndarray | None = None
+    perm: np.ndarray | None = None      # (D, n, n)
+    symf: np.ndarray | None = None      # (D,)
+    act: np.ndarray | None = None
+    nact: int | None = None
+    threads: int = 0
+    _keep: list = field(default_factory=list, repr=False)
+
+    @property
+    def D(self) -> int:
+        return int(np.asarray(self.x0).shape[0])
+
+    @property
+    def n(self) -> int:
+        return int(np.asarray(self.x0).shape[1])
+
+    @property
+    def k(self) -> int:
+        return int(np.asarray(self.type_ids).shape[0])
+
+    @property
+    def nd(self) -> int:
+        return 0 if self.dyc is None else int(np.asarray(self.dyc).shape[0])
+
+    def build(self) -> cusna_saom_multi_problem:
+        n, k, M, D = self.n, self.k, int(self.M), self.D
+        W = (n + 31) // 32
+        keep = self._keep = []
+
+        def hold(a):
+            keep.append(a)
+            return _ptr(a)
+
+        th = np.asarray(self.theta, np.float32)
+        th = np.broadcast_to(th, (M, k)) if th.ndim == 1 else th
+        lam = np.asarray(self.lam, np.float32)
+        lam = np.broadcast_to(lam, (M, D)) if lam.ndim == 1 else lam
+        if th.shape != (M, k) or lam.shape != (M, D):
+            raise ValueError("theta must be (M, k) or (k,), lam (M, D) or (D,)")
+        x0 = np.asarray(self.x0)
+        p = cusna_saom_multi_problem()
+        p.struct_version = ABI_VERSION
+        p.n, p.W, p.k, p.M, p.D, p.threads = n, W, k, M, D, int(self.threads)
+        p.x0_out = hold(np.concatenate([pack_rows(x0[d]) for d in range(D)]))
+        p.x0_in = hold(np.concatenate([pack_rows(x0[d].T) for d in range(D)]))
+        p.lam, p.theta = hold(_arr(lam, np.float32)), hold(_arr(th, np.float32))
+        p.type_ids = hold(_arr(self.type_ids, np.int32))
+        z = np.zeros(k)
+        p.etypes = hold(_arr(self.etypes if self.etypes is not None else z, np.int32))
+        p.netof = hold(_arr(self.netof, np.int32))
+        p.netref = hold(_arr(self.netref if self.netref is not None else z, np.int32))
+        p.wtab = hold(_arr(self.wtab if self.wtab is not None else np.zeros((k, n + 1)),
+                           np.float32))
+        p.p1 = hold(_arr(self.p1 if self.p1 is not None else z, np.float32))
+        p.p2 = hold(_arr(self.p2 if self.p2 is not None else z, np.float32))
+        p.vmat = hold(_arr(self.vmat if self.vmat is not None else np.zeros((k, n)),
+                           np.float32))
+        if self.dyc is not None:
+            p.dyc = hold(_arr(self.dyc, np.float32))
+        p.dyc_idx = hold(_arr(self.dyc_idx if self.dyc_idx is not None else -np.ones(k),
+                              np.int32))
+        if self.perm is not None:
+            pm = np.asarray(self.perm)
+            p.perm = hold(np.concatenate([pack_rows(pm[d]) for d in range(D)]))
+        p.symf = hold(_arr(self.symf if self.symf is not None else np.zeros(D), np.int32))
+        p.seed = int(self.seed) & 0xFFFFFFFFFFFFFFFF
+        if self.act is not None:
+            p.act = hold(_arr(self.act, np.int32))
+        p.nact = int(self.nact if self.nact is not None else n)
+        return p
+
+
+class MultiResult:
+    """Output buffers of one multi-network period call."""
+
+    def __init__(self, n: int, k: int, M: int, D: int, want_end_nets: bool = True):
+        self.rscores = np.zeros((M, D), np.float32)
+        self.scores = np.zeros((M, k), np.float32)
+        self.ntim = np.zeros(M, np.float64)
+        self.steps = np.zeros(M, np.int32)
+        self.a_ends = np.zeros((M, D, n, n), np.uint8) if want_end_nets else None
+        r = cusna_saom_multi_result()
+        r.struct_version = ABI_VERSION
+        r.rscores, r.scores = _ptr(self.rscores), _ptr(self.scores)
+        r.ntim, r.steps, r.a_ends = _ptr(self.ntim), _ptr(self.steps), _ptr(self.a_ends)
+        self.c = r
+
+    def as_dict(self) -> dict:
+        d = {"rscores": self.rscores, "scores": self.scores, "ntim": self.ntim,
+             "steps": self.steps}
+        if self.a_ends is not None:
+            d["a_ends"] = self.a_ends
+        return d
+
+
+def run_multi(fn, prob: MultiProblem, want_end_nets: bool = True, name: str = "multi",
+              extra_args=()) -> dict:
+    """Call ``fn(multi_problem*, multi_result*, *extra_args)``; return the outputs."""
+    p = prob.build()
+    res = MultiResult(prob.n, prob.k, int(prob.M), prob.D, want_end_nets)
+    st = fn(C.byref(p), C.byref(res.c), *extra_args)
+    if st != 0:
+        raise CusnaError(name, st)
+    return res.as_dict()

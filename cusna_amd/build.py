@@ -28,8 +28,10 @@ HIPCC_FLAGS = [
 
 # (source, extra defines, object name)
 UNITS = [("capi.hip", [], "capi.o"), ("saom_stats.hip", [], "saom_stats.o"),
-         ("saom_launch.hip", [], "saom_launch.o"), ("mom.hip", [], "mom.o"), ("mlp.hip", [], "mlp.o"), ("rtc.hip", [], "rtc.o")] + [
-    ("saom_period.hip", [f"-DSAOM_WMAX={w}"], f"saom_period_w{w}.o") for w in (1, 2, 4, 8, 16, 32, 64, 128)]
+         ("saom_launch.hip", [], "saom_launch.o"), ("mom.hip", [], "mom.o"), ("mlp.hip", [], "mlp.o"), ("rtc.hip", [], "rtc.o"),
+         ("multi.hip", [], "multi.o")] + [
+    ("saom_period.hip", [f"-DSAOM_WMAX={w}"], f"saom_period_w{w}.o") for w in (1, 2, 4, 8, 16, 32, 64, 128)] + [
+    ("saom_multi.hip", [f"-DSAOM_WMAX={w}"], f"saom_multi_w{w}.o") for w in (1, 2, 4, 8, 16, 32, 64, 128)]
 
 
 def hipcc() -> str:

@@ -5,6 +5,7 @@
 
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "../../include/libcusna.h"
 #include "saom_kernels.h"
@@ -40,6 +41,15 @@ class DeviceBlock {
 void set_error(const std::string& s);
 int device_count();
 hipStream_t thread_stream();
+
+// ABI bit rows (n*W u32) -> kernel rows (n*W64 u64); NULL -> off-diagonal
+// (dflt 1) or zero (dflt 0).
+std::vector<uint64_t> rows64(const uint32_t* rows, int n, int W, int W64, int dflt);
+std::vector<uint64_t> dense_rows64(const uint8_t* a, int n, int W64);
+// Smallest power of two >= W64 (the compiled word bound).
+int wmax_for(int W64);
+// Feature flags (saom::F_*) an effect list needs.
+int effect_flags(const int32_t* t, int k, int kb, const int32_t* dyn);
 
 }  // namespace cusna_impl
 

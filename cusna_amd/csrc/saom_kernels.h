@@ -100,6 +100,32 @@ bool specialize_policy(long long chains_x_actors);
 // Same effect list (what the specialised kernel bakes in).
 bool same_effects(const SimParams& a, const SimParams& b);
 
+// Multi-network period (saom_multi.hip): D networks over one actor set
+// (libcusna_saom_cpu.cpp:427-510). `base` carries the shared shape and the
+// per-effect descriptors read by the single-network change statistics
+// (n, k, W64, RS, nact, flags, tid/ety/dci/p1/p2, vmat, wtab, dyc, act);
+// its dyn flags are zero.
+constexpr int kMaxNet = 4;  // CUSNA_MAXNET
+struct MultiParams {
+  SimParams base;
+  int D;
+  int rows_global;  // 1: working rows in `work` (chain state above the LDS)
+  int netof[kMaxEff], netref[kMaxEff];
+  int symf[kMaxNet];
+  const uint64_t *x0, *pm, *kd;  // D*n*W64 each (kd: distance dyads, or null)
+  const double *theta, *lam;     // M*k, M*D
+  uint64_t* work;                // rows_global: M*D*WS*n words (WS = row words stored)
+  uint64_t* end_rows;            // D*M*n*W64: network d of chain c at (d*M + c)*n*W64
+  double *scores, *rscores, *ntim;  // M*k, M*D, M
+  int32_t* steps;                   // M
+  int32_t* dist_stat;               // [opt] D*M: |end xor start| over kd, per network
+  int M;
+};
+// Launch the multi-network period kernel (one workgroup of 64*nw threads
+// per chain). Pd is the device copy of P. Returns 0 or a hipError_t.
+int launch_multi(const MultiParams& P, const MultiParams* Pd, int nw, int wmax, void* stream);
+size_t multi_lds_bytes(const MultiParams& P, int nw, int wmax);
+
 // Statistics of M packed networks (rows u64, n*W64 per network).
 struct StatParams {
   int n, k, M, W64, symflag;
@@ -150,6 +176,19 @@ struct GofParams {
 int launch_gof(const GofParams& P, void* stream);
 // LDS bytes of the geodesic kernel (0: the network is too large for it).
 size_t gof_geodesic_lds(int n, int W64, int kmax);
+
+// Cross-network statistic of M packed networks against a predictor network
+// (libcusna_saom_cpu.cpp:819-857). Networks are (rows & keep) | force.
+struct CrossParams {
+  int n, M, W64, t;
+  double parm;
+  const uint64_t* rows;          // network c at rows + c * n * W64
+  const uint64_t *keep, *force;  // n*W64 or null
+  const uint64_t *wp, *wpT;      // predictor rows and their transpose, n*W64
+  double* out;                   // out[c * out_stride]
+  int out_stride;
+};
+int launch_cross_stats(const CrossParams& P, void* stream);
 
 // Expand packed end rows (M*n*W64) to dense uint8 (M*n*n).
 int launch_expand_dense(const uint64_t* rows, int n, int W64, int M, uint8_t* dense,

@@ -64,6 +64,34 @@ int launch_period_ex(const PeriodLaunch& L, LaunchShape s, bool replay, bool sim
   }
 }
 
+#define SAOM_MDECL(W) \
+  int launch_multi_w##W(const MultiParams* Pd, int grid, size_t lds, int nw, void* stream);
+SAOM_MDECL(1)
+SAOM_MDECL(2)
+SAOM_MDECL(4)
+SAOM_MDECL(8)
+SAOM_MDECL(16)
+SAOM_MDECL(32)
+SAOM_MDECL(64)
+SAOM_MDECL(128)
+#undef SAOM_MDECL
+
+int launch_multi(const MultiParams& P, const MultiParams* Pd, int nw, int wmax, void* stream) {
+  if (P.M < 1) return 0;
+  const size_t lds = multi_lds_bytes(P, nw, wmax);
+  switch (wmax) {
+    case 1: return launch_multi_w1(Pd, P.M, lds, nw, stream);
+    case 2: return launch_multi_w2(Pd, P.M, lds, nw, stream);
+    case 4: return launch_multi_w4(Pd, P.M, lds, nw, stream);
+    case 8: return launch_multi_w8(Pd, P.M, lds, nw, stream);
+    case 16: return launch_multi_w16(Pd, P.M, lds, nw, stream);
+    case 32: return launch_multi_w32(Pd, P.M, lds, nw, stream);
+    case 64: return launch_multi_w64(Pd, P.M, lds, nw, stream);
+    case 128: return launch_multi_w128(Pd, P.M, lds, nw, stream);
+    default: return (int)hipErrorInvalidValue;
+  }
+}
+
 int launch_period(const SimParams& P, const SimParams* Pd, LaunchShape s, bool replay,
                   uint32_t k0, uint32_t k1, void* stream, bool specialize, bool* used) {
   PeriodLaunch L;

@@ -1092,7 +1092,11 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
   }
 }
 
-#ifndef SAOM_RTC
+#if defined(SAOM_HELPERS_ONLY)
+// included by another kernel file for the helpers above (no period kernels)
+}  // namespace
+}  // namespace saom
+#elif !defined(SAOM_RTC)
 // Register budget: two waves per SIMD for the two-wave shape, three for the
 // four-wave shape (LDS admits four chains of n = 500 per CU).
 template <int NW, int R, int WMAX, bool REPLAY, bool SIMPLE>

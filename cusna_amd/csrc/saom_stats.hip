@@ -718,6 +718,94 @@ int launch_gof(const GofParams& P, void* stream) {
   }
 }
 
+// ------------------------------------------------------- cross statistics
+
+__device__ __forceinline__ uint64_t cross_row(const CrossParams& P, const uint64_t* src, int i,
+                                              int w) {
+  uint64_t v = src[(size_t)i * P.W64 + w];
+  if (P.keep) v &= P.keep[(size_t)i * P.W64 + w];
+  if (P.force) v |= P.force[(size_t)i * P.W64 + w];
+  if (w == P.W64 - 1 && (P.n & 63)) v &= (1ull << (P.n & 63)) - 1ull;
+  return v;
+}
+
+// One workgroup per network. Integer-valued statistics are exact int64 sums
+// in any order; `from` with the square root (parm >= 2) is summed by one lane
+// in the reference's order (ties row-major), so it rounds identically.
+__global__ void __launch_bounds__(kStatThreads) cross_stats_kernel(const CrossParams P) {
+  const int n = P.n, W64 = P.W64, net = blockIdx.x, t = P.t;
+  const uint64_t* src = P.rows + (size_t)net * n * W64;
+  double* out = P.out + (size_t)net * P.out_stride;
+  if (t == 36 && P.parm >= 2.0) {
+    if (threadIdx.x == 0) {
+      double acc = 0.0;
+      for (int i = 0; i < n; ++i)
+        for (int w = 0; w < W64; ++w) {
+          uint64_t m = cross_row(P, src, i, w);
+          while (m) {
+            const int j = w * 64 + __ffsll((long long)m) - 1;
+            m &= m - 1;
+            int c = 0;
+            for (int q = 0; q < W64; ++q)
+              c += __popcll(P.wp[(size_t)i * W64 + q] & P.wp[(size_t)j * W64 + q]);
+            acc += sqrt((double)c);
+          }
+        }
+      *out = acc;
+    }
+    return;
+  }
+  long long c = 0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    for (int w = 0; w < W64; ++w) {
+      const uint64_t a = cross_row(P, src, i, w);
+      const uint64_t wi = P.wp[(size_t)i * W64 + w], wt = P.wpT[(size_t)i * W64 + w];
+      switch (t) {
+        case 33: c += __popcll(a & wi); break;
+        case 34: c += __popcll(a & wt); break;
+        case 35: c += __popcll(a & wi & wt); break;
+        case 36: {  // sum over ties i->j of |out_wp(i) & out_wp(j)|
+          uint64_t m = a;
+          while (m) {
+            const int j = w * 64 + __ffsll((long long)m) - 1;
+            m &= m - 1;
+            for (int q = 0; q < W64; ++q)
+              c += __popcll(P.wp[(size_t)i * W64 + q] & P.wp[(size_t)j * W64 + q]);
+          }
+          break;
+        }
+        default: {  // 37: sum over h in out_wp(i) of |out_a(h) & out_a(i)|
+          uint64_t m = wi;
+          while (m) {
+            const int h = w * 64 + __ffsll((long long)m) - 1;
+            m &= m - 1;
+            for (int q = 0; q < W64; ++q)
+              c += __popcll(cross_row(P, src, h, q) & cross_row(P, src, i, q));
+          }
+          break;
+        }
+      }
+    }
+  }
+  for (int o = 32; o >= 1; o >>= 1) c += __shfl_xor(c, o, 64);
+  __shared__ long long part[kStatThreads / 64];
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long tot = 0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) tot += part[q];
+    *out = (double)tot;
+  }
+}
+
+int launch_cross_stats(const CrossParams& P, void* stream) {
+  if (P.M < 1) return 0;
+  if (P.t < 33 || P.t > 37) return (int)hipErrorInvalidValue;
+  hipLaunchKernelGGL(cross_stats_kernel, dim3(P.M), dim3(kStatThreads), 0, (hipStream_t)stream,
+                     P);
+  return (int)hipGetLastError();
+}
+
 int launch_expand_dense(const uint64_t* rows, int n, int W64, int M, uint8_t* dense,
                         void* stream) {
   const long long total = (long long)M * n * n;

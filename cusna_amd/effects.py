@@ -30,6 +30,9 @@ STRUCTURAL["gwespFF"] = 24
 COVARIATE = {k: EFF[k] for k in ("egoX", "altX", "simX", "sameX", "higher", "egoXaltX",
                                  "egoPlusAltX", "altRThresholdX")}
 DYADIC = {"X": 22}
+# cross-network effects of the multi-network model (R/effects.R:33-37)
+CROSSNET = dict(crprod=33, crprodRecip=34, crprodMutual=35, **{"from": 36, "to": 37})
+CROSS_IDS = (33, 34, 35, 36, 37)
 GW_FAMILY = (24, 26, 27, 28, 29, 30)
 BEH = dict(linear=0, quad=1, avAlt=2, avSim=3, totSim=4)
 ETYPES = dict(eval=0, creation=1, endow=2)
@@ -64,6 +67,7 @@ class Effect:
     dyn: bool = False
     etype: str = "eval"
     comps: tuple | None = None  # interaction components (0-based, earlier effects)
+    net_ref: int | None = None  # cross-network effects: source network (0-based)
     type_id: int = field(init=False)
     needs_cov: bool = field(init=False)
 
@@ -76,7 +80,12 @@ class Effect:
                 raise ValueError("interaction needs 2 or 3 components")
             self.comps, self.type_id, self.needs_cov, self.parm = c, 32, False, 1.0
             return
-        if self.name in STRUCTURAL:
+        if self.name in CROSSNET:
+            if self.net_ref is None:
+                raise ValueError(f"{self.name} requires net_ref (the other dependent network "
+                                 "index)")
+            self.type_id, self.needs_cov = CROSSNET[self.name], False
+        elif self.name in STRUCTURAL:
             self.type_id, self.needs_cov = STRUCTURAL[self.name], False
         elif self.name in COVARIATE:
             self.type_id, self.needs_cov = COVARIATE[self.name], not self.dyn
@@ -161,6 +170,12 @@ class Descriptors:
     int_comp: np.ndarray
 
 
+def effect_netrefs(effects: list) -> np.ndarray:
+    """Source network of each cross-network effect, -1 otherwise
+    (R/effects.R:277-283)."""
+    return np.array([-1 if e.net_ref is None else int(e.net_ref) for e in effects], np.int32)
+
+
 def prepare_effects(effects: list, n: int, p: int = 1, beh=None, z_start=None) -> Descriptors:
     """Float64 descriptor bundle for period p (R/effects.R:176-238, 241-294)."""
     k = len(effects)
@@ -185,6 +200,8 @@ def prepare_effects(effects: list, n: int, p: int = 1, beh=None, z_start=None) -
             p1[idx] = float(e.parm)
         elif t == 31:
             p1[idx] = float(math.trunc(e.parm))
+        elif t in CROSS_IDS:
+            p1[idx] = float(e.parm)
         if t == 22:
             w = np.asarray(e.cov, dtype=float)
             if w.ndim == 3:

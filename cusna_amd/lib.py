@@ -30,7 +30,9 @@ EXPORTS = (
     "cusna_mlp_layer_shape", "cusna_mlp_get_params", "cusna_mlp_set_params",
     "cusna_mlp_forward_backward", "cusna_mlp_train", "cusna_mlp_predict", "cusna_mlp_destroy",
     "cusna_period_plan_specialize", "cusna_period_plan_specialized", "cusna_gof_distribution",
-    "cusna_period_plan_gof",
+    "cusna_period_plan_gof", "cusna_saom_simulate_multi", "cusna_cross_stats",
+    "cusna_multi_plan_create", "cusna_multi_plan_set_params", "cusna_multi_plan_run",
+    "cusna_multi_plan_fetch", "cusna_multi_plan_dist", "cusna_multi_plan_destroy",
 )
 
 _lib = None
@@ -89,6 +91,22 @@ def load(path: str | None = None) -> C.CDLL:
     L.cusna_period_plan_autotune.argtypes = [_vp, C.c_int32, C.POINTER(C.c_int32)]
     L.cusna_saom_simulate_groups.restype = C.c_int
     L.cusna_saom_simulate_groups.argtypes = [_vp, C.c_int32, _vp]
+    L.cusna_saom_simulate_multi.restype = C.c_int
+    L.cusna_saom_simulate_multi.argtypes = [_vp, _vp]
+    L.cusna_cross_stats.restype = C.c_int
+    L.cusna_cross_stats.argtypes = [_vp, _vp, C.c_int32, C.c_int32, C.c_int32, C.c_double, _vp]
+    L.cusna_multi_plan_create.restype = C.c_int
+    L.cusna_multi_plan_create.argtypes = [_vp, C.c_int32, _vp, _vp]
+    L.cusna_multi_plan_set_params.restype = C.c_int
+    L.cusna_multi_plan_set_params.argtypes = [_vp, _vp, _vp]
+    L.cusna_multi_plan_run.restype = C.c_int
+    L.cusna_multi_plan_run.argtypes = [_vp, C.c_uint64]
+    L.cusna_multi_plan_fetch.restype = C.c_int
+    L.cusna_multi_plan_fetch.argtypes = [_vp, _vp]
+    L.cusna_multi_plan_dist.restype = C.c_int
+    L.cusna_multi_plan_dist.argtypes = [_vp, _vp]
+    L.cusna_multi_plan_destroy.restype = C.c_int
+    L.cusna_multi_plan_destroy.argtypes = [_vp]
     L.cusna_gof_distribution.restype = C.c_int
     L.cusna_gof_distribution.argtypes = [_vp, C.c_int32, C.c_int32, C.c_int32, _vp, C.c_int32,
                                          _vp]
@@ -145,6 +163,82 @@ def simulate_period(prob: SaomProblem, want_end_nets: bool = False) -> dict:
     L = load()
     out = run_period(L.cusna_saom_simulate_period, prob, want_end_nets, "cusna_saom_simulate_period")
     return out
+
+
+def simulate_multi(prob, want_end_nets: bool = True) -> dict:
+    """One multi-network period for prob.M chains on the GPU
+    (cusna_saom_simulate_multi); prob is a MultiProblem."""
+    from .abi import run_multi
+
+    return run_multi(load().cusna_saom_simulate_multi, prob, want_end_nets,
+                     "cusna_saom_simulate_multi")
+
+
+def cross_stats(a_batch, wp, t: int, parm: float = 1.0) -> np.ndarray:
+    """Cross-network statistic of M dense networks against wp (cusna_cross_stats)."""
+    a = np.ascontiguousarray(np.asarray(a_batch) != 0, np.uint8)
+    M, n, _ = a.shape
+    w = np.ascontiguousarray(np.asarray(wp) != 0, np.uint8)
+    out = np.zeros(M, np.float64)
+    _check("cusna_cross_stats",
+           load().cusna_cross_stats(_p(a), _p(w), n, M, int(t), float(parm), _p(out)))
+    return out
+
+
+class MultiPlan:
+    """Device-resident multi-network problem (cusna_multi_plan_*). kdist
+    (D, n, n), when given, enables the per-network period distances."""
+
+    def __init__(self, prob, want_end_nets: bool = False, kdist=None):
+        from .abi import pack_rows
+
+        self.L = load()
+        self.prob = prob
+        self._c = prob.build()
+        self._kd = None
+        if kdist is not None:
+            kd = np.asarray(kdist)
+            self._kd = np.ascontiguousarray(np.concatenate([pack_rows(kd[d])
+                                                            for d in range(prob.D)]))
+        self.h = _vp()
+        _check("cusna_multi_plan_create",
+               self.L.cusna_multi_plan_create(C.byref(self._c), int(want_end_nets),
+                                              _p(self._kd), C.byref(self.h)))
+        self.want_end_nets = want_end_nets
+
+    def set_params(self, theta, lam) -> None:
+        M, k, D = self.prob.M, self.prob.k, self.prob.D
+        th = np.ascontiguousarray(np.broadcast_to(np.asarray(theta, np.float32), (M, k)))
+        lm = np.ascontiguousarray(np.broadcast_to(np.asarray(lam, np.float32), (M, D)))
+        _check("cusna_multi_plan_set_params",
+               self.L.cusna_multi_plan_set_params(self.h, _p(th), _p(lm)))
+
+    def run(self, seed: int) -> None:
+        _check("cusna_multi_plan_run",
+               self.L.cusna_multi_plan_run(self.h, int(seed) & 0xFFFFFFFFFFFFFFFF))
+
+    def fetch(self, want_end_nets: bool = False) -> dict:
+        from .abi import MultiResult
+
+        res = MultiResult(self.prob.n, self.prob.k, self.prob.M, self.prob.D, want_end_nets)
+        _check("cusna_multi_plan_fetch", self.L.cusna_multi_plan_fetch(self.h, C.byref(res.c)))
+        return res.as_dict()
+
+    def dist(self) -> np.ndarray:
+        out = np.zeros((self.prob.D, self.prob.M), np.int32)
+        _check("cusna_multi_plan_dist", self.L.cusna_multi_plan_dist(self.h, _p(out)))
+        return out
+
+    def close(self) -> None:
+        if self.h:
+            self.L.cusna_multi_plan_destroy(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def simulate_period_replay(prob: SaomProblem, log_u0: np.ndarray, loglog_u: np.ndarray,

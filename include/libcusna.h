@@ -163,6 +163,99 @@ typedef struct cusna_saom_result {
 CUSNA_API cusna_status cusna_saom_simulate_period(const cusna_saom_problem *problem,
                                                   cusna_saom_result *result);
 
+/*
+ * Multi-network co-evolution: D one-mode networks (2..CUSNA_MAXNET) over one
+ * actor set. Layout byte-identical to the reference (libcusna.h:238-278).
+ * Each ministep picks a network in proportion to its rate, an active actor,
+ * and one toggle of that network. Effect e belongs to network netof[e]; type
+ * ids 33-37 are cross-network effects (crprod, crprodRecip, crprodMutual,
+ * from, to) reading the current state of network netref[e].
+ */
+typedef struct cusna_saom_multi_problem {
+  int32_t struct_version; /* CUSNA_ABI_VERSION                               */
+  int32_t n;              /* actors, shared by the D networks                */
+  int32_t W;              /* (n + 31) / 32                                   */
+  int32_t k;              /* effects over all networks (<= CUSNA_MAXEFF)     */
+  int32_t M;              /* chains                                          */
+  int32_t D;              /* networks, 2..CUSNA_MAXNET                       */
+  int32_t threads;        /* block size hint; 0 = automatic                  */
+
+  const uint32_t *x0_out; /* D*n*W packed out-rows of the start networks     */
+  const uint32_t *x0_in;  /* D*n*W packed in-rows                            */
+  const float *lam;       /* M*D rate of network d in chain c                */
+  const float *theta;     /* M*k parameters of chain c                       */
+
+  const int32_t *type_ids; /* k effect type ids                              */
+  const int32_t *etypes;   /* k: 0 evaluation, 1 creation, 2 endowment       */
+  const int32_t *netof;    /* k: network of effect e                         */
+  const int32_t *netref;   /* k: source network of a cross effect            */
+  const float *wtab;       /* k*(n+1) geometric weight tables                */
+  const float *p1;         /* k                                              */
+  const float *p2;         /* k                                              */
+  const float *vmat;       /* k*n covariate values                           */
+  const float *dyc;        /* [opt] dyadic covariates, nd*n*n                */
+  const int32_t *dyc_idx;  /* k: dyadic covariate of effect e, or -1         */
+  const uint32_t *perm;    /* [opt] D*n*W permitted toggles per network      */
+  const int32_t *symf;     /* D: symmetric network (paired flips)            */
+  uint64_t seed;           /* Philox key                                     */
+  const int32_t *act;      /* [opt] n: active actors first                   */
+  int32_t nact;            /* active actors (n without composition change)   */
+} cusna_saom_multi_problem;
+
+/* Outputs, caller-allocated; NULL skips one (reference libcusna.h:271-278). */
+typedef struct cusna_saom_multi_result {
+  int32_t struct_version;
+  float *rscores;  /* [opt] [M*D] rate score of each network                 */
+  float *scores;   /* [opt] [M*k] effect scores                              */
+  double *ntim;    /* [opt] [M] simulated time                               */
+  int32_t *steps;  /* [opt] [M] ministeps                                    */
+  uint8_t *a_ends; /* [opt] [M*D*n*n] dense end networks, [chain][d][i][j]   */
+} cusna_saom_multi_result;
+
+/*
+ * M chains of one multi-network period on the GPU (reference libcusna.h:
+ * 284-285; the CPU mirror it must equal is cusna_saom_simulate_multi_cpu,
+ * libcusna_saom_cpu.cpp:548-604): same trajectories, end networks, steps,
+ * ntim; scores to float32. As in that mirror, a type-19 (reciAct) effect sees
+ * reciprocated degree 0 and there is no composition restore. Dyadic covariate
+ * q is read at dyc + q*n*n as the ABI documents (the mirror copies only the
+ * first n*n values; with dyc_idx in {-1, 0} both agree).
+ */
+CUSNA_API cusna_status cusna_saom_simulate_multi(const cusna_saom_multi_problem *problem,
+                                                 cusna_saom_multi_result *result);
+
+/*
+ * Cross-network statistic of M dense networks a_batch (M*n*n) against one
+ * predictor network wp (n*n) on the GPU (reference cusna_cross_stats,
+ * libcusna_saom_cpu.cpp:819-857): t = 33 sum a*wp, 34 sum a*wp^T,
+ * 35 sum a*wp*wp^T, 36 sum over ties of |out(i) & out(j)| in wp (its square
+ * root when parm >= 2), 37 sum over ties of (wp a)_ij. out[M].
+ */
+CUSNA_API cusna_status cusna_cross_stats(const uint8_t *a_batch, const uint8_t *wp, int32_t n,
+                                         int32_t M, int32_t t, double parm, double *out);
+
+/*
+ * Device-resident multi-network problem (no counterpart in the reference,
+ * whose estimator calls the one-shot function per batch, R/mom.R:352-364):
+ * create uploads the problem once (and its theta/lam rows); set_params
+ * replaces the per-chain rows (M*k theta, M*D lam); run simulates one period
+ * with the given Philox key; fetch copies outputs like the one-shot call.
+ * kdist ([opt] D*n*W packed rows): dyads counted in each network's period
+ * distance; cusna_multi_plan_dist then returns D*M distances (network-major)
+ * of the last run.
+ */
+typedef struct cusna_multi_plan cusna_multi_plan;
+CUSNA_API cusna_status cusna_multi_plan_create(const cusna_saom_multi_problem *problem,
+                                               int32_t want_end_nets, const uint32_t *kdist,
+                                               cusna_multi_plan **plan);
+CUSNA_API cusna_status cusna_multi_plan_set_params(cusna_multi_plan *plan, const float *theta,
+                                                   const float *lam);
+CUSNA_API cusna_status cusna_multi_plan_run(cusna_multi_plan *plan, uint64_t seed);
+CUSNA_API cusna_status cusna_multi_plan_fetch(cusna_multi_plan *plan,
+                                              cusna_saom_multi_result *result);
+CUSNA_API cusna_status cusna_multi_plan_dist(cusna_multi_plan *plan, int32_t *dist_stat);
+CUSNA_API cusna_status cusna_multi_plan_destroy(cusna_multi_plan *plan);
+
 /* Pack dense n*n uint8 into out/in bit rows (reference libcusna.h:299-301). */
 CUSNA_API cusna_status cusna_pack_bits(const uint8_t *dense, int32_t n, uint32_t *out_rows,
                                        uint32_t *in_rows, int32_t *W_out);

@@ -37,5 +37,23 @@ int main(void) {
   P(cusna_saom_result, steps); P(cusna_saom_result, bdist); P(cusna_saom_result, bscores);
   P(cusna_saom_result, z_end); P(cusna_saom_result, rscores); P(cusna_saom_result, actor_dist);
   P(cusna_saom_result, want_end_nets); P(cusna_saom_result, a_ends);
+  printf("sizeof.multi_problem %zu\nsizeof.multi_result %zu\n",
+         sizeof(cusna_saom_multi_problem), sizeof(cusna_saom_multi_result));
+  P(cusna_saom_multi_problem, struct_version); P(cusna_saom_multi_problem, n);
+  P(cusna_saom_multi_problem, W); P(cusna_saom_multi_problem, k); P(cusna_saom_multi_problem, M);
+  P(cusna_saom_multi_problem, D); P(cusna_saom_multi_problem, threads);
+  P(cusna_saom_multi_problem, x0_out); P(cusna_saom_multi_problem, x0_in);
+  P(cusna_saom_multi_problem, lam); P(cusna_saom_multi_problem, theta);
+  P(cusna_saom_multi_problem, type_ids); P(cusna_saom_multi_problem, etypes);
+  P(cusna_saom_multi_problem, netof); P(cusna_saom_multi_problem, netref);
+  P(cusna_saom_multi_problem, wtab); P(cusna_saom_multi_problem, p1);
+  P(cusna_saom_multi_problem, p2); P(cusna_saom_multi_problem, vmat);
+  P(cusna_saom_multi_problem, dyc); P(cusna_saom_multi_problem, dyc_idx);
+  P(cusna_saom_multi_problem, perm); P(cusna_saom_multi_problem, symf);
+  P(cusna_saom_multi_problem, seed); P(cusna_saom_multi_problem, act);
+  P(cusna_saom_multi_problem, nact);
+  P(cusna_saom_multi_result, struct_version); P(cusna_saom_multi_result, rscores);
+  P(cusna_saom_multi_result, scores); P(cusna_saom_multi_result, ntim);
+  P(cusna_saom_multi_result, steps); P(cusna_saom_multi_result, a_ends);
   return 0;
 }

@@ -58,6 +58,11 @@ def oracle() -> C.CDLL:
                                             _vp]
         lib.oracle_set_threads.restype = C.c_int
         lib.oracle_set_threads.argtypes = [C.c_int]
+        lib.oracle_saom_simulate_multi.restype = C.c_int
+        lib.oracle_saom_simulate_multi.argtypes = [_vp, _vp, C.c_int32]
+        lib.oracle_cross_stats.restype = C.c_int
+        lib.oracle_cross_stats.argtypes = [_vp, _vp, C.c_int32, C.c_int32, C.c_int32,
+                                           C.c_double, _vp]
         lib.oracle_gof_distribution.restype = C.c_int
         lib.oracle_gof_distribution.argtypes = [_vp, C.c_int32, C.c_int32, C.c_int32, _vp,
                                                 C.c_int32, _vp]
@@ -79,6 +84,11 @@ def reference():
             lib.cusna_behavior_stats.restype = C.c_int
             lib.cusna_behavior_stats.argtypes = [_vp, C.c_int32, C.c_int32, C.c_int32, _vp, _vp,
                                                  _vp, C.c_double, C.c_double, C.c_double, _vp]
+            lib.cusna_saom_simulate_multi_cpu.restype = C.c_int
+            lib.cusna_saom_simulate_multi_cpu.argtypes = [_vp, _vp]
+            lib.cusna_cross_stats.restype = C.c_int
+            lib.cusna_cross_stats.argtypes = [_vp, _vp, C.c_int32, C.c_int32, C.c_int32,
+                                              C.c_double, _vp]
             lib.cusna_gof_distribution.restype = C.c_int
             lib.cusna_gof_distribution.argtypes = [_vp, C.c_int32, C.c_int32, C.c_int32, _vp,
                                                    C.c_int32, _vp]
@@ -183,6 +193,37 @@ def gof_distribution(a_batch, kind, levls, lib=None, fn="oracle_gof_distribution
     out = np.zeros((M, nlev + (1 if kind == 2 else 0)), np.float64)
     f = getattr(lib or oracle(), fn)
     st = f(_p(a), n, M, int(kind), _p(lv), nlev, _p(out))
+    if st != 0:
+        raise RuntimeError(f"{fn} status {st}")
+    return out
+
+
+def simulate_multi(prob, want_end_nets=True) -> dict:
+    """The oracle's multi-network period (MultiProblem)."""
+    from cusna_amd.abi import run_multi
+
+    L = oracle()
+    return run_multi(L.oracle_saom_simulate_multi, prob, want_end_nets, "oracle multi",
+                     extra_args=(C.c_int32(prob.nd),))
+
+
+def simulate_multi_reference(prob, want_end_nets=True) -> dict:
+    from cusna_amd.abi import run_multi
+
+    R = reference()
+    if R is None:
+        raise FileNotFoundError("oracle/_ref not built")
+    return run_multi(R.cusna_saom_simulate_multi_cpu, prob, want_end_nets, "reference multi")
+
+
+def cross_stats(a_batch, wp, t, parm=0.0, lib=None, fn="oracle_cross_stats"):
+    """Cross-network statistic of M dense networks (M, n, n) against wp (n, n);
+    the reference's own function with lib=reference(), fn="cusna_cross_stats"."""
+    a = np.ascontiguousarray(np.asarray(a_batch) != 0, np.uint8)
+    M, n, _ = a.shape
+    w = np.ascontiguousarray(np.asarray(wp) != 0, np.uint8)
+    out = np.zeros(M, np.float64)
+    st = getattr(lib or oracle(), fn)(_p(a), _p(w), n, M, int(t), float(parm), _p(out))
     if st != 0:
         raise RuntimeError(f"{fn} status {st}")
     return out

@@ -15,6 +15,11 @@
  *   oracle_saom_simulate_period <- cusna_saom_simulate_period_cpu (:315-425)
  *   oracle_network_stats  <- cusna_network_stats        (:606-689)
  *   oracle_behavior_stats <- cusna_behavior_stats       (:767-817)
+ *   oracle_gof_distribution <- cusna_gof_distribution   (:716-765)
+ *   cross_change          <- cross_ds                   (:430-443)
+ *   run_chain_multi       <- simulate_one_multi         (:460-510)
+ *   oracle_saom_simulate_multi <- cusna_saom_simulate_multi_cpu (:548-604)
+ *   oracle_cross_stats    <- cusna_cross_stats          (:819-857)
  * Summation orders, float32->double promotions and the Philox stream layout
  * are the reference's. Build with -ffp-contract=off so that no a*b+c is fused.
  *
@@ -1136,4 +1141,295 @@ ORACLE_API int oracle_gof_distribution(const uint8_t *a_batch, int n, int M, int
     return CUSNA_OK;
   }
   return CUSNA_ERR_UNSUPPORTED;
+}
+
+/* ------------------------------------------------- multi-network periods */
+
+/* Cross-network change statistic: ego i toggles i->j in its own network x
+ * (dl = +1 adds, -1 removes) while source network w is read in its current
+ * state. libcusna_saom_cpu.cpp:430-443. */
+static double cross_change(int t, const uint8_t *w, const uint8_t *x, int n, int i, int j, double dl,
+                           double parm) {
+  switch (t) {
+    case 33: /* crprod: w_ij */
+      return dl * (double)w[(size_t)i * n + j];
+    case 34: /* crprodRecip: w_ji */
+      return dl * (double)w[(size_t)j * n + i];
+    case 35: /* crprodMutual: w_ij w_ji */
+      return dl * (double)(w[(size_t)i * n + j] & w[(size_t)j * n + i]);
+    case 36: { /* from: shared out-ties of i and j in w */
+      int64_t c = 0;
+      for (int h = 0; h < n; ++h) c += w[(size_t)i * n + h] & w[(size_t)j * n + h];
+      return dl * (parm >= 2.0 ? sqrt((double)c) : (double)c);
+    }
+    case 37: { /* to: two-paths i -w-> h -x-> j */
+      int64_t c = 0;
+      for (int h = 0; h < n; ++h) c += w[(size_t)i * n + h] & x[(size_t)h * n + j];
+      return dl * (double)c;
+    }
+  }
+  return 0.0;
+}
+
+typedef struct multi_ctx {
+  int n, k, D, nact, need_tp, nd;
+  const uint8_t *x0, *pm; /* D*n*n dense start networks and permitted toggles */
+  const double *lam, *th; /* M*D, M*k */
+  const int32_t *tid, *ety, *netof, *netref, *dci, *symf, *act;
+  const double *wtab, *p1, *p2, *vmat, *dyc;
+  uint32_t k0, k1;
+} multi_ctx;
+
+/* One chain of a multi-network period: end networks a (D*n*n), effect scores
+ * sc (k), rate scores rsc (D). libcusna_saom_cpu.cpp:460-510. */
+static void run_chain_multi(const multi_ctx *C, uint32_t c, uint8_t *a, double *sc, double *rsc,
+                            double *ntim, int64_t *nsteps) {
+  const int n = C->n, k = C->k, D = C->D;
+  const size_t nn = (size_t)n * n;
+  const double *lam = C->lam + (size_t)c * D;
+  const double *th = C->th + (size_t)c * k;
+  int64_t *indeg = calloc((size_t)D * n, sizeof(int64_t));
+  int64_t *outdeg = calloc((size_t)D * n, sizeof(int64_t));
+  int64_t *steps_d = calloc((size_t)D, sizeof(int64_t));
+  int *tp = calloc((size_t)n, sizeof(int));
+  double *fj = malloc(sizeof(double) * (size_t)n);
+  double *ds = malloc(sizeof(double) * (size_t)n * (size_t)k);
+  double *wsum = malloc(sizeof(double) * (size_t)k);
+  uint8_t *okj = malloc((size_t)n);
+
+  memcpy(a, C->x0, (size_t)D * nn);
+  for (int d = 0; d < D; ++d)
+    for (int r = 0; r < n; ++r)
+      for (int s = 0; s < n; ++s)
+        if (a[d * nn + (size_t)r * n + s]) {
+          outdeg[(size_t)d * n + r]++;
+          indeg[(size_t)d * n + s]++;
+        }
+  double lam_tot = 0.0;
+  for (int d = 0; d < D; ++d) lam_tot += lam[d];
+  for (int e = 0; e < k; ++e) sc[e] = 0.0;
+
+  int64_t steps = 0;
+  double t = 0.0;
+  uint32_t r4[4];
+  for (;;) {
+    draw4((uint32_t)steps, c, 0u, 0u, C->k0, C->k1, r4);
+    t += -log(unit_open(r4[0])) / ((double)C->nact * lam_tot);
+    if (t > 1.0) break;
+    /* network in proportion to its rate, then an active actor */
+    const double uv = unit_open(r4[1]) * lam_tot;
+    int d = 0;
+    double acc = lam[0];
+    while (d < D - 1 && uv >= acc) acc += lam[++d];
+    int64_t x = (int64_t)(unit_open(r4[2]) * C->nact);
+    if (x >= C->nact) x = C->nact - 1;
+    const int i = C->act[x];
+    uint8_t *ad = a + (size_t)d * nn;
+    chain_view V = {n, ad, indeg + (size_t)d * n, outdeg + (size_t)d * n, tp, 0};
+    if (C->need_tp) {
+      for (int h = 0; h < n; ++h) tp[h] = 0;
+      for (int y = 0; y < n; ++y)
+        if (ad[(size_t)i * n + y])
+          for (int h = 0; h < n; ++h) tp[h] += ad[(size_t)y * n + h];
+    }
+    double gbest = -1e30, lmax = -1e30;
+    int jstar = i;
+    for (int j = 0; j < n; ++j) {
+      double f = 0.0;
+      int ok = 1;
+      if (j == i) {
+        for (int e = 0; e < k; ++e) ds[(size_t)j * k + e] = 0.0;
+      } else if (!C->pm[(size_t)d * nn + (size_t)i * n + j]) {
+        ok = 0;
+      } else {
+        const int aij = ad[(size_t)i * n + j];
+        const double dl = 1.0 - 2.0 * aij;
+        for (int e = 0; e < k; ++e) {
+          if (C->netof[e] != d) {
+            ds[(size_t)j * k + e] = 0.0;
+            continue;
+          }
+          double v;
+          if (C->tid[e] >= 33) {
+            v = cross_change(C->tid[e], a + (size_t)C->netref[e] * nn, ad, n, i, j, dl, C->p1[e]);
+          } else {
+            const int dc = C->dci[e];
+            const double dyw = (dc >= 0 && C->dyc) ? C->dyc[((size_t)dc * n + i) * n + j] : 0.0;
+            v = ego_change(C->tid[e], &V, i, j, dl, C->vmat[(size_t)e * n + i],
+                           C->vmat[(size_t)e * n + j], dyw, C->p1[e], C->p2[e],
+                           C->wtab + (size_t)e * (n + 1));
+          }
+          if ((C->ety[e] == 1 && aij == 1) || (C->ety[e] == 2 && aij == 0)) v = 0.0;
+          ds[(size_t)j * k + e] = v;
+          const double prod = th[e] * v;
+          f = f + prod;
+        }
+      }
+      okj[j] = (uint8_t)ok;
+      fj[j] = f;
+      if (ok && f > lmax) lmax = f;
+      draw4((uint32_t)steps, c, 1u, (uint32_t)j, C->k0, C->k1, r4);
+      if (ok) {
+        const double g = f - log(-log(unit_open(r4[0])));
+        if (g > gbest || (g == gbest && j < jstar)) {
+          gbest = g;
+          jstar = j;
+        }
+      }
+    }
+    double se = 0.0;
+    for (int e = 0; e < k; ++e) wsum[e] = 0.0;
+    for (int j = 0; j < n; ++j) {
+      if (!okj[j]) continue;
+      const double w = exp(fj[j] - lmax);
+      se += w;
+      for (int e = 0; e < k; ++e) wsum[e] += w * ds[(size_t)j * k + e];
+    }
+    for (int e = 0; e < k; ++e)
+      if (C->netof[e] == d) sc[e] += ds[(size_t)jstar * k + e] - wsum[e] / se;
+    steps++;
+    steps_d[d]++;
+    if (jstar != i) {
+      const int j = jstar;
+      const int64_t dd = 1 - 2 * (int64_t)ad[(size_t)i * n + j];
+      indeg[(size_t)d * n + j] += dd;
+      outdeg[(size_t)d * n + i] += dd;
+      ad[(size_t)i * n + j] ^= 1u;
+      if (C->symf[d]) {
+        indeg[(size_t)d * n + i] += dd;
+        outdeg[(size_t)d * n + j] += dd;
+        ad[(size_t)j * n + i] ^= 1u;
+      }
+    }
+    if (steps >= ORACLE_MAX_STEPS) break;
+  }
+  for (int d = 0; d < D; ++d) rsc[d] = (double)steps_d[d] / lam[d] - (double)C->nact;
+  *ntim = t;
+  *nsteps = steps;
+  free(indeg);
+  free(outdeg);
+  free(steps_d);
+  free(tp);
+  free(fj);
+  free(ds);
+  free(wsum);
+  free(okj);
+}
+
+/* M chains of a multi-network period. libcusna_saom_cpu.cpp:548-604, except
+ * that dyadic covariate q of nd is read at dyc + q*n*n as the ABI documents
+ * (the reference copies only the first n*n values; with dyc_idx in {-1, 0}
+ * both agree). */
+ORACLE_API int oracle_saom_simulate_multi(const cusna_saom_multi_problem *pb,
+                                          cusna_saom_multi_result *r, int32_t nd) {
+  if (!pb || !r) return CUSNA_ERR_NULL_ARG;
+  if (pb->struct_version != CUSNA_ABI_VERSION) return CUSNA_ERR_SHAPE;
+  const int n = pb->n, W = pb->W, k = pb->k, M = pb->M, D = pb->D;
+  if (n < 1 || n > CUSNA_MAX_N || W != (n + 31) / 32 || k < 1 || k > CUSNA_MAXEFF || M < 1 ||
+      D < 2 || D > CUSNA_MAXNET)
+    return CUSNA_ERR_SHAPE;
+  const size_t nn = (size_t)n * n;
+  uint8_t *x0 = malloc((size_t)D * nn), *pm = malloc((size_t)D * nn);
+  for (int d = 0; d < D; ++d) {
+    unpack_rows(pb->x0_out + (size_t)d * n * W, n, W, x0 + d * nn);
+    if (pb->perm) unpack_rows(pb->perm + (size_t)d * n * W, n, W, pm + d * nn);
+    else offdiag(n, pm + d * nn);
+  }
+  double *lam = malloc(sizeof(double) * (size_t)M * D), *th = malloc(sizeof(double) * (size_t)M * k);
+  double *wtab = calloc((size_t)k * (n + 1), sizeof(double)), *p1 = calloc((size_t)k, 8),
+         *p2 = calloc((size_t)k, 8), *vmat = calloc((size_t)k * n, 8);
+  for (size_t q = 0; q < (size_t)M * D; ++q) lam[q] = pb->lam[q];
+  for (size_t q = 0; q < (size_t)M * k; ++q) th[q] = pb->theta[q];
+  for (int e = 0; e < k; ++e) {
+    if (pb->p1) p1[e] = pb->p1[e];
+    if (pb->p2) p2[e] = pb->p2[e];
+  }
+  if (pb->wtab)
+    for (size_t q = 0; q < (size_t)k * (n + 1); ++q) wtab[q] = pb->wtab[q];
+  if (pb->vmat)
+    for (size_t q = 0; q < (size_t)k * n; ++q) vmat[q] = pb->vmat[q];
+  double *dyc = NULL;
+  if (pb->dyc && nd > 0) {
+    dyc = malloc(sizeof(double) * nd * nn);
+    for (size_t q = 0; q < (size_t)nd * nn; ++q) dyc[q] = pb->dyc[q];
+  }
+  int32_t *act = malloc(sizeof(int32_t) * (size_t)n);
+  int32_t *dci = malloc(sizeof(int32_t) * (size_t)k);
+  const int nact = pb->act ? pb->nact : n;
+  for (int q = 0; q < n; ++q) act[q] = pb->act ? pb->act[q] : q;
+  for (int e = 0; e < k; ++e) dci[e] = pb->dyc_idx ? pb->dyc_idx[e] : -1;
+  int need_tp = 0;
+  for (int e = 0; e < k; ++e) need_tp |= pb->type_ids[e] == 5;
+
+  multi_ctx C = {n, k, D, nact, need_tp, nd, x0, pm, lam, th, pb->type_ids, pb->etypes,
+                 pb->netof, pb->netref, dci, pb->symf, act, wtab, p1, p2, vmat, dyc,
+                 (uint32_t)(pb->seed & 0xFFFFFFFFu), (uint32_t)(pb->seed >> 32)};
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < M; ++c) {
+    uint8_t *a = malloc((size_t)D * nn);
+    double *sc = malloc(sizeof(double) * (size_t)k), *rs = malloc(sizeof(double) * (size_t)D);
+    double nt;
+    int64_t st;
+    run_chain_multi(&C, (uint32_t)c, a, sc, rs, &nt, &st);
+    if (r->ntim) r->ntim[c] = nt;
+    if (r->steps) r->steps[c] = (int32_t)st;
+    if (r->rscores)
+      for (int d = 0; d < D; ++d) r->rscores[(size_t)c * D + d] = (float)rs[d];
+    if (r->scores)
+      for (int e = 0; e < k; ++e) r->scores[(size_t)c * k + e] = (float)sc[e];
+    if (r->a_ends) memcpy(r->a_ends + (size_t)c * D * nn, a, (size_t)D * nn);
+    free(a);
+    free(sc);
+    free(rs);
+  }
+  free(x0);
+  free(pm);
+  free(lam);
+  free(th);
+  free(wtab);
+  free(p1);
+  free(p2);
+  free(vmat);
+  free(dyc);
+  free(act);
+  free(dci);
+  return CUSNA_OK;
+}
+
+/* Cross-network statistic of M networks against predictor wp.
+ * libcusna_saom_cpu.cpp:819-857. */
+ORACLE_API int oracle_cross_stats(const uint8_t *a_batch, const uint8_t *wp, int32_t n, int32_t M,
+                                  int32_t t, double parm, double *out) {
+  if (!a_batch || !wp || !out || n < 1 || n > CUSNA_MAX_N || M < 1) return CUSNA_ERR_NULL_ARG;
+  if (t < 33 || t > 37) return CUSNA_ERR_UNSUPPORTED;
+  const size_t nn = (size_t)n * n;
+  for (int c = 0; c < M; ++c) {
+    const uint8_t *a = a_batch + (size_t)c * nn;
+    double acc = 0.0;
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) {
+        const uint8_t aij = a[(size_t)i * n + j];
+        switch (t) {
+          case 33: acc += aij & wp[(size_t)i * n + j]; break;
+          case 34: acc += aij & wp[(size_t)j * n + i]; break;
+          case 35: acc += aij & wp[(size_t)i * n + j] & wp[(size_t)j * n + i]; break;
+          case 36:
+            if (aij) {
+              int64_t s = 0;
+              for (int h = 0; h < n; ++h) s += wp[(size_t)i * n + h] & wp[(size_t)j * n + h];
+              acc += parm >= 2.0 ? sqrt((double)s) : (double)s;
+            }
+            break;
+          case 37:
+            if (aij) {
+              int64_t s = 0;
+              for (int h = 0; h < n; ++h) s += wp[(size_t)i * n + h] & a[(size_t)h * n + j];
+              acc += (double)s;
+            }
+            break;
+        }
+      }
+    out[c] = acc;
+  }
+  return CUSNA_OK;
 }

@@ -9,8 +9,13 @@ the reference sources; the fixtures, not the reference, travel).
                     reference's fixtures do not cover: cycle3/inPop/outAct and
                     every other effect id, behaviour avAlt/totSim, symmetric and
                     conditional runs, and statistics of random networks
+  multi_cases.json.gz  outputs of the reference's multi-network engine and
+                    cross-network statistics compiled from source, on synthetic
+                    multi-network problems (every cross effect, D = 2..4,
+                    symmetric networks, masks, composition, per-chain rows)
   abi_offsets.txt   field offsets of the reference ABI structs
-  targets.json.gz, effects.json.gz, estimate.json.gz, data_s50.json.gz:
+  targets.json.gz, effects.json.gz, estimate.json.gz, data_s50.json.gz,
+  multinet.json.gz:
                     copies of the reference's host-layer fixtures (inputs and
                     expected outputs)
 """
@@ -47,10 +52,15 @@ def main():
     for src, dst in [("targets/targets.json.gz", "targets.json.gz"),
                      ("effects/effects.json.gz", "effects.json.gz"),
                      ("estimate/estimate.json.gz", "estimate.json.gz"),
-                     ("data/s50.json.gz", "data_s50.json.gz")]:
+                     ("data/s50.json.gz", "data_s50.json.gz"),
+                     ("multinet/multinet.json.gz", "multinet.json.gz")]:
         shutil.copyfile(os.path.join(REF_FIX, src), os.path.join(HERE, dst))
     # 2. reference-engine outputs on synthetic cases
     dump("ref_cases.json.gz", build_cases(O))
+    from multi_cases import build_cross_cases, build_multi_cases
+
+    dump("multi_cases.json.gz", {"multi": build_multi_cases(O)["cases"],
+                                 "cross": build_cross_cases(O)["cases"]})
     # 3. ABI layout of the reference header
     out = subprocess.run([os.path.join(ROOT, "oracle", "_ref", "abi_offsets_ref")],
                          capture_output=True, text=True, check=True).stdout
