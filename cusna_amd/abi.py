@@ -372,6 +372,15 @@ class cusna_saom_multi_result(C.Structure):
     ]
 
 
+class cusna_multi_stat_spec(C.Structure):
+    """Statistics of one network of a multi-network plan (include/libcusna.h)."""
+    _fields_ = [
+        ("struct_version", C.c_int32), ("k", C.c_int32), ("type_ids", _p), ("p1", _p),
+        ("p2", _p), ("vmat", _p), ("vmiss", _p), ("wtab", _p), ("od0", _p), ("id0", _p),
+        ("symflag", C.c_int32), ("keep", _p), ("force", _p), ("cross_wp", _p),
+    ]
+
+
 @dataclass
 class MultiProblem:
     """One multi-network period (D networks over one actor set) as the R

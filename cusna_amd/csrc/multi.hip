@@ -22,8 +22,20 @@
 
 using namespace cusna_impl;
 
+// Statistics of one network's end states (set by cusna_multi_plan_set_stats):
+// its non-cross effects through the network statistics kernel, its cross
+// effects through the cross-statistics kernel, all on the masked network.
+struct MultiStats {
+  std::vector<int> cols_nc, cols_cross;  // columns (global effect index)
+  saom::StatParams SP{};
+  std::vector<saom::CrossParams> CP;
+  DeviceBlock B;
+  size_t o_out_nc = 0, o_out_cross = 0;
+};
+
 struct cusna_multi_plan {
   int n = 0, k = 0, D = 0, M = 0, W64 = 0, wmax = 0, nw = 0, nd = 0;
+  std::vector<std::unique_ptr<MultiStats>> stats;
   bool want_dense = false;
   hipStream_t s = nullptr;
   DeviceBlock Dv;
@@ -397,6 +409,195 @@ CUSNA_API cusna_status cusna_cross_stats(const uint8_t* a_batch, const uint8_t* 
   }
   HIP_TRY(hipMemcpyAsync(out, B.at<char>(o_out), (size_t)M * 8, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
+  return CUSNA_OK;
+}
+
+CUSNA_API cusna_status cusna_multi_plan_set_stats(cusna_multi_plan* plan, int32_t d,
+                                                  const cusna_multi_stat_spec* sp) {
+  if (!plan || !sp) return CUSNA_ERR_NULL_ARG;
+  if (sp->struct_version != CUSNA_ABI_VERSION || d < 0 || d >= plan->D) return CUSNA_ERR_SHAPE;
+  const int n = plan->n, W = (n + 31) / 32, W64 = plan->W64, M = plan->M;
+  const saom::MultiParams& Q = plan->P;
+  std::vector<int> cols;
+  for (int e = 0; e < plan->k; ++e)
+    if (Q.netof[e] == d) cols.push_back(e);
+  if (sp->k != (int)cols.size()) {
+    set_error("statistic spec: k differs from the network's effect count");
+    return CUSNA_ERR_SHAPE;
+  }
+  if (sp->k > 0 && (!sp->type_ids || !sp->p1 || !sp->p2 || !sp->vmat)) return CUSNA_ERR_NULL_ARG;
+  std::unique_ptr<MultiStats> st(new MultiStats());
+  std::vector<int> nc, cr;  // positions within the spec
+  for (int q = 0; q < sp->k; ++q) {
+    const int t = sp->type_ids[q];
+    if (t != Q.base.tid[cols[q]]) {
+      set_error("statistic spec: effect ids differ from the plan's");
+      return CUSNA_ERR_SHAPE;
+    }
+    if (t >= 33 && t <= 37) {
+      if (!sp->cross_wp) return CUSNA_ERR_NULL_ARG;
+      cr.push_back(q);
+      st->cols_cross.push_back(cols[q]);
+    } else {
+      nc.push_back(q);
+      st->cols_nc.push_back(cols[q]);
+    }
+  }
+  const int knc = (int)nc.size(), kcr = (int)cr.size();
+  const size_t rowsb = (size_t)n * W64 * 8;
+  std::vector<int32_t> tnc(knc);
+  for (int q = 0; q < knc; ++q) tnc[q] = sp->type_ids[nc[q]];
+  const size_t words = knc ? saom::net_stats_scratch_words(n, W64, saom::stats_need_cols(
+                                                                       tnc.data(), knc))
+                           : 0;
+  // host staging of the non-cross descriptors (sub-list) and the masks
+  std::vector<double> vmat((size_t)knc * n), wtab((size_t)knc * (n + 1), 0.0), vb(n, 0.0);
+  std::vector<uint8_t> vmiss((size_t)knc * n, 0);
+  for (int q = 0; q < knc; ++q) {
+    std::copy(sp->vmat + (size_t)nc[q] * n, sp->vmat + (size_t)(nc[q] + 1) * n,
+              vmat.begin() + (size_t)q * n);
+    if (sp->wtab)
+      std::copy(sp->wtab + (size_t)nc[q] * (n + 1), sp->wtab + (size_t)(nc[q] + 1) * (n + 1),
+                wtab.begin() + (size_t)q * (n + 1));
+    if (sp->vmiss)
+      std::copy(sp->vmiss + (size_t)nc[q] * n, sp->vmiss + (size_t)(nc[q] + 1) * n,
+                vmiss.begin() + (size_t)q * n);
+  }
+  const std::vector<uint64_t> keep = rows64(sp->keep, n, W, W64, 1);
+  const std::vector<uint64_t> force = rows64(sp->force, n, W, W64, 0);
+  std::vector<uint64_t> wp((size_t)kcr * n * W64), wpt((size_t)kcr * n * W64);
+  for (int q = 0; q < kcr; ++q) {
+    const uint32_t* src = sp->cross_wp + (size_t)cr[q] * n * W;
+    const std::vector<uint64_t> r = rows64(src, n, W, W64, 0);
+    std::copy(r.begin(), r.end(), wp.begin() + (size_t)q * n * W64);
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j)
+        if ((r[(size_t)i * W64 + (j >> 6)] >> (j & 63)) & 1)
+          wpt[(size_t)q * n * W64 + (size_t)j * W64 + (i >> 6)] |= 1ull << (i & 63);
+  }
+  DeviceBlock& B = st->B;
+  const size_t o_keep = B.add(rowsb), o_force = B.add(rowsb);
+  const size_t o_vmat = B.add(vmat.size() * 8), o_wtab = B.add(wtab.size() * 8),
+               o_vb = B.add(vb.size() * 8), o_vmiss = B.add(vmiss.size());
+  const size_t o_od0 = B.add(sp->od0 ? (size_t)n * 4 : 0), o_id0 = B.add(sp->id0 ? (size_t)n * 4 : 0);
+  const size_t o_wp = B.add(wp.size() * 8), o_wpt = B.add(wpt.size() * 8);
+  st->o_out_nc = B.add((size_t)M * knc * 8);
+  st->o_out_cross = B.add((size_t)M * kcr * 8);
+  const size_t o_scr = B.add((size_t)M * words * 8);
+  HIP_TRY(B.alloc());
+  hipStream_t s = plan->s;
+  auto up = [&](size_t off, const void* src, size_t bytes) -> hipError_t {
+    if (!bytes) return hipSuccess;
+    return hipMemcpyAsync(B.at<char>(off), src, bytes, hipMemcpyHostToDevice, s);
+  };
+  HIP_TRY(up(o_keep, keep.data(), rowsb));
+  HIP_TRY(up(o_force, force.data(), rowsb));
+  HIP_TRY(up(o_vmat, vmat.data(), vmat.size() * 8));
+  HIP_TRY(up(o_wtab, wtab.data(), wtab.size() * 8));
+  HIP_TRY(up(o_vb, vb.data(), vb.size() * 8));
+  HIP_TRY(up(o_vmiss, vmiss.data(), vmiss.size()));
+  if (sp->od0) HIP_TRY(up(o_od0, sp->od0, (size_t)n * 4));
+  if (sp->id0) HIP_TRY(up(o_id0, sp->id0, (size_t)n * 4));
+  HIP_TRY(up(o_wp, wp.data(), wp.size() * 8));
+  HIP_TRY(up(o_wpt, wpt.data(), wpt.size() * 8));
+  HIP_TRY(hipStreamSynchronize(s));
+  const uint64_t* rows = Q.end_rows + (size_t)d * M * n * W64;
+  saom::StatParams& SP = st->SP;
+  std::memset(&SP, 0, sizeof(SP));
+  SP.n = n;
+  SP.k = knc;
+  SP.M = M;
+  SP.W64 = W64;
+  SP.symflag = sp->symflag;
+  for (int q = 0; q < knc; ++q) {
+    SP.tid[q] = tnc[q];
+    SP.dyn[q] = 0;
+    SP.p1[q] = sp->p1[nc[q]];
+    SP.p2[q] = sp->p2[nc[q]];
+  }
+  SP.rows = rows;
+  SP.keep = B.at<uint64_t>(o_keep);
+  SP.force = B.at<uint64_t>(o_force);
+  SP.vmat = B.at<double>(o_vmat);
+  SP.vb = B.at<double>(o_vb);
+  SP.wtab = B.at<double>(o_wtab);
+  SP.vmiss = sp->vmiss ? B.at<uint8_t>(o_vmiss) : nullptr;
+  SP.od0 = sp->od0 ? B.at<int32_t>(o_od0) : nullptr;
+  SP.id0 = sp->id0 ? B.at<int32_t>(o_id0) : nullptr;
+  SP.out = B.at<double>(st->o_out_nc);
+  SP.scratch = words ? B.at<uint64_t>(o_scr) : nullptr;
+  for (int q = 0; q < kcr; ++q) {
+    saom::CrossParams C{};
+    C.n = n;
+    C.M = M;
+    C.W64 = W64;
+    C.t = sp->type_ids[cr[q]];
+    C.parm = sp->p1[cr[q]];
+    C.rows = rows;
+    C.keep = SP.keep;
+    C.force = SP.force;
+    C.wp = B.at<uint64_t>(o_wp) + (size_t)q * n * W64;
+    C.wpT = B.at<uint64_t>(o_wpt) + (size_t)q * n * W64;
+    C.out = B.at<double>(st->o_out_cross) + q;
+    C.out_stride = kcr;
+    st->CP.push_back(C);
+  }
+  if (plan->stats.size() != (size_t)plan->D) plan->stats.resize(plan->D);
+  plan->stats[d] = std::move(st);
+  return CUSNA_OK;
+}
+
+CUSNA_API cusna_status cusna_multi_plan_stats(cusna_multi_plan* plan, double* out) {
+  if (!plan || !out) return CUSNA_ERR_NULL_ARG;
+  const int M = plan->M, k = plan->k;
+  if (plan->stats.size() != (size_t)plan->D) {
+    set_error("statistics not set for every network");
+    return CUSNA_ERR_SHAPE;
+  }
+  for (const auto& st : plan->stats)
+    if (!st) {
+      set_error("statistics not set for every network");
+      return CUSNA_ERR_SHAPE;
+    }
+  hipStream_t s = plan->s;
+  for (const auto& st : plan->stats) {
+    if (st->SP.k > 0) {
+      const int le = saom::launch_net_stats(st->SP, s);
+      if (le != 0) {
+        set_error("network statistics launch failed");
+        return CUSNA_ERR_CUDA;
+      }
+    }
+    for (const auto& C : st->CP) {
+      const int le = saom::launch_cross_stats(C, s);
+      if (le != 0) {
+        set_error("cross statistics launch failed");
+        return CUSNA_ERR_CUDA;
+      }
+    }
+  }
+  std::vector<std::vector<double>> host(plan->D * 2);
+  for (int d = 0; d < plan->D; ++d) {
+    const MultiStats& st = *plan->stats[d];
+    host[2 * d].resize((size_t)M * st.cols_nc.size());
+    host[2 * d + 1].resize((size_t)M * st.cols_cross.size());
+    if (!host[2 * d].empty())
+      HIP_TRY(hipMemcpyAsync(host[2 * d].data(), st.B.at<char>(st.o_out_nc),
+                             host[2 * d].size() * 8, hipMemcpyDeviceToHost, s));
+    if (!host[2 * d + 1].empty())
+      HIP_TRY(hipMemcpyAsync(host[2 * d + 1].data(), st.B.at<char>(st.o_out_cross),
+                             host[2 * d + 1].size() * 8, hipMemcpyDeviceToHost, s));
+  }
+  HIP_TRY(hipStreamSynchronize(s));
+  for (int d = 0; d < plan->D; ++d) {
+    const MultiStats& st = *plan->stats[d];
+    const size_t a = st.cols_nc.size(), b = st.cols_cross.size();
+    for (int c = 0; c < M; ++c) {
+      for (size_t q = 0; q < a; ++q) out[(size_t)c * k + st.cols_nc[q]] = host[2 * d][c * a + q];
+      for (size_t q = 0; q < b; ++q)
+        out[(size_t)c * k + st.cols_cross[q]] = host[2 * d + 1][c * b + q];
+    }
+  }
   return CUSNA_OK;
 }
 

@@ -33,6 +33,7 @@ EXPORTS = (
     "cusna_period_plan_gof", "cusna_saom_simulate_multi", "cusna_cross_stats",
     "cusna_multi_plan_create", "cusna_multi_plan_set_params", "cusna_multi_plan_run",
     "cusna_multi_plan_fetch", "cusna_multi_plan_dist", "cusna_multi_plan_destroy",
+    "cusna_multi_plan_set_stats", "cusna_multi_plan_stats",
 )
 
 _lib = None
@@ -105,6 +106,10 @@ def load(path: str | None = None) -> C.CDLL:
     L.cusna_multi_plan_fetch.argtypes = [_vp, _vp]
     L.cusna_multi_plan_dist.restype = C.c_int
     L.cusna_multi_plan_dist.argtypes = [_vp, _vp]
+    L.cusna_multi_plan_set_stats.restype = C.c_int
+    L.cusna_multi_plan_set_stats.argtypes = [_vp, C.c_int32, _vp]
+    L.cusna_multi_plan_stats.restype = C.c_int
+    L.cusna_multi_plan_stats.argtypes = [_vp, _vp]
     L.cusna_multi_plan_destroy.restype = C.c_int
     L.cusna_multi_plan_destroy.argtypes = [_vp]
     L.cusna_gof_distribution.restype = C.c_int
@@ -223,6 +228,47 @@ class MultiPlan:
         res = MultiResult(self.prob.n, self.prob.k, self.prob.M, self.prob.D, want_end_nets)
         _check("cusna_multi_plan_fetch", self.L.cusna_multi_plan_fetch(self.h, C.byref(res.c)))
         return res.as_dict()
+
+    def set_stats(self, d: int, type_ids, p1, p2, vmat, vmiss=None, wtab=None, od0=None,
+                  id0=None, symflag: int = 0, keep=None, force=None, cross_wp=None) -> None:
+        """Statistics of network d's end states (cusna_multi_plan_set_stats):
+        its effects in list order; cross_wp (k, n, n) predictor networks of
+        the cross effects (other rows unused); keep/force (n, n) masks."""
+        from .abi import ABI_VERSION, cusna_multi_stat_spec, pack_rows
+
+        n = self.prob.n
+        k = len(type_ids)
+        hold = []
+
+        def arr(x, dt):
+            if x is None:
+                return None
+            a = np.ascontiguousarray(np.asarray(x, dt))
+            hold.append(a)
+            return a.ctypes.data
+
+        spec = cusna_multi_stat_spec()
+        spec.struct_version = ABI_VERSION
+        spec.k = k
+        spec.type_ids = arr(type_ids, np.int32)
+        spec.p1, spec.p2 = arr(p1, np.float64), arr(p2, np.float64)
+        spec.vmat = arr(np.reshape(vmat, (k, n)), np.float64)
+        spec.vmiss = arr(None if vmiss is None else np.asarray(vmiss) != 0, np.uint8)
+        spec.wtab = arr(wtab, np.float64)
+        spec.od0, spec.id0 = arr(od0, np.int32), arr(id0, np.int32)
+        spec.symflag = int(symflag)
+        spec.keep = None if keep is None else arr(pack_rows(np.asarray(keep)), np.uint32)
+        spec.force = None if force is None else arr(pack_rows(np.asarray(force)), np.uint32)
+        if cross_wp is not None:
+            cw = np.asarray(cross_wp)
+            spec.cross_wp = arr(np.concatenate([pack_rows(cw[q]) for q in range(k)]), np.uint32)
+        _check("cusna_multi_plan_set_stats",
+               self.L.cusna_multi_plan_set_stats(self.h, int(d), C.byref(spec)))
+
+    def stats(self) -> np.ndarray:
+        out = np.zeros((self.prob.M, self.prob.k), np.float64)
+        _check("cusna_multi_plan_stats", self.L.cusna_multi_plan_stats(self.h, _p(out)))
+        return out
 
     def dist(self) -> np.ndarray:
         out = np.zeros((self.prob.D, self.prob.M), np.int32)

@@ -256,6 +256,33 @@ CUSNA_API cusna_status cusna_multi_plan_fetch(cusna_multi_plan *plan,
 CUSNA_API cusna_status cusna_multi_plan_dist(cusna_multi_plan *plan, int32_t *dist_stat);
 CUSNA_API cusna_status cusna_multi_plan_destroy(cusna_multi_plan *plan);
 
+/*
+ * Statistics of network d's end states for the multi-network estimator, on
+ * the device (the reference computes them on the host from dense a_ends,
+ * R/mom.R:280-308, 371-391): spec lists network d's effects in list order;
+ * ids 0-31 use the network statistics (as cusna_network_stats), ids 33-37 the
+ * cross statistics (as cusna_cross_stats) against predictor cross_wp[q]; all
+ * on the statistic network (A & keep) | force. cusna_multi_plan_stats then
+ * writes M*k statistics of the last run, column e = effect e of the plan.
+ */
+typedef struct cusna_multi_stat_spec {
+  int32_t struct_version;
+  int32_t k;                /* effects of network d                         */
+  const int32_t *type_ids;  /* k                                            */
+  const double *p1, *p2;    /* k                                            */
+  const double *vmat;       /* k*n                                          */
+  const uint8_t *vmiss;     /* [opt] k*n                                    */
+  const double *wtab;       /* [opt] k*(n+1)                                */
+  const int32_t *od0, *id0; /* [opt] n start degrees                        */
+  int32_t symflag;
+  const uint32_t *keep;     /* [opt] n*W; NULL = off-diagonal               */
+  const uint32_t *force;    /* [opt] n*W; NULL = none                       */
+  const uint32_t *cross_wp; /* [opt] k*n*W predictor rows (ids 33-37)       */
+} cusna_multi_stat_spec;
+CUSNA_API cusna_status cusna_multi_plan_set_stats(cusna_multi_plan *plan, int32_t d,
+                                                  const cusna_multi_stat_spec *spec);
+CUSNA_API cusna_status cusna_multi_plan_stats(cusna_multi_plan *plan, double *out);
+
 /* Pack dense n*n uint8 into out/in bit rows (reference libcusna.h:299-301). */
 CUSNA_API cusna_status cusna_pack_bits(const uint8_t *dense, int32_t n, uint32_t *out_rows,
                                        uint32_t *in_rows, int32_t *W_out);
