@@ -114,6 +114,97 @@ def groups_measure(lib) -> dict:
             "streamed_groups": sum(1 for v in ns if v > 1024)}
 
 
+def multi_problem_c2(M: int, x0s, seed: int = 77):
+    """Two co-evolving networks of 500 actors (the config-2 waves as starts),
+    rate 4 each, density/recip/transTrip/crprod on network 0 and
+    density/recip/to/crprodRecip on network 1."""
+    import numpy as np
+
+    from cusna_amd.abi import MultiProblem
+    from cusna_amd.effects import Effect, effect_netrefs, prepare_effects
+
+    effs = [Effect("density"), Effect("recip"), Effect("transTrip"), Effect("crprod", net_ref=1),
+            Effect("density"), Effect("recip"), Effect("to", net_ref=0),
+            Effect("crprodRecip", net_ref=0)]
+    n = x0s[0].shape[0]
+    d = prepare_effects(effs, n, 1)
+    return MultiProblem(x0=np.stack(x0s).astype(np.uint8), lam=np.array([4.0, 4.0]),
+                        theta=np.array([-2.6, 1.8, 0.3, 0.8, -2.7, 1.6, 0.1, 0.5]),
+                        type_ids=d.type_ids, netof=np.array([0, 0, 0, 0, 1, 1, 1, 1], np.int32),
+                        M=M, seed=seed, netref=effect_netrefs(effs), p1=d.p1, p2=d.p2,
+                        vmat=d.vmat, wtab=d.wtab, symf=np.zeros(2, np.int32))
+
+
+def multi_measure(lib, x0s, cpu: bool, threads: int) -> dict:
+    """Multi-network periods (two networks of 500 actors, 4096 chains):
+    one cusna_multi_plan_run per step (launch + completion), after one
+    warm-up run; the CPU leg runs the reference's multi-network engine
+    (oracle/_ref) on a bounded sample of the same problem. Then the
+    multi-network MoM fit of the reference's fixture (test-multinet.R)."""
+    import numpy as np
+
+    M = 4096
+    prob = multi_problem_c2(M, x0s)
+    plan = lib.MultiPlan(prob)
+    plan.run(1)
+    steps = 3
+    t0 = time.perf_counter()
+    for s in range(steps):
+        plan.run(100 + s)
+    wall = (time.perf_counter() - t0) / steps
+    out = plan.fetch()
+    plan.close()
+    res = {"workload": "multi-network periods: D=2 networks of n=500 (config-2 waves as "
+                       "starts), %d chains, rate 4+4, 4+4 effects incl. crprod/to/crprodRecip"
+                       % M,
+           "ms_per_period": round(1e3 * wall, 2), "chain_periods_per_s": round(M / wall, 1),
+           "alternatives_per_s": float("%.4g" % (float(out["steps"].astype(np.int64).sum())
+                                                 * prob.n / wall)),
+           "mean_ministeps": round(float(out["steps"].mean()), 1)}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+
+        if O.reference() is not None:
+            os.environ["OMP_NUM_THREADS"] = str(threads)
+            chains = 2 * threads
+            t0 = time.perf_counter()
+            O.simulate_multi_reference(multi_problem_c2(chains, x0s), want_end_nets=False)
+            cw = time.perf_counter() - t0
+            res["cpu"] = {"kind": "reference", "cores": threads,
+                          "chain_periods_per_s": round(chains / cw, 3),
+                          "sample": f"{chains} chains of the same problem through "
+                                    f"cusna_saom_simulate_multi_cpu, {threads} threads, "
+                                    f"{cw:.1f} s wall"}
+    # the multinet fixture fit
+    import gzip
+
+    from cusna_amd.effects import Effect
+    from cusna_amd.mom import MomControl
+    from cusna_amd.multinet import MultiMomEstimator, saom_multinet_data
+
+    with gzip.open(os.path.join(ROOT, "tests", "golden", "multinet.json.gz"), "rt") as f:
+        g = json.load(f)
+    S = g["settings"]
+    mn = saom_multinet_data([[np.array(w, float) for w in g["netA_waves"]],
+                             [np.array(w, float) for w in g["netB_waves"]]])
+    ctl = MomControl(firstg=S["firstg"], n2=tuple(S["n2"]), batch2=S["batch2"], n1=S["n1"],
+                     n3=S["n3"], nD=S["nD"], diagonalize=S["diagonalize"], seed=S["seed"])
+    est = MultiMomEstimator(mn, [[Effect("density"), Effect("recip")]] * 2, ctl)
+    est.run()  # plans created, code objects loaded
+    fit = est.run()
+    est.close()
+    th_ref, se_ref = np.array(g["theta"]), np.array(g["se"])
+    res["mom"] = {"workload": "multinet fixture fit: 2 networks of 50 actors, density+recip "
+                              "each, schedule n1=nD=300, n2=15/20 x 128, n3=600",
+                  "wall_s": round(fit.wall_time, 3), "chains_simulated": fit.n_sims,
+                  "max_abs_tconv": round(fit.max_abs_tconv, 4),
+                  "theta": [round(float(x), 4) for x in fit.theta],
+                  "within_reference_bar": bool(np.all(np.abs(fit.theta - th_ref)
+                                                      < se_ref + 0.2))}
+    return res
+
+
 def mom_measure(lib, cpu: bool, threads: int) -> dict:
     """Config-1 fit: 50 actors, 3 waves, conditional MoM on density/recip/
     transTrip with the reference's default schedule (n3 = 1000), repeated
@@ -186,6 +277,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mom", action="store_true")
     ap.add_argument("--no-groups", action="store_true")
+    ap.add_argument("--no-multi", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has one GPU, and the "
@@ -260,6 +352,9 @@ def main() -> int:
             lambda m: c2_problem(M=m, n=n, x0=w1, period_seed=11), threads, chains)
     for pl in plans:
         pl.close()
+    if not args.no_multi:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+        line["multi"] = multi_measure(lib, (w1, w2), not args.no_cpu, threads)
     if not args.no_groups:
         line["groups"] = groups_measure(lib)
     if not args.no_mom:

@@ -73,9 +73,16 @@ cusna_status validate_multi(const cusna_saom_multi_problem* pb) {
   return CUSNA_OK;
 }
 
-// Default block shape: about four alternatives per lane.
-int multi_nw(int wmax) {
+// Block shape: `threads` / 64 waves per chain rounded down to 1, 2 or 4 (at
+// most the word bound); 0 = about four alternatives per lane. Streamed word
+// bounds run 16 waves.
+int multi_nw(int wmax, int threads) {
   if (wmax > 16) return 16;
+  if (threads > 0) {
+    int nw = 1;
+    while (nw * 2 <= threads / 64 && nw * 2 <= std::min(4, wmax)) nw *= 2;
+    return nw;
+  }
   return std::max(1, std::min(4, wmax / 4));
 }
 
@@ -100,7 +107,7 @@ cusna_status multi_plan_create(const cusna_saom_multi_problem* pb, bool want_den
   pl->M = M;
   pl->W64 = W64;
   pl->wmax = wmax;
-  pl->nw = multi_nw(wmax);
+  pl->nw = multi_nw(wmax, pb->threads);
   pl->want_dense = want_dense;
   pl->s = stream;
   saom::MultiParams& Q = pl->P;
