@@ -188,7 +188,16 @@ class SaomProblem:
     act: np.ndarray | None = None
     nact: int | None = None
     inact: np.ndarray | None = None  # (n,) 0/1
+    # non-basic rate effects: kinds 0 covariate (rate_u row), 1 out-degree,
+    # 2 in-degree; alpha the rate parameters
+    rate_kind: np.ndarray | None = None
+    rate_alpha: np.ndarray | None = None
+    rate_u: np.ndarray | None = None  # (nrate, n)
     _keep: list = field(default_factory=list, repr=False)
+
+    @property
+    def nrate(self) -> int:
+        return 0 if self.rate_kind is None else int(np.asarray(self.rate_kind).shape[0])
 
     @property
     def n(self) -> int:
@@ -269,12 +278,20 @@ class SaomProblem:
         p.maxdegree, p.conditional = int(self.maxdegree), int(self.conditional)
         p.target_dist, p.symflag = int(self.target_dist), int(self.symflag)
         p.seed = int(self.seed) & 0xFFFFFFFFFFFFFFFF
+        if self.nrate > 0:
+            p.nrate = self.nrate
+            p.rkind = hold(_arr(self.rate_kind, np.int32))
+            p.ralpha = hold(_arr(self.rate_alpha, np.float32))
+            if self.rate_u is not None:
+                p.ru = hold(_arr(np.reshape(self.rate_u, (self.nrate, n)), np.float32))
         p.has_comp = int(self.has_comp)
         p.nact = int(self.nact if self.nact is not None else n)
         if self.act is not None:
             p.act = hold(_arr(self.act, np.int32))
         if self.inact is not None:
-            p.inactw = hold(pack_rows(np.asarray(self.inact).reshape(1, n))[0].copy())
+            bits = np.zeros(((n + 31) // 32) * 32, np.uint8)
+            bits[:n] = np.asarray(self.inact).reshape(n) != 0
+            p.inactw = hold(np.packbits(bits, bitorder="little").view("<u4").copy())
         p.kb = int(self.kb)
         p.brange = 1.0
         if self.kb > 0:
@@ -300,8 +317,9 @@ class SaomResult:
     """Output buffers for one period call (all optional outputs allocated)."""
 
     def __init__(self, n: int, k: int, M: int, kb: int = 0, want_end_nets: bool = False,
-                 rate_outputs: bool = True):
-        self.n, self.k, self.M, self.kb = n, k, M, kb
+                 rate_outputs: bool = True, nrate: int = 0):
+        self.n, self.k, self.M, self.kb, self.nrate = n, k, M, kb, nrate
+        self.rscores = np.zeros((M, max(nrate, 1)), np.float32)
         self.stats = np.zeros((M, k), np.float64)
         self.ntim = np.zeros(M, np.float64)
         self.scores = np.zeros((M, k + 1), np.float32)
@@ -320,6 +338,8 @@ class SaomResult:
         if kb > 0:
             r.bdist, r.bscores, r.z_end = _ptr(self.bdist), _ptr(self.bscores), _ptr(self.z_end)
         r.actor_dist = _ptr(self.actor_dist)
+        if nrate > 0:
+            r.rscores = _ptr(self.rscores)
         r.want_end_nets = 1 if want_end_nets else 0
         r.a_ends = _ptr(self.a_ends)
         self.c = r
@@ -331,6 +351,8 @@ class SaomResult:
         }
         if self.kb > 0:
             d.update(bdist=self.bdist, bscores=self.bscores, z_end=self.z_end)
+        if self.nrate > 0:
+            d.update(rscores=self.rscores, actor_dist=self.actor_dist)
         if self.a_ends is not None:
             d["a_ends"] = self.a_ends
         return d
@@ -340,7 +362,8 @@ def run_period(fn, prob: SaomProblem, want_end_nets: bool = False, name: str = "
                extra_args=()) -> dict:
     """Call ``fn(problem*, result*, *extra_args)`` and return the outputs."""
     p = prob.build()
-    res = SaomResult(prob.n, prob.k, int(prob.M), int(prob.kb), want_end_nets)
+    res = SaomResult(prob.n, prob.k, int(prob.M), int(prob.kb), want_end_nets,
+                     nrate=prob.nrate)
     st = fn(C.byref(p), C.byref(res.c), *extra_args)
     if st != 0:
         raise CusnaError(name, st)

@@ -157,9 +157,13 @@ cusna_status validate(const cusna_saom_problem* pb, const cusna_saom_result* r) 
   if (n < 1 || n > CUSNA_MAX_N || W != (n + 31) / 32 || k < 1 || M < 1) return CUSNA_ERR_SHAPE;
   if (k > CUSNA_MAXEFF || pb->kb < 0 || pb->kb > CUSNA_MAXBEFF) return CUSNA_ERR_SHAPE;
   if (!pb->x0_out || !pb->type_ids || !pb->theta) return CUSNA_ERR_NULL_ARG;
-  if (pb->nrate != 0) {
-    set_error("non-basic rate effects (nrate > 0) are not supported by the GPU path");
-    return CUSNA_ERR_UNSUPPORTED;
+  if (pb->nrate < 0 || pb->nrate > CUSNA_MAXRATE) return CUSNA_ERR_SHAPE;
+  if (pb->nrate > 0) {  // rate effects (libcusna_saom_cpu.cpp:170-193)
+    if (!pb->rkind || !pb->ralpha) return CUSNA_ERR_NULL_ARG;
+    for (int q = 0; q < pb->nrate; ++q) {
+      if (pb->rkind[q] < 0 || pb->rkind[q] > 2) return CUSNA_ERR_UNSUPPORTED;
+      if (pb->rkind[q] == 0 && !pb->ru) return CUSNA_ERR_NULL_ARG;
+    }
   }
   if (pb->kb > 0 && (!pb->lamb || !pb->thetab || !pb->bty || !pb->z0)) return CUSNA_ERR_NULL_ARG;
   for (int e = 0; e < k; ++e) {
@@ -243,6 +247,14 @@ cusna_status plan_create(const cusna_saom_problem* pb, bool want_dense, const do
   P.brange = pb->brange;
   P.bsim = pb->bsimmean;
   P.flags = effect_flags(pb->type_ids, k, kb, pb->dyn_flags);
+  P.nrate = pb->nrate;
+  if (P.nrate > 0) {
+    P.flags |= saom::F_RATE;
+    for (int q = 0; q < P.nrate; ++q) {
+      P.rkind[q] = pb->rkind[q];
+      P.ralpha[q] = (double)pb->ralpha[q];
+    }
+  }
   int nd = 0;
   for (int e = 0; e < k; ++e) {
     P.tid[e] = pb->type_ids[e];
@@ -293,6 +305,11 @@ cusna_status plan_create(const cusna_saom_problem* pb, bool want_dense, const do
     dyc.resize((size_t)nd * n * n);
     for (size_t q = 0; q < dyc.size(); ++q) dyc[q] = pb->dyc[q];
   }
+  std::vector<double> ru;
+  if (P.nrate > 0 && pb->ru) {
+    ru.resize((size_t)P.nrate * n);
+    for (size_t q = 0; q < ru.size(); ++q) ru[q] = pb->ru[q];
+  }
   std::vector<int32_t> z0(n, 0), bkd(n, 1);
   for (int q = 0; q < n; ++q) {
     if (pb->z0) z0[q] = pb->z0[q];
@@ -324,6 +341,9 @@ cusna_status plan_create(const cusna_saom_problem* pb, bool want_dense, const do
   const size_t o_steps = D.add((size_t)M * 4);
   const size_t o_bdist = D.add((size_t)M * 4), o_zend = D.add((size_t)M * n * 4);
   pl->o_stats = D.add((size_t)M * k * 8);
+  const size_t o_ru = D.add(ru.size() * 8);
+  pl->o_rsc = D.add((size_t)M * P.nrate * 8);
+  pl->o_adist = D.add(P.nrate > 0 ? (size_t)M * n * 4 : 0);
   pl->o_params = D.add(sizeof(saom::SimParams));
   const size_t o_work = D.add(stream_rows ? (size_t)M * rowsb : 0);
   pl->o_sscr = D.add((size_t)M * stat_words * 8);
@@ -359,6 +379,7 @@ cusna_status plan_create(const cusna_saom_problem* pb, bool want_dense, const do
   HIP_TRY(up(o_wtab, wtab.data(), wtab.size() * 8));
   HIP_TRY(up(pl->o_vb, vb.data(), (size_t)n * 8));
   if (!dyc.empty()) HIP_TRY(up(o_dyc, dyc.data(), dyc.size() * 8));
+  HIP_TRY(up(o_ru, ru.data(), ru.size() * 8));
   HIP_TRY(up(o_z0, z0.data(), (size_t)n * 4));
   HIP_TRY(up(o_bkd, bkd.data(), (size_t)n * 4));
   if (pb->act) HIP_TRY(up(o_act, pb->act, (size_t)P.nact * 4));
@@ -402,7 +423,10 @@ cusna_status plan_create(const cusna_saom_problem* pb, bool want_dense, const do
   P.steps = D.at<int32_t>(o_steps);
   P.bdist = D.at<int32_t>(o_bdist);
   P.z_end = kb > 0 ? D.at<int32_t>(o_zend) : nullptr;
-  P.actor_dist = nullptr;  // written by the reference only when nrate > 0 (:406-409)
+  // written by the reference only when nrate > 0 (:406-409), with the rate scores
+  P.actor_dist = P.nrate > 0 ? D.at<int32_t>(pl->o_adist) : nullptr;
+  P.ru = ru.empty() ? nullptr : D.at<double>(o_ru);
+  P.rscores = P.nrate > 0 ? D.at<double>(pl->o_rsc) : nullptr;
   P.work = stream_rows ? D.at<uint64_t>(o_work) : nullptr;
   HIP_TRY(hipMemcpyAsync(D.at<char>(pl->o_params), &P, sizeof(P), hipMemcpyHostToDevice, s));
   HIP_TRY(hipStreamSynchronize(s));
@@ -527,7 +551,14 @@ cusna_status plan_fetch(cusna_period_plan* pl, cusna_saom_result* r) {
     HIP_TRY(down(r->z_end, P.z_end, (size_t)M * n * 4));
   }
   if (dense) HIP_TRY(down(r->a_ends, pl->D.at<char>(pl->o_dense), (size_t)M * n * n));
+  std::vector<double> rsc((size_t)M * P.nrate);
+  if (P.nrate > 0) {
+    if (r->rscores) HIP_TRY(down(rsc.data(), P.rscores, rsc.size() * 8));
+    HIP_TRY(down(r->actor_dist, P.actor_dist, (size_t)M * n * 4));
+  }
   HIP_TRY(hipStreamSynchronize(s));
+  if (P.nrate > 0 && r->rscores)
+    for (size_t q = 0; q < rsc.size(); ++q) r->rscores[q] = (float)rsc[q];
   if (r->scores)
     for (size_t q = 0; q < sc.size(); ++q) r->scores[q] = (float)sc[q];
   if (kb > 0 && r->bscores)

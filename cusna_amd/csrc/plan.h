@@ -78,6 +78,7 @@ struct cusna_period_plan {
   size_t o_keep = 0, o_force = 0, o_vb = 0, o_vmiss = 0, o_od0 = 0, o_id0 = 0, o_stats = 0;
   size_t o_dense = 0, o_th = 0, o_lam = 0, o_thb = 0, o_lamb = 0, o_params = 0;
   size_t o_sscr = 0;       // statistics scratch (networks above the LDS)
+  size_t o_rsc = 0, o_adist = 0;  // rate-effect scores, per-actor distances
   bool has_sscr = false;
   bool specialize = false;  // launch the runtime-specialised period kernel
   bool last_rtc = false;    // the last run did

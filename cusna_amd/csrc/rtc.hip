@@ -124,7 +124,8 @@ void* rtc_period_kernel(const SimParams& P, LaunchShape s, bool replay, bool sim
       "-DSAOM_FIXED_K=" + std::to_string(P.k),
       "-DSAOM_FIXED_TYPES=" + list(P.tid, P.k), "-DSAOM_FIXED_ETYPES=" + list(P.ety, P.k),
       "-DSAOM_FIXED_DYN=" + list(P.dyn, P.k),
-      std::string("-DSAOM_CACHE_D=") + (P.k * R <= 32 ? "1" : "0")};
+      std::string("-DSAOM_CACHE_D=") + (P.k * R <= 32 ? "1" : "0"),
+      std::string("-DSAOM_HAS_RATE=") + ((P.flags & F_RATE) ? "1" : "0")};
   int dev = 0;
   (void)hipGetDevice(&dev);
   std::string key = std::to_string(dev);

@@ -18,6 +18,7 @@ namespace saom {
 
 constexpr int kMaxEff = 24;   // CUSNA_MAXEFF
 constexpr int kMaxBeff = 6;   // CUSNA_MAXBEFF
+constexpr int kMaxRate = 4;   // CUSNA_MAXRATE
 constexpr long long kMaxSteps = 4000000;  // runaway cap per chain-period
 
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11). The two multipliers and

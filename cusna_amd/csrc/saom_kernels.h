@@ -17,6 +17,7 @@ enum : int {
   F_SOALL = 1 << 5,  // shared out-ties of every actor with i in LDS (gwdspFB)
   F_Z = 1 << 6,      // behaviour values live in LDS
   F_RDEG = 1 << 7,   // reciprocated degree of the ego (reciAct)
+  F_RATE = 1 << 8,   // non-basic rate effects: per-actor rates in LDS
 };
 
 // One SAOM period for a batch of chains (see saom_period.hip). Rows are u64
@@ -33,6 +34,14 @@ struct SimParams {
   int c0[kMaxEff], c1[kMaxEff], c2[kMaxEff];
   double p1[kMaxEff], p2[kMaxEff];
   int bty[kMaxBeff];
+  // non-basic rate effects (libcusna_saom_cpu.cpp:170-193): actor a's rate is
+  // lam * exp(clamp(sum_q ralpha[q] u_q(a), -30, 30)), u_q = ru[q*n + a]
+  // (rkind 0), out-degree (1) or in-degree (2)
+  int nrate;
+  int rkind[kMaxRate];
+  double ralpha[kMaxRate];
+  const double* ru;    // nrate*n
+  double* rscores;     // M*nrate
   const uint64_t *x0, *xs, *kd, *pm;  // n*W64 words each
   const uint64_t *inactw;             // W64 words or null
   const double *theta, *lam, *thetab, *lamb;  // chain c reads row c * stride

@@ -533,7 +533,7 @@ __device__ __forceinline__ double beh_d(int t, double dz, double zic, long long 
 
 struct LdsLayout {
   size_t rows, indeg, outdeg, z, tpall, soall, coli, rowi, tpm0, tpm1, red, tsl, dstar, sc, bsc,
-      cnt, total;
+      cnt, crate, rate, total;
 };
 
 // Word bounds above this stream the network from device memory.
@@ -567,8 +567,19 @@ __host__ __device__ inline LdsLayout lds_layout(int n, int k, int kb, int wmax, 
   L.sc = o; o += (size_t)k * 8;
   L.bsc = o; o += (size_t)(kb > 0 ? kb : 1) * 8;
   L.cnt = o; o += (size_t)nw * 8 * 8;
+  // rate effects: per-actor rates (act order), then S0, the kMaxRate
+  // rate-weighted sums, the rate scores, rho and the chosen actor
+  L.crate = o; if (flags & F_RATE) o += (size_t)n * 8;
+  L.rate = o; if (flags & F_RATE) o += (size_t)(2 * kMaxRate + 3) * 8;
   L.total = o;
   return L;
+}
+
+// Rate-effect covariate q of actor a (libcusna_saom_cpu.cpp:175-176).
+__device__ __forceinline__ double rate_cov(const SimParams& P, int q, int a, const int* outdeg,
+                                           const int* indeg) {
+  const int kd = P.rkind[q];
+  return kd == 0 ? P.ru[(size_t)q * P.n + a] : (kd == 1 ? (double)outdeg[a] : (double)indeg[a]);
 }
 
 // The period of one chain (the kernel body; see the file comment).
@@ -615,7 +626,18 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
   double* sc = (double*)(smem + L.sc);
   double* bsc = (double*)(smem + L.bsc);
   long long* cnt = (long long*)(smem + L.cnt);
-  const int flags = SIMPLE ? (P.flags & (F_TP | F_Z | F_RDEG)) : P.flags;
+  double* crate = (double*)(smem + L.crate);
+  double* rsum_s = (double*)(smem + L.rate);      // [0] S0, [1 + q] sum c * u_q
+  double* rsc_s = rsum_s + kMaxRate + 1;          // rate scores
+  double* rho_s = rsc_s + kMaxRate;               // integrated total rate
+  int* isel_s = (int*)(rho_s + 1);                // chosen actor
+#if defined(SAOM_FIXED_K) && defined(SAOM_HAS_RATE)
+  // specialised build: rate effects compiled in or out
+  constexpr int kRateMask = SAOM_HAS_RATE ? ~0 : ~F_RATE;
+#else
+  constexpr int kRateMask = ~0;
+#endif
+  const int flags = (SIMPLE ? (P.flags & (F_TP | F_Z | F_RDEG | F_RATE)) : P.flags) & kRateMask;
   const ChainLds S{rows, RS, W64, NW, wv, lane, indeg, outdeg, z, tpall, soall, tpm0, tpm1};
 
   // ---- chain set-up (libcusna_saom_cpu.cpp:146-160)
@@ -632,6 +654,10 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
   }
   for (int q = tid; q < k; q += NT) sc[q] = 0.0;
   for (int q = tid; q < kb; q += NT) bsc[q] = 0.0;
+  if ((flags & F_RATE) && tid == 0) {
+    for (int q = 0; q < kMaxRate; ++q) rsc_s[q] = 0.0;
+    *rho_s = 0.0;
+  }
   __syncthreads();
   for (int h = tid; h < n; h += NT) {
     int od = 0;
@@ -665,12 +691,75 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
     double lu0;
     if (REPLAY) lu0 = P.lu0[(size_t)chain * P.smax + steps];
     else lu0 = log(unit_open(c0));
-    t += -lu0 / ((double)nact * (lam + lamb));
-    if (!P.cond && t > 1.0) break;
-    const bool net = kb == 0 || unit_open(c1) < lam / (lam + lamb);
-    long long ix = (long long)(unit_open(c2) * nact);
-    if (ix >= nact) ix = nact - 1;
-    const int i = P.act ? P.act[ix] : (int)ix;
+    int i;
+    bool net;
+    if (flags & F_RATE) {
+      // rate effects (:170-193): per-actor rates in parallel; their sums and
+      // the cumulative actor choice in act order by single lanes, as the
+      // reference sums them (bit-identical totals)
+      const int nr = P.nrate;
+      for (int x = tid; x < nact; x += NT) {
+        const int a2 = P.act ? P.act[x] : x;
+        double sx = 0.0;
+        for (int q = 0; q < nr; ++q) {
+          const double prod = P.ralpha[q] * rate_cov(P, q, a2, outdeg, indeg);
+          sx = sx + prod;
+        }
+        if (sx > 30.0) sx = 30.0;
+        else if (sx < -30.0) sx = -30.0;
+        crate[x] = exp(sx);
+      }
+      __syncthreads();
+      if (tid <= nr) {
+        double acc = 0.0;
+        for (int x = 0; x < nact; ++x) {
+          const double cx = crate[x];
+          if (tid == 0) {
+            acc += cx;
+          } else {
+            const int a2 = P.act ? P.act[x] : x;
+            const double prod = cx * rate_cov(P, tid - 1, a2, outdeg, indeg);
+            acc += prod;
+          }
+        }
+        rsum_s[tid] = acc;
+      }
+      __syncthreads();
+      const double S0 = rsum_s[0];
+      const double tau = -lu0 / (lam * S0);
+      double tau_eff = tau;
+      if (!P.cond && t + tau > 1.0) tau_eff = 1.0 - t;
+      if (tid == 0) {
+        *rho_s += tau_eff * S0;
+        for (int q = 0; q < nr; ++q) {
+          const double dec = tau_eff * lam * rsum_s[1 + q];
+          rsc_s[q] -= dec;
+        }
+        const double tgt = unit_open(c2) * S0;
+        double acc = 0.0;
+        int sel = P.act ? P.act[nact - 1] : nact - 1;
+        for (int x = 0; x < nact; ++x) {
+          acc += crate[x];
+          if (acc >= tgt) {
+            sel = P.act ? P.act[x] : x;
+            break;
+          }
+        }
+        *isel_s = sel;
+      }
+      t += tau;
+      if (!P.cond && t > 1.0) break;
+      __syncthreads();
+      i = *isel_s;
+      net = true;
+    } else {
+      t += -lu0 / ((double)nact * (lam + lamb));
+      if (!P.cond && t > 1.0) break;
+      net = kb == 0 || unit_open(c1) < lam / (lam + lamb);
+      long long ix = (long long)(unit_open(c2) * nact);
+      if (ix >= nact) ix = nact - 1;
+      i = P.act ? P.act[ix] : (int)ix;
+    }
     auto ROWI = [&](int w) -> uint64_t { return rows[(size_t)w * RS + i]; };
 
     if (!net) {
@@ -1032,6 +1121,9 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
           outdeg[jstar] += dd;
         }
       }
+      // rate-effect scores: covariate of the ego after the step (:283-284)
+      if (flags & F_RATE)
+        for (int q = 0; q < P.nrate; ++q) rsc_s[q] += rate_cov(P, q, i, outdeg, indeg);
     }
     steps++;
     if (jstar != i && kdij) {
@@ -1081,7 +1173,10 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
     P.steps[chain] = (int32_t)steps;
     P.ntim[chain] = t;
     double* so = P.scores + (size_t)chain * (k + 1);
-    so[0] = (double)(steps - steps_beh) / lam - (double)nact;
+    so[0] = (flags & F_RATE) ? (double)(steps - steps_beh) / lam - *rho_s
+                             : (double)(steps - steps_beh) / lam - (double)nact;
+    if (flags & F_RATE)
+      for (int q = 0; q < P.nrate; ++q) P.rscores[(size_t)chain * P.nrate + q] = rsc_s[q];
     for (int e = 0; e < k; ++e) so[1 + e] = sc[e];
     if (kb > 0) {
       P.bdist[chain] = (int32_t)bdist;

@@ -452,7 +452,7 @@ def simulate_groups(problems: list, want_end_nets: bool = False) -> list:
     L = load()
     G = len(problems)
     built = [pr.build() for pr in problems]
-    res = [SaomResult(pr.n, pr.k, pr.M, pr.kb, want_end_nets) for pr in problems]
+    res = [SaomResult(pr.n, pr.k, pr.M, pr.kb, want_end_nets, nrate=pr.nrate) for pr in problems]
     parr = (_vp * G)(*[C.addressof(b) for b in built])
     rarr = (_vp * G)(*[C.addressof(r.c) for r in res])
     _check("cusna_saom_simulate_groups", L.cusna_saom_simulate_groups(parr, G, rarr))
@@ -580,7 +580,8 @@ class PeriodPlan:
         _check("cusna_period_plan_sync", self.L.cusna_period_plan_sync(self.h))
 
     def fetch(self, want_end_nets: bool = False) -> dict:
-        res = SaomResult(self.prob.n, self.prob.k, self.prob.M, self.prob.kb, want_end_nets)
+        res = SaomResult(self.prob.n, self.prob.k, self.prob.M, self.prob.kb, want_end_nets,
+                         nrate=self.prob.nrate)
         _check("cusna_period_plan_fetch", self.L.cusna_period_plan_fetch(self.h, C.byref(res.c)))
         return res.as_dict()
 

@@ -13,6 +13,10 @@ the reference sources; the fixtures, not the reference, travel).
                     cross-network statistics compiled from source, on synthetic
                     multi-network problems (every cross effect, D = 2..4,
                     symmetric networks, masks, composition, per-chain rows)
+  rate_cases.json.gz   outputs of the reference engine compiled from source on
+                    periods with non-basic rate effects (covariate, out-degree,
+                    in-degree kinds, clamping, conditional, symmetric) and
+                    composition change (active lists, restore of inactive actors)
   abi_offsets.txt   field offsets of the reference ABI structs
   targets.json.gz, effects.json.gz, estimate.json.gz, data_s50.json.gz,
   multinet.json.gz:
@@ -58,6 +62,9 @@ def main():
     # 2. reference-engine outputs on synthetic cases
     dump("ref_cases.json.gz", build_cases(O))
     from multi_cases import build_cross_cases, build_multi_cases
+    from rate_cases import build_rate_cases
+
+    dump("rate_cases.json.gz", build_rate_cases(O))
 
     dump("multi_cases.json.gz", {"multi": build_multi_cases(O)["cases"],
                                  "cross": build_cross_cases(O)["cases"]})

@@ -85,7 +85,13 @@ def test_argument_errors_without_device_work():
     p.k = 25
     assert L.cusna_saom_simulate_period(ctypes.byref(p), ctypes.byref(res.c)) == 2
     p = prob.build()
-    p.nrate = 1
+    p.nrate = 5  # above CUSNA_MAXRATE
+    assert L.cusna_saom_simulate_period(ctypes.byref(p), ctypes.byref(res.c)) == 2
+    p = prob.build()
+    p.nrate = 1  # rate effects without kinds / parameters
+    assert L.cusna_saom_simulate_period(ctypes.byref(p), ctypes.byref(res.c)) == 1
+    kinds, alpha = (ctypes.c_int32 * 1)(3), (ctypes.c_float * 1)(0.1)
+    p.rkind, p.ralpha = ctypes.addressof(kinds), ctypes.addressof(alpha)
     assert L.cusna_saom_simulate_period(ctypes.byref(p), ctypes.byref(res.c)) == 5
     # network stats: the reference returns NULL_ARG for shape errors too
     assert L.cusna_network_stats(None, 3, 1, None, None, None, None, None, None, None, None,
