@@ -115,6 +115,34 @@ class BehEffect:
         self.type_id = BEH[self.name]
 
 
+RATE_KINDS = dict(RateX=0, outRate=1, inRate=2)
+
+
+@dataclass
+class RateEffect:
+    """A non-basic rate effect (R/effects.R:120-127): RateX (covariate),
+    outRate (out-degree), inRate (in-degree)."""
+
+    name: str
+    cov: np.ndarray | None = None
+    kind: int = field(init=False)
+
+    def __post_init__(self):
+        if self.name not in RATE_KINDS:
+            raise ValueError(f"unknown rate effect {self.name}")
+        self.kind = RATE_KINDS[self.name]
+        if self.kind == 0 and self.cov is None:
+            raise ValueError("RateX requires a covariate")
+
+
+def rate_cov_values(e: RateEffect, p: int = 1) -> np.ndarray:
+    """Centred rate covariate for period p, missing values 0 (R/effects.R:168-173)."""
+    v = np.asarray(e.cov, dtype=float)
+    gmean = np.nanmean(v)
+    vm = v - gmean if v.ndim == 1 else v[:, min(p, v.shape[1]) - 1] - gmean
+    return np.where(np.isnan(vm), 0.0, vm)
+
+
 def effect_cov_values(e: Effect, p: int = 1) -> np.ndarray:
     """Centred covariate for 1-based period p; grand mean over all columns
     (R/effects.R:139-147)."""

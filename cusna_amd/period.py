@@ -18,8 +18,17 @@ from .effects import prepare_effects
 
 def period_problem(data: SaomData, effects: list, p: int, theta, lam, M: int, seed: int,
                    beh_effects: list | None = None, thetab=None, lamb=0.0,
-                   conditional: bool = False, maxdegree: int = 0) -> SaomProblem:
-    """SaomProblem for 1-based period p (R/mom.R:91-106, cusna_saom_glue.cpp:67-147)."""
+                   conditional: bool = False, maxdegree: int = 0,
+                   rate_effects: list | None = None, rate_theta=None,
+                   composition: bool = False) -> SaomProblem:
+    """SaomProblem for 1-based period p (R/mom.R:91-106, cusna_saom_glue.cpp:67-147).
+
+    Beyond what the R glue passes (cusna_saom_glue.cpp:106 leaves them unset),
+    ``composition=True`` with composition change in ``data`` sets the period's
+    active list and inactive mask (restored at the period end, libcusna_saom_cpu.cpp:290-291, 356-361),
+    and ``rate_effects`` with their parameters ``rate_theta`` set the
+    non-basic rate effects (:170-193): RateX uses the centred covariate,
+    outRate / inRate the current degrees."""
     n = data.n
     beh = data.beh
     kb = len(beh_effects) if beh_effects else 0
@@ -35,6 +44,17 @@ def period_problem(data: SaomData, effects: list, p: int, theta, lam, M: int, se
         conditional=1 if conditional else 0,
         target_dist=data.observed_distance(p) if conditional else 0,
         symflag=1 if data.symmetric else 0)
+    comp = data.period_actors(p) if composition else None
+    if comp is not None:
+        prob.act, prob.nact, prob.inact = comp
+        prob.has_comp = 1
+    if rate_effects:
+        from .effects import rate_cov_values
+
+        prob.rate_kind = np.array([e.kind for e in rate_effects], np.int32)
+        prob.rate_alpha = np.asarray(rate_theta, dtype=float)
+        prob.rate_u = np.stack([rate_cov_values(e, p) if e.kind == 0 else np.zeros(n)
+                                for e in rate_effects])
     if kb:
         prob.kb = kb
         prob.lamb = float(lamb)

@@ -18,8 +18,9 @@ the reference sources; the fixtures, not the reference, travel).
                     in-degree kinds, clamping, conditional, symmetric) and
                     composition change (active lists, restore of inactive actors)
   abi_offsets.txt   field offsets of the reference ABI structs
-  targets.json.gz, effects.json.gz, estimate.json.gz, data_s50.json.gz,
-  multinet.json.gz:
+  targets.json.gz, effects.json.gz, estimate.json.gz, multinet.json.gz,
+  data/*.json.gz (the seven data-engine datasets of test-data.R: s50, s50_beh,
+  s50miss, s50str, knecht, cow, synth):
                     copies of the reference's host-layer fixtures (inputs and
                     expected outputs)
 """
@@ -56,9 +57,12 @@ def main():
     for src, dst in [("targets/targets.json.gz", "targets.json.gz"),
                      ("effects/effects.json.gz", "effects.json.gz"),
                      ("estimate/estimate.json.gz", "estimate.json.gz"),
-                     ("data/s50.json.gz", "data_s50.json.gz"),
                      ("multinet/multinet.json.gz", "multinet.json.gz")]:
         shutil.copyfile(os.path.join(REF_FIX, src), os.path.join(HERE, dst))
+    os.makedirs(os.path.join(HERE, "data"), exist_ok=True)
+    for name in ("s50", "s50_beh", "s50miss", "s50str", "knecht", "cow", "synth"):
+        shutil.copyfile(os.path.join(REF_FIX, "data", f"{name}.json.gz"),
+                        os.path.join(HERE, "data", f"{name}.json.gz"))
     # 2. reference-engine outputs on synthetic cases
     dump("ref_cases.json.gz", build_cases(O))
     from multi_cases import build_cross_cases, build_multi_cases

@@ -91,15 +91,17 @@ def test_moment_targets_match_reference_fixture(name, oracle_lib):
 
     g = load_golden("targets.json.gz")
     cfg = [c for c in g["configs"] if c["name"] == name][0]
-    if cfg["composition_input"] is not None:
-        pytest.skip("composition change is not part of the period path")
-    if cfg["rate_effects"] is not None:
-        pytest.skip("non-basic rate effects (nrate > 0) are not part of the period path")
     covs = dict(alc_const=np.array(g["inputs"]["alc_const"], float),
                 alc_change=_mat(g["inputs"]["alc_change"]))
     waves = [_mat(w) for w in cfg["waves_input"]]
     behavior = _mat(cfg["behavior_input"]) if cfg["behavior_input"] is not None else None
-    data = build_saom_data(waves, behavior=behavior)
+    data = build_saom_data(waves, behavior=behavior, composition=cfg["composition_input"])
+    rate_effects = None
+    if cfg["rate_effects"] is not None:
+        from cusna_amd.effects import RateEffect
+
+        rate_effects = [RateEffect(r["name"], covs[r["cov"]] if "cov" in r else None)
+                        for r in cfg["rate_effects"]]
     effects = [Effect(r["name"], covs[r["cov"]] if "cov" in r else None, parm=r.get("parm"),
                       dyn=r.get("dyn", False)) for r in cfg["effects"]]
     beh_effects = [BehEffect(b) for b in cfg["beh_effects"]] if cfg["beh_effects"] else None
@@ -110,7 +112,10 @@ def test_moment_targets_match_reference_fixture(name, oracle_lib):
                                         kw["id0"], kw["wtab"], kw["symflag"])
 
     mt = mom_targets(data, effects, beh_effects, cfg["conditional"] in (True, "True"),
-                     net_stats=net, beh_stats=oracle_lib.behavior_stats)
+                     net_stats=net, beh_stats=oracle_lib.behavior_stats,
+                     rate_effects=rate_effects)
+    if o_rate := cfg["out"].get("rate_eff_targets"):
+        np.testing.assert_allclose(mt.rate_eff.sum(0), o_rate, rtol=1e-12)
     o = cfg["out"]
     np.testing.assert_allclose(mt.eval, _mat(o["eval_targets"]), rtol=1e-7)
     np.testing.assert_allclose(mt.basic_rate, o["rate_targets"], rtol=1e-7)
