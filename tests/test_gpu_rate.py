@@ -62,3 +62,30 @@ def test_rate_large_matches_oracle(n, specialize, gpu_lib, oracle_lib):
     _assert_rate_equal(got, ref, spec["name"])
     np.testing.assert_array_equal(got["a_ends"], ref["a_ends"])
     assert np.all(got["steps"] > 100)
+
+
+def test_cow_composition_with_rate_effect(gpu_lib, oracle_lib):
+    """A period built by the host layer from the reference's COW data
+    (symmetric, n = 116, composition change) with an outRate effect."""
+    import gzip
+    import json
+    import os
+
+    from conftest import GOLDEN
+    from cusna_amd.data import build_saom_data
+    from cusna_amd.effects import Effect, RateEffect
+    from cusna_amd.period import period_problem
+
+    with gzip.open(os.path.join(GOLDEN, "data", "cow.json.gz"), "rt") as f:
+        g = json.load(f)
+    mat = [np.array([[np.nan if v is None else v for v in r] for r in w], float)
+           for w in g["waves_input"]]
+    data = build_saom_data(mat, composition=g["composition_input"])
+    for p in (1, 2):
+        prob = period_problem(data, [Effect("density"), Effect("transTriads")], p, [-2.0, 0.3],
+                              3.0, 6, 11 + p, rate_effects=[RateEffect("outRate")],
+                              rate_theta=[0.05], composition=True)
+        got = gpu_lib.simulate_period(prob, want_end_nets=True)
+        ref = oracle_lib.simulate(prob, want_end_nets=True)
+        _assert_rate_equal(got, ref, f"cow p{p}")
+        np.testing.assert_array_equal(got["a_ends"], ref["a_ends"])
