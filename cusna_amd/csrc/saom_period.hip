@@ -110,6 +110,65 @@ __device__ __forceinline__ double wave_max(double v) {
               fmax(readlane_f64(v, 32), readlane_f64(v, 48)));
 }
 
+// Lane exchange with the partner that differs in lane bit B: row_mirror
+// (15 - l) for bit 3, row_half_mirror (7 - l) for bit 2, quad_perm xor 2 /
+// xor 1 for bits 1 / 0, ds_bpermute xor 16 / xor 32 for bits 4 / 5.
+template <int B>
+__device__ __forceinline__ double lane_exchange(double v) {
+  if constexpr (B == 3) return dpp_f64<0x140>(v);
+  else if constexpr (B == 2) return dpp_f64<0x141>(v);
+  else if constexpr (B == 1) return dpp_f64<0x4E>(v);
+  else if constexpr (B == 0) return dpp_f64<0xB1>(v);
+  else return __shfl_xor(v, 1 << B, 64);
+}
+
+// Transposing all-reduction of KP (a power of two, <= 32) values per lane:
+// level t exchanges half of the values still held with the partner across
+// lane bit B[t] (B = 3, 2, 1, 0, 4, 5) and keeps the other half, so after
+// log2(KP) levels each lane holds one value index; the remaining levels add
+// the partner's copy. Returns the wave total of value index
+// v(lane) = sum_t bit(lane, B[t]) << (log2(KP) - 1 - t); every lane whose
+// remaining bits are zero holds a distinct index. Costs about 7 VALU per value
+// on the first level instead of a full reduction per value.
+template <int KP, int T = 0>
+__device__ __forceinline__ double transpose_sum(double (&x)[KP], int lane) {
+  constexpr int kBits[6] = {3, 2, 1, 0, 4, 5};
+  if constexpr (T == 6) {
+    return x[0];
+  } else {
+    constexpr int B = kBits[T];
+    if constexpr (KP > 1) {
+      constexpr int H = KP / 2;
+      const bool hi = (lane >> B) & 1;
+      double y[H];
+#pragma unroll
+      for (int i = 0; i < H; ++i) {
+        const double send = hi ? x[i] : x[i + H];
+        const double keep = hi ? x[i + H] : x[i];
+        y[i] = keep + lane_exchange<B>(send);
+      }
+      return transpose_sum<H, T + 1>(y, lane);
+    } else {
+      double y[1] = {x[0] + lane_exchange<B>(x[0])};
+      return transpose_sum<1, T + 1>(y, lane);
+    }
+  }
+}
+
+// Value index held by `lane` after transpose_sum<KP> (-1: a duplicate).
+template <int KP>
+__device__ __forceinline__ int transpose_index(int lane) {
+  constexpr int kBits[6] = {3, 2, 1, 0, 4, 5};
+  int v = 0, L = 0;
+  while ((1 << L) < KP) ++L;
+  for (int t = 0; t < 6; ++t) {
+    const int b = (lane >> kBits[t]) & 1;
+    if (t < L) v = (v << 1) | b;
+    else if (b) return -1;
+  }
+  return v;
+}
+
 // lexicographic max of (g, -j): larger g wins, lower j breaks ties.
 __device__ __forceinline__ void arg_combine(double& g, int& j, double g2, int j2) {
   if (g2 > g || (g2 == g && j2 < j)) {
@@ -997,6 +1056,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
     // 5. Gumbel-max choice (:265-268) and lmax
     double gb = -1e30, lm = -1e30;
     int jb = i;
+    double shift = 0.0;  // softmax shift: lmax (replay) or the noisy maximum's bound
     if constexpr (REPLAY) {
 #pragma unroll
       for (int r = 0; r < R; ++r) {
@@ -1019,7 +1079,6 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
         if (A.j[r] == i) f[r] = 0.0;
         ga[r] = -INFINITY;
         if (!((A.ok >> r) & 1)) continue;
-        if (f[r] > lm) lm = f[r];
         ga[r] = f[r] - (double)approx_loglog(uj[r]);
         M = fmax(M, ga[r]);
       }
@@ -1038,9 +1097,14 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
           arg_combine(gb, jb, f[r] - ll, A.j[r]);
         }
       }
+      // M >= max_j f_j - 3.14 (the noise -log(-log u) is at least
+      // -log(33 ln 2) for every u >= 2^-33 the stream yields), so
+      // exp(f_j - M) <= e^3.14 and M serves as the softmax shift: the scores
+      // are ratios of the weighted sums, invariant to the shift up to rounding
+      shift = M;
     }
     wave_argmax(gb, jb);
-    lm = wave_max(lm);
+    if constexpr (REPLAY) lm = wave_max(lm);
     if (NW > 1) {
       if (lane == 0) {
         red[wv * 4 + 0] = gb;
@@ -1053,9 +1117,10 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
       jb = (int)red[2];
       for (int q = 1; q < NW; ++q) {
         arg_combine(gb, jb, red[q * 4 + 0], (int)red[q * 4 + 2]);
-        lm = fmax(lm, red[q * 4 + 1]);
+        if constexpr (REPLAY) lm = fmax(lm, red[q * 4 + 1]);
       }
     }
+    if constexpr (REPLAY) shift = lm;
     const int jstar = jb;
 
     // 6. softmax-weighted expected change statistics (:271-274)
@@ -1063,9 +1128,33 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
     double Sw = 0.0;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      wgt[r] = ((A.ok >> r) & 1) ? exp(f[r] - lm) : 0.0;
+      wgt[r] = ((A.ok >> r) & 1) ? exp(f[r] - shift) : 0.0;
       Sw += wgt[r];
     }
+#if defined(SAOM_FIXED_K) && SAOM_CACHE_D
+    {
+      // all k + 1 sums in one transposing reduction
+      constexpr int KP = SAOM_FIXED_K + 1 <= 2 ? 2 : SAOM_FIXED_K + 1 <= 4 ? 4
+                       : SAOM_FIXED_K + 1 <= 8 ? 8 : SAOM_FIXED_K + 1 <= 16 ? 16 : 32;
+      double x[KP];
+#pragma unroll
+      for (int e = 0; e < KP; ++e) x[e] = 0.0;
+#pragma unroll
+      for (int e = 0; e < SAOM_FIXED_K; ++e) {
+        double T = 0.0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          T += wgt[r] * dc[e][r];
+          if (((A.in >> r) & 1) && A.j[r] == jstar) dstar[e] = dc[e][r];
+        }
+        x[e] = T;
+      }
+      x[SAOM_FIXED_K] = Sw;
+      const double tot = transpose_sum<KP>(x, lane);
+      const int v = transpose_index<KP>(lane);
+      if (v >= 0 && v <= SAOM_FIXED_K) tsl[wv * (k + 1) + v] = tot;
+    }
+#else
     Sw = wave_sum(Sw);
     if (lane == 0) tsl[wv * (k + 1) + k] = Sw;
 #ifdef SAOM_FIXED_K
@@ -1092,6 +1181,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
       T = wave_sum(T);
       if (lane == 0) tsl[wv * (k + 1) + e] = T;
     }
+#endif
     // 7. state of the chosen dyad before the toggle (:277-281)
     int aij = 0, x0ij = 0, kdij = 0;
     if (jstar != i) {
