@@ -199,8 +199,10 @@ __device__ __forceinline__ void wave_argmax(double& g, int& j) {
 }
 
 // Word 0 of the Philox4x32-10 blocks (c0, c1, 1, j[r]) of the lane's R
-// alternatives: the round loop stays rolled (two key registers) and the R
-// independent blocks are interleaved inside it.
+// alternatives. The rounds are unrolled with the R independent blocks
+// interleaved: the key schedule and the parts of the first rounds that do not
+// depend on j (c0, c1 and the key are wave-uniform) then stay in scalar
+// registers, and no register rotation is needed between rounds.
 template <int R>
 __device__ __forceinline__ void philox_alternatives(uint32_t c0, uint32_t c1, const int (&j)[R],
                                                     uint32_t k0, uint32_t k1, uint32_t (&u)[R]) {
@@ -212,7 +214,7 @@ __device__ __forceinline__ void philox_alternatives(uint32_t c0, uint32_t c1, co
     c[r] = 1u;
     d[r] = (uint32_t)j[r];
   }
-#pragma unroll 1
+#pragma unroll
   for (int round = 0; round < 9; ++round) {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -655,8 +657,11 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
   // the Philox key comes from the launch (one group) or the group's
   // parameters, or from device memory when a captured graph replays the
   // launch with keys a device kernel has written
-  const uint32_t key0 = P.keys ? P.keys[0] : (items ? P.key0 : key0_arg);
-  const uint32_t key1 = P.keys ? P.keys[1] : (items ? P.key1 : key1_arg);
+  // wave-uniform: keep the key in scalar registers
+  const uint32_t key0 = __builtin_amdgcn_readfirstlane(
+      P.keys ? P.keys[0] : (items ? P.key0 : key0_arg));
+  const uint32_t key1 = __builtin_amdgcn_readfirstlane(
+      P.keys ? P.keys[1] : (items ? P.key1 : key1_arg));
   constexpr int NT = 64 * NW;
   extern __shared__ __align__(16) unsigned char smem[];
   // word-major bit rows: word w of actor h at rows[w * RS + h], RS = n, so the
