@@ -219,7 +219,8 @@ __global__ void __launch_bounds__(64 * NW)
   double lam_tot = 0.0;
   for (int d = 0; d < D; ++d) lam_tot += lam[d];
   const int nact = P.nact;
-  const uint32_t key0 = P.key0, key1 = P.key1;
+  const uint32_t key0 = __builtin_amdgcn_readfirstlane(P.key0);
+  const uint32_t key1 = __builtin_amdgcn_readfirstlane(P.key1);
   long long steps = 0;
   double t = 0.0;
 
@@ -361,11 +362,13 @@ __global__ void __launch_bounds__(64 * NW)
       }
     }
     // 4. Gumbel-max choice with the float32 bound (as the single-network kernel)
-    double gb = -1e30, lm = -1e30;
+    double gb = -1e30, shift = 0.0;
     int jb = i;
     {
       uint32_t uj[R];
-      philox_alternatives<R>((uint32_t)steps, (uint32_t)chain, A.j, key0, key1, uj);
+      philox_alternatives<R>(__builtin_amdgcn_readfirstlane((uint32_t)steps),
+                             __builtin_amdgcn_readfirstlane((uint32_t)chain), A.j, key0, key1,
+                             uj);
       double ga[R];
       double Mx = -INFINITY;
 #pragma unroll
@@ -373,7 +376,6 @@ __global__ void __launch_bounds__(64 * NW)
         if (A.j[r] == i) f[r] = 0.0;
         ga[r] = -INFINITY;
         if (!((A.ok >> r) & 1)) continue;
-        if (f[r] > lm) lm = f[r];
         ga[r] = f[r] - (double)approx_loglog(uj[r]);
         Mx = fmax(Mx, ga[r]);
       }
@@ -392,23 +394,20 @@ __global__ void __launch_bounds__(64 * NW)
           arg_combine(gb, jb, f[r] - ll, A.j[r]);
         }
       }
+      // the noise bound's maximum is within 3.14 of max_j f_j: the softmax
+      // shift (as in the single-network kernel)
+      shift = Mx;
     }
     wave_argmax(gb, jb);
-    lm = wave_max(lm);
     if (NW > 1) {
       if (lane == 0) {
         red[wv * 4 + 0] = gb;
-        red[wv * 4 + 1] = lm;
         red[wv * 4 + 2] = (double)jb;
       }
       __syncthreads();
       gb = red[0];
-      lm = red[1];
       jb = (int)red[2];
-      for (int q = 1; q < NW; ++q) {
-        arg_combine(gb, jb, red[q * 4 + 0], (int)red[q * 4 + 2]);
-        lm = fmax(lm, red[q * 4 + 1]);
-      }
+      for (int q = 1; q < NW; ++q) arg_combine(gb, jb, red[q * 4 + 0], (int)red[q * 4 + 2]);
     }
     const int jstar = jb;
 
@@ -417,7 +416,7 @@ __global__ void __launch_bounds__(64 * NW)
     double Sw = 0.0;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      wgt[r] = ((A.ok >> r) & 1) ? exp(f[r] - lm) : 0.0;
+      wgt[r] = ((A.ok >> r) & 1) ? exp(f[r] - shift) : 0.0;
       Sw += wgt[r];
     }
     Sw = wave_sum(Sw);

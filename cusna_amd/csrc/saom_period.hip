@@ -1076,7 +1076,9 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
       // exact float64 noise only where g_j can reach the maximum: every j
       // with the largest exact g (ties included) satisfies approx >= M - 2e.
       uint32_t uj[R];
-      philox_alternatives<R>((uint32_t)steps, (uint32_t)chain, A.j, key0, key1, uj);
+      philox_alternatives<R>(__builtin_amdgcn_readfirstlane((uint32_t)steps),
+                             __builtin_amdgcn_readfirstlane((uint32_t)chain), A.j, key0, key1,
+                             uj);
       double ga[R];
       double M = -INFINITY;
 #pragma unroll
