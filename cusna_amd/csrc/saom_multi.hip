@@ -291,7 +291,7 @@ __global__ void __launch_bounds__(64 * NW)
     if (flags & F_TP) {
 #pragma unroll
       for (int wd = 0; wd < U.words(); ++wd) {
-        uint64_t m = U.rowi(wd);
+        uint64_t m = uniform64(U.rowi(wd));  // scalar loop over i's out-ties
         while (m) {
           const int h = wd * 64 + __ffsll((long long)m) - 1;
           m &= m - 1;

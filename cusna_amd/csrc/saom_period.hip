@@ -59,6 +59,14 @@ __device__ __forceinline__ int eff_dyn(const SimParams& P, int e) { return P.dyn
 
 __device__ __forceinline__ int popc64(uint64_t x) { return __popcll(x); }
 
+// A wave-uniform 64-bit value moved to scalar registers (loops over its set
+// bits then run on the scalar unit).
+__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
 // Wave64 all-reductions: four DPP steps combine within each row of 16 lanes
 // (quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror, row_mirror), then
 // the four row results are read into scalar registers and combined. Every lane
@@ -969,7 +977,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
     if (flags & F_TP) {
 #pragma unroll
       for (int wd = 0; wd < U.words(); ++wd) {
-        uint64_t m = U.rowi(wd);
+        uint64_t m = uniform64(U.rowi(wd));  // scalar loop over i's out-ties
         while (m) {
           const int h = wd * 64 + __ffsll((long long)m) - 1;
           m &= m - 1;
