@@ -114,6 +114,95 @@ def groups_measure(lib) -> dict:
             "streamed_groups": sum(1 for v in ns if v > 1024)}
 
 
+def gof_measure(lib, plan, x0, cpu: bool) -> dict:
+    """sienaGOF distributions of the 4096 end networks of the last config-2
+    period, where they lie in device memory (cusna_period_plan_gof): in- and
+    out-degree counts at levels 0..10 and geodesic counts at levels 1..5.
+    CPU leg: the reference's cusna_gof_distribution (single-threaded, as the
+    reference runs it) on 4 end networks of the same problem."""
+    import numpy as np
+
+    from cusna_amd.workloads import c2_problem
+
+    res = {"workload": "GoF of 4096 config-2 end networks (n=500): in/out-degree levels "
+                       "0..10, geodesic levels 1..5"}
+    plan.gof(0, range(11))  # code objects loaded
+    for name, kind, lv in (("indegree", 0, range(11)), ("outdegree", 1, range(11)),
+                           ("geodesic", 2, range(1, 6))):
+        t0 = time.perf_counter()
+        plan.gof(kind, lv, masked=True)
+        w = time.perf_counter() - t0
+        res[name] = {"wall_ms": round(1e3 * w, 2), "networks_per_s": round(4096 / w, 1)}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+
+        R = O.reference()
+        if R is not None:
+            ends = lib.simulate_period(c2_problem(M=4, x0=x0, period_seed=11),
+                                       want_end_nets=True)["a_ends"]
+            t0 = time.perf_counter()
+            O.gof_distribution(ends, 2, list(range(1, 6)), lib=R, fn="cusna_gof_distribution")
+            cw = time.perf_counter() - t0
+            t1 = time.perf_counter()
+            O.gof_distribution(ends, 0, list(range(11)), lib=R, fn="cusna_gof_distribution")
+            cd = time.perf_counter() - t1
+            res["cpu"] = {"kind": "reference", "cores": 1,
+                          "geodesic_networks_per_s": round(4 / cw, 3),
+                          "indegree_networks_per_s": round(4 / cd, 1),
+                          "sample": f"4 end networks of the same problem, {cw:.1f} s geodesic, "
+                                    f"{cd:.3f} s in-degree"}
+    return res
+
+
+def rate_measure(lib, x0, cpu: bool, threads: int) -> dict:
+    """Config-2 period with two non-basic rate effects (outRate and a
+    covariate RateX), 4096 chains: plan run (launch + completion) after one
+    warm-up; CPU leg: the reference engine on a bounded sample."""
+    import numpy as np
+
+    from cusna_amd.workloads import c2_problem
+
+    def prob(M):
+        pr = c2_problem(M=M, x0=x0, period_seed=11)
+        pr.rate_kind = np.array([1, 0], np.int32)
+        pr.rate_alpha = np.array([0.05, 0.2])
+        pr.rate_u = np.stack([np.zeros(pr.n), np.asarray(pr.vmat)[4]])
+        pr.lam = 0.6
+        return pr
+
+    plan = lib.PeriodPlan(prob(4096))
+    plan.run(1)
+    plan.sync()
+    t0 = time.perf_counter()
+    for s in range(2):
+        plan.run(100 + s)
+        plan.sync()
+    wall = (time.perf_counter() - t0) / 2
+    out = plan.fetch()
+    res = {"workload": "config-2 period (n=500, 4096 chains) with outRate and RateX rate "
+                       "effects, basic rate 0.6", "ms_per_period": round(1e3 * wall, 2),
+           "chain_periods_per_s": round(4096 / wall, 1),
+           "mean_ministeps": round(float(out["steps"].mean()), 1),
+           "specialised": plan.specialized}
+    plan.close()
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+
+        if O.reference() is not None:
+            os.environ["OMP_NUM_THREADS"] = str(threads)
+            chains = 2 * threads
+            t0 = time.perf_counter()
+            O.simulate_reference(prob(chains))
+            cw = time.perf_counter() - t0
+            res["cpu"] = {"kind": "reference", "cores": threads,
+                          "chain_periods_per_s": round(chains / cw, 3),
+                          "sample": f"{chains} chains of the same problem, {threads} threads, "
+                                    f"{cw:.1f} s wall"}
+    return res
+
+
 def multi_problem_c2(M: int, x0s, seed: int = 77):
     """Two co-evolving networks of 500 actors (the config-2 waves as starts),
     rate 4 each, density/recip/transTrip/crprod on network 0 and
@@ -278,6 +367,8 @@ def main() -> int:
     ap.add_argument("--no-mom", action="store_true")
     ap.add_argument("--no-groups", action="store_true")
     ap.add_argument("--no-multi", action="store_true")
+    ap.add_argument("--no-gof", action="store_true")
+    ap.add_argument("--no-rate", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has one GPU, and the "
@@ -320,6 +411,7 @@ def main() -> int:
         pl.sync()
     elapsed = time.perf_counter() - t0
     outs = [pl.fetch() for pl in plans]
+    gof = None if args.no_gof else gof_measure(lib, plans[1], w1, not args.no_cpu)
     chain_periods = 2 * M * args.steps
     value = chain_periods / elapsed
     # algorithmic work of the last launches: alternatives = ministeps x n
@@ -352,6 +444,11 @@ def main() -> int:
             lambda m: c2_problem(M=m, n=n, x0=w1, period_seed=11), threads, chains)
     for pl in plans:
         pl.close()
+    if gof is not None:
+        line["gof"] = gof
+    if not args.no_rate:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+        line["rate"] = rate_measure(lib, w1, not args.no_cpu, threads)
     if not args.no_multi:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
         line["multi"] = multi_measure(lib, (w1, w2), not args.no_cpu, threads)

@@ -249,7 +249,8 @@ cusna_status plan_create(const cusna_saom_problem* pb, bool want_dense, const do
   P.flags = effect_flags(pb->type_ids, k, kb, pb->dyn_flags);
   P.nrate = pb->nrate;
   if (P.nrate > 0) {
-    P.flags |= saom::F_RATE;
+    // rate effects; the count rides in bits 16-18 for the LDS layout
+    P.flags |= saom::F_RATE | (P.nrate << 16);
     for (int q = 0; q < P.nrate; ++q) {
       P.rkind[q] = pb->rkind[q];
       P.ralpha[q] = (double)pb->ralpha[q];

@@ -17,7 +17,7 @@ enum : int {
   F_SOALL = 1 << 5,  // shared out-ties of every actor with i in LDS (gwdspFB)
   F_Z = 1 << 6,      // behaviour values live in LDS
   F_RDEG = 1 << 7,   // reciprocated degree of the ego (reciAct)
-  F_RATE = 1 << 8,   // non-basic rate effects: per-actor rates in LDS
+  F_RATE = 1 << 8,   // non-basic rate effects: per-actor rates in LDS (count in bits 16-18)
 };
 
 // One SAOM period for a batch of chains (see saom_period.hip). Rows are u64

@@ -602,7 +602,7 @@ __device__ __forceinline__ double beh_d(int t, double dz, double zic, long long 
 
 struct LdsLayout {
   size_t rows, indeg, outdeg, z, tpall, soall, coli, rowi, tpm0, tpm1, red, tsl, dstar, sc, bsc,
-      cnt, crate, rate, total;
+      cnt, crate, ract, ru, rate, total;
 };
 
 // Word bounds above this stream the network from device memory.
@@ -636,19 +636,24 @@ __host__ __device__ inline LdsLayout lds_layout(int n, int k, int kb, int wmax, 
   L.sc = o; o += (size_t)k * 8;
   L.bsc = o; o += (size_t)(kb > 0 ? kb : 1) * 8;
   L.cnt = o; o += (size_t)nw * 8 * 8;
-  // rate effects: per-actor rates (act order), then S0, the kMaxRate
-  // rate-weighted sums, the rate scores, rho and the chosen actor
+  // rate effects: per-actor rates in act order (then their prefix sums),
+  // the active list, the rate covariates (nrate rows), then S0, the
+  // rate-weighted sums, the rate scores and rho
+  const int nrate = (flags & F_RATE) ? (flags >> 16) & 7 : 0;
   L.crate = o; if (flags & F_RATE) o += (size_t)n * 8;
+  L.ru = o; o += (size_t)nrate * n * 8;
+  L.ract = o; if (flags & F_RATE) o += (size_t)n * 4;
+  o = al8(o);
   L.rate = o; if (flags & F_RATE) o += (size_t)(2 * kMaxRate + 3) * 8;
   L.total = o;
   return L;
 }
 
-// Rate-effect covariate q of actor a (libcusna_saom_cpu.cpp:175-176).
-__device__ __forceinline__ double rate_cov(const SimParams& P, int q, int a, const int* outdeg,
+// Rate-effect covariate of kind kd (0 covariate row ru, 1 out-degree,
+// 2 in-degree) of actor a (libcusna_saom_cpu.cpp:175-176).
+__device__ __forceinline__ double rate_cov(int kd, const double* ru, int a, const int* outdeg,
                                            const int* indeg) {
-  const int kd = P.rkind[q];
-  return kd == 0 ? P.ru[(size_t)q * P.n + a] : (kd == 1 ? (double)outdeg[a] : (double)indeg[a]);
+  return kd == 0 ? ru[a] : (kd == 1 ? (double)outdeg[a] : (double)indeg[a]);
 }
 
 // The period of one chain (the kernel body; see the file comment).
@@ -698,11 +703,12 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
   double* sc = (double*)(smem + L.sc);
   double* bsc = (double*)(smem + L.bsc);
   long long* cnt = (long long*)(smem + L.cnt);
-  double* crate = (double*)(smem + L.crate);
+  double* crate = (double*)(smem + L.crate);    // rates, then their prefix sums
+  double* ru_s = (double*)(smem + L.ru);         // rate covariates, q * n + actor
+  int* ract = (int*)(smem + L.ract);             // active actors
   double* rsum_s = (double*)(smem + L.rate);      // [0] S0, [1 + q] sum c * u_q
   double* rsc_s = rsum_s + kMaxRate + 1;          // rate scores
   double* rho_s = rsc_s + kMaxRate;               // integrated total rate
-  int* isel_s = (int*)(rho_s + 1);                // chosen actor
 #if defined(SAOM_FIXED_K) && defined(SAOM_HAS_RATE)
   // specialised build: rate effects compiled in or out
   constexpr int kRateMask = SAOM_HAS_RATE ? ~0 : ~F_RATE;
@@ -726,9 +732,15 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
   }
   for (int q = tid; q < k; q += NT) sc[q] = 0.0;
   for (int q = tid; q < kb; q += NT) bsc[q] = 0.0;
-  if ((flags & F_RATE) && tid == 0) {
-    for (int q = 0; q < kMaxRate; ++q) rsc_s[q] = 0.0;
-    *rho_s = 0.0;
+  if (flags & F_RATE) {
+    if (tid == 0) {
+      for (int q = 0; q < kMaxRate; ++q) rsc_s[q] = 0.0;
+      *rho_s = 0.0;
+    }
+    for (int x = tid; x < n; x += NT) {
+      ract[x] = P.act ? P.act[x] : x;
+      for (int q = 0; q < P.nrate; ++q) ru_s[(size_t)q * n + x] = P.ru ? P.ru[(size_t)q * n + x] : 0.0;
+    }
   }
   __syncthreads();
   for (int h = tid; h < n; h += NT) {
@@ -766,15 +778,18 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
     int i;
     bool net;
     if (flags & F_RATE) {
-      // rate effects (:170-193): per-actor rates in parallel; their sums and
-      // the cumulative actor choice in act order by single lanes, as the
-      // reference sums them (bit-identical totals)
+      // rate effects (:170-193): per-actor rates in parallel; their sums by
+      // single lanes in act order (lane 0: S0, writing the running sums over
+      // the rates as it goes; lane 1 + q: sum of c * u_q), as the reference
+      // sums them; the ego is the first actor whose running sum reaches
+      // u * S0, found by counting the running sums below it in parallel
       const int nr = P.nrate;
       for (int x = tid; x < nact; x += NT) {
-        const int a2 = P.act ? P.act[x] : x;
+        const int a2 = ract[x];
         double sx = 0.0;
         for (int q = 0; q < nr; ++q) {
-          const double prod = P.ralpha[q] * rate_cov(P, q, a2, outdeg, indeg);
+          const double prod = P.ralpha[q] * rate_cov(P.rkind[q], ru_s + (size_t)q * n, a2,
+                                                     outdeg, indeg);
           sx = sx + prod;
         }
         if (sx > 30.0) sx = 30.0;
@@ -783,16 +798,36 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
       }
       __syncthreads();
       if (tid <= nr) {
+        // branch-free so the loads of eight actors overlap: lane 0 weighs
+        // the rates by 1 (exact), lane 1 + q by u_q
+        const int kq = tid > 0 ? P.rkind[tid - 1] : -1;
+        const double* ruq = ru_s + (size_t)(tid > 0 ? tid - 1 : 0) * n;
+        auto weight = [&](int a2) -> double {
+          if (kq < 0) return 1.0;
+          const double c0 = ruq[a2];
+          const int dg = kq == 1 ? outdeg[a2] : indeg[a2];
+          return kq == 0 ? c0 : (double)dg;
+        };
         double acc = 0.0;
-        for (int x = 0; x < nact; ++x) {
-          const double cx = crate[x];
-          if (tid == 0) {
-            acc += cx;
-          } else {
-            const int a2 = P.act ? P.act[x] : x;
-            const double prod = cx * rate_cov(P, tid - 1, a2, outdeg, indeg);
-            acc += prod;
+        int x = 0;
+        for (; x + 8 <= nact; x += 8) {
+          double cx[8], ux[8];
+#pragma unroll
+          for (int m = 0; m < 8; ++m) {
+            cx[m] = crate[x + m];
+            ux[m] = weight(ract[x + m]);
           }
+#pragma unroll
+          for (int m = 0; m < 8; ++m) {
+            const double prod = cx[m] * ux[m];
+            acc += prod;
+            if (tid == 0) crate[x + m] = acc;
+          }
+        }
+        for (; x < nact; ++x) {
+          const double prod = crate[x] * weight(ract[x]);
+          acc += prod;
+          if (tid == 0) crate[x] = acc;
         }
         rsum_s[tid] = acc;
       }
@@ -807,22 +842,20 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
           const double dec = tau_eff * lam * rsum_s[1 + q];
           rsc_s[q] -= dec;
         }
-        const double tgt = unit_open(c2) * S0;
-        double acc = 0.0;
-        int sel = P.act ? P.act[nact - 1] : nact - 1;
-        for (int x = 0; x < nact; ++x) {
-          acc += crate[x];
-          if (acc >= tgt) {
-            sel = P.act ? P.act[x] : x;
-            break;
-          }
-        }
-        *isel_s = sel;
       }
       t += tau;
       if (!P.cond && t > 1.0) break;
-      __syncthreads();
-      i = *isel_s;
+      const double tgt = unit_open(c2) * S0;
+      long long below = 0;  // running sums are non-decreasing: count those below tgt
+      for (int x = tid; x < nact; x += NT) below += crate[x] < tgt ? 1 : 0;
+      below = wave_sum_ll(below);
+      if (NW > 1) {
+        if (lane == 0) cnt[wv * 8 + 5] = below;
+        __syncthreads();
+        below = 0;
+        for (int q = 0; q < NW; ++q) below += cnt[q * 8 + 5];
+      }
+      i = ract[below < nact ? (int)below : nact - 1];
       net = true;
     } else {
       t += -lu0 / ((double)nact * (lam + lamb));
@@ -1228,7 +1261,8 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
       }
       // rate-effect scores: covariate of the ego after the step (:283-284)
       if (flags & F_RATE)
-        for (int q = 0; q < P.nrate; ++q) rsc_s[q] += rate_cov(P, q, i, outdeg, indeg);
+        for (int q = 0; q < P.nrate; ++q)
+          rsc_s[q] += rate_cov(P.rkind[q], ru_s + (size_t)q * n, i, outdeg, indeg);
     }
     steps++;
     if (jstar != i && kdij) {
