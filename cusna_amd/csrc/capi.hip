@@ -608,8 +608,10 @@ cusna_status simulate_groups(const cusna_saom_problem* const* pbs, int G,
     if (st != CUSNA_OK) return st;
   }
   // buckets of one launch shape
+  // (one word count per bucket, so a specialised launch can use the exact
+  // word bound of its networks)
   struct Bucket {
-    int nw, wmax;
+    int nw, wmax, W64;
     bool simple;
     std::vector<int> groups;
   };
@@ -619,9 +621,9 @@ cusna_status simulate_groups(const cusna_saom_problem* const* pbs, int G,
     const bool simple = saom::simple_effects(pl->P.tid, pl->P.k);
     Bucket* b = nullptr;
     for (auto& x : buckets)
-      if (x.nw == pl->nw && x.wmax == pl->wmax && x.simple == simple) b = &x;
+      if (x.nw == pl->nw && x.wmax == pl->wmax && x.W64 == pl->W64 && x.simple == simple) b = &x;
     if (!b) {
-      buckets.push_back(Bucket{pl->nw, pl->wmax, simple, {}});
+      buckets.push_back(Bucket{pl->nw, pl->wmax, pl->W64, simple, {}});
       b = &buckets.back();
     }
     b->groups.push_back(g);
@@ -649,10 +651,12 @@ cusna_status simulate_groups(const cusna_saom_problem* const* pbs, int G,
     std::stable_sort(b.groups.begin(), b.groups.end(),
                      [&](int a, int c) { return cost(a) > cost(c); });
     const size_t first = items.size();
-    size_t lds = 0;
+    size_t lds = 0, lds_exact = 0;
+    const saom::LaunchShape ex = saom::exact_shape(b.W64, saom::LaunchShape{b.nw, b.wmax});
     for (int g : b.groups) {
       for (int c = 0; c < plans[g]->M; ++c) items.push_back(((uint32_t)g << 20) | (uint32_t)c);
       lds = std::max(lds, saom::sim_lds_bytes(plans[g]->P, b.nw, b.wmax));
+      lds_exact = std::max(lds_exact, saom::sim_lds_bytes(plans[g]->P, ex.nw, ex.wmax));
     }
     saom::PeriodLaunch L;
     L.Pd = D.at<saom::SimParams>(o_grp);
@@ -668,6 +672,8 @@ cusna_status simulate_groups(const cusna_saom_problem* const* pbs, int G,
     }
     L.Ph = &plans[b.groups[0]]->P;
     L.specialize = uniform && saom::specialize_policy(work);
+    L.exact = ex;
+    L.lds_exact = lds_exact;
     HIP_TRY(hipMemcpyAsync(D.at<uint32_t>(o_items) + first, items.data() + first,
                            (items.size() - first) * 4, hipMemcpyHostToDevice, s));
     const int le =

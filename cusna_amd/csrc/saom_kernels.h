@@ -94,7 +94,18 @@ struct PeriodLaunch {
   const SimParams* Ph = nullptr;
   bool specialize = false;
   bool* used = nullptr;
+  // shape and LDS bytes of the specialised kernel (exact_shape); nw = 0: the
+  // generic shape
+  LaunchShape exact{0, 0};
+  size_t lds_exact = 0;
 };
+// Word bound and block shape of a specialised kernel for networks of W64
+// words: the generic shape rounds the word bound up to a power of two; the
+// specialised kernel is compiled for any bound, so it pads only to a multiple
+// of the waves per chain (R words per lane): R = ceil(W64 / nw) with the
+// generic waves up to 8 words, two words per lane from 9 words (more waves,
+// less LDS padding), and at most 16 waves for streamed networks.
+LaunchShape exact_shape(int W64, LaunchShape generic);
 int launch_period_ex(const PeriodLaunch& L, LaunchShape s, bool replay, bool simple,
                      void* stream);
 

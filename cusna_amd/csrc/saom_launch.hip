@@ -34,16 +34,36 @@ bool simple_effects(const int* t, int k) {
   return true;
 }
 
+LaunchShape exact_shape(int W64, LaunchShape g) {
+  int nw, R;
+  if (W64 <= 8) {
+    nw = g.nw;
+    R = (W64 + nw - 1) / nw;
+  } else if (W64 <= 32) {
+    R = 2;
+    nw = (W64 + 1) / 2;
+  } else {
+    R = (W64 + 15) / 16;
+    nw = (W64 + R - 1) / R;
+  }
+  return LaunchShape{nw, nw * R};
+}
+
 int launch_period_ex(const PeriodLaunch& L, LaunchShape s, bool replay, bool simple,
                      void* stream) {
   if (L.grid < 1) return 0;
   if (L.used) *L.used = false;
-  // the specialised kernel (module API); a launch the runtime refuses (for
-  // instance for its dynamic LDS size) falls back to the generic kernel
+  // the specialised kernel (module API), in its exact shape when the caller
+  // gave one; a launch the runtime refuses (for instance for its dynamic LDS
+  // size) falls back to the generic kernel
   if (L.specialize && L.Ph) {
-    void* fn = rtc_period_kernel(*L.Ph, s, replay, simple);
+    const bool ex = L.exact.nw > 0 && L.lds_exact <= 160 * 1024;
+    const LaunchShape rs = ex ? L.exact : s;
+    void* fn = rtc_period_kernel(*L.Ph, rs, replay, simple);
     if (fn) {
-      const int e = launch_period_rtc(fn, L, s.nw, stream);
+      PeriodLaunch Lr = L;
+      if (ex) Lr.lds = L.lds_exact;
+      const int e = launch_period_rtc(fn, Lr, rs.nw, stream);
       if (e == 0) {
         if (L.used) *L.used = true;
         return 0;
@@ -104,6 +124,8 @@ int launch_period(const SimParams& P, const SimParams* Pd, LaunchShape s, bool r
   L.Ph = &P;
   L.specialize = specialize;
   L.used = used;
+  L.exact = exact_shape(P.W64, s);
+  L.lds_exact = sim_lds_bytes(P, L.exact.nw, L.exact.wmax);
   return launch_period_ex(L, s, replay, simple_effects(P.tid, P.k), stream);
 }
 
