@@ -67,12 +67,16 @@ typedef __hip_internal::int64_t int64_t;
 """
 
 
-def write_rtc_source() -> str:
-    """Generate csrc/rtc_source.inc (a raw string literal of the kernel source)."""
+# The multi-network kernel's source: the period source's helpers (no period
+# kernel) followed by saom_multi.hip.
+RTC_MULTI_SOURCES = RTC_SOURCES + ["saom_multi.hip"]
+
+
+def _rtc_body(files: list, preamble: str) -> str:
     import re
 
-    parts = [RTC_PREAMBLE]
-    for f in RTC_SOURCES:
+    parts = [preamble]
+    for f in files:
         with open(os.path.join(CSRC, f)) as fh:
             src = fh.read()
         src = re.sub(r"^#pragma once\s*$", "", src, flags=re.M)
@@ -81,9 +85,18 @@ def write_rtc_source() -> str:
     body = "\n".join(parts)
     if ")CUSNASRC" in body:
         raise RuntimeError("kernel source contains the raw-string delimiter")
+    return body
+
+
+def write_rtc_source() -> str:
+    """Generate csrc/rtc_source.inc (raw string literals of the kernel sources)."""
     out = os.path.join(CSRC, "rtc_source.inc")
-    text = ("// generated by cusna_amd/build.py from " + ", ".join(RTC_SOURCES) + "\n"
-            'static const char kRtcPeriodSource[] = R"CUSNASRC(' + body + ')CUSNASRC";\n')
+    text = ("// generated by cusna_amd/build.py from " + ", ".join(RTC_MULTI_SOURCES) + "\n"
+            'static const char kRtcPeriodSource[] = R"CUSNASRC('
+            + _rtc_body(RTC_SOURCES, RTC_PREAMBLE) + ')CUSNASRC";\n'
+            'static const char kRtcMultiSource[] = R"CUSNASRC('
+            + _rtc_body(RTC_MULTI_SOURCES, RTC_PREAMBLE + "#define SAOM_HELPERS_ONLY 1\n")
+            + ')CUSNASRC";\n')
     old = open(out).read() if os.path.exists(out) else None
     if old != text:
         with open(out, "w") as fh:

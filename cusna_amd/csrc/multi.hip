@@ -37,6 +37,7 @@ struct cusna_multi_plan {
   int n = 0, k = 0, D = 0, M = 0, W64 = 0, wmax = 0, nw = 0, nd = 0;
   std::vector<std::unique_ptr<MultiStats>> stats;
   bool want_dense = false;
+  bool last_rtc = false;  // the last run launched the specialised kernel
   hipStream_t s = nullptr;
   DeviceBlock Dv;
   saom::MultiParams P;  // host copy (device pointers inside)
@@ -250,7 +251,20 @@ cusna_status multi_plan_run(cusna_multi_plan* pl, uint64_t seed) {
   Q.base.key0 = (uint32_t)(seed & 0xFFFFFFFFu);
   Q.base.key1 = (uint32_t)(seed >> 32);
   HIP_TRY(hipMemcpyAsync(pl->Pd, &Q, sizeof(Q), hipMemcpyHostToDevice, pl->s));
-  const int e = saom::launch_multi(Q, pl->Pd, pl->nw, pl->wmax, pl->s);
+  // the kernel specialised to the effect list when the launch is large
+  // (CUSNA_SPECIALIZE=0/1 overrides); the generic kernel otherwise or when
+  // the specialised one is unavailable
+  pl->last_rtc = false;
+  if (saom::specialize_policy((long long)pl->M * pl->n)) {
+    void* fn = saom::rtc_multi_kernel(Q, pl->nw, pl->wmax);
+    if (fn && saom::launch_multi_rtc(fn, pl->Pd, pl->M, saom::multi_lds_bytes(Q, pl->nw, pl->wmax),
+                                     pl->nw, pl->s) == 0) {
+      pl->last_rtc = true;
+    } else {
+      (void)hipGetLastError();
+    }
+  }
+  const int e = pl->last_rtc ? 0 : saom::launch_multi(Q, pl->Pd, pl->nw, pl->wmax, pl->s);
   if (e != 0) {
     set_error(std::string("multi-network period launch: ") + hipGetErrorString((hipError_t)e));
     return CUSNA_ERR_CUDA;
@@ -606,6 +620,10 @@ CUSNA_API cusna_status cusna_multi_plan_stats(cusna_multi_plan* plan, double* ou
     }
   }
   return CUSNA_OK;
+}
+
+CUSNA_API int32_t cusna_multi_plan_specialized(const cusna_multi_plan* plan) {
+  return plan && plan->last_rtc ? 1 : 0;
 }
 
 CUSNA_API cusna_status cusna_multi_plan_destroy(cusna_multi_plan* plan) {

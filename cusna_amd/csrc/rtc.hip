@@ -93,10 +93,10 @@ std::string list(const int* v, int k) {
   return s;
 }
 
-bool compile(const std::vector<std::string>& opts, std::vector<char>& code) {
+bool compile(const char* source, const std::vector<std::string>& opts,
+             std::vector<char>& code) {
   hiprtcProgram prog;
-  if (hiprtcCreateProgram(&prog, kRtcPeriodSource, "saom_period_rtc.hip", 0, nullptr,
-                          nullptr) != HIPRTC_SUCCESS)
+  if (hiprtcCreateProgram(&prog, source, "saom_rtc.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS)
     return false;
   std::vector<const char*> o;
   for (const auto& x : opts) o.push_back(x.c_str());
@@ -109,6 +109,51 @@ bool compile(const std::vector<std::string>& opts, std::vector<char>& code) {
   }
   (void)hiprtcDestroyProgram(&prog);
   return ok;
+}
+
+// Compiled kernel `name` of `source` for these options, from the in-memory
+// cache, the disk cache or a fresh compile (null on any failure).
+void* rtc_get(const char* source, const char* name, const std::vector<std::string>& opts) {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::string key = std::to_string(dev) + " " + name;
+  for (const auto& x : opts) key += " " + x;
+  std::lock_guard<std::mutex> lk(g_rtc_mu);
+  RtcEntry& ent = g_rtc[key];
+  if (ent.fn || ent.failed) return (void*)ent.fn;
+  ent.failed = true;  // until loaded
+  std::vector<char> code;
+  std::string path;
+  const std::string dir = cache_dir();
+  if (!dir.empty()) {
+    char fname[64];
+    std::snprintf(fname, sizeof(fname), "/%s_%016llx.co", name,
+                  (unsigned long long)fnv1a(key.substr(key.find(' ')), fnv1a(source)));
+    path = dir + fname;
+  }
+  if (path.empty() || !read_file(path, code)) {
+    if (!compile(source, opts, code)) return nullptr;
+    if (!path.empty()) write_file(path, code);
+  }
+  if (hipModuleLoadData(&ent.mod, code.data()) != hipSuccess) return nullptr;
+  if (hipModuleGetFunction(&ent.fn, ent.mod, name) != hipSuccess) {
+    ent.fn = nullptr;
+    return nullptr;
+  }
+  ent.failed = false;
+  return (void*)ent.fn;
+}
+
+// Raise a module function's dynamic-LDS limit to the CU's 160 KB once.
+void raise_lds(void* fn) {
+  static std::mutex mu;
+  static std::map<void*, bool> raised;
+  std::lock_guard<std::mutex> lk(mu);
+  if (!raised[fn]) {
+    raised[fn] = true;
+    (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    (void)hipGetLastError();
+  }
 }
 
 }  // namespace
@@ -126,34 +171,33 @@ void* rtc_period_kernel(const SimParams& P, LaunchShape s, bool replay, bool sim
       "-DSAOM_FIXED_DYN=" + list(P.dyn, P.k),
       std::string("-DSAOM_CACHE_D=") + (P.k * R <= 32 ? "1" : "0"),
       std::string("-DSAOM_HAS_RATE=") + ((P.flags & F_RATE) ? "1" : "0")};
-  int dev = 0;
-  (void)hipGetDevice(&dev);
-  std::string key = std::to_string(dev);
-  for (const auto& x : opts) key += " " + x;
-  std::lock_guard<std::mutex> lk(g_rtc_mu);
-  RtcEntry& ent = g_rtc[key];
-  if (ent.fn || ent.failed) return (void*)ent.fn;
-  ent.failed = true;  // until loaded
-  std::vector<char> code;
-  std::string path;
-  const std::string dir = cache_dir();
-  if (!dir.empty()) {
-    char name[64];
-    std::snprintf(name, sizeof(name), "/period_%016llx.co",
-                  (unsigned long long)fnv1a(key.substr(key.find(' ')), fnv1a(kRtcPeriodSource)));
-    path = dir + name;
-  }
-  if (path.empty() || !read_file(path, code)) {
-    if (!compile(opts, code)) return nullptr;
-    if (!path.empty()) write_file(path, code);
-  }
-  if (hipModuleLoadData(&ent.mod, code.data()) != hipSuccess) return nullptr;
-  if (hipModuleGetFunction(&ent.fn, ent.mod, "saom_period_rtc") != hipSuccess) {
-    ent.fn = nullptr;
-    return nullptr;
-  }
-  ent.failed = false;
-  return (void*)ent.fn;
+  return rtc_get(kRtcPeriodSource, "saom_period_rtc", opts);
+}
+
+void* rtc_multi_kernel(const MultiParams& Q, int nw, int wmax) {
+  const SimParams& P = Q.base;
+  const int R = wmax / nw;
+  std::vector<int> netref(P.k);
+  for (int e = 0; e < P.k; ++e) netref[e] = Q.netref[e];
+  std::vector<std::string> opts = {
+      "--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
+      "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+      "-DSAOM_WMAX=" + std::to_string(wmax), "-DSAOM_NW=" + std::to_string(nw),
+      "-DSAOM_FIXED_K=" + std::to_string(P.k),
+      "-DSAOM_FIXED_TYPES=" + list(P.tid, P.k), "-DSAOM_FIXED_ETYPES=" + list(P.ety, P.k),
+      "-DSAOM_FIXED_DYN=" + list(P.dyn, P.k),
+      "-DSAOM_FIXED_NETOF=" + list(Q.netof, P.k), "-DSAOM_FIXED_NETREF=" + list(netref.data(), P.k),
+      std::string("-DSAOM_CACHE_D=") + (P.k * R <= 32 ? "1" : "0")};
+  return rtc_get(kRtcMultiSource, "saom_multi_rtc", opts);
+}
+
+int launch_multi_rtc(void* fn, const MultiParams* Pd, int grid, size_t lds, int nw,
+                     void* stream) {
+  raise_lds(fn);
+  const MultiParams* pd = Pd;
+  void* args[] = {&pd};
+  return (int)hipModuleLaunchKernel((hipFunction_t)fn, (unsigned)grid, 1, 1, 64u * (unsigned)nw,
+                                    1, 1, (unsigned)lds, (hipStream_t)stream, args, nullptr);
 }
 
 bool specialize_policy(long long chains_x_actors) {
@@ -171,17 +215,7 @@ bool same_effects(const SimParams& a, const SimParams& b) {
 }
 
 int launch_period_rtc(void* fn, const PeriodLaunch& L, int nw, void* stream) {
-  // raise the function's dynamic LDS limit to the CU's 160 KB once
-  static std::mutex mu;
-  static std::map<void*, bool> raised;
-  {
-    std::lock_guard<std::mutex> lk(mu);
-    if (!raised[fn]) {
-      raised[fn] = true;
-      (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-      (void)hipGetLastError();
-    }
-  }
+  raise_lds(fn);
   const SimParams* pd = L.Pd;
   const uint32_t* items = L.items;
   uint32_t k0 = L.key0, k1 = L.key1;

@@ -145,6 +145,11 @@ struct MultiParams {
 // per chain). Pd is the device copy of P. Returns 0 or a hipError_t.
 int launch_multi(const MultiParams& P, const MultiParams* Pd, int nw, int wmax, void* stream);
 size_t multi_lds_bytes(const MultiParams& P, int nw, int wmax);
+// Runtime-specialised multi-network kernel for P's effect list and this
+// shape (null when unavailable), and its launch.
+void* rtc_multi_kernel(const MultiParams& P, int nw, int wmax);
+int launch_multi_rtc(void* fn, const MultiParams* Pd, int grid, size_t lds, int nw,
+                     void* stream);
 
 // Statistics of M packed networks (rows u64, n*W64 per network).
 struct StatParams {

@@ -31,12 +31,31 @@
 // w * n + h) in LDS when they fit, else in a per-chain device buffer (rows
 // padded to WMAX words for WMAX <= 16; streamed layout with W64 words above);
 // in/out degrees of every network in LDS.
+//
+// Runtime specialisation (rtc.hip) compiles this file with SAOM_RTC and the
+// effect list fixed (SAOM_FIXED_* as for the single-network kernel, plus
+// SAOM_FIXED_NETOF / SAOM_FIXED_NETREF): the effect loops unroll and, when
+// k x alternatives-per-lane <= 32, the change statistics stay in registers
+// between the choice and score passes and the score sums take one transposing
+// reduction. Same arithmetic in the same order as the generic kernel.
+#ifndef SAOM_RTC
 #define SAOM_HELPERS_ONLY
 #include "saom_period.hip"
 #undef SAOM_HELPERS_ONLY
+#endif
 
 namespace saom {
 namespace {
+
+#if defined(SAOM_FIXED_K) && defined(SAOM_FIXED_NETOF)
+constexpr int kFixedNetof[] = {SAOM_FIXED_NETOF};
+constexpr int kFixedNetref[] = {SAOM_FIXED_NETREF};
+__device__ __forceinline__ int eff_netof(const MultiParams&, int e) { return kFixedNetof[e]; }
+__device__ __forceinline__ int eff_netref(const MultiParams&, int e) { return kFixedNetref[e]; }
+#else
+__device__ __forceinline__ int eff_netof(const MultiParams& Q, int e) { return Q.netof[e]; }
+__device__ __forceinline__ int eff_netref(const MultiParams& Q, int e) { return Q.netref[e]; }
+#endif
 
 struct MultiLds {
   size_t rows, indeg, outdeg, tpall, soall, coli, rowi, tpm0, tpm1, red, tsl, dstar, sc, sd,
@@ -77,11 +96,11 @@ __device__ __forceinline__ void multi_eff(const MultiParams& Q, int e, const Alt
                                           const uint64_t* rows_all, size_t netw,
                                           double (&d)[R]) {
   const SimParams& P = Q.base;
-  const int t = P.tid[e];
+  const int t = eff_type(P, e);
   if (t < 33 || t > 37) {
     eff_base<R, WMAX, STREAM, false>(P, e, A, U, S, d);
   } else {
-    const uint64_t* rw = rows_all + (size_t)Q.netref[e] * netw;
+    const uint64_t* rw = rows_all + (size_t)eff_netref(Q, e) * netw;
     const int RS = S.RS, i = U.i, iw = i >> 6, ib = i & 63;
     auto wij = [&](int r) -> int {  // bit j of row i of the source network
       const int word = S.NW * r + S.wv;
@@ -136,7 +155,7 @@ __device__ __forceinline__ void multi_eff(const MultiParams& Q, int e, const Alt
         break;
       }
     }
-    const int et = P.ety[e];
+    const int et = eff_ety(P, e);
     if (et != 0) {
 #pragma unroll
       for (int r = 0; r < R; ++r)
@@ -149,8 +168,7 @@ __device__ __forceinline__ void multi_eff(const MultiParams& Q, int e, const Alt
 }
 
 template <int NW, int R, int WMAX>
-__global__ void __launch_bounds__(64 * NW)
-    saom_multi_kernel(const MultiParams* __restrict__ Qp) {
+__device__ __forceinline__ void multi_body(const MultiParams* __restrict__ Qp) {
   const MultiParams& Q = *Qp;
   const SimParams& P = Q.base;
   constexpr int NT = 64 * NW;
@@ -350,8 +368,29 @@ __global__ void __launch_bounds__(64 * NW)
     double f[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) f[r] = 0.0;
+#ifdef SAOM_FIXED_K
+    constexpr int KC = SAOM_CACHE_D ? SAOM_FIXED_K : 1;
+    double dc[KC][R];
+#pragma unroll
+    for (int e = 0; e < SAOM_FIXED_K; ++e) {
+      if (SAOM_CACHE_D) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) dc[SAOM_CACHE_D ? e : 0][r] = 0.0;
+      }
+      if (eff_netof(Q, e) != d) continue;
+      double dv[R];
+      multi_eff<R, WMAX, STREAM>(Q, e, A, U, S, rows_all, netw, dv);
+      const double te = th[e];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const double prod = te * dv[r];
+        f[r] = f[r] + prod;
+        if (SAOM_CACHE_D) dc[SAOM_CACHE_D ? e : 0][r] = dv[r];
+      }
+    }
+#else
     for (int e = 0; e < k; ++e) {
-      if (Q.netof[e] != d) continue;
+      if (eff_netof(Q, e) != d) continue;
       double dv[R];
       multi_eff<R, WMAX, STREAM>(Q, e, A, U, S, rows_all, netw, dv);
       const double te = th[e];
@@ -361,6 +400,7 @@ __global__ void __launch_bounds__(64 * NW)
         f[r] = f[r] + prod;
       }
     }
+#endif
     // 4. Gumbel-max choice with the float32 bound (as the single-network kernel)
     double gb = -1e30, shift = 0.0;
     int jb = i;
@@ -419,10 +459,40 @@ __global__ void __launch_bounds__(64 * NW)
       wgt[r] = ((A.ok >> r) & 1) ? exp(f[r] - shift) : 0.0;
       Sw += wgt[r];
     }
+#if defined(SAOM_FIXED_K) && SAOM_CACHE_D
+    {
+      // all k + 1 sums in one transposing reduction (zeros for the other
+      // networks' effects, which the score update skips)
+      constexpr int KP = SAOM_FIXED_K + 1 <= 2 ? 2 : SAOM_FIXED_K + 1 <= 4 ? 4
+                       : SAOM_FIXED_K + 1 <= 8 ? 8 : SAOM_FIXED_K + 1 <= 16 ? 16 : 32;
+      double x[KP];
+#pragma unroll
+      for (int e = 0; e < KP; ++e) x[e] = 0.0;
+#pragma unroll
+      for (int e = 0; e < SAOM_FIXED_K; ++e) {
+        double T = 0.0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          T += wgt[r] * dc[e][r];
+          if (((A.in >> r) & 1) && A.j[r] == jstar) dstar[e] = dc[e][r];
+        }
+        x[e] = T;
+      }
+      x[SAOM_FIXED_K] = Sw;
+      const double tot = transpose_sum<KP>(x, lane);
+      const int v = transpose_index<KP>(lane);
+      if (v >= 0 && v <= SAOM_FIXED_K) tsl[wv * (k + 1) + v] = tot;
+    }
+#else
     Sw = wave_sum(Sw);
     if (lane == 0) tsl[wv * (k + 1) + k] = Sw;
+#ifdef SAOM_FIXED_K
+#pragma unroll
+    for (int e = 0; e < SAOM_FIXED_K; ++e) {
+#else
     for (int e = 0; e < k; ++e) {
-      if (Q.netof[e] != d) continue;
+#endif
+      if (eff_netof(Q, e) != d) continue;
       double dv[R];
       multi_eff<R, WMAX, STREAM>(Q, e, A, U, S, rows_all, netw, dv);
       double T = 0.0;
@@ -434,10 +504,11 @@ __global__ void __launch_bounds__(64 * NW)
       T = wave_sum(T);
       if (lane == 0) tsl[wv * (k + 1) + e] = T;
     }
+#endif
     int aij = 0;
     if (jstar != i) aij = (int)((ROWI(jstar >> 6) >> (jstar & 63)) & 1);
     __syncthreads();
-    if (tid < k && Q.netof[tid] == d) {
+    if (tid < k && Q.netof[tid] == d) {  // (netof of a lane's own effect)
       double St = 0.0, Tt = 0.0;
       for (int q = 0; q < NW; ++q) {
         St += tsl[q * (k + 1) + k];
@@ -492,6 +563,13 @@ __global__ void __launch_bounds__(64 * NW)
   }
 }
 
+#ifndef SAOM_RTC
+template <int NW, int R, int WMAX>
+__global__ void __launch_bounds__(64 * NW)
+    saom_multi_kernel(const MultiParams* __restrict__ Qp) {
+  multi_body<NW, R, WMAX>(Qp);
+}
+
 template <int NW, int WMAX>
 hipError_t launch_multi_shape(const MultiParams* Pd, int grid, size_t lds, hipStream_t st) {
   auto kern = saom_multi_kernel<NW, WMAX / NW, WMAX>;
@@ -540,3 +618,13 @@ size_t multi_lds_bytes(const MultiParams& P, int nw, int wmax) {
 #endif
 
 }  // namespace saom
+#else   // SAOM_RTC: one specialised kernel, shape and effect list from macros
+}  // namespace
+
+extern "C" __global__ void __launch_bounds__(64 * SAOM_NW)
+    saom_multi_rtc(const MultiParams* __restrict__ Qp) {
+  multi_body<SAOM_NW, SAOM_WMAX / SAOM_NW, SAOM_WMAX>(Qp);
+}
+
+}  // namespace saom
+#endif

@@ -33,7 +33,7 @@ EXPORTS = (
     "cusna_period_plan_gof", "cusna_saom_simulate_multi", "cusna_cross_stats",
     "cusna_multi_plan_create", "cusna_multi_plan_set_params", "cusna_multi_plan_run",
     "cusna_multi_plan_fetch", "cusna_multi_plan_dist", "cusna_multi_plan_destroy",
-    "cusna_multi_plan_set_stats", "cusna_multi_plan_stats",
+    "cusna_multi_plan_set_stats", "cusna_multi_plan_stats", "cusna_multi_plan_specialized",
 )
 
 _lib = None
@@ -110,6 +110,8 @@ def load(path: str | None = None) -> C.CDLL:
     L.cusna_multi_plan_set_stats.argtypes = [_vp, C.c_int32, _vp]
     L.cusna_multi_plan_stats.restype = C.c_int
     L.cusna_multi_plan_stats.argtypes = [_vp, _vp]
+    L.cusna_multi_plan_specialized.restype = C.c_int32
+    L.cusna_multi_plan_specialized.argtypes = [_vp]
     L.cusna_multi_plan_destroy.restype = C.c_int
     L.cusna_multi_plan_destroy.argtypes = [_vp]
     L.cusna_gof_distribution.restype = C.c_int
@@ -269,6 +271,11 @@ class MultiPlan:
         out = np.zeros((self.prob.M, self.prob.k), np.float64)
         _check("cusna_multi_plan_stats", self.L.cusna_multi_plan_stats(self.h, _p(out)))
         return out
+
+    @property
+    def specialized(self) -> bool:
+        """Whether the last run launched the specialised kernel."""
+        return bool(self.L.cusna_multi_plan_specialized(self.h))
 
     def dist(self) -> np.ndarray:
         out = np.zeros((self.prob.D, self.prob.M), np.int32)

@@ -255,6 +255,10 @@ CUSNA_API cusna_status cusna_multi_plan_fetch(cusna_multi_plan *plan,
                                               cusna_saom_multi_result *result);
 CUSNA_API cusna_status cusna_multi_plan_dist(cusna_multi_plan *plan, int32_t *dist_stat);
 CUSNA_API cusna_status cusna_multi_plan_destroy(cusna_multi_plan *plan);
+/* 1 when the last run launched the kernel specialised (hipRTC) to the plan's
+ * effect list: launches of at least 2^18 chains x actors, CUSNA_SPECIALIZE=0/1
+ * overriding; results are identical either way. */
+CUSNA_API int32_t cusna_multi_plan_specialized(const cusna_multi_plan *plan);
 
 /*
  * Statistics of network d's end states for the multi-network estimator, on

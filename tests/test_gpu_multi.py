@@ -133,3 +133,21 @@ def test_multi_argument_checks(gpu_lib):
     a, wp = cross_inputs(1, n=6, M=1)
     with pytest.raises(CusnaError, match="status 5"):
         gpu_lib.cross_stats(a, wp, 38)
+
+
+@pytest.mark.parametrize("q", range(len(multi_specs())))
+def test_specialised_multi_matches_generic(q, gpu_lib, oracle_lib, monkeypatch):
+    """The kernel specialised to the effect list (hipRTC) against the generic
+    kernel and the oracle: trajectories bit for bit, scores to the bar (the
+    specialised kernel sums the score terms in another tree order)."""
+    pr = multi_problem(multi_specs()[q])
+    out = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("CUSNA_SPECIALIZE", mode)
+        plan = gpu_lib.MultiPlan(pr, want_end_nets=True)
+        plan.run(pr.seed)
+        out[mode] = plan.fetch(want_end_nets=True)
+        assert plan.specialized == (mode == "1")
+        plan.close()
+    assert_multi_equal(out["1"], out["0"], "specialised vs generic")
+    assert_multi_equal(out["1"], oracle_lib.simulate_multi(pr), "specialised vs oracle")

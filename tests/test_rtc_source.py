@@ -13,10 +13,10 @@ INC = os.path.join(ROOT, "cusna_amd", "csrc", "rtc_source.inc")
 HIPRTC = "/opt/rocm/lib/libhiprtc.so"
 
 
-def _source() -> bytes:
+def _source(name: str = "kRtcPeriodSource") -> bytes:
     text = open(INC).read()
-    m = re.search(r'R"CUSNASRC\((.*)\)CUSNASRC"', text, re.S)
-    assert m, "rtc_source.inc has no embedded source"
+    m = re.search(name + r'\[\] = R"CUSNASRC\((.*?)\)CUSNASRC"', text, re.S)
+    assert m, f"rtc_source.inc has no embedded {name}"
     return m.group(1).encode()
 
 
@@ -50,6 +50,23 @@ CASES = {
     "stream": dict(SAOM_WMAX=32, SAOM_NW=16, SAOM_REPLAY=0, SAOM_SIMPLE=1, SAOM_FIXED_K=4,
                    SAOM_FIXED_TYPES="0,1,2,12", SAOM_FIXED_ETYPES="0,0,0,0",
                    SAOM_FIXED_DYN="0,0,0,1", SAOM_CACHE_D=1),
+    # exact word bounds (not powers of two), rate effects compiled in
+    "exact": dict(SAOM_WMAX=18, SAOM_NW=9, SAOM_REPLAY=0, SAOM_SIMPLE=1, SAOM_FIXED_K=3,
+                  SAOM_FIXED_TYPES="0,1,2", SAOM_FIXED_ETYPES="0,0,0", SAOM_FIXED_DYN="0,0,0",
+                  SAOM_CACHE_D=1, SAOM_HAS_RATE=1),
+}
+
+MULTI_CASES = {
+    # two networks, cross effects, cached statistics
+    "multi_lds": dict(SAOM_WMAX=8, SAOM_NW=2, SAOM_FIXED_K=8,
+                      SAOM_FIXED_TYPES="0,1,2,33,0,1,37,34", SAOM_FIXED_ETYPES="0,0,0,0,0,0,0,1",
+                      SAOM_FIXED_DYN="0,0,0,0,0,0,0,0", SAOM_FIXED_NETOF="0,0,0,0,1,1,1,1",
+                      SAOM_FIXED_NETREF="-1,-1,-1,1,-1,-1,0,0", SAOM_CACHE_D=1),
+    # streamed, four networks, recomputed statistics
+    "multi_stream": dict(SAOM_WMAX=32, SAOM_NW=16, SAOM_FIXED_K=5,
+                         SAOM_FIXED_TYPES="0,36,0,35,5", SAOM_FIXED_ETYPES="0,0,0,0,0",
+                         SAOM_FIXED_DYN="0,0,0,0,0", SAOM_FIXED_NETOF="0,0,1,2,3",
+                         SAOM_FIXED_NETREF="-1,3,-1,0,-1", SAOM_CACHE_D=0),
 }
 
 
@@ -60,4 +77,14 @@ def test_embedded_source_compiles(name):
     if not os.path.exists(INC):
         pytest.skip("library not built (rtc_source.inc is generated by the build)")
     rc, log = _compile(_source(), CASES[name])
+    assert rc == 0, log[-3000:]
+
+
+@pytest.mark.parametrize("name", sorted(MULTI_CASES))
+def test_embedded_multi_source_compiles(name):
+    if not os.path.exists(HIPRTC):
+        pytest.skip("hipRTC not installed")
+    if not os.path.exists(INC):
+        pytest.skip("library not built (rtc_source.inc is generated by the build)")
+    rc, log = _compile(_source("kRtcMultiSource"), MULTI_CASES[name])
     assert rc == 0, log[-3000:]
