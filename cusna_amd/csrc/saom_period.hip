@@ -1004,36 +1004,70 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
     }
 
     // 3. features: tp_j (two-paths i->h->j), so_j = |row_i & row_j|,
-    //    cy_j = |row_j & col_i|
+    //    cy_j = |row_j & col_i|. When the ego has few ties, so and cy come
+    //    from walking its out- and in-ties (bit h of row j per tie, three
+    //    instructions) instead of popcounts over every row word (eight per
+    //    word); the counts are the same either way.
+    const bool walk =
+        !STREAM && __builtin_amdgcn_readfirstlane(U.outdeg_i + U.indeg_i) * 3 < U.words() * 8;
 #pragma unroll
-    for (int r = 0; r < R; ++r) A.tp[r] = 0;
-    if (flags & F_TP) {
+    for (int r = 0; r < R; ++r) {
+      A.tp[r] = 0;
+      A.so[r] = 0;
+      A.cy[r] = 0;
+    }
+    auto bit_of_row_j = [&](int r, int h) -> int {  // x_{j h}
+      const uint64_t x = rows[(size_t)(h >> 6) * RS + A.j[r]];
+      const uint32_t half = (h & 32) ? (uint32_t)(x >> 32) : (uint32_t)x;
+      return (int)((half >> (h & 31)) & 1u);
+    };
+    if ((flags & F_TP) || walk) {
 #pragma unroll
       for (int wd = 0; wd < U.words(); ++wd) {
         uint64_t m = uniform64(U.rowi(wd));  // scalar loop over i's out-ties
         while (m) {
           const int h = wd * 64 + __ffsll((long long)m) - 1;
           m &= m - 1;
+          if (flags & F_TP) {
 #pragma unroll
-          for (int r = 0; r < R; ++r) {
-            const int word = NW * r + wv;
-            const uint64_t x = (!STREAM || word < W64) ? rows[(size_t)(word) * RS + h] : 0ull;
-            A.tp[r] += (int)((x >> lane) & 1);
+            for (int r = 0; r < R; ++r) {
+              const int word = NW * r + wv;
+              const uint64_t x =
+                  (!STREAM || word < W64) ? rows[(size_t)(word) * RS + h] : 0ull;
+              A.tp[r] += (int)((x >> lane) & 1);
+            }
+          }
+          if (walk) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) A.so[r] += bit_of_row_j(r, h);
           }
         }
       }
     }
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      int so = 0, cy = 0;
+    if (walk) {
 #pragma unroll
       for (int wd = 0; wd < U.words(); ++wd) {
-        const uint64_t x = rows[(size_t)(wd) * RS + A.j[r]];
-        so += popc64(U.rowi(wd) & x);
-        cy += popc64(x & U.coli(wd));
+        uint64_t m = uniform64(U.coli(wd));  // scalar loop over i's in-ties
+        while (m) {
+          const int h = wd * 64 + __ffsll((long long)m) - 1;
+          m &= m - 1;
+#pragma unroll
+          for (int r = 0; r < R; ++r) A.cy[r] += bit_of_row_j(r, h);
+        }
       }
-      A.so[r] = so;
-      A.cy[r] = cy;
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        int so = 0, cy = 0;
+#pragma unroll
+        for (int wd = 0; wd < U.words(); ++wd) {
+          const uint64_t x = rows[(size_t)(wd) * RS + A.j[r]];
+          so += popc64(U.rowi(wd) & x);
+          cy += popc64(x & U.coli(wd));
+        }
+        A.so[r] = so;
+        A.cy[r] = cy;
+      }
     }
     if (flags & (F_TPALL | F_SOALL)) {
 #pragma unroll
