@@ -304,35 +304,68 @@ __device__ __forceinline__ void multi_body(const MultiParams* __restrict__ Qp) {
     U.indeg_i = indeg[i];
     U.zvi = 0.0;
     U.rdeg_i = 0;
+    // features of network d: tp, so and cy as the single-network kernel
+    // computes them, with the same tie walk for egos with few ties.
+    const bool walk =
+        !STREAM && __builtin_amdgcn_readfirstlane(U.outdeg_i + U.indeg_i) * 3 < U.words() * 8;
 #pragma unroll
-    for (int r = 0; r < R; ++r) A.tp[r] = 0;
-    if (flags & F_TP) {
+    for (int r = 0; r < R; ++r) {
+      A.tp[r] = 0;
+      A.so[r] = 0;
+      A.cy[r] = 0;
+    }
+    auto bit_of_row_j = [&](int r, int h) -> int {  // x_{j h}
+      const uint64_t x = rows[(size_t)(h >> 6) * RS + A.j[r]];
+      const uint32_t half = (h & 32) ? (uint32_t)(x >> 32) : (uint32_t)x;
+      return (int)((half >> (h & 31)) & 1u);
+    };
+    if ((flags & F_TP) || walk) {
 #pragma unroll
       for (int wd = 0; wd < U.words(); ++wd) {
         uint64_t m = uniform64(U.rowi(wd));  // scalar loop over i's out-ties
         while (m) {
           const int h = wd * 64 + __ffsll((long long)m) - 1;
           m &= m - 1;
+          if (flags & F_TP) {
 #pragma unroll
-          for (int r = 0; r < R; ++r) {
-            const int word = NW * r + wv;
-            const uint64_t x = (!STREAM || word < W64) ? rows[(size_t)(word) * RS + h] : 0ull;
-            A.tp[r] += (int)((x >> lane) & 1);
+            for (int r = 0; r < R; ++r) {
+              const int word = NW * r + wv;
+              const uint64_t x =
+                  (!STREAM || word < W64) ? rows[(size_t)(word) * RS + h] : 0ull;
+              A.tp[r] += (int)((x >> lane) & 1);
+            }
+          }
+          if (walk) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) A.so[r] += bit_of_row_j(r, h);
           }
         }
       }
     }
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      int so = 0, cy = 0;
+    if (walk) {
 #pragma unroll
       for (int wd = 0; wd < U.words(); ++wd) {
-        const uint64_t x = rows[(size_t)(wd) * RS + A.j[r]];
-        so += popc64(U.rowi(wd) & x);
-        cy += popc64(x & U.coli(wd));
+        uint64_t m = uniform64(U.coli(wd));  // scalar loop over i's in-ties
+        while (m) {
+          const int h = wd * 64 + __ffsll((long long)m) - 1;
+          m &= m - 1;
+#pragma unroll
+          for (int r = 0; r < R; ++r) A.cy[r] += bit_of_row_j(r, h);
+        }
       }
-      A.so[r] = so;
-      A.cy[r] = cy;
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        int so = 0, cy = 0;
+#pragma unroll
+        for (int wd = 0; wd < U.words(); ++wd) {
+          const uint64_t x = rows[(size_t)(wd) * RS + A.j[r]];
+          so += popc64(U.rowi(wd) & x);
+          cy += popc64(x & U.coli(wd));
+        }
+        A.so[r] = so;
+        A.cy[r] = cy;
+      }
     }
     if (flags & (F_TPALL | F_SOALL)) {
 #pragma unroll
@@ -371,13 +404,13 @@ __device__ __forceinline__ void multi_body(const MultiParams* __restrict__ Qp) {
 #ifdef SAOM_FIXED_K
     constexpr int KC = SAOM_CACHE_D ? SAOM_FIXED_K : 1;
     double dc[KC][R];
-#pragma unroll
-    for (int e = 0; e < SAOM_FIXED_K; ++e) {
+    static_for<0, SAOM_FIXED_K>([&](auto EC) {
+      constexpr int e = EC;
       if (SAOM_CACHE_D) {
 #pragma unroll
         for (int r = 0; r < R; ++r) dc[SAOM_CACHE_D ? e : 0][r] = 0.0;
       }
-      if (eff_netof(Q, e) != d) continue;
+      if (eff_netof(Q, e) != d) return;
       double dv[R];
       multi_eff<R, WMAX, STREAM>(Q, e, A, U, S, rows_all, netw, dv);
       const double te = th[e];
@@ -387,7 +420,7 @@ __device__ __forceinline__ void multi_body(const MultiParams* __restrict__ Qp) {
         f[r] = f[r] + prod;
         if (SAOM_CACHE_D) dc[SAOM_CACHE_D ? e : 0][r] = dv[r];
       }
-    }
+    });
 #else
     for (int e = 0; e < k; ++e) {
       if (eff_netof(Q, e) != d) continue;
@@ -486,13 +519,9 @@ __device__ __forceinline__ void multi_body(const MultiParams* __restrict__ Qp) {
 #else
     Sw = wave_sum(Sw);
     if (lane == 0) tsl[wv * (k + 1) + k] = Sw;
-#ifdef SAOM_FIXED_K
-#pragma unroll
-    for (int e = 0; e < SAOM_FIXED_K; ++e) {
-#else
-    for (int e = 0; e < k; ++e) {
-#endif
-      if (eff_netof(Q, e) != d) continue;
+    auto expected = [&](auto ev) {  // ev: an int, or IntC<e> in a fixed list
+      const int e = ev;
+      if (eff_netof(Q, e) != d) return;
       double dv[R];
       multi_eff<R, WMAX, STREAM>(Q, e, A, U, S, rows_all, netw, dv);
       double T = 0.0;
@@ -503,7 +532,12 @@ __device__ __forceinline__ void multi_body(const MultiParams* __restrict__ Qp) {
       }
       T = wave_sum(T);
       if (lane == 0) tsl[wv * (k + 1) + e] = T;
-    }
+    };
+#ifdef SAOM_FIXED_K
+    static_for<0, SAOM_FIXED_K>(expected);
+#else
+    for (int e = 0; e < k; ++e) expected(e);
+#endif
 #endif
     int aij = 0;
     if (jstar != i) aij = (int)((ROWI(jstar >> 6) >> (jstar & 63)) & 1);

@@ -51,6 +51,24 @@ constexpr int kFixedD[] = {SAOM_FIXED_DYN};
 __device__ __forceinline__ int eff_type(const SimParams&, int e) { return kFixedT[e]; }
 __device__ __forceinline__ int eff_ety(const SimParams&, int e) { return kFixedE[e]; }
 __device__ __forceinline__ int eff_dyn(const SimParams&, int e) { return kFixedD[e]; }
+
+// f(IntC<B>), ..., f(IntC<E-1>): a loop whose index is a compile-time
+// constant in every iteration. The fixed effect list is walked this way so
+// that each iteration holds only its own effect's code: a "#pragma unroll"
+// loop is sized before the effect type folds, so with a dozen effects it
+// exceeds the unroller's limit, stays a loop over every effect's code and
+// indexes the cached change statistics dynamically, i.e. through scratch.
+template <int V>
+struct IntC {
+  __device__ constexpr operator int() const { return V; }
+};
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(IntC<B>{});
+    static_for<B + 1, E>(f);
+  }
+}
 #else
 __device__ __forceinline__ int eff_type(const SimParams& P, int e) { return P.tid[e]; }
 __device__ __forceinline__ int eff_ety(const SimParams& P, int e) { return P.ety[e]; }
@@ -1109,8 +1127,8 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
 #ifdef SAOM_FIXED_K
     constexpr int KC = SAOM_CACHE_D ? SAOM_FIXED_K : 1;
     double dc[KC][R];
-#pragma unroll
-    for (int e = 0; e < SAOM_FIXED_K; ++e) {
+    static_for<0, SAOM_FIXED_K>([&](auto EC) {
+      constexpr int e = EC;
       double d[R];
       eff_full<R, WMAX, STREAM, SIMPLE>(P, e, A, U, S, d);
       const double te = th[e];
@@ -1120,7 +1138,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
         f[r] = f[r] + prod;
         if (SAOM_CACHE_D) dc[SAOM_CACHE_D ? e : 0][r] = d[r];
       }
-    }
+    });
 #else
     for (int e = 0; e < k; ++e) {
       double d[R];
@@ -1239,21 +1257,10 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
 #else
     Sw = wave_sum(Sw);
     if (lane == 0) tsl[wv * (k + 1) + k] = Sw;
-#ifdef SAOM_FIXED_K
-#pragma unroll
-    for (int e = 0; e < SAOM_FIXED_K; ++e) {
-      double d[R];
-      if (SAOM_CACHE_D) {
-#pragma unroll
-        for (int r = 0; r < R; ++r) d[r] = dc[SAOM_CACHE_D ? e : 0][r];
-      } else {
-        eff_full<R, WMAX, STREAM, SIMPLE>(P, e, A, U, S, d);
-      }
-#else
-    for (int e = 0; e < k; ++e) {
+    auto expected = [&](auto ev) {  // ev: an int, or IntC<e> in a fixed list
+      const int e = ev;
       double d[R];
       eff_full<R, WMAX, STREAM, SIMPLE>(P, e, A, U, S, d);
-#endif
       double T = 0.0;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
@@ -1262,7 +1269,12 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
       }
       T = wave_sum(T);
       if (lane == 0) tsl[wv * (k + 1) + e] = T;
-    }
+    };
+#ifdef SAOM_FIXED_K
+    static_for<0, SAOM_FIXED_K>(expected);
+#else
+    for (int e = 0; e < k; ++e) expected(e);
+#endif
 #endif
     // 7. state of the chosen dyad before the toggle (:277-281)
     int aij = 0, x0ij = 0, kdij = 0;

@@ -32,13 +32,13 @@ EFF_CFGS = [c["name"] for c in load_golden("effects.json.gz")["configs"]]
 
 @pytest.mark.parametrize("name", EFF_CFGS)
 def test_effect_descriptors_match_reference_fixture(name):
-    """.prepare_effects_v2 & friends (test-effects.R:43-92)."""
-    from cusna_amd.effects import Effect, prepare_effects
+    """.prepare_effects_v2 & friends (test-effects.R:43-92), cross-network
+    effects included (their source network passes through unchanged,
+    R/effects.R:277-283)."""
+    from cusna_amd.effects import Effect, effect_netrefs, prepare_effects
 
     g = load_golden("effects.json.gz")
     cfg = [c for c in g["configs"] if c["name"] == name][0]
-    if any("net_ref" in r for r in cfg["recipe"]):
-        pytest.skip("cross-network effects belong to the multi-network simulator (out of scope)")
     n, covs, beh = _effects_inputs(g)
     effects = []
     for r in cfg["recipe"]:
@@ -47,7 +47,7 @@ def test_effect_descriptors_match_reference_fixture(name):
         else:
             effects.append(Effect(r["name"], covs[r["cov"]] if "cov" in r else None,
                                   parm=r.get("parm"), dyn=r.get("dyn", False),
-                                  etype=r.get("etype", "eval")))
+                                  etype=r.get("etype", "eval"), net_ref=r.get("net_ref")))
     p = cfg["period"] + 1
     z_start = None
     if cfg["use_beh"]:
@@ -67,6 +67,8 @@ def test_effect_descriptors_match_reference_fixture(name):
     np.testing.assert_array_equal(d.int_comp.ravel(),
                                   np.array([x for r in cfg["int_idx"] for x in r]))
     np.testing.assert_array_equal(d.etypes, cfg["etypes"])
+    if "netrefs" in cfg:
+        np.testing.assert_array_equal(effect_netrefs(effects), cfg["netrefs"])
 
 
 def test_gwesp_weights_match_reference_fixture():
