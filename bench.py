@@ -1,6 +1,6 @@
 """Benchmark: SAOM chain-periods per second at n = 500 on one MI355X.
 
-Workload (config 2): 500 directed actors, 4096 replicate chains, 2 periods,
+Workload: 500 directed actors, 4096 replicate chains, 2 periods,
 rate 8 per period; wave 1 Bernoulli with mean outdegree 5; effects density
 -2.8, recip 1.8, transTrip 0.3, cycle3 -0.1, egoX/altX/simX 0.1/0.1/0.6 on a
 covariate uniform on 1..5; wave 2 is one seeded simulated period. Choice in
@@ -16,7 +16,7 @@ plan's stream), the reference CPU engine (compiled from the reference's own
 source under oracle/_ref) timed on this host's cores on a bounded sample, and
 ("mom") the wall time of a method-of-moments fit to max|t| <= 0.1 on the
 device next to one pass of the same schedule through the reference engine,
-and ("groups") the config-3 batch of 256 networks of 30..1200 actors.
+and ("groups") a batch of 256 networks of 30..1200 actors.
 """
 from __future__ import annotations
 
@@ -49,7 +49,7 @@ def algorithmic_ops_per_alternative(n: int, k: int) -> float:
 
 
 def measured_traffic(n: int, chains: int):
-    """HBM bytes per period launch of the config-2 workload, from the committed
+    """HBM bytes per period launch of the n = 500 workload, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a plain
     copy); None for other shapes."""
     path = os.path.join(ROOT, "profiles", "period_kernel_rtc_c2_n500_traffic.md")
@@ -87,25 +87,25 @@ def cpu_baseline(prob_factory, threads: int, chains: int) -> dict:
 
 
 def groups_measure(lib) -> dict:
-    """Config-3 batch: 256 independent networks (n uniform on 30..1200, 500
+    """Group batch: 256 independent networks (n uniform on 30..1200, 500
     chains each) through one cusna_saom_simulate_groups call per run: one
     period launch per launch shape, items longest-first. Timed: the whole
     call (plan set-up and uploads, launches, statistics, copies back), after
     a 4-chain warm-up call that loads the code objects."""
     import numpy as np
 
-    from cusna_amd.workloads import c3_problems
+    from cusna_amd.workloads import mixed_group_problems
 
-    small = c3_problems(chains=4)
+    small = mixed_group_problems(chains=4)
     lib.simulate_groups(small)
-    probs = c3_problems()
+    probs = mixed_group_problems()
     t0 = time.perf_counter()
     outs = lib.simulate_groups(probs)
     wall = time.perf_counter() - t0
     cp = sum(p.M for p in probs)
     alts = sum(float(o["steps"].astype(np.int64).sum()) * p.n for p, o in zip(probs, outs))
     ns = [p.n for p in probs]
-    return {"workload": "config-3 groups: 256 networks, n uniform 30..1200 (%d..%d here), "
+    return {"workload": "groups: 256 networks, n uniform 30..1200 (%d..%d here), "
                         "500 chains each, rate 4, density/recip/transTrip/inPop/egoX/altX"
                         % (min(ns), max(ns)),
             "chain_periods": cp, "wall_s": round(wall, 3),
@@ -115,16 +115,16 @@ def groups_measure(lib) -> dict:
 
 
 def gof_measure(lib, plan, x0, cpu: bool) -> dict:
-    """sienaGOF distributions of the 4096 end networks of the last config-2
+    """sienaGOF distributions of the 4096 end networks of the last n = 500
     period, where they lie in device memory (cusna_period_plan_gof): in- and
     out-degree counts at levels 0..10 and geodesic counts at levels 1..5.
     CPU leg: the reference's cusna_gof_distribution (single-threaded, as the
     reference runs it) on 4 end networks of the same problem."""
     import numpy as np
 
-    from cusna_amd.workloads import c2_problem
+    from cusna_amd.workloads import n500_problem
 
-    res = {"workload": "GoF of 4096 config-2 end networks (n=500): in/out-degree levels "
+    res = {"workload": "GoF of 4096 end networks of the n=500 period: in/out-degree levels "
                        "0..10, geodesic levels 1..5"}
     plan.gof(0, range(11))  # code objects loaded
     for name, kind, lv in (("indegree", 0, range(11)), ("outdegree", 1, range(11)),
@@ -139,7 +139,7 @@ def gof_measure(lib, plan, x0, cpu: bool) -> dict:
 
         R = O.reference()
         if R is not None:
-            ends = lib.simulate_period(c2_problem(M=4, x0=x0, period_seed=11),
+            ends = lib.simulate_period(n500_problem(M=4, x0=x0, period_seed=11),
                                        want_end_nets=True)["a_ends"]
             t0 = time.perf_counter()
             O.gof_distribution(ends, 2, list(range(1, 6)), lib=R, fn="cusna_gof_distribution")
@@ -156,15 +156,15 @@ def gof_measure(lib, plan, x0, cpu: bool) -> dict:
 
 
 def rate_measure(lib, x0, cpu: bool, threads: int) -> dict:
-    """Config-2 period with two non-basic rate effects (outRate and a
+    """The n = 500 period with two non-basic rate effects (outRate and a
     covariate RateX), 4096 chains: plan run (launch + completion) after one
     warm-up; CPU leg: the reference engine on a bounded sample."""
     import numpy as np
 
-    from cusna_amd.workloads import c2_problem
+    from cusna_amd.workloads import n500_problem
 
     def prob(M):
-        pr = c2_problem(M=M, x0=x0, period_seed=11)
+        pr = n500_problem(M=M, x0=x0, period_seed=11)
         pr.rate_kind = np.array([1, 0], np.int32)
         pr.rate_alpha = np.array([0.05, 0.2])
         pr.rate_u = np.stack([np.zeros(pr.n), np.asarray(pr.vmat)[4]])
@@ -180,7 +180,7 @@ def rate_measure(lib, x0, cpu: bool, threads: int) -> dict:
         plan.sync()
     wall = (time.perf_counter() - t0) / 2
     out = plan.fetch()
-    res = {"workload": "config-2 period (n=500, 4096 chains) with outRate and RateX rate "
+    res = {"workload": "n=500 period (4096 chains, 7 effects) with outRate and RateX rate "
                        "effects, basic rate 0.6", "ms_per_period": round(1e3 * wall, 2),
            "chain_periods_per_s": round(4096 / wall, 1),
            "mean_ministeps": round(float(out["steps"].mean()), 1),
@@ -204,7 +204,7 @@ def rate_measure(lib, x0, cpu: bool, threads: int) -> dict:
 
 
 def multi_problem_c2(M: int, x0s, seed: int = 77):
-    """Two co-evolving networks of 500 actors (the config-2 waves as starts),
+    """Two co-evolving networks of 500 actors (the waves of the n = 500 workload as starts),
     rate 4 each, density/recip/transTrip/crprod on network 0 and
     density/recip/to/crprodRecip on network 1."""
     import numpy as np
@@ -243,7 +243,7 @@ def multi_measure(lib, x0s, cpu: bool, threads: int) -> dict:
     wall = (time.perf_counter() - t0) / steps
     out = plan.fetch()
     plan.close()
-    res = {"workload": "multi-network periods: D=2 networks of n=500 (config-2 waves as "
+    res = {"workload": "multi-network periods: D=2 networks of n=500 (the n=500 waves as "
                        "starts), %d chains, rate 4+4, 4+4 effects incl. crprod/to/crprodRecip"
                        % M,
            "ms_per_period": round(1e3 * wall, 2), "chain_periods_per_s": round(M / wall, 1),
@@ -295,22 +295,22 @@ def multi_measure(lib, x0s, cpu: bool, threads: int) -> dict:
 
 
 def mom_measure(lib, cpu: bool, threads: int) -> dict:
-    """Config-1 fit: 50 actors, 3 waves, conditional MoM on density/recip/
+    """Fit of 50 actors, 3 waves, conditional MoM on density/recip/
     transTrip with the reference's default schedule (n3 = 1000), repeated
     last-gain sub-phases until max|tconv| <= 0.1. Wall time of the whole
     cusna_mom_estimate call (set-up, three phases, host solves)."""
     import numpy as np
 
     from cusna_amd.mom import MomControl, MomEstimator
-    from cusna_amd.workloads import c1_estimation
+    from cusna_amd.workloads import fit50_estimation
 
-    data, eff = c1_estimation(lambda p: lib.simulate_period(p, want_end_nets=True)["a_ends"])
+    data, eff = fit50_estimation(lambda p: lib.simulate_period(p, want_end_nets=True)["a_ends"])
     est = MomEstimator(data, eff, None, True, MomControl(n3=1000))
     est.run(max_repeats=0)  # warm-up: code objects and first-launch set-up
     t0 = time.perf_counter()
     fit = est.run(max_repeats=20, conv_tol=0.1)
     wall = time.perf_counter() - t0
-    out = {"workload": "config-1 fit: n=50, 3 waves, conditional, density/recip/transTrip, "
+    out = {"workload": "MoM fit: n=50, 3 waves, conditional, density/recip/transTrip, "
                        "schedule n1=nD=1000, n2=40/40/60/80 x 256, n3=1000, then last-gain "
                        "sub-phase + phase 3 until max|t|<=0.1",
            "wall_s": round(wall, 3), "max_abs_tconv": round(float(fit.max_abs_tconv), 4),
@@ -379,14 +379,14 @@ def main() -> int:
     import numpy as np
 
     from cusna_amd import lib
-    from cusna_amd.workloads import c2_problem, c2_waves
+    from cusna_amd.workloads import n500_problem, n500_waves
 
     if not lib.has_device():
         print("no HIP device", file=sys.stderr)
         return 3
     n, M = args.n, args.chains
-    w1, w2 = c2_waves(lambda p: lib.simulate_period(p, want_end_nets=True)["a_ends"], n=n)
-    probs = [c2_problem(M=M, n=n, x0=w, period_seed=11 + q) for q, w in enumerate((w1, w2))]
+    w1, w2 = n500_waves(lambda p: lib.simulate_period(p, want_end_nets=True)["a_ends"], n=n)
+    probs = [n500_problem(M=M, n=n, x0=w, period_seed=11 + q) for q, w in enumerate((w1, w2))]
     for p in probs:
         p.threads = args.threads
     plans = [lib.PeriodPlan(p) for p in probs]
@@ -424,7 +424,7 @@ def main() -> int:
         "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(1e3 * elapsed / args.steps, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"config-2 period simulation: n={n}, {M} chains x 2 periods, "
+        "config": {"workload": f"period simulation: n={n}, {M} chains x 2 periods, "
                                "rate 8, density/recip/transTrip/cycle3/egoX/altX/simX, "
                                "unconditional", "n": n, "chains": M, "periods": 2,
                    "block_threads": plans[0].block_threads},
@@ -441,7 +441,7 @@ def main() -> int:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
         chains = args.cpu_chains or 4 * threads
         line["cpu_baseline"] = cpu_baseline(
-            lambda m: c2_problem(M=m, n=n, x0=w1, period_seed=11), threads, chains)
+            lambda m: n500_problem(M=m, n=n, x0=w1, period_seed=11), threads, chains)
     for pl in plans:
         pl.close()
     if gof is not None:

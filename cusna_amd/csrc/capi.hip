@@ -275,6 +275,7 @@ cusna_status plan_create(const cusna_saom_problem* pb, bool want_dense, const do
   if (pb->threads <= 0) {
     const int tuned = cached_nw(ShapeKey{wmax, W64, k, P.flags});
     if (tuned > 0) pl->nw = tuned;
+    pl->free_shape = tuned <= 0;
   }
   if (saom::sim_lds_bytes(P, pl->nw, wmax) > kLdsLimit) {
     set_error("chain state exceeds the 160 KB LDS of a CU");
@@ -470,7 +471,7 @@ cusna_status plan_run(cusna_period_plan* pl, uint64_t seed, bool want_stats, int
   if (timed) HIP_TRY(hipEventRecord(pl->ev0, s));
   int le = saom::launch_period(P, pl->D.at<saom::SimParams>(pl->o_params),
                                saom::LaunchShape{pl->nw, pl->wmax}, pl->replay, P.key0, P.key1,
-                               s, pl->specialize, &pl->last_rtc);
+                               s, pl->specialize, &pl->last_rtc, pl->free_shape);
   if (le != 0) {
     set_error(std::string("period kernel launch: ") + hipGetErrorString((hipError_t)le));
     return CUSNA_ERR_CUDA;
@@ -652,7 +653,16 @@ cusna_status simulate_groups(const cusna_saom_problem* const* pbs, int G,
                      [&](int a, int c) { return cost(a) > cost(c); });
     const size_t first = items.size();
     size_t lds = 0, lds_exact = 0;
-    const saom::LaunchShape ex = saom::exact_shape(b.W64, saom::LaunchShape{b.nw, b.wmax});
+    bool uniform = true, free_shape = true;
+    long long work = 0;
+    for (int g : b.groups) {
+      uniform = uniform && saom::same_effects(plans[g]->P, plans[b.groups[0]]->P);
+      free_shape = free_shape && plans[g]->free_shape;
+      work += (long long)plans[g]->M * plans[g]->n;
+    }
+    const saom::LaunchShape ex =
+        saom::exact_shape(b.W64, saom::LaunchShape{b.nw, b.wmax},
+                          uniform && free_shape ? plans[b.groups[0]]->P.k : 0);
     for (int g : b.groups) {
       for (int c = 0; c < plans[g]->M; ++c) items.push_back(((uint32_t)g << 20) | (uint32_t)c);
       lds = std::max(lds, saom::sim_lds_bytes(plans[g]->P, b.nw, b.wmax));
@@ -664,12 +674,6 @@ cusna_status simulate_groups(const cusna_saom_problem* const* pbs, int G,
     L.grid = (int)(items.size() - first);
     L.lds = lds;
     L.key0 = L.key1 = 0u;
-    bool uniform = true;
-    long long work = 0;
-    for (int g : b.groups) {
-      uniform = uniform && saom::same_effects(plans[g]->P, plans[b.groups[0]]->P);
-      work += (long long)plans[g]->M * plans[g]->n;
-    }
     L.Ph = &plans[b.groups[0]]->P;
     L.specialize = uniform && saom::specialize_policy(work);
     L.exact = ex;
@@ -704,6 +708,8 @@ cusna_status autotune(cusna_period_plan* pl, int probe, int* best_threads) {
   if (probe <= 0) probe = std::min(pl->M, 2048);
   probe = std::min(probe, pl->M);
   const int keep = pl->nw;
+  const bool keep_free = pl->free_shape;
+  pl->free_shape = false;  // time exactly the shapes tried
   int best = keep;
   double best_ms = 1e300;
   for (int nw = 1; nw <= 16; nw *= 2) {
@@ -715,6 +721,7 @@ cusna_status autotune(cusna_period_plan* pl, int probe, int* best_threads) {
       cusna_status st = plan_run(pl, 0x5eedull + (uint64_t)rep, false, probe, nullptr, true);
       if (st != CUSNA_OK) {
         pl->nw = keep;
+        pl->free_shape = keep_free;
         return st;
       }
       HIP_TRY(hipEventSynchronize(pl->ev1));

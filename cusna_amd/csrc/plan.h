@@ -81,6 +81,7 @@ struct cusna_period_plan {
   size_t o_rsc = 0, o_adist = 0;  // rate-effect scores, per-actor distances
   bool has_sscr = false;
   bool specialize = false;  // launch the runtime-specialised period kernel
+  bool free_shape = false;  // block shape left to the library (no threads, none tuned)
   bool last_rtc = false;    // the last run did
   bool stats_valid = false;
   ~cusna_period_plan() {

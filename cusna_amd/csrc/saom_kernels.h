@@ -74,7 +74,7 @@ bool simple_effects(const int* type_ids, int k);
 // struct in device memory, read by the kernel. Returns 0 or a hipError_t.
 int launch_period(const SimParams& P, const SimParams* Pd, LaunchShape s, bool replay,
                   uint32_t key0, uint32_t key1, void* stream, bool specialize = false,
-                  bool* used = nullptr);
+                  bool* used = nullptr, bool free_shape = false);
 
 // General form. One group: Pd, grid = chains, key0/key1 the Philox key.
 // Several groups: Pd[g] are group g's device parameters (its Philox key in
@@ -104,8 +104,13 @@ struct PeriodLaunch {
 // specialised kernel is compiled for any bound, so it pads only to a multiple
 // of the waves per chain (R words per lane): R = ceil(W64 / nw) with the
 // generic waves up to 8 words, two words per lane from 9 words (more waves,
-// less LDS padding), and at most 16 waves for streamed networks.
-LaunchShape exact_shape(int W64, LaunchShape generic);
+// less LDS padding), and at most 16 waves for streamed networks. k_free > 0
+// (the caller left the block shape to the library; k_free effects) doubles
+// the waves per chain up to 8 words when that brings k x R within the 32
+// change statistics per lane that the specialised kernel keeps in registers
+// between its choice and score passes (without them it computes every change
+// statistic twice).
+LaunchShape exact_shape(int W64, LaunchShape generic, int k_free = 0);
 int launch_period_ex(const PeriodLaunch& L, LaunchShape s, bool replay, bool simple,
                      void* stream);
 

@@ -34,11 +34,16 @@ bool simple_effects(const int* t, int k) {
   return true;
 }
 
-LaunchShape exact_shape(int W64, LaunchShape g) {
+LaunchShape exact_shape(int W64, LaunchShape g, int k_free) {
   int nw, R;
   if (W64 <= 8) {
     nw = g.nw;
     R = (W64 + nw - 1) / nw;
+    const int R2 = (W64 + 2 * nw - 1) / (2 * nw);
+    if (k_free > 0 && k_free * R > 32 && 2 * nw <= 8 && k_free * R2 <= 32) {
+      nw *= 2;
+      R = R2;
+    }
   } else if (W64 <= 32) {
     R = 2;
     nw = (W64 + 1) / 2;
@@ -113,7 +118,8 @@ int launch_multi(const MultiParams& P, const MultiParams* Pd, int nw, int wmax, 
 }
 
 int launch_period(const SimParams& P, const SimParams* Pd, LaunchShape s, bool replay,
-                  uint32_t k0, uint32_t k1, void* stream, bool specialize, bool* used) {
+                  uint32_t k0, uint32_t k1, void* stream, bool specialize, bool* used,
+                  bool free_shape) {
   PeriodLaunch L;
   L.Pd = Pd;
   L.items = nullptr;
@@ -124,7 +130,7 @@ int launch_period(const SimParams& P, const SimParams* Pd, LaunchShape s, bool r
   L.Ph = &P;
   L.specialize = specialize;
   L.used = used;
-  L.exact = exact_shape(P.W64, s);
+  L.exact = exact_shape(P.W64, s, free_shape ? P.k : 0);
   L.lds_exact = sim_lds_bytes(P, L.exact.nw, L.exact.wmax);
   return launch_period_ex(L, s, replay, simple_effects(P.tid, P.k), stream);
 }

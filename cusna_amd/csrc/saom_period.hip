@@ -1377,11 +1377,14 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
 }  // namespace
 }  // namespace saom
 #elif !defined(SAOM_RTC)
-// Register budget: two waves per SIMD for the two-wave shape, three for the
-// four-wave shape (LDS admits four chains of n = 500 per CU).
+// Register budget: 256 VGPRs (two waves per SIMD) for every multi-wave
+// shape. Measured at n = 500: a 12-effect list in four waves per chain ran
+// 310 ms with three waves' budget (168 VGPRs, spills) and 284 ms with two,
+// 527 ms with four (the LDS-bound occupancy); the batch of 256 groups of 30-1200 actors (up to 16
+// waves per chain) 13.3 s and 13.0 s.
 template <int NW, int R, int WMAX, bool REPLAY, bool SIMPLE>
 __global__ void __launch_bounds__(64 * NW)
-    __attribute__((amdgpu_waves_per_eu(NW == 1 ? 1 : (NW == 2 ? 2 : 3))))
+    __attribute__((amdgpu_waves_per_eu(NW == 1 ? 1 : 2)))
     saom_period_kernel(const SimParams* __restrict__ Pp, const uint32_t* __restrict__ items,
                        uint32_t key0_arg, uint32_t key1_arg) {
   period_body<NW, R, WMAX, REPLAY, SIMPLE>(Pp, items, key0_arg, key1_arg);
@@ -1453,7 +1456,7 @@ size_t SAOM_CAT(sim_lds_bytes_w, SAOM_WMAX)(const SimParams& P, int nw, int wmax
 }  // namespace
 
 extern "C" __global__ void __launch_bounds__(64 * SAOM_NW)
-    __attribute__((amdgpu_waves_per_eu(SAOM_NW == 1 ? 1 : (SAOM_NW == 2 ? 2 : 3))))
+    __attribute__((amdgpu_waves_per_eu(SAOM_NW == 1 ? 1 : 2)))  // as the generic kernel
     saom_period_rtc(const SimParams* __restrict__ Pp, const uint32_t* __restrict__ items,
                     uint32_t key0_arg, uint32_t key1_arg) {
   period_body<SAOM_NW, SAOM_WMAX / SAOM_NW, SAOM_WMAX, SAOM_REPLAY != 0, SAOM_SIMPLE != 0>(

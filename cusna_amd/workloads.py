@@ -4,13 +4,13 @@ Each builder returns the inputs of one period call (a ``SaomProblem``) from a
 fixed seed, using the same marshalling as the R glue (float64 descriptors,
 NULL masks, float32 theta/lambda):
 
-  c1_waves         50 actors, 3 waves: wave 1 Poisson(2.3) out-ties capped at
+  fit50_waves         50 actors, 3 waves: wave 1 Poisson(2.3) out-ties capped at
                    6, reciprocated with p = 0.4; waves 2-3 simulated.
-  c2_problem       500 actors, directed, rate 8 per period, mean outdegree 5;
+  n500_problem       500 actors, directed, rate 8 per period, mean outdegree 5;
                    density -2.8, recip 1.8, transTrip 0.3, cycle3 -0.1,
                    egoX/altX/simX 0.1/0.1/0.6 on a covariate uniform on 1..5.
-  c3_problems      256 networks of 30..1200 actors, 500 chains each (groups).
-  c4_problem       300 actors with an ordinal behaviour 1..5 = Binomial(4, 0.4)+1;
+  mixed_group_problems      256 networks of 30..1200 actors, 500 chains each (groups).
+  coevo300_problem       300 actors with an ordinal behaviour 1..5 = Binomial(4, 0.4)+1;
                    network density/recip/transTrip/simX(behaviour), behaviour
                    rate 4, linear -0.2, quad -0.3, avAlt 0.8.
 """
@@ -22,8 +22,8 @@ from .abi import SaomProblem
 from .data import build_saom_data
 from .effects import BehEffect, Effect, prepare_effects
 
-C2_THETA = [-2.8, 1.8, 0.3, -0.1, 0.1, 0.1, 0.6]
-C2_EFFECTS = ["density", "recip", "transTrip", "cycle3", "egoX", "altX", "simX"]
+N500_THETA = [-2.8, 1.8, 0.3, -0.1, 0.1, 0.1, 0.6]
+N500_EFFECTS = ["density", "recip", "transTrip", "cycle3", "egoX", "altX", "simX"]
 
 
 def bernoulli_network(n: int, mean_outdeg: float, rng: np.random.Generator) -> np.ndarray:
@@ -74,61 +74,61 @@ def effect_problem(x0: np.ndarray, effects: list, theta, lam: float, M: int, see
     return prob
 
 
-C1_THETA = [-2.5, 2.0, 0.4]
-C1_EFFECTS = ["density", "recip", "transTrip"]
+FIT50_THETA = [-2.5, 2.0, 0.4]
+FIT50_EFFECTS = ["density", "recip", "transTrip"]
 
 
-def c1_waves(simulate, n: int = 50, seed: int = 1) -> list:
-    """Three waves of config 1: wave 1 with min(Poisson(2.3), 6) out-ties per
+def fit50_waves(simulate, n: int = 50, seed: int = 1) -> list:
+    """Three waves of 50 actors: wave 1 with min(Poisson(2.3), 6) out-ties per
     actor, each reciprocated with p = 0.4; waves 2 and 3 are seeded simulated
     periods at rate 6 with density -2.5, recip 2.0, transTrip 0.4
     (``simulate`` runs a SaomProblem and returns a_ends)."""
     rng = np.random.default_rng(seed)
     waves = [poisson_capped_network(n, 2.3, 6, 0.4, rng)]
     for q in range(2):
-        prob = effect_problem(waves[-1], [Effect(e) for e in C1_EFFECTS], C1_THETA, 6.0, 1,
+        prob = effect_problem(waves[-1], [Effect(e) for e in FIT50_EFFECTS], FIT50_THETA, 6.0, 1,
                               seed * 1000 + q)
         waves.append(np.asarray(simulate(prob)[0], np.uint8))
     return waves
 
 
-def c1_estimation(simulate, n: int = 50, seed: int = 1):
-    """(data, effects) of the config-1 fit: conditional, density/recip/transTrip."""
-    return build_saom_data(c1_waves(simulate, n, seed)), [Effect(e) for e in C1_EFFECTS]
+def fit50_estimation(simulate, n: int = 50, seed: int = 1):
+    """(data, effects) of the 50-actor fit: conditional, density/recip/transTrip."""
+    return build_saom_data(fit50_waves(simulate, n, seed)), [Effect(e) for e in FIT50_EFFECTS]
 
 
-def c2_covariate(n: int = 500, seed: int = 2) -> np.ndarray:
+def n500_covariate(n: int = 500, seed: int = 2) -> np.ndarray:
     rng = np.random.default_rng(seed + 1000)
     return rng.integers(1, 6, size=n).astype(float)
 
 
-def c2_effects(x: np.ndarray) -> list:
+def n500_effects(x: np.ndarray) -> list:
     return [Effect("density"), Effect("recip"), Effect("transTrip"), Effect("cycle3"),
             Effect("egoX", x), Effect("altX", x), Effect("simX", x)]
 
 
-def c2_problem(M: int = 4096, n: int = 500, seed: int = 2, x0: np.ndarray | None = None,
+def n500_problem(M: int = 4096, n: int = 500, seed: int = 2, x0: np.ndarray | None = None,
                period_seed: int = 11) -> SaomProblem:
-    """Config-2 period problem from wave 1 (or from the given start network)."""
+    """The n = 500 period problem (seven effects, rate 8) from wave 1 (or from the given start network)."""
     rng = np.random.default_rng(seed)
     w1 = bernoulli_network(n, 5.0, rng)
-    x = c2_covariate(n, seed)
+    x = n500_covariate(n, seed)
     start = w1 if x0 is None else x0
-    return effect_problem(start, c2_effects(x), C2_THETA, 8.0, M, period_seed)
+    return effect_problem(start, n500_effects(x), N500_THETA, 8.0, M, period_seed)
 
 
-def c2_waves(simulate, n: int = 500, seed: int = 2) -> list:
-    """Waves 1 and 2 of config 2; wave 2 is one seeded simulated period from
+def n500_waves(simulate, n: int = 500, seed: int = 2) -> list:
+    """Waves 1 and 2 of the n = 500 workload; wave 2 is one seeded simulated period from
     wave 1 (``simulate`` runs a SaomProblem and returns a_ends)."""
     rng = np.random.default_rng(seed)
     w1 = bernoulli_network(n, 5.0, rng)
-    prob = c2_problem(M=1, n=n, seed=seed, x0=w1, period_seed=seed + 7)
+    prob = n500_problem(M=1, n=n, seed=seed, x0=w1, period_seed=seed + 7)
     w2 = simulate(prob)[0]
     return [w1, np.asarray(w2, np.uint8)]
 
 
-def c3_problems(groups: int = 256, chains: int = 500, seed: int = 3) -> list:
-    """Config 3: `groups` independent networks, n uniform on 30..1200, mean
+def mixed_group_problems(groups: int = 256, chains: int = 500, seed: int = 3) -> list:
+    """`groups` independent networks, n uniform on 30..1200, mean
     outdegree 4, density/recip/transTrip/inPop/egoX/altX on a Bernoulli(0.5)
     covariate, rate 4, `chains` chains each (one period)."""
     rng = np.random.default_rng(seed)
@@ -144,8 +144,8 @@ def c3_problems(groups: int = 256, chains: int = 500, seed: int = 3) -> list:
     return out
 
 
-def c4_problem(M: int = 2048, n: int = 300, seed: int = 4, period_seed: int = 13) -> SaomProblem:
-    """Config-4 co-evolution period problem (period 1 from wave 1)."""
+def coevo300_problem(M: int = 2048, n: int = 300, seed: int = 4, period_seed: int = 13) -> SaomProblem:
+    """Network-behaviour co-evolution period problem at n = 300 (period 1 from wave 1)."""
     rng = np.random.default_rng(seed)
     w1 = bernoulli_network(n, 4.0, rng)
     z = rng.binomial(4, 0.4, size=(n, 3)) + 1

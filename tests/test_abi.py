@@ -69,11 +69,11 @@ def test_argument_errors_without_device_work():
     """Status codes of the reference's checks; none of these reaches a kernel."""
     from cusna_amd import lib
     from cusna_amd.abi import SaomResult, cusna_saom_problem
-    from cusna_amd.workloads import c2_problem
+    from cusna_amd.workloads import n500_problem
 
     L = lib.load()
     assert L.cusna_saom_simulate_period(None, None) == 1
-    prob = c2_problem(M=2, n=40)
+    prob = n500_problem(M=2, n=40)
     p = prob.build()
     res = SaomResult(40, prob.k, 2)
     p.struct_version = 2
@@ -102,12 +102,12 @@ def test_argument_errors_without_device_work():
 def test_no_device_status_on_cpu():
     from cusna_amd import lib
     from cusna_amd.abi import SaomResult
-    from cusna_amd.workloads import c2_problem
+    from cusna_amd.workloads import n500_problem
 
     L = lib.load()
     if L.cusna_has_cuda():
         pytest.skip("a device is present")
-    prob = c2_problem(M=2, n=40)
+    prob = n500_problem(M=2, n=40)
     p = prob.build()
     res = SaomResult(40, prob.k, 2)
     assert L.cusna_saom_simulate_period(ctypes.byref(p), ctypes.byref(res.c)) == 3

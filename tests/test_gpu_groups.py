@@ -17,7 +17,7 @@ pytestmark = pytest.mark.gpu
 
 def _groups():
     from cusna_amd.effects import Effect
-    from cusna_amd.workloads import (bernoulli_network, c2_problem, c4_problem,
+    from cusna_amd.workloads import (bernoulli_network, n500_problem, coevo300_problem,
                                      effect_problem, poisson_capped_network)
 
     rng = np.random.default_rng(21)
@@ -32,8 +32,8 @@ def _groups():
             Effect("outAct"), Effect("egoX", cov), Effect("simX", cov)]
     out.append(effect_problem(x2, eff2, [-2.6, 1.7, -0.1, 0.05, -0.05, 0.1, 0.5], 5.0, 17,
                               103))
-    out.append(c2_problem(M=12, n=500))
-    out.append(c4_problem(M=8, n=300))
+    out.append(n500_problem(M=12, n=500))
+    out.append(coevo300_problem(M=8, n=300))
     # conditional group of the first network
     out.append(effect_problem(x, eff, [-2.5, 2.0, 0.4], 1.0, 11, 104, conditional=True,
                               target_dist=40))
@@ -66,9 +66,9 @@ def test_groups_single_group_and_errors(gpu_lib):
 
 
 def test_autotune_keeps_trajectories(gpu_lib):
-    from cusna_amd.workloads import c2_problem
+    from cusna_amd.workloads import n500_problem
 
-    pr = c2_problem(M=48, n=200)
+    pr = n500_problem(M=48, n=200)
     plan = gpu_lib.PeriodPlan(pr)
     try:
         plan.run(77)

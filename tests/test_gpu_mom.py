@@ -98,7 +98,7 @@ def test_rejects_bad_schedule(gpu_lib):
 
 def test_surrogate_ablation(gpu_lib):
     """Surrogate phase 2 (perceptron root + simulation-checked Newton
-    corrections) against the plain Robbins-Monro phase 2 on the config-1 fit
+    corrections) against the plain Robbins-Monro phase 2 on the 50-actor fit
     (50 actors, conditional, reference default schedule, n3 = 1000): both
     reach max|t| <= 0.1 (repeats allowed), the estimates agree within one
     standard error of the plain fit, and the surrogate uses fewer simulated
@@ -106,9 +106,9 @@ def test_surrogate_ablation(gpu_lib):
     import time
 
     from cusna_amd.mom import MomControl, MomEstimator
-    from cusna_amd.workloads import c1_estimation
+    from cusna_amd.workloads import fit50_estimation
 
-    data, eff = c1_estimation(lambda p: gpu_lib.simulate_period(p, want_end_nets=True)["a_ends"])
+    data, eff = fit50_estimation(lambda p: gpu_lib.simulate_period(p, want_end_nets=True)["a_ends"])
     est = MomEstimator(data, eff, None, True, MomControl(n3=1000))
     est.run(max_repeats=0)  # code objects loaded
     t0 = time.perf_counter()

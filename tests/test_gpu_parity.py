@@ -119,19 +119,19 @@ def test_reference_engine_cases(gpu_lib, idx):
 
 
 def test_replay_n500_bit_exact(gpu_lib, oracle_lib):
-    """Config-2 shape (n = 500, seven effects) on 8 chains in replay mode: the
+    """The n = 500, seven-effect workload on 8 chains in replay mode: the
     device reads the host's recorded log(u0) and log(-log u_j), so the whole
     trajectory, ntim and statistics must be identical to the oracle's."""
-    from cusna_amd.workloads import c2_problem
+    from cusna_amd.workloads import n500_problem
 
-    prob = c2_problem(M=8)
-    print("case replay c2 n=500 M=8", flush=True)
+    prob = n500_problem(M=8)
+    print("case replay n500 M=8", flush=True)
     free = oracle_lib.simulate(prob)
     smax = int(free["steps"].max()) + 64
     lu0, llu = oracle_lib.replay_noise(prob.seed, prob.M, prob.n, smax)
     ref = oracle_lib.simulate(prob, want_end_nets=True, replay=(lu0, llu, smax))
     got = gpu_lib.simulate_period_replay(prob, lu0, llu, want_end_nets=True)
-    assert_period_equal(got, ref, exact_ntim=True, label="replay c2")
+    assert_period_equal(got, ref, exact_ntim=True, label="replay n500")
     np.testing.assert_array_equal(got["a_ends"], ref["a_ends"])
     # replay reproduces the free-running oracle (same recorded noise)
     np.testing.assert_array_equal(ref["steps"], free["steps"])
@@ -139,32 +139,32 @@ def test_replay_n500_bit_exact(gpu_lib, oracle_lib):
 
 def test_free_n500_against_oracle(gpu_lib, oracle_lib):
     """Free running at n = 500: device log/exp, trajectories still identical."""
-    from cusna_amd.workloads import c2_problem
+    from cusna_amd.workloads import n500_problem
 
-    prob = c2_problem(M=16)
-    print("case free c2 n=500 M=16", flush=True)
+    prob = n500_problem(M=16)
+    print("case free n500 M=16", flush=True)
     ref = oracle_lib.simulate(prob, want_end_nets=True)
     got = gpu_lib.simulate_period(prob, want_end_nets=True)
-    assert_period_equal(got, ref, label="free c2")
+    assert_period_equal(got, ref, label="free n500")
     np.testing.assert_array_equal(got["a_ends"], ref["a_ends"])
 
 
 def test_coevolution_avalt_n300(gpu_lib, oracle_lib):
-    """Config-4 shape: behaviour avAlt dynamics at n = 300."""
-    from cusna_amd.workloads import c4_problem
+    """Co-evolution at n = 300: behaviour avAlt dynamics at n = 300."""
+    from cusna_amd.workloads import coevo300_problem
 
-    prob = c4_problem(M=8)
-    print("case c4 n=300 M=8", flush=True)
+    prob = coevo300_problem(M=8)
+    print("case coevo n=300 M=8", flush=True)
     ref = oracle_lib.simulate(prob)
     got = gpu_lib.simulate_period(prob)
-    assert_period_equal(got, ref, label="c4")
+    assert_period_equal(got, ref, label="coevo n300")
 
 
 def test_per_chain_theta(gpu_lib, oracle_lib):
     """Chain c uses theta row c: equal to a run where every row is row c."""
-    from cusna_amd.workloads import c2_problem
+    from cusna_amd.workloads import n500_problem
 
-    prob = c2_problem(M=6, n=200)
+    prob = n500_problem(M=6, n=200)
     rng = np.random.default_rng(5)
     th = np.array(prob.theta, float)[None, :] + rng.normal(scale=0.1, size=(6, prob.k))
     prob.theta = th
@@ -226,9 +226,9 @@ def test_packed_stats_match_dense(gpu_lib, oracle_lib):
 
 def test_period_plan_reuse(gpu_lib, oracle_lib):
     """A device-resident plan run twice with new parameters equals one-shot calls."""
-    from cusna_amd.workloads import c2_problem
+    from cusna_amd.workloads import n500_problem
 
-    prob = c2_problem(M=8, n=150)
+    prob = n500_problem(M=8, n=150)
     plan = gpu_lib.PeriodPlan(prob)
     plan.run(prob.seed)
     a = plan.fetch()

@@ -43,10 +43,10 @@ def test_specialised_matches_generic_and_reference(gpu_lib, idx):
 
 
 def test_specialised_c2_and_streamed(gpu_lib):
-    from cusna_amd.workloads import c2_problem
+    from cusna_amd.workloads import n500_problem
     from test_gpu_stream import CASES, _problem
 
-    _both(gpu_lib, c2_problem(M=64, n=500), seed=11)
+    _both(gpu_lib, n500_problem(M=64, n=500), seed=11)
     n, names, theta, lam, M = CASES["n1100_simple"]
     _both(gpu_lib, _problem(n, names, theta, lam, M, 31), seed=31)
 

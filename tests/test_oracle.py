@@ -76,12 +76,12 @@ def test_reference_stats_cases(oracle_lib):
 
 def test_against_compiled_reference_live(oracle_lib):
     """Where oracle/_ref is built (it needs the reference sources), compare on
-    a fresh config-2-shaped problem, including a conditional run."""
+    a fresh n = 500, seven-effect problem, including a conditional run."""
     if oracle_lib.reference() is None:
         pytest.skip("oracle/_ref not built here")
-    from cusna_amd.workloads import c2_problem
+    from cusna_amd.workloads import n500_problem
 
-    prob = c2_problem(M=3, n=200)
+    prob = n500_problem(M=3, n=200)
     a = oracle_lib.simulate(prob, want_end_nets=True)
     b = oracle_lib.simulate_reference(prob, want_end_nets=True)
     for key in a:
@@ -133,9 +133,9 @@ def test_conditional_stops_at_target(oracle_lib):
 
 def test_thread_count_invariance(oracle_lib):
     """Results do not depend on the OpenMP thread count (test-nets.R:108-114)."""
-    from cusna_amd.workloads import c2_problem
+    from cusna_amd.workloads import n500_problem
 
-    prob = c2_problem(M=6, n=120)
+    prob = n500_problem(M=6, n=120)
     prev = oracle_lib.oracle().oracle_set_threads(1)
     a = oracle_lib.simulate(prob, want_end_nets=True)
     oracle_lib.oracle().oracle_set_threads(max(prev, 2))
