@@ -203,7 +203,7 @@ def rate_measure(lib, x0, cpu: bool, threads: int) -> dict:
     return res
 
 
-def multi_problem_c2(M: int, x0s, seed: int = 77):
+def multi_problem_n500(M: int, x0s, seed: int = 77):
     """Two co-evolving networks of 500 actors (the waves of the n = 500 workload as starts),
     rate 4 each, density/recip/transTrip/crprod on network 0 and
     density/recip/to/crprodRecip on network 1."""
@@ -233,7 +233,7 @@ def multi_measure(lib, x0s, cpu: bool, threads: int) -> dict:
     import numpy as np
 
     M = 4096
-    prob = multi_problem_c2(M, x0s)
+    prob = multi_problem_n500(M, x0s)
     plan = lib.MultiPlan(prob)
     plan.run(1)
     steps = 3
@@ -258,7 +258,7 @@ def multi_measure(lib, x0s, cpu: bool, threads: int) -> dict:
             os.environ["OMP_NUM_THREADS"] = str(threads)
             chains = 2 * threads
             t0 = time.perf_counter()
-            O.simulate_multi_reference(multi_problem_c2(chains, x0s), want_end_nets=False)
+            O.simulate_multi_reference(multi_problem_n500(chains, x0s), want_end_nets=False)
             cw = time.perf_counter() - t0
             res["cpu"] = {"kind": "reference", "cores": threads,
                           "chain_periods_per_s": round(chains / cw, 3),

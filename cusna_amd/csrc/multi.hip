@@ -75,8 +75,8 @@ cusna_status validate_multi(const cusna_saom_multi_problem* pb) {
 }
 
 // Block shape: `threads` / 64 waves per chain rounded down to 1, 2 or 4 (at
-// most the word bound); 0 = about four alternatives per lane. Streamed word
-// bounds run 16 waves.
+// most the word bound); 0 = the default below. Streamed word bounds run 16
+// waves.
 int multi_nw(int wmax, int threads) {
   if (wmax > 16) return 16;
   if (threads > 0) {
@@ -84,7 +84,11 @@ int multi_nw(int wmax, int threads) {
     while (nw * 2 <= threads / 64 && nw * 2 <= std::min(4, wmax)) nw *= 2;
     return nw;
   }
-  return std::max(1, std::min(4, wmax / 4));
+  // default: two words per lane, at most four waves (D networks' rows leave
+  // room for few chains per CU, so the waves of one chain hide the latency;
+  // two networks of 500 actors: 368 / 262 / 218 ms per 4096-chain period at
+  // one / two / four waves per chain)
+  return std::max(1, std::min(4, wmax / 2));
 }
 
 }  // namespace
