@@ -317,16 +317,21 @@ def mom_measure(lib, cpu: bool, threads: int) -> dict:
            "repeats": fit.repeats, "chains_simulated": fit.n_sims,
            "phase_s": [round(x, 3) for x in fit.phase_seconds],
            "theta": [round(float(x), 4) for x in fit.theta]}
-    # same fit with the surrogate phase 2 (perceptron on 2^14 simulated pairs,
-    # then simulation-checked Newton corrections)
-    t0 = time.perf_counter()
-    sf = est.run(max_repeats=20, conv_tol=0.1, surrogate=True)
-    out["surrogate"] = {"wall_s": round(time.perf_counter() - t0, 3),
-                        "max_abs_tconv": round(float(sf.max_abs_tconv), 4),
-                        "repeats": sf.repeats, "chains_simulated": sf.n_sims,
-                        "theta": [round(float(x), 4) for x in sf.theta],
-                        "within_one_se_of_plain": bool(np.all(np.abs(sf.theta - fit.theta)
-                                                              <= fit.se))}
+    # same fit with the surrogate phase 2 (perceptron on 2^13 simulated pairs,
+    # 200 Adam steps, then simulation-checked Newton corrections), and the
+    # ablation without a trained network (64 pairs, one step: the corrections
+    # alone, from the phase-1 Newton point)
+    for key, kw in (("surrogate", {}),
+                    ("newton_only", dict(surrogate_pairs_log2=6, surrogate_steps=1))):
+        t0 = time.perf_counter()
+        sf = est.run(max_repeats=20, conv_tol=0.1, surrogate=True, **kw)
+        out[key] = {"wall_s": round(time.perf_counter() - t0, 3),
+                    "max_abs_tconv": round(float(sf.max_abs_tconv), 4),
+                    "repeats": sf.repeats, "chains_simulated": sf.n_sims,
+                    "theta": [round(float(x), 4) for x in sf.theta],
+                    "within_one_se_of_plain": bool(np.all(np.abs(sf.theta - fit.theta)
+                                                          <= fit.se)),
+                    "wall_fraction_of_plain": round((time.perf_counter() - t0) / wall, 3)}
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import mom_oracle as MO

@@ -75,6 +75,8 @@ class Mlp {
   hipStream_t s_ = nullptr;
   DeviceBlock D_;
   float *params_ = nullptr, *m_ = nullptr, *v_ = nullptr, *grads_ = nullptr;
+  float* part_ = nullptr;  // partial blocks of split products
+  size_t part_floats_ = 0;
   __bf16* Wb_[kMlpLayers] = {};   // W_l [K][N] (products with W^T in the backward pass)
   __bf16* Wtb_[kMlpLayers] = {};  // W_l^T [N][K] (forward products)
   __bf16 *X_ = nullptr, *XT_ = nullptr;

@@ -11,6 +11,12 @@
 // adds the bias, keeps the float32 pre-activation, applies SiLU (forward) or
 // multiplies by SiLU'(z) (backward), and writes bfloat16 row-major and/or
 // transposed copies, so that every later product reads contiguous K.
+//
+// Latency, not throughput, bounds these products (a 1024 x 256 x 256 product
+// is ~0.13 GFLOP): the K loop is unrolled so that a wave's loads for U K
+// steps are all in flight before its MFMAs, and products with a long K and a
+// plain float32 result (the weight gradients, K = batch) are split over K
+// into partial blocks that a second kernel sums in a fixed order.
 #include "mlp.h"
 
 #include <algorithm>
@@ -42,8 +48,12 @@ struct GemmArgs {
   float* Cf;          // M x N float32 result before the activation, or null
   __bf16* Cb;         // M x N bfloat16 result, or null
   __bf16* CbT;        // N x M bfloat16 result transposed, or null
+  int Ks;             // K per split (blockIdx.z); Ks = K without a split
+  float* part;        // split products: partial blocks [gridDim.z][M][N]
 };
 
+// U: K steps of 32 whose loads are issued together (Ks a multiple of 32 U).
+template <int U>
 __global__ void __launch_bounds__(kMlpThreads) gemm_bf16_kernel(const GemmArgs g) {
   const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int m0 = blockIdx.x * 64 + 16 * w, n0 = blockIdx.y * 64;
@@ -53,13 +63,32 @@ __global__ void __launch_bounds__(kMlpThreads) gemm_bf16_kernel(const GemmArgs g
   for (int t = 0; t < 4; ++t) acc[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
   const __bf16* arow = g.A + (size_t)(m0 + r) * g.K + 8 * q;
   const __bf16* brow = g.Bt + (size_t)(n0 + r) * g.K + 8 * q;
-  for (int k0 = 0; k0 < g.K; k0 += 32) {
-    const bf16x8 a = *reinterpret_cast<const bf16x8*>(arow + k0);
+  const int kbeg = blockIdx.z * g.Ks, kend = kbeg + g.Ks;
+  for (int k0 = kbeg; k0 < kend; k0 += 32 * U) {
+    bf16x8 a[U], b[U][4];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      a[u] = *reinterpret_cast<const bf16x8*>(arow + k0 + 32 * u);
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+        b[u][t] = *reinterpret_cast<const bf16x8*>(brow + (size_t)16 * t * g.K + k0 + 32 * u);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u], b[u][t], acc[t], 0, 0, 0);
+    }
+  }
+  if (gridDim.z > 1) {  // partial block of split blockIdx.z
+    float* part = g.part + (size_t)blockIdx.z * g.M * g.N;
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-      const bf16x8 b = *reinterpret_cast<const bf16x8*>(brow + (size_t)16 * t * g.K + k0);
-      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        part[(size_t)(m0 + 4 * q + v) * g.N + n0 + 16 * t + r] = acc[t][v];
     }
+    return;
   }
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
@@ -78,6 +107,15 @@ __global__ void __launch_bounds__(kMlpThreads) gemm_bf16_kernel(const GemmArgs g
       if (g.CbT) g.CbT[(size_t)col * g.M + row] = hv;
     }
   }
+}
+
+// out[i] = sum over s = 0..S-1 of part[s][i], in that order.
+__global__ void sum_parts_kernel(const float* part, int S, int MN, float* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= MN) return;
+  float v = part[i];
+  for (int s = 1; s < S; ++s) v += part[(size_t)s * MN + i];
+  out[i] = v;
 }
 
 // x (B x p_in, float32) -> X (B x Kin) and X^T (Kin x B), bfloat16, zero padded.
@@ -141,9 +179,19 @@ __global__ void __launch_bounds__(kMlpThreads) rowsum_kernel(const __bf16* XT, i
   if (threadIdx.x == 0) out[n] = sh[0];
 }
 
-// Adam (bias-corrected): w -= lr * m_hat / (sqrt(v_hat) + eps).
+// Where the weights of each layer sit in the flat parameter vector and their
+// bfloat16 copies W [K][N] and W^T [N][K].
+struct CastMap {
+  size_t off[kMlpLayers];
+  int K[kMlpLayers], N[kMlpLayers];
+  __bf16* Wb[kMlpLayers];
+  __bf16* Wtb[kMlpLayers];
+};
+
+// Adam (bias-corrected): w -= lr * m_hat / (sqrt(v_hat) + eps); a weight's
+// new value is also written to its two bfloat16 copies (as cast_kernel).
 __global__ void adam_kernel(float* w, float* m, float* v, const float* g, size_t n, float lr,
-                            float b1, float b2, float eps, float c1, float c2) {
+                            float b1, float b2, float eps, float c1, float c2, const CastMap cm) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float gi = g[i];
@@ -151,7 +199,18 @@ __global__ void adam_kernel(float* w, float* m, float* v, const float* g, size_t
   const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
   m[i] = mi;
   v[i] = vi;
-  w[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+  const float wi = w[i] - lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+  w[i] = wi;
+#pragma unroll
+  for (int l = 0; l < kMlpLayers; ++l) {
+    const size_t e = i - cm.off[l];
+    if (i >= cm.off[l] && e < (size_t)cm.K[l] * cm.N[l]) {
+      const int k = (int)(e / cm.N[l]), c = (int)(e - (size_t)k * cm.N[l]);
+      const __bf16 hv = (__bf16)wi;
+      cm.Wb[l][e] = hv;
+      cm.Wtb[l][(size_t)c * cm.K[l] + k] = hv;
+    }
+  }
 }
 
 // Master weights W [K][N] -> bfloat16 W [K][N] and W^T [N][K].
@@ -233,7 +292,14 @@ cusna_status Mlp::init(int p_in, int p_out, int width, int batch, uint64_t seed,
   const size_t o_o = D_.add((size_t)B * N_[3] * 4), o_do = D_.add((size_t)B * N_[3] * 2);
   const size_t o_dot = D_.add((size_t)B * N_[3] * 2), o_loss = D_.add((size_t)B * 4);
   const size_t o_yb = D_.add((size_t)B * p_out * 4), o_xb = D_.add((size_t)B * p_in * 4);
+  // partial blocks of the split weight-gradient products (K = B)
+  part_floats_ = 0;
+  if (B >= 512 && B % 128 == 0)
+    for (int l = 0; l < kMlpLayers; ++l)
+      part_floats_ = std::max(part_floats_, (size_t)(B / 128) * K_[l] * N_[l]);
+  const size_t o_part = D_.add(part_floats_ * 4);
   HIP_TRY(D_.alloc());
+  part_ = D_.at<float>(o_part);
   params_ = D_.at<float>(o_p);
   m_ = D_.at<float>(o_m);
   v_ = D_.at<float>(o_v);
@@ -320,8 +386,24 @@ cusna_status Mlp::gemm(const __bf16* A, const __bf16* Bt, int M, int N, int K, c
     set_error("mlp: product shape not a multiple of the 64 x 64 x 32 tile");
     return CUSNA_ERR_INTERNAL;
   }
-  GemmArgs g{A, Bt, M, N, K, bias, dz, act, Cf, Cb, CbT};
-  hipLaunchKernelGGL(gemm_bf16_kernel, dim3(M / 64, N / 64), dim3(kMlpThreads), 0, s_, g);
+  // split K (128 per split) for long plain float32 products
+  const bool plain = Cf && !bias && !dz && !act && !Cb && !CbT;
+  const int S = (plain && K >= 512 && K % 128 == 0) ? K / 128 : 1;
+  if (S > 1 && (size_t)S * M * N > part_floats_) {
+    set_error("mlp: split-product scratch too small");
+    return CUSNA_ERR_INTERNAL;
+  }
+  GemmArgs g{A, Bt, M, N, K, bias, dz, act, Cf, Cb, CbT, K / S, part_};
+  const dim3 grid(M / 64, N / 64, S);
+  if (g.Ks % 128 == 0)
+    hipLaunchKernelGGL(gemm_bf16_kernel<4>, grid, dim3(kMlpThreads), 0, s_, g);
+  else if (g.Ks % 64 == 0)
+    hipLaunchKernelGGL(gemm_bf16_kernel<2>, grid, dim3(kMlpThreads), 0, s_, g);
+  else
+    hipLaunchKernelGGL(gemm_bf16_kernel<1>, grid, dim3(kMlpThreads), 0, s_, g);
+  if (S > 1)
+    hipLaunchKernelGGL(sum_parts_kernel, dim3((M * N + 255) / 256), dim3(256), 0, s_, part_, S,
+                       M * N, Cf);
   HIP_TRY(hipGetLastError());
   return CUSNA_OK;
 }
@@ -382,10 +464,18 @@ cusna_status Mlp::adam_step(float lr) {
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   const float c1 = (float)(1.0 / (1.0 - std::pow(0.9, (double)t_)));
   const float c2 = (float)(1.0 / (1.0 - std::pow(0.999, (double)t_)));
+  CastMap cm;
+  for (int l = 0; l < kMlpLayers; ++l) {
+    cm.off[l] = off_w_[l];
+    cm.K[l] = K_[l];
+    cm.N[l] = N_[l];
+    cm.Wb[l] = Wb_[l];
+    cm.Wtb[l] = Wtb_[l];
+  }
   hipLaunchKernelGGL(adam_kernel, dim3((unsigned)((n_params_ + 255) / 256)), dim3(256), 0, s_,
-                     params_, m_, v_, grads_, n_params_, lr, b1, b2, eps, c1, c2);
+                     params_, m_, v_, grads_, n_params_, lr, b1, b2, eps, c1, c2, cm);
   HIP_TRY(hipGetLastError());
-  return cast_weights();
+  return CUSNA_OK;
 }
 
 cusna_status Mlp::train(const float* x_all, const float* y_all, int64_t n, int steps, float lr,

@@ -722,18 +722,46 @@ std::vector<double> sandwich_se(const Mat& A, const Mat& S, int p) {
 // residual mean(s) - target and the score-function derivative, the step is
 // guarded as the Robbins-Monro steps are; they stop once every |residual| is
 // below conv_tol moment standard deviations. Leaves theta in the engine.
+// theta on the device: float64 and the float32-rounded copy the kernels read
+// (with the trailing 1 of the per-chain layout).
+cusna_status put_theta(cusna_mom_engine* E, const std::vector<double>& theta) {
+  const int p = E->p;
+  std::vector<double> ts(p + 1);
+  for (int a = 0; a < p; ++a) ts[a] = (double)(float)theta[a];
+  ts[p] = 1.0;
+  HIP_TRY(hipMemcpyAsync(E->theta, theta.data(), p * 8, hipMemcpyHostToDevice, E->s));
+  HIP_TRY(hipMemcpyAsync(E->theta_sim, ts.data(), (p + 1) * 8, hipMemcpyHostToDevice, E->s));
+  HIP_TRY(hipStreamSynchronize(E->s));  // the host vectors go out of scope
+  return CUSNA_OK;
+}
+
+// One Newton step theta -= D^-1 (sbar - t), each coordinate capped at
+// max(1, |theta| / 2), rates kept above a tenth of their value.
+void newton_step(std::vector<double>& theta, const Mat& Dm, const std::vector<double>& sbar,
+                 const std::vector<double>& tgt, int p, int n_rates) {
+  const Mat Di = safe_solve(Dm, p);
+  for (int a = 0; a < p; ++a) {
+    double stp = 0.0;
+    for (int b = 0; b < p; ++b) stp += Di[(size_t)a * p + b] * (sbar[b] - tgt[b]);
+    const double cap = std::max(1.0, 0.5 * std::fabs(theta[a]));
+    stp = std::min(cap, std::max(-cap, stp));
+    if (a < n_rates && theta[a] - stp < 0.1 * theta[a]) stp = 0.5 * theta[a];
+    theta[a] -= stp;
+  }
+}
+
 cusna_status surrogate_phase(cusna_mom_engine* E, const cusna_saom_problem* const* periods,
                              const cusna_mom_options* opt, std::vector<double> c,
                              std::vector<double> h, const std::vector<double>& tgt) {
   const int p = E->p;
   hipStream_t s = E->s;
-  const int q = opt->surrogate_pairs_log2 > 0 ? opt->surrogate_pairs_log2 : 14;
+  const int q = opt->surrogate_pairs_log2 > 0 ? opt->surrogate_pairs_log2 : 13;
   if (q > 20 || ((long long)E->P << q) > (1ll << 30)) {
     set_error("cusna_mom_estimate: surrogate_pairs_log2 too large");
     return CUSNA_ERR_SHAPE;
   }
   const int Mtr = 1 << q;
-  const int steps = opt->surrogate_steps > 0 ? opt->surrogate_steps : 400;
+  const int steps = opt->surrogate_steps > 0 ? opt->surrogate_steps : 200;
   const int rounds = opt->surrogate_rounds > 0 ? opt->surrogate_rounds : 1;
   const int corrections = opt->surrogate_corrections >= 0 ? opt->surrogate_corrections : 4;
   if (E->bscratch_doubles > 0 && Mtr > E->Mmax) {
@@ -799,15 +827,7 @@ cusna_status surrogate_phase(cusna_mom_engine* E, const cusna_saom_problem* cons
   // simulation-checked Newton corrections: residual and score-function
   // derivative from nD plain chains at theta
   std::vector<double> theta = c, sbar(p), msf((size_t)p * p), Dm((size_t)p * p);
-  auto put_theta = [&]() -> cusna_status {
-    std::vector<double> ts(p + 1);
-    for (int a = 0; a < p; ++a) ts[a] = (double)(float)theta[a];
-    ts[p] = 1.0;
-    HIP_TRY(hipMemcpyAsync(E->theta, theta.data(), p * 8, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(E->theta_sim, ts.data(), (p + 1) * 8, hipMemcpyHostToDevice, s));
-    return CUSNA_OK;
-  };
-  if ((st = put_theta()) != CUSNA_OK) return st;
+  if ((st = put_theta(E, theta)) != CUSNA_OK) return st;
   SimSet& X = E->main;
   for (int it = 0; it < corrections; ++it) {
     if ((st = engine_simulate(E, X, opt->nD)) != CUSNA_OK) return st;
@@ -823,16 +843,8 @@ cusna_status surrogate_phase(cusna_mom_engine* E, const cusna_saom_problem* cons
       worst = std::max(worst, std::fabs(sbar[a] - tgt[a]) / (v > 0.0 ? std::sqrt(v) : 1.0));
     }
     if (worst <= opt->conv_tol) break;
-    const Mat Di = safe_solve(Dm, p);
-    for (int a = 0; a < p; ++a) {
-      double stp = 0.0;
-      for (int b = 0; b < p; ++b) stp += Di[(size_t)a * p + b] * (sbar[b] - tgt[b]);
-      const double cap = std::max(1.0, 0.5 * std::fabs(theta[a]));
-      stp = std::min(cap, std::max(-cap, stp));
-      if (a < E->n_rates && theta[a] - stp < 0.1 * theta[a]) stp = 0.5 * theta[a];
-      theta[a] -= stp;
-    }
-    if ((st = put_theta()) != CUSNA_OK) return st;
+    newton_step(theta, Dm, sbar, tgt, p, E->n_rates);
+    if ((st = put_theta(E, theta)) != CUSNA_OK) return st;
   }
   HIP_TRY(hipStreamSynchronize(s));
   return CUSNA_OK;
@@ -1095,9 +1107,16 @@ CUSNA_API cusna_status cusna_mom_estimate(const cusna_saom_problem* const* perio
     if (max_abs_t <= opt->conv_tol || repeats >= opt->max_repeats) break;
     ++repeats;
     t_phase = clk::now();
-    const int last = opt->nsub > 0 ? opt->nsub - 1 : 0;
-    const double gain = opt->firstg * std::pow(0.5, last);
-    if ((st = subphase(gain, opt->nsub > 0 ? opt->n2[last] : 10)) != CUSNA_OK) return st;
+    if (opt->surrogate) {
+      // surrogate path: one more simulation-checked Newton step, from this
+      // phase 3's own means and score-function derivative
+      newton_step(theta_h, Dm, sim_means, tgt, p, E->n_rates);
+      if ((st = put_theta(E.get(), theta_h)) != CUSNA_OK) return st;
+    } else {
+      const int last = opt->nsub > 0 ? opt->nsub - 1 : 0;
+      const double gain = opt->firstg * std::pow(0.5, last);
+      if ((st = subphase(gain, opt->nsub > 0 ? opt->n2[last] : 10)) != CUSNA_OK) return st;
+    }
     phase_t[1] += std::chrono::duration<double>(clk::now() - t_phase).count();
   }
 

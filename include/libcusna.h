@@ -471,8 +471,10 @@ CUSNA_API cusna_status cusna_saom_simulate_groups(const cusna_saom_problem *cons
  * phase 3 (n3 chains) follow R/mom.R:218-269; every simulation and moment
  * reduction runs on the device, phase-2 iterations replayed from a HIP graph
  * when use_graph is set. With max_repeats > 0, a phase-3 result with
- * max|tconv| > conv_tol is followed by one more last-gain sub-phase and a new
- * phase 3, up to max_repeats times (no counterpart in the reference).
+ * max|tconv| > conv_tol is followed by one more last-gain sub-phase (with
+ * the surrogate: one Newton step from that phase 3's means and score-function
+ * derivative) and a new phase 3, up to max_repeats times (no counterpart in
+ * the reference).
  */
 typedef struct cusna_mom_options {
   int32_t struct_version; /* CUSNA_ABI_VERSION */
@@ -493,14 +495,15 @@ typedef struct cusna_mom_options {
   double beh_zbar, beh_brange, beh_bsimmean;
   /* Surrogate phase 2 (no counterpart in the reference) in place of the
    * Robbins-Monro sub-phases: per round, 2^surrogate_pairs_log2 chains (0:
-   * 2^14) with per-chain theta drawn uniformly from a box around the phase-1
+   * 2^13) with per-chain theta drawn uniformly from a box around the phase-1
    * Newton point, the surrogate perceptron trained on them for
-   * surrogate_steps Adam steps (0: 400), E s(theta) = target solved on its
+   * surrogate_steps Adam steps (0: 200), E s(theta) = target solved on its
    * mean; the box re-centres on the solution and shrinks to two standard
    * errors over surrogate_rounds rounds (0: 1). Then up to
    * surrogate_corrections (negative: 4) Newton steps with residual and
    * score-function derivative from nD plain chains, stopping once every
-   * residual is below conv_tol moment standard deviations. Phase 3 as usual. */
+   * residual is below conv_tol moment standard deviations. Phase 3 as usual,
+   * its repeats as above. */
   int32_t surrogate;
   int32_t surrogate_pairs_log2;
   int32_t surrogate_steps;

@@ -5,7 +5,8 @@ Tolerances: the device accumulates in float32 (the oracle in float64), and a
 float32-vs-float64 difference can move a bfloat16 rounding by one unit
 (2^-8 relative) for an occasional element; the comparisons are relative to
 each array's scale: outputs and loss 1e-3, gradients 2e-2 per element of the
-largest magnitude and 5e-3 in norm.
+largest magnitude and 5e-3 in norm; outputs 1e-3 of the scale for 99.9 % of
+the elements and 5e-3 for all (a flipped rounding of one hidden unit).
 """
 import numpy as np
 import pytest
@@ -15,10 +16,13 @@ import mlp_oracle as MO
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("p_in,p_out", [(3, 3), (13, 13), (40, 7)])
-def test_forward_backward_matches_oracle(gpu_lib, p_in, p_out):
-    rng = np.random.default_rng(p_in)
-    B, width = 128, 256
+@pytest.mark.parametrize("p_in,p_out,B", [(3, 3, 128), (13, 13, 128), (40, 7, 128),
+                                          (13, 13, 1024)])
+def test_forward_backward_matches_oracle(gpu_lib, p_in, p_out, B):
+    """Batch 1024: the weight gradients (K = batch) run as split products
+    summed by a second kernel."""
+    rng = np.random.default_rng(p_in + B)
+    width = 256
     m = gpu_lib.Mlp(p_in, p_out, width=width, batch=B, seed=3)
     try:
         flat = MO.init_flat(p_in, p_out, width, rng).astype(np.float32)
@@ -33,7 +37,10 @@ def test_forward_backward_matches_oracle(gpu_lib, p_in, p_out):
     N3 = MO.pad64(2 * p_out)
     out, rout = got["out"].astype(np.float64), ref["out"]
     assert out.shape == (B, N3)
-    assert np.max(np.abs(out - rout)) <= 1e-3 * np.max(np.abs(rout))
+    err, scale = np.abs(out - rout), np.max(np.abs(rout))
+    # a flipped bfloat16 rounding of one hidden unit moves an output by ~1e-3
+    # of the scale; at batch 1024 a few elements see one
+    assert np.quantile(err, 0.999) <= 1e-3 * scale and err.max() <= 5e-3 * scale, err.max()
     assert abs(got["loss"] - ref["loss"]) <= 1e-3 * abs(ref["loss"])
     g, rg = got["grads"].astype(np.float64), ref["grads"]
     o = 0
@@ -47,6 +54,29 @@ def test_forward_backward_matches_oracle(gpu_lib, p_in, p_out):
     # padding rows of the first layer and padding output columns stay at zero gradient
     K0 = MO.pad64(p_in)
     assert np.all(g[:K0 * width].reshape(K0, width)[p_in:] == 0.0)
+
+
+def test_adam_keeps_bfloat16_copies_of_the_weights(gpu_lib):
+    """The Adam kernel writes each updated weight to its bfloat16 copies (row-
+    major and transposed, which the products read): after training steps the
+    device's outputs equal the oracle's forward pass on the master weights."""
+    rng = np.random.default_rng(8)
+    p_in, p_out, B = 5, 4, 1024
+    m = gpu_lib.Mlp(p_in, p_out, width=256, batch=B, seed=2)
+    try:
+        xs = rng.uniform(-1, 1, (4096, p_in))
+        ys = rng.normal(size=(4096, p_out))
+        m.train(xs, ys, steps=5, lr=1e-2, seed=4)
+        flat = m.get_params()
+        x = rng.uniform(-1, 1, (B, p_in)).astype(np.float32)
+        y = rng.normal(size=(B, p_out)).astype(np.float32)
+        got = m.forward_backward(x, y)
+    finally:
+        m.close()
+    ref = MO.forward_backward(flat.astype(np.float64), x.astype(np.float64),
+                              y.astype(np.float64), p_in, p_out, 256, bf16=True)
+    err, scale = np.abs(got["out"] - ref["out"]), np.max(np.abs(ref["out"]))
+    assert np.quantile(err, 0.999) <= 1e-3 * scale and err.max() <= 5e-3 * scale, err.max()
 
 
 def test_training_fits_a_smooth_map(gpu_lib):
