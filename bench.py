@@ -294,6 +294,36 @@ def multi_measure(lib, x0s, cpu: bool, threads: int) -> dict:
     return res
 
 
+def mlp_measure(lib) -> dict:
+    """Training of the surrogate perceptron at the shape the surrogate phase
+    uses (3 parameters in, 3 moments out, width 256, batch 1024, 2^13
+    samples, 200 Adam steps): wall time of the train call after one warm-up
+    call. Per step: 11 BF16 MFMA products (4 forward, 7 backward)."""
+    import numpy as np
+
+    rng = np.random.default_rng(3)
+    p, n, steps, B, width = 3, 1 << 13, 200, 1024, 256
+    x = rng.uniform(-1, 1, (n, p)).astype(np.float32)
+    y = (np.tanh(x @ rng.normal(size=(p, p))) + 0.1 * rng.normal(size=(n, p))).astype(np.float32)
+    m = lib.Mlp(p, p, width=width, batch=B, seed=5)
+    try:
+        m.train(x, y, steps=10, lr=2e-3, seed=1)
+        t0 = time.perf_counter()
+        loss = m.train(x, y, steps=steps, lr=2e-3, seed=2)
+        wall = time.perf_counter() - t0
+    finally:
+        m.close()
+    # forward 2 B (64 x 256 + 2 x 256^2 + 256 x 64) FLOP (padded shapes), backward
+    # twice that: dW for all four layers, dH for the upper three
+    fwd = 2.0 * B * (64 * width + 2 * width * width + width * 64)
+    flops = fwd + 2.0 * fwd - 2.0 * B * 64 * width
+    return {"workload": "perceptron training: p=3 -> 256 -> 256 -> 256 -> 2p, batch 1024, "
+                        "2^13 samples, 200 Adam steps (the surrogate phase's shape)",
+            "ms_per_step": round(1e3 * wall / steps, 4), "final_loss": round(float(loss), 4),
+            "gflop_per_step": round(flops / 1e9, 4),
+            "tflops": round(flops * steps / wall / 1e12, 3)}
+
+
 def mom_measure(lib, cpu: bool, threads: int) -> dict:
     """Fit of 50 actors, 3 waves, conditional MoM on density/recip/
     transTrip with the reference's default schedule (n3 = 1000), repeated
@@ -374,6 +404,9 @@ def main() -> int:
     ap.add_argument("--no-multi", action="store_true")
     ap.add_argument("--no-gof", action="store_true")
     ap.add_argument("--no-rate", action="store_true")
+    ap.add_argument("--no-mlp", action="store_true")
+    ap.add_argument("--mlp-only", action="store_true",
+                    help="only the perceptron training section (for counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has one GPU, and the "
@@ -389,6 +422,9 @@ def main() -> int:
     if not lib.has_device():
         print("no HIP device", file=sys.stderr)
         return 3
+    if args.mlp_only:
+        print(json.dumps({"mlp": mlp_measure(lib)}), flush=True)
+        return 0
     n, M = args.n, args.chains
     w1, w2 = n500_waves(lambda p: lib.simulate_period(p, want_end_nets=True)["a_ends"], n=n)
     probs = [n500_problem(M=M, n=n, x0=w, period_seed=11 + q) for q, w in enumerate((w1, w2))]
@@ -462,6 +498,8 @@ def main() -> int:
     if not args.no_mom:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
         line["mom"] = mom_measure(lib, not args.no_cpu, threads)
+    if not args.no_mlp:
+        line["mlp"] = mlp_measure(lib)
     print(json.dumps(line), flush=True)
     return 0
 
