@@ -16,7 +16,9 @@
 // is ~0.13 GFLOP): the K loop is unrolled so that a wave's loads for U K
 // steps are all in flight before its MFMAs, and products with a long K and a
 // plain float32 result (the weight gradients, K = batch) are split over K
-// into partial blocks that a second kernel sums in a fixed order.
+// into partial blocks that a second kernel sums in split order
+// (deterministic). The bfloat16 outputs pass through LDS so that the
+// row-major and transposed copies are written as 16-byte stores.
 #include "mlp.h"
 
 #include <algorithm>
@@ -73,6 +75,9 @@ __global__ void __launch_bounds__(kMlpThreads) gemm_bf16_kernel(const GemmArgs g
       for (int t = 0; t < 4; ++t)
         b[u][t] = *reinterpret_cast<const bf16x8*>(brow + (size_t)16 * t * g.K + k0 + 32 * u);
     }
+    // every load of the U steps is issued before the first MFMA (the default
+    // schedule keeps only ~7 loads in flight to save registers)
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
 #pragma unroll
@@ -80,8 +85,10 @@ __global__ void __launch_bounds__(kMlpThreads) gemm_bf16_kernel(const GemmArgs g
         acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u], b[u][t], acc[t], 0, 0, 0);
     }
   }
+  const int mt = blockIdx.x * 64;  // the workgroup's 64 x 64 tile: rows mt.., columns n0..
   if (gridDim.z > 1) {  // partial block of split blockIdx.z
-    float* part = g.part + (size_t)blockIdx.z * g.M * g.N;
+    const size_t MN = (size_t)g.M * g.N;
+    float* part = g.part + (size_t)blockIdx.z * MN;
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
 #pragma unroll
@@ -90,6 +97,7 @@ __global__ void __launch_bounds__(kMlpThreads) gemm_bf16_kernel(const GemmArgs g
     }
     return;
   }
+  __shared__ __align__(16) __bf16 tile[64][72];  // bfloat16 outputs, rows padded
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
     const int col = n0 + 16 * t + r;
@@ -102,10 +110,28 @@ __global__ void __launch_bounds__(kMlpThreads) gemm_bf16_kernel(const GemmArgs g
       if (g.Cf) g.Cf[ix] = val;
       if (g.dz) val *= dsilu(g.dz[ix]);
       else if (g.act) val = silu(val);
-      const __bf16 hv = (__bf16)val;
-      if (g.Cb) g.Cb[ix] = hv;
-      if (g.CbT) g.CbT[(size_t)col * g.M + row] = hv;
+      tile[row - mt][col - n0] = (__bf16)val;
     }
+  }
+  if (!g.Cb && !g.CbT) return;
+  __syncthreads();
+  const int a = threadIdx.x >> 2, c16 = 16 * (threadIdx.x & 3);
+  if (g.Cb) {  // row a of the tile, columns c16..c16+15
+    const bf16x8* src = reinterpret_cast<const bf16x8*>(&tile[a][c16]);
+    bf16x8* dst = reinterpret_cast<bf16x8*>(g.Cb + (size_t)(mt + a) * g.N + n0 + c16);
+    dst[0] = src[0];
+    dst[1] = src[1];
+  }
+  if (g.CbT) {  // column a of the tile, rows c16..c16+15, as a row of the transpose
+    bf16x8 lo, hi;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      lo[e] = tile[c16 + e][a];
+      hi[e] = tile[c16 + 8 + e][a];
+    }
+    bf16x8* dst = reinterpret_cast<bf16x8*>(g.CbT + (size_t)(n0 + a) * g.M + mt + c16);
+    dst[0] = lo;
+    dst[1] = hi;
   }
 }
 
@@ -395,7 +421,9 @@ cusna_status Mlp::gemm(const __bf16* A, const __bf16* Bt, int M, int N, int K, c
   }
   GemmArgs g{A, Bt, M, N, K, bias, dz, act, Cf, Cb, CbT, K / S, part_};
   const dim3 grid(M / 64, N / 64, S);
-  if (g.Ks % 128 == 0)
+  if (g.Ks % 256 == 0)
+    hipLaunchKernelGGL(gemm_bf16_kernel<8>, grid, dim3(kMlpThreads), 0, s_, g);
+  else if (g.Ks % 128 == 0)
     hipLaunchKernelGGL(gemm_bf16_kernel<4>, grid, dim3(kMlpThreads), 0, s_, g);
   else if (g.Ks % 64 == 0)
     hipLaunchKernelGGL(gemm_bf16_kernel<2>, grid, dim3(kMlpThreads), 0, s_, g);
