@@ -324,6 +324,61 @@ def mlp_measure(lib) -> dict:
             "tflops": round(flops * steps / wall / 1e12, 3)}
 
 
+GROUPS_MOM_RUNS = {
+    "plain": {},
+    # one plain sub-phase, then the surrogate (2^12 pairs) around its theta
+    "surrogate": dict(surrogate=True, surrogate_warmup=1, surrogate_pairs_log2=12),
+    "surrogate_2e13": dict(surrogate=True, surrogate_warmup=1),
+    # the surrogate straight from phase 1 (its plain fallback engages when
+    # the corrections leave a large residual)
+    "surrogate_from_phase1": dict(surrogate=True),
+}
+
+
+def groups_mom_measure(lib, runs: list) -> dict:
+    """Surrogate-assisted against plain MoM on 16 groups of 1000 actors, 12
+    effects, one period each (28 parameters: 16 rates + 12 effects),
+    phase 3 with 2000 chains, the reference's default schedule. Minutes of
+    device time: run with --groups-mom only. Progress lines on stderr."""
+    import numpy as np
+
+    from cusna_amd.mom import GroupMomEstimator, MomControl
+    from cusna_amd.workloads import groups1000_estimation
+
+    t0 = time.perf_counter()
+    datas, effs, theta, rates = groups1000_estimation(
+        lambda p: lib.simulate_period(p, want_end_nets=True)["a_ends"])
+    print(f"[groups-mom] data built in {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+          flush=True)
+    est = GroupMomEstimator(datas, effs, MomControl(n3=2000))
+    out = {"workload": "16 groups x 1000 actors, one period each, 12 effects (density, recip, "
+                       "transTrip, cycle3, inPop, outAct, egoX/altX/simX on N(0,1) and on a "
+                       "1..5 covariate) + 16 rates; unconditional; n1=nD=1000, n2=40/40/60/80 "
+                       "x 256, n3=2000; then repeats until max|t|<=0.1",
+           "true_theta": [round(float(x), 4) for x in theta]}
+    fits = {}
+    for key in runs:
+        kw = GROUPS_MOM_RUNS[key]
+        print(f"[groups-mom] {key}", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        fit = est.run(max_repeats=10, conv_tol=0.1, verbose=True, **kw)
+        wall = time.perf_counter() - t0
+        fits[key] = fit
+        out[key] = {"wall_s": round(wall, 2), "max_abs_tconv": round(float(fit.max_abs_tconv), 4),
+                    "repeats": fit.repeats, "chains_simulated": fit.n_sims,
+                    "theta": [round(float(x), 4) for x in fit.theta[est.P:]]}
+    if "plain" in fits:
+        plain = fits["plain"]
+        out["plain"]["se"] = [round(float(x), 4) for x in plain.se[est.P:]]
+        for key, fit in fits.items():
+            if key != "plain":
+                out[key]["within_one_se_of_plain"] = bool(np.all(np.abs(fit.theta - plain.theta)
+                                                                 <= plain.se))
+                out[key]["wall_fraction_of_plain"] = round(out[key]["wall_s"]
+                                                           / out["plain"]["wall_s"], 3)
+    return out
+
+
 def mom_measure(lib, cpu: bool, threads: int) -> dict:
     """Fit of 50 actors, 3 waves, conditional MoM on density/recip/
     transTrip with the reference's default schedule (n3 = 1000), repeated
@@ -407,6 +462,9 @@ def main() -> int:
     ap.add_argument("--no-mlp", action="store_true")
     ap.add_argument("--mlp-only", action="store_true",
                     help="only the perceptron training section (for counter runs)")
+    ap.add_argument("--groups-mom", default="", metavar="RUNS",
+                    help="only the 16 x 1000-actor MoM comparison, comma-separated runs of "
+                         + "/".join(GROUPS_MOM_RUNS) + " (minutes each)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has one GPU, and the "
@@ -424,6 +482,13 @@ def main() -> int:
         return 3
     if args.mlp_only:
         print(json.dumps({"mlp": mlp_measure(lib)}), flush=True)
+        return 0
+    if args.groups_mom:
+        runs = args.groups_mom.split(",")
+        if any(r not in GROUPS_MOM_RUNS for r in runs):
+            print(f"--groups-mom runs: {sorted(GROUPS_MOM_RUNS)}", file=sys.stderr)
+            return 2
+        print(json.dumps({"groups_mom": groups_mom_measure(lib, runs)}), flush=True)
         return 0
     n, M = args.n, args.chains
     w1, w2 = n500_waves(lambda p: lib.simulate_period(p, want_end_nets=True)["a_ends"], n=n)

@@ -89,7 +89,8 @@ class cusna_mom_options(C.Structure):
         ("beh_zbar", C.c_double), ("beh_brange", C.c_double), ("beh_bsimmean", C.c_double),
         ("surrogate", C.c_int32), ("surrogate_pairs_log2", C.c_int32),
         ("surrogate_steps", C.c_int32), ("surrogate_rounds", C.c_int32),
-        ("surrogate_corrections", C.c_int32), ("specialize", C.c_int32),
+        ("surrogate_corrections", C.c_int32), ("specialize", C.c_int32), ("verbose", C.c_int32),
+        ("surrogate_warmup", C.c_int32),
     ]
 
 

@@ -32,6 +32,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -492,6 +493,7 @@ struct SimSet {
 
 struct cusna_mom_engine {
   int P = 0, k = 0, kb = 0, p = 0, mode = 0, n = 0, Mmax = 0, n_rates = 0, W64 = 0;
+  std::chrono::steady_clock::time_point t0;  // start of the estimate (progress lines)
   SimSet main;
   DeviceBlock D;
   hipStream_t s = nullptr;
@@ -722,6 +724,24 @@ std::vector<double> sandwich_se(const Mat& A, const Mat& S, int p) {
 // residual mean(s) - target and the score-function derivative, the step is
 // guarded as the Robbins-Monro steps are; they stop once every |residual| is
 // below conv_tol moment standard deviations. Leaves theta in the engine.
+// Seconds since the estimate started (prefix of the progress lines).
+double elapsed(const cusna_mom_engine* E) {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - E->t0).count();
+}
+
+// A progress line on stderr (cusna_mom_options.verbose): the tag and the
+// current theta, as the R estimator's verbose output (mom.R:230, 246).
+cusna_status log_theta(cusna_mom_engine* E, const char* tag) {
+  std::vector<double> th(E->p);
+  cusna_status st = fetch(E, th.data(), E->theta, E->p);
+  if (st != CUSNA_OK) return st;
+  std::fprintf(stderr, "[%.2f s] %s theta =", elapsed(E), tag);
+  for (double v : th) std::fprintf(stderr, " %.3f", v);
+  std::fprintf(stderr, "\n");
+  std::fflush(stderr);
+  return CUSNA_OK;
+}
+
 // theta on the device: float64 and the float32-rounded copy the kernels read
 // (with the trailing 1 of the per-chain layout).
 cusna_status put_theta(cusna_mom_engine* E, const std::vector<double>& theta) {
@@ -736,23 +756,32 @@ cusna_status put_theta(cusna_mom_engine* E, const std::vector<double>& theta) {
 }
 
 // One Newton step theta -= D^-1 (sbar - t), each coordinate capped at
-// max(1, |theta| / 2), rates kept above a tenth of their value.
+// max(1, |theta| / 2) and, given standard errors, at two of them; rates kept
+// above a tenth of their value.
 void newton_step(std::vector<double>& theta, const Mat& Dm, const std::vector<double>& sbar,
-                 const std::vector<double>& tgt, int p, int n_rates) {
+                 const std::vector<double>& tgt, int p, int n_rates,
+                 const std::vector<double>* se = nullptr) {
   const Mat Di = safe_solve(Dm, p);
   for (int a = 0; a < p; ++a) {
     double stp = 0.0;
     for (int b = 0; b < p; ++b) stp += Di[(size_t)a * p + b] * (sbar[b] - tgt[b]);
-    const double cap = std::max(1.0, 0.5 * std::fabs(theta[a]));
+    double cap = std::max(1.0, 0.5 * std::fabs(theta[a]));
+    if (se && (*se)[a] > 0.0) cap = std::min(cap, 2.0 * (*se)[a]);
     stp = std::min(cap, std::max(-cap, stp));
     if (a < n_rates && theta[a] - stp < 0.1 * theta[a]) stp = 0.5 * theta[a];
     theta[a] -= stp;
   }
 }
 
+// Surrogate phase 2; *worst_out: the smallest largest residual (in moment
+// standard deviations) the corrections measured (the caller falls back to
+// the plain sub-phases above kSurrogateFallback).
+constexpr double kSurrogateFallback = 3.0;
+
 cusna_status surrogate_phase(cusna_mom_engine* E, const cusna_saom_problem* const* periods,
                              const cusna_mom_options* opt, std::vector<double> c,
-                             std::vector<double> h, const std::vector<double>& tgt) {
+                             std::vector<double> h, const std::vector<double>& tgt,
+                             double* worst_out) {
   const int p = E->p;
   hipStream_t s = E->s;
   const int q = opt->surrogate_pairs_log2 > 0 ? opt->surrogate_pairs_log2 : 13;
@@ -807,6 +836,11 @@ cusna_status surrogate_phase(cusna_mom_engine* E, const cusna_saom_problem* cons
       tn[a] = (tgt[a] - mean[a]) / sd[a];
     }
     surrogate_newton(H, tn, x, Jn, lv);
+    if (opt->verbose) {
+      std::fprintf(stderr, "[%.2f s] [surrogate %d] %d pairs simulated, %d steps trained\n",
+                   elapsed(E), r + 1, Mtr, steps);
+      std::fflush(stderr);
+    }
     bool interior = true;
     for (int a = 0; a < p; ++a) {
       c[a] = c[a] + h[a] * x[a];
@@ -829,6 +863,7 @@ cusna_status surrogate_phase(cusna_mom_engine* E, const cusna_saom_problem* cons
   std::vector<double> theta = c, sbar(p), msf((size_t)p * p), Dm((size_t)p * p);
   if ((st = put_theta(E, theta)) != CUSNA_OK) return st;
   SimSet& X = E->main;
+  double best = 1e300;
   for (int it = 0; it < corrections; ++it) {
     if ((st = engine_simulate(E, X, opt->nD)) != CUSNA_OK) return st;
     engine_derivative(E, X, opt->nD);
@@ -842,10 +877,18 @@ cusna_status surrogate_phase(cusna_mom_engine* E, const cusna_saom_problem* cons
       const double v = msf[(size_t)a * p + a];
       worst = std::max(worst, std::fabs(sbar[a] - tgt[a]) / (v > 0.0 ? std::sqrt(v) : 1.0));
     }
+    if (opt->verbose) {
+      std::fprintf(stderr, "[%.2f s] [correction %d] worst residual %.4f sd\n", elapsed(E),
+                   it + 1, worst);
+      std::fflush(stderr);
+    }
+    best = std::min(best, worst);
     if (worst <= opt->conv_tol) break;
     newton_step(theta, Dm, sbar, tgt, p, E->n_rates);
     if ((st = put_theta(E, theta)) != CUSNA_OK) return st;
   }
+  if (corrections == 0) best = 0.0;  // nothing measured: no fallback
+  *worst_out = best;
   HIP_TRY(hipStreamSynchronize(s));
   return CUSNA_OK;
 }
@@ -896,6 +939,7 @@ CUSNA_API cusna_status cusna_mom_estimate(const cusna_saom_problem* const* perio
   }
 
   std::unique_ptr<cusna_mom_engine> E(new cusna_mom_engine());
+  E->t0 = t_start;
   E->P = P;
   E->k = k;
   E->kb = kb;
@@ -1000,6 +1044,7 @@ CUSNA_API cusna_status cusna_mom_estimate(const cusna_saom_problem* const* perio
   hipLaunchKernelGGL(mom_update_kernel, dim3(1), dim3(64), 0, s, E->theta, E->theta_sim,
                      E->thav, E->dinv, X.mean, E->targets, p, 0.5 * opt->firstg, E->n_rates, 0);
   phase_t[0] = std::chrono::duration<double>(clk::now() - t_phase).count();
+  if (opt->verbose && (st = log_theta(E.get(), "[phase 1]")) != CUSNA_OK) return st;
   t_phase = clk::now();
 
   // ---- phase 2 (mom.R:232-248), sub-phases replayed from a captured graph
@@ -1045,12 +1090,37 @@ CUSNA_API cusna_status cusna_mom_estimate(const cusna_saom_problem* const* perio
                        E->thav, p, (double)n2);
     return CUSNA_OK;
   };
+  auto run_sub = [&](int sub) -> cusna_status {
+    const double gain = opt->firstg * std::pow(0.5, sub);
+    cusna_status sst = subphase(gain, opt->n2[sub]);
+    if (sst != CUSNA_OK) return sst;
+    if (opt->verbose) {
+      char tag[64];
+      std::snprintf(tag, sizeof tag, "[phase 2.%d] gain=%.3f", sub + 1, gain);
+      return log_theta(E.get(), tag);
+    }
+    return CUSNA_OK;
+  };
   if (opt->surrogate) {
-    // box from phase 1: Newton point of the phase-1 moments and derivative,
-    // half-widths covering the step and three standard errors
+    // plain sub-phases first when asked; then the box: Newton point of the
+    // moments and derivative at the current theta (phase 1's, or nD chains at
+    // the sub-phases' theta), half-widths covering the step and three
+    // standard errors
+    const int warm = std::min(std::max(opt->surrogate_warmup, 0), opt->nsub);
+    for (int sub = 0; sub < warm; ++sub)
+      if ((st = run_sub(sub)) != CUSNA_OK) return st;
+    std::vector<double> base = th0;
+    int nb = opt->n1;
+    if (warm > 0) {
+      if ((st = fetch(E.get(), base.data(), E->theta, p)) != CUSNA_OK) return st;
+      if ((st = engine_simulate(E.get(), X, opt->nD)) != CUSNA_OK) return st;
+      engine_derivative(E.get(), X, opt->nD);
+      if ((st = fetch(E.get(), Dm.data(), E->Dm, (size_t)p * p)) != CUSNA_OK) return st;
+      nb = opt->nD;
+    }
     std::vector<double> m1(p), msf1((size_t)p * p);
     hipLaunchKernelGGL(mom_cov_kernel, dim3(p * p), dim3(kMomThreads), 0, s, X.S, X.mean, X.S,
-                       X.mean, opt->n1, p, (double)(opt->n1 - 1), 0, E->msf);
+                       X.mean, nb, p, (double)(nb - 1), 0, E->msf);
     if ((st = fetch(E.get(), m1.data(), X.mean, p)) != CUSNA_OK) return st;
     if ((st = fetch(E.get(), msf1.data(), E->msf, (size_t)p * p)) != CUSNA_OK) return st;
     const Mat D1i = safe_solve(Dm, p);
@@ -1059,18 +1129,32 @@ CUSNA_API cusna_status cusna_mom_estimate(const cusna_saom_problem* const* perio
     for (int a = 0; a < p; ++a) {
       double step = 0.0;
       for (int b = 0; b < p; ++b) step += D1i[(size_t)a * p + b] * (m1[b] - tgt[b]);
-      const double cap = 2.0 * std::max(1.0, std::fabs(th0[a]));
+      const double cap = 2.0 * std::max(1.0, std::fabs(base[a]));
       step = std::min(cap, std::max(-cap, step));
-      c[a] = th0[a] - step;
-      if (a < E->n_rates && c[a] < 0.2 * th0[a]) c[a] = 0.2 * th0[a];
-      h[a] = std::max(std::max(2.0 * std::fabs(c[a] - th0[a]), 3.0 * se1[a]),
+      c[a] = base[a] - step;
+      if (a < E->n_rates && c[a] < 0.2 * base[a]) c[a] = 0.2 * base[a];
+      h[a] = std::max(std::max(2.0 * std::fabs(c[a] - base[a]), 3.0 * se1[a]),
                       1e-3 * (1.0 + std::fabs(c[a])));
     }
-    if ((st = surrogate_phase(E.get(), periods, opt, c, h, tgt)) != CUSNA_OK) return st;
-  } else {
-    for (int sub = 0; sub < opt->nsub; ++sub) {
-      if ((st = subphase(opt->firstg * std::pow(0.5, sub), opt->n2[sub])) != CUSNA_OK) return st;
+    std::vector<double> start(p);
+    if ((st = fetch(E.get(), start.data(), E->theta, p)) != CUSNA_OK) return st;
+    double worst = 0.0;
+    if ((st = surrogate_phase(E.get(), periods, opt, c, h, tgt, &worst)) != CUSNA_OK) return st;
+    if (worst > kSurrogateFallback) {
+      // the plain estimator is the fallback, from the theta the surrogate
+      // started at (a failed surrogate's points can be far off)
+      if ((st = put_theta(E.get(), start)) != CUSNA_OK) return st;
+      if (opt->verbose) {
+        std::fprintf(stderr, "[%.2f s] [surrogate] residual %.2f sd: plain sub-phases %d..%d\n",
+                     elapsed(E.get()), worst, warm + 1, opt->nsub);
+        std::fflush(stderr);
+      }
+      for (int sub = warm; sub < opt->nsub; ++sub)
+        if ((st = run_sub(sub)) != CUSNA_OK) return st;
     }
+  } else {
+    for (int sub = 0; sub < opt->nsub; ++sub)
+      if ((st = run_sub(sub)) != CUSNA_OK) return st;
   }
   phase_t[1] = std::chrono::duration<double>(clk::now() - t_phase).count();
 
@@ -1104,13 +1188,18 @@ CUSNA_API cusna_status cusna_mom_estimate(const cusna_saom_problem* const* perio
     tconv_max = std::sqrt(q);
     se = sandwich_se(Dm, msf, p);  // diag(D^-1 msf D^-T)
     phase_t[2] += std::chrono::duration<double>(clk::now() - t_phase).count();
+    if (opt->verbose) {
+      std::fprintf(stderr, "[%.2f s] [phase 3] max|tconv| = %.4f (repeat %d)\n",
+                   elapsed(E.get()), max_abs_t, repeats);
+      std::fflush(stderr);
+    }
     if (max_abs_t <= opt->conv_tol || repeats >= opt->max_repeats) break;
     ++repeats;
     t_phase = clk::now();
     if (opt->surrogate) {
       // surrogate path: one more simulation-checked Newton step, from this
       // phase 3's own means and score-function derivative
-      newton_step(theta_h, Dm, sim_means, tgt, p, E->n_rates);
+      newton_step(theta_h, Dm, sim_means, tgt, p, E->n_rates, &se);
       if ((st = put_theta(E.get(), theta_h)) != CUSNA_OK) return st;
     } else {
       const int last = opt->nsub > 0 ? opt->nsub - 1 : 0;

@@ -472,7 +472,8 @@ def mom_estimate(problems: list, targets, theta0, mode: int, n1: int, nD: int, b
                  use_graph: bool = True, max_repeats: int = 0, conv_tol: float = 0.1,
                  surrogate: bool = False, surrogate_pairs_log2: int = 0,
                  surrogate_steps: int = 0, surrogate_rounds: int = 0,
-                 surrogate_corrections: int = -1, specialize: int = 0) -> dict:
+                 surrogate_corrections: int = -1, specialize: int = 0,
+                 verbose: bool = False, surrogate_warmup: int = 0) -> dict:
     """Method-of-moments estimate on the device (cusna_mom_estimate).
 
     ``problems``: one SaomProblem per period (inputs only); ``xnets``: the
@@ -508,6 +509,8 @@ def mom_estimate(problems: list, targets, theta0, mode: int, n1: int, nD: int, b
     o.surrogate_steps, o.surrogate_rounds = int(surrogate_steps), int(surrogate_rounds)
     o.surrogate_corrections = int(surrogate_corrections)
     o.specialize = int(specialize)
+    o.verbose = int(bool(verbose))
+    o.surrogate_warmup = int(surrogate_warmup)
     if mode == 2:
         pr0 = problems[0]
         o.beh_zbar, o.beh_brange, o.beh_bsimmean = (float(pr0.zbar), float(pr0.brange),

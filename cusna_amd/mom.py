@@ -126,7 +126,8 @@ class MomEstimator:
     def run(self, theta0=None, use_graph: bool = True, max_repeats: int = 0,
             conv_tol: float = 0.1, surrogate: bool = False, surrogate_pairs_log2: int = 0,
             surrogate_steps: int = 0, surrogate_rounds: int = 0,
-            surrogate_corrections: int = -1, specialize: int = 0) -> MomFit:
+            surrogate_corrections: int = -1, specialize: int = 0,
+            verbose: bool = False, surrogate_warmup: int = 0) -> MomFit:
         """Estimate. ``surrogate`` replaces the Robbins-Monro sub-phases by the
         perceptron surrogate (see cusna_mom_options in include/libcusna.h)."""
         c = self.control
@@ -138,7 +139,8 @@ class MomEstimator:
                              surrogate=surrogate, surrogate_pairs_log2=surrogate_pairs_log2,
                              surrogate_steps=surrogate_steps, surrogate_rounds=surrogate_rounds,
                              surrogate_corrections=surrogate_corrections,
-                             specialize=specialize)
+                             specialize=specialize, verbose=verbose,
+                             surrogate_warmup=surrogate_warmup)
         return MomFit(theta=r["theta"], se=r["se"], tconv=r["tconv"],
                       tconv_max=r["tconv_max"], max_abs_tconv=r["max_abs_tconv"],
                       param_names=self.param_names, targets=self.targets,
@@ -155,3 +157,60 @@ def mom_estimate(data: SaomData, effects: list, beh_effects: list | None = None,
                  theta0=None, **kw) -> MomFit:
     """One-call estimate (the reference's mom_estimate, R/estimate.R)."""
     return MomEstimator(data, effects, beh_effects, conditional, control).run(theta0, **kw)
+
+
+class GroupMomEstimator:
+    """Several independent groups (networks over different actor sets, each
+    with its own waves) fitted with one parameter vector: the groups' periods
+    are the periods of one cusna_mom_estimate call, each with its own start
+    network and rate parameter, and the effect targets are the sums over the
+    groups, as for the periods of one group (R/mom.R:152-166 sums the
+    statistics over periods). The reference's R layer fits one group; this
+    composes that per-period algebra over groups. Unconditional only."""
+
+    def __init__(self, datas: list, effects_per_group: list, control: MomControl | None = None,
+                 **target_carriers):
+        if len(datas) != len(effects_per_group) or not datas:
+            raise ValueError("one effect list per group")
+        self.control = control or MomControl()
+        self.groups = [MomEstimator(d, e, None, False, self.control, **target_carriers)
+                       for d, e in zip(datas, effects_per_group)]
+        k = self.groups[0].k
+        if any(g.k != k for g in self.groups):
+            raise ValueError("every group needs the same number of effects")
+        self.k = k
+        self.P = sum(g.P for g in self.groups)
+        if self.P > 16:
+            raise ValueError("at most 16 periods over all groups")
+        self.p = self.P + k
+        self.problems = [pr for g in self.groups for pr in g.problems]
+        rates = np.concatenate([g.targets[:g.P] for g in self.groups])
+        eff = np.sum([g.targets[g.P:] for g in self.groups], axis=0)
+        self.targets = np.concatenate([rates, eff])
+        g0 = self.groups[0]
+        self.param_names = ([f"rate (group {gi + 1}, period {q})"
+                             for gi, g in enumerate(self.groups) for q in range(1, g.P + 1)]
+                            + g0.param_names[g0.P:])
+
+    def default_theta0(self) -> np.ndarray:
+        """R/mom.R:199-212 per group: rates 1.5 x target / n + 1, density -1."""
+        lam0 = np.concatenate([1.5 * g.targets[:g.P] / g.n + 1.0 for g in self.groups])
+        return np.concatenate([lam0, self.groups[0].default_theta0()[self.groups[0].P:]])
+
+    def run(self, theta0=None, use_graph: bool = True, max_repeats: int = 0,
+            conv_tol: float = 0.1, verbose: bool = False, **surrogate_kw) -> MomFit:
+        c = self.control
+        th0 = self.default_theta0() if theta0 is None else np.asarray(theta0, float)
+        r = lib.mom_estimate(self.problems, self.targets, th0, 0, c.n1, c.nD, c.batch2, c.n3,
+                             list(c.n2), c.firstg, c.diagonalize, c.seed,
+                             rate_consts=np.zeros(self.P), use_graph=use_graph,
+                             max_repeats=max_repeats, conv_tol=conv_tol, verbose=verbose,
+                             **surrogate_kw)
+        return MomFit(theta=r["theta"], se=r["se"], tconv=r["tconv"],
+                      tconv_max=r["tconv_max"], max_abs_tconv=r["max_abs_tconv"],
+                      param_names=self.param_names, targets=self.targets,
+                      sim_means=r["sim_means"], D=r["D"], msf=r["msf"], conditional=False,
+                      rate_est=None, rate_sd=None, n_sims=r["n_sims"], repeats=r["repeats"],
+                      phase_seconds=r["phase_seconds"], wall_time=r["wall_seconds"],
+                      phase3_stats=r["phase3_stats"], phase3_sp=r["phase3_sp"],
+                      phase3_scp=r["phase3_scp"])

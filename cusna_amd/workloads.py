@@ -13,6 +13,10 @@ NULL masks, float32 theta/lambda):
   coevo300_problem       300 actors with an ordinal behaviour 1..5 = Binomial(4, 0.4)+1;
                    network density/recip/transTrip/simX(behaviour), behaviour
                    rate 4, linear -0.2, quad -0.3, avAlt 0.8.
+  groups1000_estimation  16 groups of 1000 actors, two waves each, 12 effects
+                   (density, recip, transTrip, cycle3, inPop, outAct, egoX/altX/
+                   simX on an N(0, 1) and on a 1..5 covariate), true theta and
+                   rates drawn from the seed.
 """
 from __future__ import annotations
 
@@ -157,3 +161,36 @@ def coevo300_problem(M: int = 2048, n: int = 300, seed: int = 4, period_seed: in
     return effect_problem(w1, effects, [-2.6, 1.6, 0.3, 0.8], 6.0, M, period_seed, beh=d.beh,
                           z0=d.beh.imputed[:, 0], beh_effects=beh_effects,
                           thetab=[-0.2, -0.3, 0.8], lamb=4.0)
+
+
+GROUPS1000_THETA = [-4.5, 2.0, 0.3, -0.2, 0.03, -0.03, 0.1, 0.1, 0.5, 0.05, 0.1, 0.4]
+
+
+def groups1000_effects(z: np.ndarray, c: np.ndarray) -> list:
+    return [Effect("density"), Effect("recip"), Effect("transTrip"), Effect("cycle3"),
+            Effect("inPop"), Effect("outAct"), Effect("egoX", z), Effect("altX", z),
+            Effect("simX", z), Effect("egoX", c), Effect("altX", c), Effect("simX", c)]
+
+
+def groups1000_estimation(simulate, groups: int = 16, n: int = 1000, seed: int = 5):
+    """(datas, effects per group, theta, rates): ``groups`` groups of n actors;
+    wave 1 Bernoulli with mean outdegree 5, covariates N(0, 1) and uniform on
+    1..5 per group; the true theta is GROUPS1000_THETA perturbed by 10 % of
+    each magnitude and the rates uniform on 2.5..3.5, both from the seed;
+    wave 2 of each group is one seeded simulated period at them (``simulate``
+    runs a SaomProblem and returns a_ends)."""
+    rng = np.random.default_rng(seed)
+    base = np.asarray(GROUPS1000_THETA)
+    theta = base + 0.1 * np.abs(base) * rng.standard_normal(base.size)
+    rates = rng.uniform(2.5, 3.5, groups)
+    datas, effs = [], []
+    for g in range(groups):
+        w1 = bernoulli_network(n, 5.0, rng)
+        z = rng.standard_normal(n)
+        c = rng.integers(1, 6, size=n).astype(float)
+        eff = groups1000_effects(z, c)
+        prob = effect_problem(w1, eff, theta, float(rates[g]), 1, seed * 100000 + g)
+        w2 = np.asarray(simulate(prob)[0], np.uint8)
+        datas.append(build_saom_data([w1, w2]))
+        effs.append(eff)
+    return datas, effs, theta, rates

@@ -513,6 +513,18 @@ typedef struct cusna_mom_options {
    * -1 off, 0 when a simulation round has >= 2^18 chains x actors or the
    * whole schedule >= 2^22 (the ~1 s compile, cached on disk, amortised) */
   int32_t specialize;
+  /* progress lines on stderr, as the R estimator's verbose = TRUE prints them
+   * (mom.R:230, 246): theta after phase 1 and after each sub-phase, then
+   * max|tconv| of each phase 3 */
+  int32_t verbose;
+  /* surrogate: plain Robbins-Monro sub-phases run before it (0: none, the
+   * box then comes from phase 1); with some, the box is centred on the Newton
+   * point of nD chains at their theta. Whatever the count, when the
+   * corrections leave a residual above 3 moment standard deviations the
+   * remaining plain sub-phases run from the theta the surrogate started at
+   * (the plain estimator is the fallback), and a phase-3 Newton repeat moves each
+   * parameter by at most two of that phase 3's standard errors. */
+  int32_t surrogate_warmup;
 } cusna_mom_options;
 
 typedef struct cusna_mom_result {
