@@ -102,7 +102,8 @@ def test_surrogate_ablation(gpu_lib):
     (50 actors, conditional, reference default schedule, n3 = 1000): both
     reach max|t| <= 0.1 (repeats allowed), the estimates agree within one
     standard error of the plain fit, and the surrogate uses fewer simulated
-    chains and less wall time."""
+    chains and less wall time; without a trained network the same path falls
+    back to the plain sub-phases."""
     import time
 
     from cusna_amd.mom import MomControl, MomEstimator
@@ -121,5 +122,43 @@ def test_surrogate_ablation(gpu_lib):
     assert np.all(np.abs(sur.theta - plain.theta) <= plain.se), (sur.theta, plain.theta)
     assert sur.n_sims < plain.n_sims
     assert t_sur < t_plain, (t_sur, t_plain)
+    # the same path without a trained network (64 pairs, one Adam step): its
+    # corrections cannot finish, the plain sub-phases run as the fallback, and
+    # it simulates more chains than plain MoM (chain counts are deterministic)
+    none = est.run(max_repeats=10, conv_tol=0.1, surrogate=True, surrogate_pairs_log2=6,
+                   surrogate_steps=1)
+    assert none.max_abs_tconv <= 0.1
+    assert none.n_sims > plain.n_sims > sur.n_sims
     print(f"plain {t_plain:.3f} s, {plain.n_sims} chains; surrogate {t_sur:.3f} s, "
-          f"{sur.n_sims} chains")
+          f"{sur.n_sims} chains; untrained network {none.n_sims} chains")
+
+
+def test_group_estimator_plain_and_surrogate(gpu_lib):
+    """Three groups of 60 actors fitted together (GroupMomEstimator: the
+    groups are the periods of one call, one rate each): the plain estimator
+    and the surrogate with one warm plain sub-phase both reach max|t| <= 0.1
+    and agree within one standard error of the plain fit; the estimates lie
+    within three standard errors of the theta the waves were simulated at."""
+    from cusna_amd.data import build_saom_data
+    from cusna_amd.effects import Effect
+    from cusna_amd.mom import GroupMomEstimator, MomControl
+    from cusna_amd.workloads import bernoulli_network, effect_problem
+
+    rng = np.random.default_rng(21)
+    theta_true = np.array([-2.2, 1.8, 0.3])
+    datas, effs = [], []
+    for g in range(3):
+        w1 = bernoulli_network(60, 3.0, rng)
+        eff = [Effect("density"), Effect("recip"), Effect("transTrip")]
+        pr = effect_problem(w1, eff, theta_true, 4.0, 1, 900 + g)
+        w2 = np.asarray(gpu_lib.simulate_period(pr, want_end_nets=True)["a_ends"][0], np.uint8)
+        datas.append(build_saom_data([w1, w2]))
+        effs.append(eff)
+    est = GroupMomEstimator(datas, effs, MomControl(n3=1000))
+    assert est.p == 6
+    plain = est.run(max_repeats=10, conv_tol=0.1)
+    sur = est.run(max_repeats=10, conv_tol=0.1, surrogate=True, surrogate_warmup=1)
+    for fit in (plain, sur):
+        assert fit.max_abs_tconv <= 0.1, fit.max_abs_tconv
+    assert np.all(np.abs(sur.theta - plain.theta) <= plain.se), (sur.theta, plain.theta)
+    assert np.all(np.abs(plain.theta[3:] - theta_true) <= 3 * plain.se[3:]), plain.theta
