@@ -326,12 +326,12 @@ def mlp_measure(lib) -> dict:
 
 GROUPS_MOM_RUNS = {
     "plain": {},
-    # one plain sub-phase, then the surrogate (2^12 pairs) around its theta
-    "surrogate": dict(surrogate=True, surrogate_warmup=1, surrogate_pairs_log2=12),
-    "surrogate_2e13": dict(surrogate=True, surrogate_warmup=1),
+    # the surrogate (2^12 pairs) with the automatic warm start: one plain
+    # sub-phase first when phase 1 lies far from the targets
+    "surrogate": dict(surrogate=True, surrogate_pairs_log2=12),
     # the surrogate straight from phase 1 (its plain fallback engages when
     # the corrections leave a large residual)
-    "surrogate_from_phase1": dict(surrogate=True),
+    "surrogate_from_phase1": dict(surrogate=True, surrogate_pairs_log2=12, surrogate_warmup=-1),
 }
 
 

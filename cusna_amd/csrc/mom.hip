@@ -777,6 +777,10 @@ void newton_step(std::vector<double>& theta, const Mat& Dm, const std::vector<do
 // standard deviations) the corrections measured (the caller falls back to
 // the plain sub-phases above kSurrogateFallback).
 constexpr double kSurrogateFallback = 3.0;
+// Phase 1 further than this (largest moment deviation, in standard deviations)
+// from the targets: its linearisation cannot centre the surrogate's box, and
+// the automatic warm start runs one plain sub-phase first.
+constexpr double kSurrogateWarmDev = 50.0;
 
 cusna_status surrogate_phase(cusna_mom_engine* E, const cusna_saom_problem* const* periods,
                              const cusna_mom_options* opt, std::vector<double> c,
@@ -1106,23 +1110,39 @@ CUSNA_API cusna_status cusna_mom_estimate(const cusna_saom_problem* const* perio
     // moments and derivative at the current theta (phase 1's, or nD chains at
     // the sub-phases' theta), half-widths covering the step and three
     // standard errors
-    const int warm = std::min(std::max(opt->surrogate_warmup, 0), opt->nsub);
+    // phase 1's moments and their covariance (the rows phase 1 left in X)
+    std::vector<double> m1(p), msf1((size_t)p * p);
+    auto moments = [&](int nb) -> cusna_status {
+      hipLaunchKernelGGL(mom_cov_kernel, dim3(p * p), dim3(kMomThreads), 0, s, X.S, X.mean,
+                         X.S, X.mean, nb, p, (double)(nb - 1), 0, E->msf);
+      cusna_status mst = fetch(E.get(), m1.data(), X.mean, p);
+      if (mst != CUSNA_OK) return mst;
+      return fetch(E.get(), msf1.data(), E->msf, (size_t)p * p);
+    };
+    if ((st = moments(opt->n1)) != CUSNA_OK) return st;
+    double dev1 = 0.0;  // largest deviation from the targets, in moment standard deviations
+    for (int a = 0; a < p; ++a) {
+      const double v = msf1[(size_t)a * p + a];
+      dev1 = std::max(dev1, std::fabs(m1[a] - tgt[a]) / (v > 0.0 ? std::sqrt(v) : 1.0));
+    }
+    int warm = opt->surrogate_warmup;
+    if (warm == 0) warm = dev1 > kSurrogateWarmDev ? 1 : 0;  // automatic
+    warm = std::min(std::max(warm, 0), opt->nsub);
+    if (opt->verbose) {
+      std::fprintf(stderr, "[%.2f s] [surrogate] phase 1 %.1f sd from the targets: %d plain "
+                   "sub-phase(s) first\n", elapsed(E.get()), dev1, warm);
+      std::fflush(stderr);
+    }
     for (int sub = 0; sub < warm; ++sub)
       if ((st = run_sub(sub)) != CUSNA_OK) return st;
     std::vector<double> base = th0;
-    int nb = opt->n1;
     if (warm > 0) {
       if ((st = fetch(E.get(), base.data(), E->theta, p)) != CUSNA_OK) return st;
       if ((st = engine_simulate(E.get(), X, opt->nD)) != CUSNA_OK) return st;
       engine_derivative(E.get(), X, opt->nD);
       if ((st = fetch(E.get(), Dm.data(), E->Dm, (size_t)p * p)) != CUSNA_OK) return st;
-      nb = opt->nD;
+      if ((st = moments(opt->nD)) != CUSNA_OK) return st;
     }
-    std::vector<double> m1(p), msf1((size_t)p * p);
-    hipLaunchKernelGGL(mom_cov_kernel, dim3(p * p), dim3(kMomThreads), 0, s, X.S, X.mean, X.S,
-                       X.mean, nb, p, (double)(nb - 1), 0, E->msf);
-    if ((st = fetch(E.get(), m1.data(), X.mean, p)) != CUSNA_OK) return st;
-    if ((st = fetch(E.get(), msf1.data(), E->msf, (size_t)p * p)) != CUSNA_OK) return st;
     const Mat D1i = safe_solve(Dm, p);
     std::vector<double> c(p), h(p);
     const std::vector<double> se1 = sandwich_se(Dm, msf1, p);

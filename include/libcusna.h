@@ -517,10 +517,12 @@ typedef struct cusna_mom_options {
    * (mom.R:230, 246): theta after phase 1 and after each sub-phase, then
    * max|tconv| of each phase 3 */
   int32_t verbose;
-  /* surrogate: plain Robbins-Monro sub-phases run before it (0: none, the
-   * box then comes from phase 1); with some, the box is centred on the Newton
-   * point of nD chains at their theta. Whatever the count, when the
-   * corrections leave a residual above 3 moment standard deviations the
+  /* surrogate: plain Robbins-Monro sub-phases run before it (< 0: none, the
+   * box then comes from phase 1; 0: one when phase 1's moments lie more than
+   * 50 standard deviations from the targets, else none); with some, the box
+   * is centred on the Newton point of nD chains at their theta. Whatever the
+   * count, when the corrections leave a residual above 3 moment standard
+   * deviations the
    * remaining plain sub-phases run from the theta the surrogate started at
    * (the plain estimator is the fallback), and a phase-3 Newton repeat moves each
    * parameter by at most two of that phase 3's standard errors. */
