@@ -306,8 +306,8 @@ __device__ __forceinline__ void multi_body(const MultiParams* __restrict__ Qp) {
     U.rdeg_i = 0;
     // features of network d: tp, so and cy as the single-network kernel
     // computes them, with the same tie walk for egos with few ties.
-    const bool walk =
-        !STREAM && __builtin_amdgcn_readfirstlane(U.outdeg_i + U.indeg_i) * 3 < U.words() * 8;
+    const int deg_i = __builtin_amdgcn_readfirstlane(U.outdeg_i + U.indeg_i);
+    const bool walk = STREAM ? deg_i < U.words() : deg_i * 3 < U.words() * 8;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       A.tp[r] = 0;

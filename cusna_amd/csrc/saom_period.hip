@@ -1025,9 +1025,11 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
     //    cy_j = |row_j & col_i|. When the ego has few ties, so and cy come
     //    from walking its out- and in-ties (bit h of row j per tie, three
     //    instructions) instead of popcounts over every row word (eight per
-    //    word); the counts are the same either way.
-    const bool walk =
-        !STREAM && __builtin_amdgcn_readfirstlane(U.outdeg_i + U.indeg_i) * 3 < U.words() * 8;
+    //    word); the counts are the same either way. Streamed rows (device
+    //    memory) are walked when the ego has fewer ties than a row has
+    //    words: each is one load per alternative either way.
+    const int deg_i = __builtin_amdgcn_readfirstlane(U.outdeg_i + U.indeg_i);
+    const bool walk = STREAM ? deg_i < U.words() : deg_i * 3 < U.words() * 8;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       A.tp[r] = 0;
