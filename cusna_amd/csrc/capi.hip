@@ -589,6 +589,32 @@ cusna_status simulate(const cusna_saom_problem* pb, cusna_saom_result* r, const 
 // Within a launch the items run longest-first: groups sorted by an estimate of
 // their per-chain work, so that the hardware's in-order block dispatch keeps
 // the CUs busy until the end.
+// Streams (default priority) and events of one multi-group call, released
+// when the call returns.
+constexpr int kGroupStreams = 4;
+struct GroupStreams {
+  hipStream_t st[kGroupStreams] = {};
+  hipEvent_t done[kGroupStreams] = {};
+  hipEvent_t ready = nullptr;
+  int n = 0;
+  hipError_t create(int count) {
+    hipError_t e = hipEventCreateWithFlags(&ready, hipEventDisableTiming);
+    for (; e == hipSuccess && n < count; ++n) {
+      e = hipStreamCreateWithFlags(&st[n], hipStreamNonBlocking);
+      if (e == hipSuccess) e = hipEventCreateWithFlags(&done[n], hipEventDisableTiming);
+    }
+    return e;
+  }
+  ~GroupStreams() {
+    for (int q = 0; q < kGroupStreams; ++q) {
+      if (st[q]) (void)hipStreamSynchronize(st[q]);
+      if (done[q]) (void)hipEventDestroy(done[q]);
+      if (st[q]) (void)hipStreamDestroy(st[q]);
+    }
+    if (ready) (void)hipEventDestroy(ready);
+  }
+};
+
 cusna_status simulate_groups(const cusna_saom_problem* const* pbs, int G,
                              cusna_saom_result* const* rs) {
   hipStream_t s = thread_stream();
@@ -648,7 +674,9 @@ cusna_status simulate_groups(const cusna_saom_problem* const* pbs, int G,
                          hipMemcpyHostToDevice, s));
   std::vector<uint32_t> items;
   items.reserve(n_items);
-  for (auto& b : buckets) {
+  std::vector<saom::PeriodLaunch> launches(buckets.size());
+  for (size_t bi = 0; bi < buckets.size(); ++bi) {
+    Bucket& b = buckets[bi];
     std::stable_sort(b.groups.begin(), b.groups.end(),
                      [&](int a, int c) { return cost(a) > cost(c); });
     const size_t first = items.size();
@@ -668,7 +696,7 @@ cusna_status simulate_groups(const cusna_saom_problem* const* pbs, int G,
       lds = std::max(lds, saom::sim_lds_bytes(plans[g]->P, b.nw, b.wmax));
       lds_exact = std::max(lds_exact, saom::sim_lds_bytes(plans[g]->P, ex.nw, ex.wmax));
     }
-    saom::PeriodLaunch L;
+    saom::PeriodLaunch& L = launches[bi];
     L.Pd = D.at<saom::SimParams>(o_grp);
     L.items = D.at<uint32_t>(o_items) + first;
     L.grid = (int)(items.size() - first);
@@ -678,14 +706,36 @@ cusna_status simulate_groups(const cusna_saom_problem* const* pbs, int G,
     L.specialize = uniform && saom::specialize_policy(work);
     L.exact = ex;
     L.lds_exact = lds_exact;
-    HIP_TRY(hipMemcpyAsync(D.at<uint32_t>(o_items) + first, items.data() + first,
-                           (items.size() - first) * 4, hipMemcpyHostToDevice, s));
-    const int le =
-        saom::launch_period_ex(L, saom::LaunchShape{b.nw, b.wmax}, false, b.simple, s);
+  }
+  HIP_TRY(hipMemcpyAsync(D.at<uint32_t>(o_items), items.data(), items.size() * 4,
+                         hipMemcpyHostToDevice, s));
+  // The buckets run concurrently on up to four streams (largest first,
+  // round-robin), so one bucket's tail and a CU's spare LDS and wave slots
+  // take work from another; chains are independent, results do not depend
+  // on the order.
+  GroupStreams gs;
+  const int nst = (int)std::min<size_t>(kGroupStreams, buckets.size());
+  HIP_TRY(gs.create(nst));
+  HIP_TRY(hipEventRecord(gs.ready, s));
+  std::vector<size_t> order(buckets.size());
+  for (size_t bi = 0; bi < order.size(); ++bi) order[bi] = bi;
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t c) {
+    return (double)launches[a].grid * buckets[a].W64 > (double)launches[c].grid * buckets[c].W64;
+  });
+  for (int q = 0; q < nst; ++q) HIP_TRY(hipStreamWaitEvent(gs.st[q], gs.ready, 0));
+  for (size_t o = 0; o < order.size(); ++o) {
+    const size_t bi = order[o];
+    const Bucket& b = buckets[bi];
+    const int le = saom::launch_period_ex(launches[bi], saom::LaunchShape{b.nw, b.wmax}, false,
+                                          b.simple, gs.st[o % nst]);
     if (le != 0) {
       set_error(std::string("period kernel launch: ") + hipGetErrorString((hipError_t)le));
       return CUSNA_ERR_CUDA;
     }
+  }
+  for (int q = 0; q < nst; ++q) {
+    HIP_TRY(hipEventRecord(gs.done[q], gs.st[q]));
+    HIP_TRY(hipStreamWaitEvent(s, gs.done[q], 0));
   }
   for (int g = 0; g < G; ++g) {
     if (rs[g]->stats) {
