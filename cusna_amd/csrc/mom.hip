@@ -796,7 +796,7 @@ cusna_status surrogate_phase(cusna_mom_engine* E, const cusna_saom_problem* cons
   const int Mtr = 1 << q;
   const int steps = opt->surrogate_steps > 0 ? opt->surrogate_steps : 200;
   const int rounds = opt->surrogate_rounds > 0 ? opt->surrogate_rounds : 1;
-  const int corrections = opt->surrogate_corrections >= 0 ? opt->surrogate_corrections : 4;
+  const int corrections = opt->surrogate_corrections >= 0 ? opt->surrogate_corrections : 6;
   if (E->bscratch_doubles > 0 && Mtr > E->Mmax) {
     set_error("cusna_mom_estimate: surrogate rounds above the behaviour scratch");
     return CUSNA_ERR_UNSUPPORTED;

@@ -500,7 +500,7 @@ typedef struct cusna_mom_options {
    * surrogate_steps Adam steps (0: 200), E s(theta) = target solved on its
    * mean; the box re-centres on the solution and shrinks to two standard
    * errors over surrogate_rounds rounds (0: 1). Then up to
-   * surrogate_corrections (negative: 4) Newton steps with residual and
+   * surrogate_corrections (negative: 6) Newton steps with residual and
    * score-function derivative from nD plain chains, stopping once every
    * residual is below conv_tol moment standard deviations. Phase 3 as usual,
    * its repeats as above. */
