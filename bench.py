@@ -203,6 +203,48 @@ def rate_measure(lib, x0, cpu: bool, threads: int) -> dict:
     return res
 
 
+def coevo_measure(lib, cpu: bool, threads: int) -> dict:
+    """Network-behaviour co-evolution period: 300 actors, 2048 chains,
+    behaviour 1..5 (linear, quad, avAlt; rate 4), network density/recip/
+    transTrip/simX(behaviour, dynamic), rate 6: plan run (launch +
+    completion) after one warm-up; CPU leg: the reference engine on a
+    bounded sample of the same problem."""
+    from cusna_amd.workloads import coevo300_problem
+
+    plan = lib.PeriodPlan(coevo300_problem(M=2048))
+    plan.run(1)
+    plan.sync()
+    t0 = time.perf_counter()
+    for s in range(2):
+        plan.run(100 + s)
+        plan.sync()
+    wall = (time.perf_counter() - t0) / 2
+    out = plan.fetch()
+    res = {"workload": "co-evolution period: n=300, 2048 chains, network rate 6 "
+                       "(density/recip/transTrip/simX dyn), behaviour rate 4 "
+                       "(linear/quad/avAlt)", "ms_per_period": round(1e3 * wall, 2),
+           "chain_periods_per_s": round(2048 / wall, 1),
+           "kernel_ms": round(plan.kernel_ms(), 2),
+           "mean_ministeps": round(float(out["steps"].mean()), 1),
+           "specialised": plan.specialized}
+    plan.close()
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+
+        if O.reference() is not None:
+            os.environ["OMP_NUM_THREADS"] = str(threads)
+            chains = 2 * threads
+            t0 = time.perf_counter()
+            O.simulate_reference(coevo300_problem(M=chains))
+            cw = time.perf_counter() - t0
+            res["cpu"] = {"kind": "reference", "cores": threads,
+                          "chain_periods_per_s": round(chains / cw, 3),
+                          "sample": f"{chains} chains of the same problem, {threads} threads, "
+                                    f"{cw:.1f} s wall"}
+    return res
+
+
 def multi_problem_n500(M: int, x0s, seed: int = 77):
     """Two co-evolving networks of 500 actors (the waves of the n = 500 workload as starts),
     rate 4 each, density/recip/transTrip/crprod on network 0 and
@@ -460,6 +502,7 @@ def main() -> int:
     ap.add_argument("--no-gof", action="store_true")
     ap.add_argument("--no-rate", action="store_true")
     ap.add_argument("--no-mlp", action="store_true")
+    ap.add_argument("--no-coevo", action="store_true")
     ap.add_argument("--mlp-only", action="store_true",
                     help="only the perceptron training section (for counter runs)")
     ap.add_argument("--groups-mom", default="", metavar="RUNS",
@@ -555,6 +598,9 @@ def main() -> int:
     if not args.no_rate:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
         line["rate"] = rate_measure(lib, w1, not args.no_cpu, threads)
+    if not args.no_coevo:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+        line["coevo"] = coevo_measure(lib, not args.no_cpu, threads)
     if not args.no_multi:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
         line["multi"] = multi_measure(lib, (w1, w2), not args.no_cpu, threads)
