@@ -816,24 +816,28 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
       }
       __syncthreads();
       if (tid <= nr) {
-        // branch-free so the loads of eight actors overlap: lane 0 weighs
-        // the rates by 1 (exact), lane 1 + q by u_q
+        // lane 0 weighs the rates by 1 (exact), lane 1 + q by u_q. Every
+        // lane loads its covariate and a degree for each actor and selects
+        // the weight without a branch, so the loads of eight actors are all
+        // in flight before their sums (a branch per actor serialised two
+        // dependent LDS round trips per actor: ~8x the cost of a ministep).
         const int kq = tid > 0 ? P.rkind[tid - 1] : -1;
         const double* ruq = ru_s + (size_t)(tid > 0 ? tid - 1 : 0) * n;
-        auto weight = [&](int a2) -> double {
-          if (kq < 0) return 1.0;
-          const double c0 = ruq[a2];
-          const int dg = kq == 1 ? outdeg[a2] : indeg[a2];
-          return kq == 0 ? c0 : (double)dg;
+        const int* degq = kq == 1 ? outdeg : indeg;
+        auto weight = [&](double c0, int dg) -> double {
+          return kq < 0 ? 1.0 : (kq == 0 ? c0 : (double)dg);
         };
         double acc = 0.0;
         int x = 0;
         for (; x + 8 <= nact; x += 8) {
+          int a2[8];
+#pragma unroll
+          for (int m = 0; m < 8; ++m) a2[m] = ract[x + m];
           double cx[8], ux[8];
 #pragma unroll
           for (int m = 0; m < 8; ++m) {
             cx[m] = crate[x + m];
-            ux[m] = weight(ract[x + m]);
+            ux[m] = weight(ruq[a2[m]], degq[a2[m]]);
           }
 #pragma unroll
           for (int m = 0; m < 8; ++m) {
@@ -843,7 +847,8 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
           }
         }
         for (; x < nact; ++x) {
-          const double prod = crate[x] * weight(ract[x]);
+          const int a2 = ract[x];
+          const double prod = crate[x] * weight(ruq[a2], degq[a2]);
           acc += prod;
           if (tid == 0) crate[x] = acc;
         }
