@@ -818,7 +818,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
       if (tid <= nr) {
         // lane 0 weighs the rates by 1 (exact), lane 1 + q by u_q. Every
         // lane loads its covariate and a degree for each actor and selects
-        // the weight without a branch, so the loads of eight actors are all
+        // the weight without a branch, so the loads of kB actors are all
         // in flight before their sums (a branch per actor serialised two
         // dependent LDS round trips per actor: ~8x the cost of a ministep).
         const int kq = tid > 0 ? P.rkind[tid - 1] : -1;
@@ -829,18 +829,19 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
         };
         double acc = 0.0;
         int x = 0;
-        for (; x + 8 <= nact; x += 8) {
-          int a2[8];
+        constexpr int kB = 16;  // actors whose loads are in flight together
+        for (; x + kB <= nact; x += kB) {
+          int a2[kB];
 #pragma unroll
-          for (int m = 0; m < 8; ++m) a2[m] = ract[x + m];
-          double cx[8], ux[8];
+          for (int m = 0; m < kB; ++m) a2[m] = ract[x + m];
+          double cx[kB], ux[kB];
 #pragma unroll
-          for (int m = 0; m < 8; ++m) {
+          for (int m = 0; m < kB; ++m) {
             cx[m] = crate[x + m];
             ux[m] = weight(ruq[a2[m]], degq[a2[m]]);
           }
 #pragma unroll
-          for (int m = 0; m < 8; ++m) {
+          for (int m = 0; m < kB; ++m) {
             const double prod = cx[m] * ux[m];
             acc += prod;
             if (tid == 0) crate[x + m] = acc;
