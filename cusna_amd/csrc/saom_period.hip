@@ -796,64 +796,72 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
     int i;
     bool net;
     if (flags & F_RATE) {
-      // rate effects (:170-193): per-actor rates in parallel; their sums by
-      // single lanes in act order (lane 0: S0, writing the running sums over
-      // the rates as it goes; lane 1 + q: sum of c * u_q), as the reference
-      // sums them; the ego is the first actor whose running sum reaches
-      // u * S0, found by counting the running sums below it in parallel
+      // rate effects (:170-193): per-actor rates in parallel. Lane 0 sums
+      // the rates in act order, as the reference does, writing the running
+      // sums as it goes (S0 and the running sums fix the waiting time and
+      // the ego: the ego is the first actor whose running sum reaches
+      // u * S0, found by counting the running sums below it in parallel).
+      // The covariate-weighted sums sum_a c_a u_q(a), which only enter the
+      // rate scores, are summed in parallel (per thread, then a fixed-order
+      // lane and wave reduction): their rounding differs from the
+      // reference's sequential sum by float64 ulps.
       const int nr = P.nrate;
+      double part[kMaxRate];
+#pragma unroll
+      for (int q = 0; q < kMaxRate; ++q) part[q] = 0.0;
       for (int x = tid; x < nact; x += NT) {
         const int a2 = ract[x];
-        double sx = 0.0;
-        for (int q = 0; q < nr; ++q) {
-          const double prod = P.ralpha[q] * rate_cov(P.rkind[q], ru_s + (size_t)q * n, a2,
-                                                     outdeg, indeg);
-          sx = sx + prod;
+        double sx = 0.0, uq[kMaxRate];
+#pragma unroll
+        for (int q = 0; q < kMaxRate; ++q) {
+          uq[q] = 0.0;
+          if (q < nr) {
+            uq[q] = rate_cov(P.rkind[q], ru_s + (size_t)q * n, a2, outdeg, indeg);
+            const double prod = P.ralpha[q] * uq[q];
+            sx = sx + prod;
+          }
         }
         if (sx > 30.0) sx = 30.0;
         else if (sx < -30.0) sx = -30.0;
-        crate[x] = exp(sx);
+        const double c = exp(sx);
+        crate[x] = c;
+#pragma unroll
+        for (int q = 0; q < kMaxRate; ++q) part[q] += c * uq[q];
+      }
+      double* wsum = (double*)cnt;  // slots 0..3 of each wave's counters
+#pragma unroll
+      for (int q = 0; q < kMaxRate; ++q) {
+        if (q < nr) {
+          const double v = wave_sum(part[q]);
+          if (lane == 0) wsum[wv * 8 + q] = v;
+        }
       }
       __syncthreads();
-      if (tid <= nr) {
-        // lane 0 weighs the rates by 1 (exact), lane 1 + q by u_q. Every
-        // lane loads its covariate and a degree for each actor and selects
-        // the weight without a branch, so the loads of kB actors are all
-        // in flight before their sums (a branch per actor serialised two
-        // dependent LDS round trips per actor: ~8x the cost of a ministep).
-        const int kq = tid > 0 ? P.rkind[tid - 1] : -1;
-        const double* ruq = ru_s + (size_t)(tid > 0 ? tid - 1 : 0) * n;
-        const int* degq = kq == 1 ? outdeg : indeg;
-        auto weight = [&](double c0, int dg) -> double {
-          return kq < 0 ? 1.0 : (kq == 0 ? c0 : (double)dg);
-        };
+      if (tid == 0) {
+        // the loads of kB rates are in flight together before their sums
         double acc = 0.0;
         int x = 0;
-        constexpr int kB = 16;  // actors whose loads are in flight together
+        constexpr int kB = 16;
         for (; x + kB <= nact; x += kB) {
-          int a2[kB];
+          double cx[kB];
 #pragma unroll
-          for (int m = 0; m < kB; ++m) a2[m] = ract[x + m];
-          double cx[kB], ux[kB];
-#pragma unroll
-          for (int m = 0; m < kB; ++m) {
-            cx[m] = crate[x + m];
-            ux[m] = weight(ruq[a2[m]], degq[a2[m]]);
-          }
+          for (int m = 0; m < kB; ++m) cx[m] = crate[x + m];
 #pragma unroll
           for (int m = 0; m < kB; ++m) {
-            const double prod = cx[m] * ux[m];
-            acc += prod;
-            if (tid == 0) crate[x + m] = acc;
+            acc += cx[m];
+            crate[x + m] = acc;
           }
         }
         for (; x < nact; ++x) {
-          const int a2 = ract[x];
-          const double prod = crate[x] * weight(ruq[a2], degq[a2]);
-          acc += prod;
-          if (tid == 0) crate[x] = acc;
+          acc += crate[x];
+          crate[x] = acc;
         }
-        rsum_s[tid] = acc;
+        rsum_s[0] = acc;
+        for (int q = 0; q < nr; ++q) {
+          double t = 0.0;
+          for (int w = 0; w < NW; ++w) t += wsum[w * 8 + q];
+          rsum_s[1 + q] = t;
+        }
       }
       __syncthreads();
       const double S0 = rsum_s[0];
