@@ -6,7 +6,9 @@ Bar as the parity tests: trajectories, end networks, distances, statistics
 and per-actor distances bit for bit; ntim to 1e-12 relative; scores (the
 rate score steps/lam - integral of the total rate included) and rate-effect
 scores within 1e-5 * max(1, |ref|) (device vs glibc exp/log in the
-rates and waiting times).
+rates and waiting times; the covariate-weighted rate sums behind the
+rate-effect scores are summed in parallel on the device, sequentially in the
+reference).
 """
 import numpy as np
 import pytest
