@@ -588,7 +588,7 @@ def main() -> int:
     }
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
-        chains = args.cpu_chains or 4 * threads
+        chains = args.cpu_chains or 8 * threads  # at least 8 chains per core
         line["cpu_baseline"] = cpu_baseline(
             lambda m: n500_problem(M=m, n=n, x0=w1, period_seed=11), threads, chains)
     for pl in plans:
