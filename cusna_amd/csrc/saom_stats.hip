@@ -5,9 +5,10 @@
 // network. Integer-valued statistics (density, recip, transTrip, cycle3,
 // transTies, transRecTrip, inPop, outAct, reciAct, transTriads) are exact
 // int64 counts from popcounts and out-neighbour walks, which equal the
-// reference's double sums of integers. Statistics that sum non-integers are
-// accumulated by one lane in the reference's order (actors ascending; ties
-// row-major), so they round exactly as the reference does.
+// reference's double sums of integers. Statistics that sum non-integers have
+// their terms computed by the workgroup in windows and added by one lane in the
+// reference's order (actors ascending; ties row-major), so they round exactly
+// as the reference does.
 //
 // beh_stats_kernel restates cusna_behavior_stats (:767-817).
 #include <hip/hip_runtime.h>
@@ -19,6 +20,8 @@ namespace saom {
 namespace {
 
 constexpr int kStatThreads = 256;
+// term window of the ordered sums, in doubles (LDS rows / rows in scratch)
+constexpr int kTermCapLds = 512, kTermCapScratch = 4096;
 
 __device__ __forceinline__ int bit_of(const uint64_t* row, int j) {
   return (int)((row[j >> 6] >> (j & 63)) & 1);
@@ -45,6 +48,28 @@ __device__ __forceinline__ long long block_sum_ll(long long v, long long* slots)
   for (int q = 0; q < (int)(blockDim.x >> 6); ++q) s += slots[q];
   __syncthreads();
   return s;
+}
+
+// Exclusive prefix sum of v over the workgroup; total receives the sum.
+__device__ __forceinline__ int block_exscan(int v, int* slots, int& total) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) slots[wv] = x;
+  __syncthreads();
+  int before = 0, all = 0;
+  for (int q = 0; q < (int)(blockDim.x >> 6); ++q) {
+    const int sq = slots[q];
+    before += q < wv ? sq : 0;
+    all += sq;
+  }
+  __syncthreads();
+  total = all;
+  return before + x - v;
 }
 
 enum : int {
@@ -80,6 +105,7 @@ __global__ void __launch_bounds__(kStatThreads) net_stats_kernel(const StatParam
   o += (size_t)n * 4;
   o = (o + 7) & ~(size_t)7;
   long long* slots = (long long*)(smem + o);
+  double* tbuf = (double*)(smem + o + 64 * 8);
 
   const uint64_t* src = P.rows + (size_t)net * n * W64;
   for (int q = tid; q < n * W64; q += NT) {
@@ -166,195 +192,220 @@ __global__ void __launch_bounds__(kStatThreads) net_stats_kernel(const StatParam
   long long tot[I_N];
   for (int q = 0; q < I_N; ++q) tot[q] = block_sum_ll(acc[q], slots);
 
-  // per effect; order-sensitive sums on one lane
-  if (tid == 0) {
-    double* out = P.out + (size_t)net * k;
-    for (int e = 0; e < k; ++e) {
-      const int t = P.tid[e];
-      const double pm = P.p1[e], q2 = P.p2[e];
-      const double* wr = P.wtab + (size_t)e * (n + 1);
-      const bool dyn = P.dyn[e] != 0;
-      const double* vrow = P.vmat + (size_t)e * n;
-      const uint8_t* mrow = P.vmiss ? P.vmiss + (size_t)e * n : nullptr;
-      auto V = [&](int i) -> double { return dyn ? (P.vb ? P.vb[i] : 0.0) : vrow[i]; };
-      auto MISS = [&](int i) -> bool { return mrow && mrow[i]; };
-      double s = 0.0;
+  // Per effect. A statistic that sums non-integers keeps the reference's
+  // order (actors ascending; ties row-major): the workgroup writes a window of
+  // terms in that order into tbuf, then lane 0 adds them to its running sum,
+  // so the sum rounds exactly as the reference's loop. A term the reference
+  // skips is written as +0.0: the running sum starts at +0.0 and can never
+  // become -0.0, so adding +0.0 leaves it bit-identical.
+  constexpr int kCap = GROWS ? kTermCapScratch : kTermCapLds;
+  int* islots = (int*)slots;
+  double* out = P.out + (size_t)net * k;
+  for (int e = 0; e < k; ++e) {
+    const int t = P.tid[e];
+    const double pm = P.p1[e], q2 = P.p2[e];
+    const double* wr = P.wtab + (size_t)e * (n + 1);
+    const bool dyn = P.dyn[e] != 0;
+    const double* vrow = P.vmat + (size_t)e * n;
+    const uint8_t* mrow = P.vmiss ? P.vmiss + (size_t)e * n : nullptr;
+    auto V = [&](int i) -> double { return dyn ? (P.vb ? P.vb[i] : 0.0) : vrow[i]; };
+    auto MISS = [&](int i) -> bool { return mrow && mrow[i]; };
+    double s = 0.0;  // lane 0's running sum
+    auto add_window = [&](int len) {
+      __syncthreads();
+      if (tid == 0) {
+        int x = 0;
+        for (; x + 8 <= len; x += 8) {
+          double c[8];
+#pragma unroll
+          for (int m = 0; m < 8; ++m) c[m] = tbuf[x + m];
+#pragma unroll
+          for (int m = 0; m < 8; ++m) s += c[m];
+        }
+        for (; x < len; ++x) s += tbuf[x];
+      }
+      __syncthreads();
+    };
+    // term of actor i in the per-actor sums (:663-671)
+    auto actor_term = [&](int i) -> double {
       switch (t) {
-        case 0:
-          s = (double)tot[I_E];
-          if (P.symflag) s *= 0.5;
-          break;
-        case 1:
-          s = (double)tot[I_RECIP];
-          break;
-        case 2:
-          s = (double)tot[I_TT];
-          break;
-        case 3:
-          s = (double)tot[I_C3] / 3.0;
-          break;
-        case 23:
-          s = (double)tot[I_TT] / 6.0;
-          break;
-        case 5:
-          s = (double)tot[I_TTIES];
-          break;
-        case 6:
-          s = (double)tot[I_TRT];
-          break;
-        case 4:
-          s = (double)tot[I_INPOP];
-          break;
-        case 16:
-          s = (double)tot[I_OUTACT];
-          break;
-        case 19:
-          s = (double)tot[I_RECIACT];
-          break;
         case 7:
-          for (int j = 0; j < n; ++j) s += pow_three_halves_libm((double)indeg[j]);
-          break;
+          return pow_three_halves_libm((double)indeg[i]);
         case 17:
-          for (int i = 0; i < n; ++i) s += pow_three_halves_libm((double)outdeg[i]);
-          break;
+          return pow_three_halves_libm((double)outdeg[i]);
         case 8:
         case 9:
-          for (int i = 0; i < n; ++i)
-            s += (double)indeg[i] *
+          return (double)indeg[i] *
                  degree_term(t, pm, (double)outdeg[i], P.od0 ? (double)P.od0[i] : 0.0);
-          break;
         case 14:
         case 15:
-          for (int i = 0; i < n; ++i)
-            s += (double)outdeg[i] *
+          return (double)outdeg[i] *
                  degree_term(t, pm, (double)indeg[i], P.id0 ? (double)P.id0[i] : 0.0);
-          break;
         case 18:
-          for (int i = 0; i < n; ++i) s += fmin((double)outdeg[i], pm);
-          break;
+          return fmin((double)outdeg[i], pm);
         case 10:
-          for (int i = 0; i < n; ++i)
-            if (!MISS(i)) s += (double)outdeg[i] * V(i);
-          break;
-        case 11:
-          for (int i = 0; i < n; ++i)
-            if (!MISS(i)) s += (double)indeg[i] * V(i);
-          break;
-        case 12: case 13: case 20: case 21: case 24: case 25: case 26: case 27: case 28:
-        case 29: case 30: case 31: {
-          // sums over ties (i, j) in row-major order
-          const bool all_pairs = (t == 29 || t == 30);
-          for (int i = 0; i < n; ++i) {
-            const uint64_t* ri = rows + (size_t)i * W64;
-            for (int wd = 0; wd < W64; ++wd) {
-              uint64_t m;
-              if (!all_pairs) {
-                m = ri[wd];
-              } else if (wr[0] != 0.0) {
-                m = ~0ull;  // every pair contributes
-                if (wd == W64 - 1 && (n & 63)) m &= (1ull << (n & 63)) - 1;
-              } else if (t == 29) {  // pairs with a two-path i->h->j
-                m = 0;
-                for (int w2 = 0; w2 < W64; ++w2) {
-                  uint64_t mm = ri[w2];
-                  while (mm) {
-                    const int h = w2 * 64 + __ffsll((long long)mm) - 1;
-                    mm &= mm - 1;
-                    m |= rows[(size_t)h * W64 + wd];
-                  }
+          return MISS(i) ? 0.0 : (double)outdeg[i] * V(i);
+        default:  // 11
+          return MISS(i) ? 0.0 : (double)indeg[i] * V(i);
+      }
+    };
+    // pairs (i, j) of word wd that the tie sums visit: the ties of i, or for
+    // 29/30 every pair (wr[0] != 0) or the pairs with a two-path i->h->j (29)
+    // or a shared out-neighbour h (30)
+    const bool all_pairs = (t == 29 || t == 30);
+    auto pair_mask = [&](const uint64_t* ri, int wd) -> uint64_t {
+      if (!all_pairs) return ri[wd];
+      if (wr[0] != 0.0) {
+        uint64_t m = ~0ull;
+        if (wd == W64 - 1 && (n & 63)) m &= (1ull << (n & 63)) - 1;
+        return m;
+      }
+      const uint64_t* base = t == 29 ? rows : cols;
+      uint64_t m = 0;
+      for (int w2 = 0; w2 < W64; ++w2) {
+        uint64_t mm = ri[w2];
+        while (mm) {
+          const int h = w2 * 64 + __ffsll((long long)mm) - 1;
+          mm &= mm - 1;
+          m |= base[(size_t)h * W64 + wd];
+        }
+      }
+      return m;
+    };
+    // term of pair (i, j) in the tie sums (:672-683)
+    auto tie_term = [&](int i, const uint64_t* ri, int j) -> double {
+      const uint64_t* rj = rows + (size_t)j * W64;
+      switch (t) {
+        case 12:
+          return (MISS(i) || MISS(j)) ? 0.0 : 1.0 - fabs(V(i) - V(j)) * pm - q2;
+        case 13:
+          return (MISS(i) || MISS(j)) ? 0.0 : (fabs(V(i) - V(j)) < 1e-9 ? 1.0 : 0.0);
+        case 20: {
+          if (MISS(i) || MISS(j)) return 0.5;
+          const double dd = V(i) - V(j);
+          return (dd > 1e-9 ? 1.0 : 0.0) + 0.5 * (fabs(dd) <= 1e-9 ? 1.0 : 0.0);
+        }
+        case 21:
+          return (MISS(i) || MISS(j)) ? 0.0 : V(i) * V(j);
+        case 25:
+          return (MISS(i) || MISS(j)) ? 0.0 : V(i) + V(j);
+        case 31:
+          return (!MISS(j) && V(j) >= pm - 1e-12) ? 1.0 : 0.0;
+        case 24:
+        case 29: {
+          if (t == 29 && i == j) return 0.0;
+          int c = 0;  // two-paths i->h->j
+          for (int w2 = 0; w2 < W64; ++w2) {
+            uint64_t mm = ri[w2];
+            while (mm) {
+              const int h = w2 * 64 + __ffsll((long long)mm) - 1;
+              mm &= mm - 1;
+              c += bit_of(rows + (size_t)h * W64, j);
+            }
+          }
+          return wr[c];
+        }
+        case 26:
+        case 30: {
+          if (t == 30 && i == j) return 0.0;
+          int c = 0;
+          for (int w2 = 0; w2 < W64; ++w2) c += __popcll(ri[w2] & rj[w2]);
+          return wr[c];
+        }
+        case 27: {
+          int c = 0;
+          for (int w2 = 0; w2 < W64; ++w2)
+            c += __popcll(cols[(size_t)i * W64 + w2] & cols[(size_t)j * W64 + w2]);
+          return wr[c];
+        }
+        default: {  // 28
+          int c = 0;
+          for (int w2 = 0; w2 < W64; ++w2) c += __popcll(cols[(size_t)i * W64 + w2] & rj[w2]);
+          return wr[c];
+        }
+      }
+    };
+    switch (t) {
+      case 0:
+        s = (double)tot[I_E];
+        if (P.symflag) s *= 0.5;
+        break;
+      case 1:
+        s = (double)tot[I_RECIP];
+        break;
+      case 2:
+        s = (double)tot[I_TT];
+        break;
+      case 3:
+        s = (double)tot[I_C3] / 3.0;
+        break;
+      case 23:
+        s = (double)tot[I_TT] / 6.0;
+        break;
+      case 5:
+        s = (double)tot[I_TTIES];
+        break;
+      case 6:
+        s = (double)tot[I_TRT];
+        break;
+      case 4:
+        s = (double)tot[I_INPOP];
+        break;
+      case 16:
+        s = (double)tot[I_OUTACT];
+        break;
+      case 19:
+        s = (double)tot[I_RECIACT];
+        break;
+      case 7: case 17: case 8: case 9: case 14: case 15: case 18: case 10: case 11:
+        // windows of NT actors (kCap >= NT)
+        for (int b = 0; b < n; b += NT) {
+          if (b + tid < n) tbuf[tid] = actor_term(b + tid);
+          add_window(min(NT, n - b));
+        }
+        break;
+      case 12: case 13: case 20: case 21: case 24: case 25: case 26: case 27: case 28:
+      case 29: case 30: case 31:
+        // windows of NT actors; a window's pairs go through tbuf kCap at a time
+        for (int b = 0; b < n; b += NT) {
+          const int i = b + tid;
+          const bool act = i < n;
+          const uint64_t* ri = rows + (size_t)(act ? i : 0) * W64;
+          int cnt = 0;
+          if (act) {
+            if (!all_pairs) cnt = outdeg[i];
+            else
+              for (int wd = 0; wd < W64; ++wd) cnt += __popcll(pair_mask(ri, wd));
+          }
+          int wtot;
+          const int off = block_exscan(cnt, islots, wtot);
+          for (int lo = 0; lo < wtot; lo += kCap) {
+            const int hi = min(lo + kCap, wtot);
+            if (act && off < hi && off + cnt > lo) {
+              int kq = off;  // window index of the next pair of i
+              for (int wd = 0; wd < W64 && kq < hi; ++wd) {
+                uint64_t m = pair_mask(ri, wd);
+                const int c = __popcll(m);
+                if (kq + c <= lo) {
+                  kq += c;
+                  continue;
                 }
-              } else {  // t == 30: pairs sharing an out-neighbour h
-                m = 0;
-                for (int w2 = 0; w2 < W64; ++w2) {
-                  uint64_t mm = ri[w2];
-                  while (mm) {
-                    const int h = w2 * 64 + __ffsll((long long)mm) - 1;
-                    mm &= mm - 1;
-                    m |= cols[(size_t)h * W64 + wd];
-                  }
-                }
-              }
-              while (m) {
-                const int j = wd * 64 + __ffsll((long long)m) - 1;
-                m &= m - 1;
-                const uint64_t* rj = rows + (size_t)j * W64;
-                switch (t) {
-                  case 12:
-                    if (!MISS(i) && !MISS(j)) s += 1.0 - fabs(V(i) - V(j)) * pm - q2;
-                    break;
-                  case 13:
-                    if (!MISS(i) && !MISS(j)) s += fabs(V(i) - V(j)) < 1e-9 ? 1.0 : 0.0;
-                    break;
-                  case 20: {
-                    double x;
-                    if (MISS(i) || MISS(j)) {
-                      x = 0.5;
-                    } else {
-                      const double dd = V(i) - V(j);
-                      x = (dd > 1e-9 ? 1.0 : 0.0) + 0.5 * (fabs(dd) <= 1e-9 ? 1.0 : 0.0);
-                    }
-                    s += x;
-                    break;
-                  }
-                  case 21:
-                    if (!MISS(i) && !MISS(j)) s += V(i) * V(j);
-                    break;
-                  case 25:
-                    if (!MISS(i) && !MISS(j)) s += V(i) + V(j);
-                    break;
-                  case 31:
-                    if (!MISS(j) && V(j) >= pm - 1e-12) s += 1.0;
-                    break;
-                  case 24:
-                  case 29: {
-                    if (t == 29 && i == j) break;
-                    int c = 0;  // two-paths i->h->j
-                    for (int w2 = 0; w2 < W64; ++w2) {
-                      uint64_t mm = ri[w2];
-                      while (mm) {
-                        const int h = w2 * 64 + __ffsll((long long)mm) - 1;
-                        mm &= mm - 1;
-                        c += bit_of(rows + (size_t)h * W64, j);
-                      }
-                    }
-                    s += wr[c];
-                    break;
-                  }
-                  case 26:
-                  case 30: {
-                    if (t == 30 && i == j) break;
-                    int c = 0;
-                    for (int w2 = 0; w2 < W64; ++w2) c += __popcll(ri[w2] & rj[w2]);
-                    s += wr[c];
-                    break;
-                  }
-                  case 27: {
-                    int c = 0;
-                    for (int w2 = 0; w2 < W64; ++w2)
-                      c += __popcll(cols[(size_t)i * W64 + w2] & cols[(size_t)j * W64 + w2]);
-                    s += wr[c];
-                    break;
-                  }
-                  case 28: {
-                    int c = 0;
-                    for (int w2 = 0; w2 < W64; ++w2)
-                      c += __popcll(cols[(size_t)i * W64 + w2] & rj[w2]);
-                    s += wr[c];
-                    break;
-                  }
-                  default:
-                    break;
+                while (m && kq < hi) {
+                  const int j = wd * 64 + __ffsll((long long)m) - 1;
+                  m &= m - 1;
+                  if (kq >= lo) tbuf[kq - lo] = tie_term(i, ri, j);
+                  ++kq;
                 }
               }
             }
+            add_window(hi - lo);
           }
-          break;
         }
-        default:
-          break;  // X (22) and interactions (32) have no statistic (:684)
-      }
-      out[e] = s;
+        break;
+      default:
+        break;  // X (22) and interactions (32) have no statistic (:684)
     }
+    if (tid == 0) out[e] = s;
   }
 }
 
@@ -517,7 +568,7 @@ int launch_net_stats(const StatParams& P, void* stream) {
       160 * 1024);
   if (attr_g != hipSuccess) return (int)attr_g;
   if (grows) {
-    const size_t lds = (size_t)P.n * 8 + 8 + 64 * 8;
+    const size_t lds = (size_t)P.n * 8 + 8 + 64 * 8 + (size_t)kTermCapScratch * 8;
     hipLaunchKernelGGL(net_stats_kernel<true>, dim3(P.M), dim3(kStatThreads), lds,
                        (hipStream_t)stream, P);
   } else {
@@ -529,7 +580,8 @@ int launch_net_stats(const StatParams& P, void* stream) {
 }
 
 size_t net_stats_lds(int n, int W64, bool need_cols) {
-  return (size_t)n * W64 * 8 * (need_cols ? 2 : 1) + (size_t)n * 8 + 8 + 64 * 8;
+  return (size_t)n * W64 * 8 * (need_cols ? 2 : 1) + (size_t)n * 8 + 8 + 64 * 8 +
+         (size_t)kTermCapLds * 8;
 }
 
 size_t net_stats_scratch_words(int n, int W64, bool need_cols) {
