@@ -505,6 +505,8 @@ def main() -> int:
     ap.add_argument("--no-coevo", action="store_true")
     ap.add_argument("--mlp-only", action="store_true",
                     help="only the perceptron training section (for counter runs)")
+    ap.add_argument("--mom-only", action="store_true",
+                    help="only the 50-actor MoM fit, without its CPU leg (for profiler runs)")
     ap.add_argument("--groups-mom", default="", metavar="RUNS",
                     help="only the 16 x 1000-actor MoM comparison, comma-separated runs of "
                          + "/".join(GROUPS_MOM_RUNS) + " (minutes each)")
@@ -525,6 +527,9 @@ def main() -> int:
         return 3
     if args.mlp_only:
         print(json.dumps({"mlp": mlp_measure(lib)}), flush=True)
+        return 0
+    if args.mom_only:
+        print(json.dumps({"mom": mom_measure(lib, False, args.threads)}), flush=True)
         return 0
     if args.groups_mom:
         runs = args.groups_mom.split(",")
