@@ -72,6 +72,19 @@ __device__ __forceinline__ int block_exscan(int v, int* slots, int& total) {
   return before + x - v;
 }
 
+// s += buf[0], s += buf[1], ... in order (one lane); eight loads in flight.
+__device__ __forceinline__ void ordered_add(double& s, const double* buf, int len) {
+  int x = 0;
+  for (; x + 8 <= len; x += 8) {
+    double c[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) c[m] = buf[x + m];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) s += c[m];
+  }
+  for (; x < len; ++x) s += buf[x];
+}
+
 enum : int {
   I_E = 0, I_RECIP, I_TT, I_C3, I_TTIES, I_TRT, I_INPOP, I_OUTACT, I_RECIACT, I_N
 };
@@ -213,17 +226,7 @@ __global__ void __launch_bounds__(kStatThreads) net_stats_kernel(const StatParam
     double s = 0.0;  // lane 0's running sum
     auto add_window = [&](int len) {
       __syncthreads();
-      if (tid == 0) {
-        int x = 0;
-        for (; x + 8 <= len; x += 8) {
-          double c[8];
-#pragma unroll
-          for (int m = 0; m < 8; ++m) c[m] = tbuf[x + m];
-#pragma unroll
-          for (int m = 0; m < 8; ++m) s += c[m];
-        }
-        for (; x < len; ++x) s += tbuf[x];
-      }
+      if (tid == 0) ordered_add(s, tbuf, len);
       __syncthreads();
     };
     // term of actor i in the per-actor sums (:663-671)
@@ -782,29 +785,57 @@ __device__ __forceinline__ uint64_t cross_row(const CrossParams& P, const uint64
 }
 
 // One workgroup per network. Integer-valued statistics are exact int64 sums
-// in any order; `from` with the square root (parm >= 2) is summed by one lane
-// in the reference's order (ties row-major), so it rounds identically.
+// in any order; for `from` with the square root (parm >= 2) the workgroup
+// computes the terms a window at a time and one lane adds them in the
+// reference's order (ties row-major), so it rounds identically.
 __global__ void __launch_bounds__(kStatThreads) cross_stats_kernel(const CrossParams P) {
   const int n = P.n, W64 = P.W64, net = blockIdx.x, t = P.t;
   const uint64_t* src = P.rows + (size_t)net * n * W64;
   double* out = P.out + (size_t)net * P.out_stride;
   if (t == 36 && P.parm >= 2.0) {
-    if (threadIdx.x == 0) {
-      double acc = 0.0;
-      for (int i = 0; i < n; ++i)
-        for (int w = 0; w < W64; ++w) {
-          uint64_t m = cross_row(P, src, i, w);
-          while (m) {
-            const int j = w * 64 + __ffsll((long long)m) - 1;
-            m &= m - 1;
-            int c = 0;
-            for (int q = 0; q < W64; ++q)
-              c += __popcll(P.wp[(size_t)i * W64 + q] & P.wp[(size_t)j * W64 + q]);
-            acc += sqrt((double)c);
+    __shared__ double tbuf[kTermCapLds];
+    __shared__ int islots[kStatThreads / 64];
+    const int tid = threadIdx.x, NT = blockDim.x;
+    double acc = 0.0;  // lane 0's running sum
+    // windows of NT actors; a window's ties go through tbuf kTermCapLds at a time
+    for (int b = 0; b < n; b += NT) {
+      const int i = b + tid;
+      const bool act = i < n;
+      int cnt = 0;
+      if (act)
+        for (int w = 0; w < W64; ++w) cnt += __popcll(cross_row(P, src, i, w));
+      int wtot;
+      const int off = block_exscan(cnt, islots, wtot);
+      for (int lo = 0; lo < wtot; lo += kTermCapLds) {
+        const int hi = min(lo + kTermCapLds, wtot);
+        if (act && off < hi && off + cnt > lo) {
+          int kq = off;  // window index of the next tie of i
+          for (int w = 0; w < W64 && kq < hi; ++w) {
+            uint64_t m = cross_row(P, src, i, w);
+            const int c = __popcll(m);
+            if (kq + c <= lo) {
+              kq += c;
+              continue;
+            }
+            while (m && kq < hi) {
+              const int j = w * 64 + __ffsll((long long)m) - 1;
+              m &= m - 1;
+              if (kq >= lo) {
+                int cc = 0;
+                for (int q = 0; q < W64; ++q)
+                  cc += __popcll(P.wp[(size_t)i * W64 + q] & P.wp[(size_t)j * W64 + q]);
+                tbuf[kq - lo] = sqrt((double)cc);
+              }
+              ++kq;
+            }
           }
         }
-      *out = acc;
+        __syncthreads();
+        if (tid == 0) ordered_add(acc, tbuf, hi - lo);
+        __syncthreads();
+      }
     }
+    if (tid == 0) *out = acc;
     return;
   }
   long long c = 0;
