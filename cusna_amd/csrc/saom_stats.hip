@@ -492,11 +492,11 @@ __global__ void __launch_bounds__(kStatThreads) beh_stats_kernel(const BehStatPa
     if (tid == 0) {
       double s = 0.0;
       if (t == 0) {
-        for (int i = 0; i < n; ++i) s += u[i];
+        ordered_add(s, u, n);
       } else if (t == 1) {
         for (int i = 0; i < n; ++i) s += u[i] * u[i];
       } else if (t == 2 || t == 3 || t == 4) {
-        for (int i = 0; i < n; ++i) s += term[i];
+        ordered_add(s, term, n);
       }
       P.out[(size_t)c * P.kb + e] = s;
     }
