@@ -209,6 +209,9 @@ bool specialize_policy(long long chains_x_actors) {
 
 bool same_effects(const SimParams& a, const SimParams& b) {
   if (a.k != b.k) return false;
+  // the specialised kernel compiles the rate-effect path in or out
+  // (SAOM_HAS_RATE): a launch is specialised only for groups that agree on it
+  if ((a.flags & F_RATE) != (b.flags & F_RATE)) return false;
   for (int e = 0; e < a.k; ++e)
     if (a.tid[e] != b.tid[e] || a.ety[e] != b.ety[e] || a.dyn[e] != b.dyn[e]) return false;
   return true;

@@ -122,7 +122,8 @@ int launch_period_rtc(void* fn, const PeriodLaunch& L, int nw, void* stream);
 // has at least this many chains x actors.
 constexpr long long kSpecializeWork = 1ll << 18;
 bool specialize_policy(long long chains_x_actors);
-// Same effect list (what the specialised kernel bakes in).
+// Same effect list and the same rate-effect switch (what the specialised
+// kernel bakes in).
 bool same_effects(const SimParams& a, const SimParams& b);
 
 // Multi-network period (saom_multi.hip): D networks over one actor set

@@ -194,3 +194,15 @@ def groups1000_estimation(simulate, groups: int = 16, n: int = 1000, seed: int =
         datas.append(build_saom_data([w1, w2]))
         effs.append(eff)
     return datas, effs, theta, rates
+
+
+def groups1000_problem(M: int = 4, n: int = 1000, seed: int = 5, rate: float = 3.0,
+                       period_seed: int = 51) -> SaomProblem:
+    """One period of one group of the 16 x 1000-actor panel's shape: wave 1
+    Bernoulli with mean outdegree 5, the 12 effects of ``groups1000_effects``
+    on an N(0, 1) and a 1..5 covariate, GROUPS1000_THETA, rate ``rate``."""
+    rng = np.random.default_rng(seed)
+    w1 = bernoulli_network(n, 5.0, rng)
+    z = rng.standard_normal(n)
+    c = rng.integers(1, 6, size=n).astype(float)
+    return effect_problem(w1, groups1000_effects(z, c), GROUPS1000_THETA, rate, M, period_seed)

@@ -17,6 +17,10 @@ the reference sources; the fixtures, not the reference, travel).
                     periods with non-basic rate effects (covariate, out-degree,
                     in-degree kinds, clamping, conditional, symmetric) and
                     composition change (active lists, restore of inactive actors)
+  large_cases.json.gz  outputs of the reference engine compiled from source at
+                    the benchmark configurations' sizes: 500 actors (8 chains),
+                    300-actor co-evolution, 1000 actors with twelve effects, and
+                    three groups of the uneven batch at 1045-1188 actors
   abi_offsets.txt   field offsets of the reference ABI structs
   targets.json.gz, effects.json.gz, estimate.json.gz, multinet.json.gz,
   data/*.json.gz (the seven data-engine datasets of test-data.R: s50, s50_beh,
@@ -69,6 +73,9 @@ def main():
     from rate_cases import build_rate_cases
 
     dump("rate_cases.json.gz", build_rate_cases(O))
+    from large_cases import build_large_cases
+
+    dump("large_cases.json.gz", build_large_cases(O))
 
     dump("multi_cases.json.gz", {"multi": build_multi_cases(O)["cases"],
                                  "cross": build_cross_cases(O)["cases"]})

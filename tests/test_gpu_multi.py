@@ -117,6 +117,23 @@ def test_cross_stats_match_reference_fixture(q, gold, gpu_lib, oracle_lib):
                                   oracle_lib.cross_stats(a2, wp2, case["t"], case["parm"]))
 
 
+@pytest.mark.parametrize("parm", [1.0, 2.0])
+def test_cross_from_several_term_windows(parm, gpu_lib, oracle_lib):
+    """`from` (t = 36) at n = 600: the square-root form (parm >= 2) adds its
+    tie terms window by window (several 256-actor windows), against the
+    oracle bit for bit; sparse and dense predictors."""
+    for seed, dens in ((4242, 0.12), (4243, 0.01)):
+        a, wp = cross_inputs(seed, n=600, M=3)
+        if dens < 0.1:
+            rng = np.random.default_rng(seed)
+            a = (rng.random(a.shape) < dens).astype(np.uint8)
+            for c in range(a.shape[0]):
+                np.fill_diagonal(a[c], 0)
+        print(f"case cross from n=600 parm={parm} density={dens}", flush=True)
+        np.testing.assert_array_equal(gpu_lib.cross_stats(a, wp, 36, parm),
+                                      oracle_lib.cross_stats(a, wp, 36, parm))
+
+
 def test_multi_argument_checks(gpu_lib):
     from cusna_amd.abi import CusnaError
 

@@ -224,6 +224,29 @@ def test_packed_stats_match_dense(gpu_lib, oracle_lib):
         np.testing.assert_array_equal(got[c].view(np.uint64), ref.view(np.uint64))
 
 
+@pytest.mark.parametrize("n", [600, 1000])
+def test_gwesp_family_stats_several_windows(n, gpu_lib, oracle_lib):
+    """The gwesp/gwdsp statistics (ids 24, 26-30), whose tie terms the
+    statistics kernel adds window by window, on networks of several
+    windows (LDS-resident rows), against the oracle bit for bit."""
+    from cusna_amd.effects import gwesp_weights
+
+    rng = np.random.default_rng(n)
+    M = 3 if n < 1000 else 1  # the oracle's gwdsp sums take ~5 s per network at n = 1000
+    nets = (rng.random((M, n, n)) < 8.0 / n).astype(np.uint8)
+    for c in range(M):
+        np.fill_diagonal(nets[c], 0)
+    ids = np.array([24, 26, 27, 28, 29, 30, 0, 2], np.int32)
+    k = len(ids)
+    wtab = np.array([gwesp_weights(69, n)] * k)
+    print(f"case gwesp family stats n={n}", flush=True)
+    got = gpu_lib.network_stats_packed(nets, ids, wtab=wtab)
+    for c in range(M):
+        ref = oracle_lib.network_stats(nets[c], ids, np.zeros(k), np.zeros(k), np.zeros((k, n)),
+                                       wtab=wtab)
+        np.testing.assert_array_equal(got[c].view(np.uint64), ref.view(np.uint64))
+
+
 def test_period_plan_reuse(gpu_lib, oracle_lib):
     """A device-resident plan run twice with new parameters equals one-shot calls."""
     from cusna_amd.workloads import n500_problem

@@ -76,16 +76,49 @@ def test_reference_stats_cases(oracle_lib):
 
 def test_against_compiled_reference_live(oracle_lib):
     """Where oracle/_ref is built (it needs the reference sources), compare on
-    a fresh n = 500, seven-effect problem, including a conditional run."""
+    a fresh n = 500, seven-effect problem (3 chains) and a conditional run of
+    the same effects at n = 200."""
     if oracle_lib.reference() is None:
         pytest.skip("oracle/_ref not built here")
     from cusna_amd.workloads import n500_problem
 
-    prob = n500_problem(M=3, n=200)
-    a = oracle_lib.simulate(prob, want_end_nets=True)
-    b = oracle_lib.simulate_reference(prob, want_end_nets=True)
-    for key in a:
-        np.testing.assert_array_equal(a[key], b[key], err_msg=key)
+    cond = n500_problem(M=3, n=200, period_seed=29)
+    cond.conditional, cond.target_dist, cond.lam = 1, 150, 1.0
+    for prob in (n500_problem(M=3, n=500, period_seed=23), cond):
+        a = oracle_lib.simulate(prob, want_end_nets=True)
+        b = oracle_lib.simulate_reference(prob, want_end_nets=True)
+        for key in a:
+            np.testing.assert_array_equal(a[key], b[key], err_msg=key)
+    assert np.all(a["dist"] == 150)
+
+
+def _large_cases():
+    return load_golden("large_cases.json.gz")["cases"]
+
+
+@pytest.mark.parametrize("idx", range(6))
+def test_large_reference_cases(oracle_lib, idx):
+    """The compiled reference's outputs at the configurations' own sizes
+    (500 actors x 8 chains, 300-actor co-evolution, 1000 actors with twelve
+    effects, groups of 1045-1188 actors): the oracle equals them bit for bit,
+    end networks by SHA-256."""
+    from large_cases import end_hashes, large_problem
+
+    case = _large_cases()[idx]
+    prob = large_problem(case["spec"])
+    assert (prob.n, prob.k) == (case["n"], case["k"])
+    r = oracle_lib.simulate(prob, want_end_nets=True)
+    o = case["out"]
+    for key in ("dist", "dist_stat", "steps", "ntim", "stats"):
+        np.testing.assert_array_equal(np.asarray(r[key]).ravel(), np.asarray(o[key]).ravel(),
+                                      err_msg=f"{case['spec']['name']} {key}")
+    np.testing.assert_array_equal(r["scores"].ravel(), np.asarray(o["scores"], np.float32).ravel())
+    assert end_hashes(r["a_ends"]) == o["ends_sha256"]
+    if prob.kb:
+        for key in ("z_end", "bdist"):
+            np.testing.assert_array_equal(np.asarray(r[key]).ravel(), np.asarray(o[key]).ravel())
+        np.testing.assert_array_equal(r["bscores"].ravel(),
+                                      np.asarray(o["bscores"], np.float32).ravel())
 
 
 def test_stats_closed_forms(oracle_lib):
