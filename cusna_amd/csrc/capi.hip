@@ -196,7 +196,7 @@ namespace cusna_impl {
 
 cusna_status plan_create(const cusna_saom_problem* pb, bool want_dense, const double* log_u0,
                          const double* loglog_u, int32_t smax, hipStream_t stream,
-                         cusna_period_plan** out) {
+                         cusna_period_plan** out, const DeviceArena* arena) {
   cusna_saom_result dummy{};
   dummy.struct_version = CUSNA_ABI_VERSION;
   cusna_status st = validate(pb, &dummy);
@@ -270,7 +270,7 @@ cusna_status plan_create(const cusna_saom_problem* pb, bool want_dense, const do
     if (P.dci[e] + 1 > nd) nd = P.dci[e] + 1;
   }
   for (int e = 0; e < kb; ++e) P.bty[e] = pb->bty[e];
-  pl->specialize = !replay && saom::specialize_policy((long long)M * n);
+  pl->specialize = !replay && !arena && saom::specialize_policy((long long)M * n);
   pl->nw = pick_nw(wmax, pb->threads);
   if (pb->threads <= 0) {
     const int tuned = cached_nw(ShapeKey{wmax, W64, k, P.flags});
@@ -356,15 +356,24 @@ cusna_status plan_create(const cusna_saom_problem* pb, bool want_dense, const do
     o_lu0 = D.add((size_t)M * smax * 8);
     o_llu = D.add((size_t)M * smax * n * 8);
   }
-  HIP_TRY(D.alloc());
-  if (stream) {
+  if (arena) {
+    if (!D.place(arena->base, arena->capacity)) {
+      set_error("the call needs " + std::to_string(D.bytes()) + " bytes of device memory, above the " +
+                std::to_string(arena->capacity) + " bytes it may use");
+      return CUSNA_ERR_SHAPE;
+    }
     pl->s = stream;
   } else {
-    HIP_TRY(hipStreamCreateWithFlags(&pl->s, hipStreamNonBlocking));
-    pl->own_stream = true;
+    HIP_TRY(D.alloc());
+    if (stream) {
+      pl->s = stream;
+    } else {
+      HIP_TRY(hipStreamCreateWithFlags(&pl->s, hipStreamNonBlocking));
+      pl->own_stream = true;
+    }
+    HIP_TRY(hipEventCreate(&pl->ev0));
+    HIP_TRY(hipEventCreate(&pl->ev1));
   }
-  HIP_TRY(hipEventCreate(&pl->ev0));
-  HIP_TRY(hipEventCreate(&pl->ev1));
   hipStream_t s = pl->s;
   auto up = [&](size_t off, const void* src, size_t bytes) -> hipError_t {
     if (!bytes) return hipSuccess;
@@ -468,6 +477,7 @@ cusna_status plan_run(cusna_period_plan* pl, uint64_t seed, bool want_stats, int
   P.key1 = (uint32_t)(seed >> 32);
   if (M_run > 0) P.M = M_run;
   hipStream_t s = stream ? stream : pl->s;
+  timed = timed && pl->ev0 && pl->ev1;  // a plan placed in an arena has no events
   if (timed) HIP_TRY(hipEventRecord(pl->ev0, s));
   int le = saom::launch_period(P, pl->D.at<saom::SimParams>(pl->o_params),
                                saom::LaunchShape{pl->nw, pl->wmax}, pl->replay, P.key0, P.key1,
@@ -580,6 +590,63 @@ cusna_status simulate(const cusna_saom_problem* pb, cusna_saom_result* r, const 
   st = plan_set_params(pl.get(), pb->theta, pb->lam, pb->thetab, pb->lamb);
   if (st != CUSNA_OK) return st;
   st = plan_run(pl.get(), pb->seed, r->stats != nullptr, -1, nullptr, true);
+  if (st != CUSNA_OK) return st;
+  return plan_fetch(pl.get(), r);
+}
+
+// ---- one period without device allocation
+// The device memory of cusna_saom_simulate_period_noalloc: one array per GPU,
+// which each GPU holds from the load of the library's code object until the
+// process ends. Host code never passes the array's name to a kernel or a copy
+// (on the host it is a stand-in, not a device address): the address comes
+// from hipGetSymbolAddress, once per device, with that device current.
+__device__ __align__(256) unsigned char scratch[CUSNA_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kNoAllocDevices = 64;
+struct NoAllocDevice {
+  std::mutex mu;  // calls on one device run one at a time
+  void* ptr = nullptr;
+};
+NoAllocDevice g_noalloc[kNoAllocDevices];
+
+cusna_status simulate_noalloc(const cusna_saom_problem* pb, cusna_saom_result* r) {
+  cusna_status st = validate(pb, r);
+  if (st != CUSNA_OK) return st;
+  if (device_count() < 1) {
+    set_error("no HIP device");
+    return CUSNA_ERR_NO_CUDA;
+  }
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocDevices) {
+    set_error("cusna_saom_simulate_period_noalloc: device index above 63");
+    return CUSNA_ERR_UNSUPPORTED;
+  }
+  NoAllocDevice& S = g_noalloc[dev];
+  std::lock_guard<std::mutex> lk(S.mu);
+  if (!S.ptr) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error(std::string("cusna_saom_simulate_period_noalloc: hipGetSymbolAddress: ") +
+                hipGetErrorString(e));
+      return CUSNA_ERR_CUDA;
+    }
+    S.ptr = p;
+  }
+  const DeviceArena arena{S.ptr, (size_t)CUSNA_NOALLOC_SCRATCH_BYTES};
+  cusna_period_plan* raw = nullptr;
+  // the null stream: no stream is created; calls on one device are serialised
+  st = plan_create(pb, r->want_end_nets && r->a_ends, nullptr, nullptr, 0, nullptr, &raw, &arena);
+  if (st != CUSNA_OK) {
+    if (st == CUSNA_ERR_SHAPE)
+      set_error("cusna_saom_simulate_period_noalloc: " + g_last_error);
+    return st;
+  }
+  std::unique_ptr<cusna_period_plan> pl(raw);
+  st = plan_set_params(pl.get(), pb->theta, pb->lam, pb->thetab, pb->lamb);
+  if (st != CUSNA_OK) return st;
+  st = plan_run(pl.get(), pb->seed, r->stats != nullptr, -1, nullptr, false);
   if (st != CUSNA_OK) return st;
   return plan_fetch(pl.get(), r);
 }
@@ -967,6 +1034,12 @@ CUSNA_API cusna_status cusna_saom_simulate_period(const cusna_saom_problem* prob
                                                   cusna_saom_result* result) {
   g_last_error.clear();
   return simulate(problem, result, nullptr, nullptr, 0);
+}
+
+CUSNA_API cusna_status cusna_saom_simulate_period_noalloc(const cusna_saom_problem* problem,
+                                                          cusna_saom_result* result) {
+  g_last_error.clear();
+  return simulate_noalloc(problem, result);
 }
 
 CUSNA_API cusna_status cusna_saom_simulate_groups(const cusna_saom_problem* const* problems,

@@ -12,8 +12,9 @@
 
 namespace cusna_impl {
 
-// Device allocation owned by one call: one hipMalloc, carved in 256-byte
-// aligned pieces, freed on scope exit.
+// Device memory of one call: one hipMalloc, carved in 256-byte aligned
+// pieces, freed on scope exit; or the same pieces placed in device memory the
+// caller provides (place), which is then neither allocated nor freed.
 class DeviceBlock {
  public:
   size_t add(size_t bytes) {
@@ -25,17 +26,33 @@ class DeviceBlock {
     if (total_ == 0) total_ = 256;
     return hipMalloc(&base_, total_);
   }
+  // false when the pieces need more than `capacity` bytes
+  bool place(void* base, size_t capacity) {
+    if (total_ == 0) total_ = 256;
+    if (total_ > capacity) return false;
+    base_ = base;
+    owned_ = false;
+    return true;
+  }
+  size_t bytes() const { return total_; }
   template <class T>
   T* at(size_t off) const {
     return reinterpret_cast<T*>(static_cast<char*>(base_) + off);
   }
   ~DeviceBlock() {
-    if (base_) (void)hipFree(base_);
+    if (base_ && owned_) (void)hipFree(base_);
   }
 
  private:
   void* base_ = nullptr;
   size_t total_ = 0;
+  bool owned_ = true;
+};
+
+// Device memory a call places its buffers in instead of allocating them.
+struct DeviceArena {
+  void* base;
+  size_t capacity;
 };
 
 void set_error(const std::string& s);
@@ -94,9 +111,13 @@ struct cusna_period_plan {
 
 namespace cusna_impl {
 
+// arena: place the plan's device buffers there (no allocation, no timing
+// events, no runtime-specialised kernel, whose module load allocates) and
+// run on `stream` as given, the null stream included; CUSNA_ERR_SHAPE when
+// they do not fit.
 cusna_status plan_create(const cusna_saom_problem* pb, bool want_dense, const double* log_u0,
                          const double* loglog_u, int32_t smax, hipStream_t stream,
-                         cusna_period_plan** out);
+                         cusna_period_plan** out, const DeviceArena* arena = nullptr);
 cusna_status plan_set_params(cusna_period_plan* pl, const float* theta, const float* lam,
                              const float* thetab, const float* lamb);
 // Launch one period (and the statistics kernel) for the first M_run chains

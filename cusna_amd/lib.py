@@ -34,6 +34,7 @@ EXPORTS = (
     "cusna_multi_plan_create", "cusna_multi_plan_set_params", "cusna_multi_plan_run",
     "cusna_multi_plan_fetch", "cusna_multi_plan_dist", "cusna_multi_plan_destroy",
     "cusna_multi_plan_set_stats", "cusna_multi_plan_stats", "cusna_multi_plan_specialized",
+    "cusna_saom_simulate_period_noalloc",
 )
 
 _lib = None
@@ -51,6 +52,8 @@ def load(path: str | None = None) -> C.CDLL:
     L = C.CDLL(p)
     L.cusna_saom_simulate_period.restype = C.c_int
     L.cusna_saom_simulate_period.argtypes = [_vp, _vp]
+    L.cusna_saom_simulate_period_noalloc.restype = C.c_int
+    L.cusna_saom_simulate_period_noalloc.argtypes = [_vp, _vp]
     L.cusna_saom_simulate_period_replay.restype = C.c_int
     L.cusna_saom_simulate_period_replay.argtypes = [_vp, _vp, _vp, _vp, C.c_int32]
     L.cusna_pack_bits.restype = C.c_int
@@ -170,6 +173,18 @@ def simulate_period(prob: SaomProblem, want_end_nets: bool = False) -> dict:
     L = load()
     out = run_period(L.cusna_saom_simulate_period, prob, want_end_nets, "cusna_saom_simulate_period")
     return out
+
+
+def simulate_period_noalloc(prob: SaomProblem, want_end_nets: bool = False) -> dict:
+    """One period without device allocation (cusna_saom_simulate_period_noalloc):
+    the call's device buffers sit in the library's static per-GPU array of
+    CUSNA_NOALLOC_SCRATCH_BYTES (64 MiB); larger calls raise CusnaError (status 2)."""
+    name = "cusna_saom_simulate_period_noalloc"
+    L = load()
+    try:
+        return run_period(L.cusna_saom_simulate_period_noalloc, prob, want_end_nets, name)
+    except CusnaError as e:
+        raise CusnaError(name, e.status, L.cusna_last_error().decode(errors="replace")) from None
 
 
 def simulate_multi(prob, want_end_nets: bool = True) -> dict:

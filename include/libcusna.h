@@ -164,6 +164,26 @@ CUSNA_API cusna_status cusna_saom_simulate_period(const cusna_saom_problem *prob
                                                   cusna_saom_result *result);
 
 /*
+ * cusna_saom_simulate_period without device allocation. No counterpart in the
+ * reference: for callers that cannot call hipMalloc (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation). Same
+ * arguments, results and status codes as cusna_saom_simulate_period; the
+ * device buffers of the call (the problem's device copy, chain state, packed
+ * and dense end networks, outputs before their copy to the caller) are placed
+ * in one static device array of CUSNA_NOALLOC_SCRATCH_BYTES per GPU, which
+ * the GPU holds from the library's load until the process ends. A call whose
+ * buffers need more returns CUSNA_ERR_SHAPE (cusna_last_error names the
+ * size). Runs the generic period kernel (no runtime specialisation: loading
+ * a new module allocates) on the null stream. Calls on one GPU share the
+ * array and are serialised by a lock within the process; calls from two
+ * processes on one GPU use separate copies. The array keeps the last call's
+ * contents; every buffer a call reads is written by that call first.
+ */
+#define CUSNA_NOALLOC_SCRATCH_BYTES (64u << 20)
+CUSNA_API cusna_status cusna_saom_simulate_period_noalloc(const cusna_saom_problem *problem,
+                                                          cusna_saom_result *result);
+
+/*
  * Multi-network co-evolution: D one-mode networks (2..CUSNA_MAXNET) over one
  * actor set. Layout byte-identical to the reference (libcusna.h:238-278).
  * Each ministep picks a network in proportion to its rate, an active actor,
