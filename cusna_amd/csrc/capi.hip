@@ -771,6 +771,7 @@ cusna_status simulate_groups(const cusna_saom_problem* const* pbs, int G,
     L.key0 = L.key1 = 0u;
     L.Ph = &plans[b.groups[0]]->P;
     L.specialize = uniform && saom::specialize_policy(work);
+    for (int g : b.groups) L.same_n = L.same_n && plans[g]->n == plans[b.groups[0]]->n;
     L.exact = ex;
     L.lds_exact = lds_exact;
   }

@@ -158,7 +158,7 @@ void raise_lds(void* fn) {
 
 }  // namespace
 
-void* rtc_period_kernel(const SimParams& P, LaunchShape s, bool replay, bool simple) {
+void* rtc_period_kernel(const SimParams& P, LaunchShape s, bool replay, bool simple, bool fixed_n) {
   const int R = s.wmax / s.nw;
   std::vector<std::string> opts = {
       "--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
@@ -170,7 +170,9 @@ void* rtc_period_kernel(const SimParams& P, LaunchShape s, bool replay, bool sim
       "-DSAOM_FIXED_TYPES=" + list(P.tid, P.k), "-DSAOM_FIXED_ETYPES=" + list(P.ety, P.k),
       "-DSAOM_FIXED_DYN=" + list(P.dyn, P.k),
       std::string("-DSAOM_CACHE_D=") + (P.k * R <= 32 ? "1" : "0"),
-      std::string("-DSAOM_HAS_RATE=") + ((P.flags & F_RATE) ? "1" : "0")};
+      std::string("-DSAOM_HAS_RATE=") + ((P.flags & F_RATE) ? "1" : "0"),
+      "-DSAOM_FIXED_KB=" + std::to_string(P.kb), "-DSAOM_FIXED_FLAGS=" + std::to_string(P.flags)};
+  if (fixed_n) opts.push_back("-DSAOM_FIXED_N=" + std::to_string(P.n));
   return rtc_get(kRtcPeriodSource, "saom_period_rtc", opts);
 }
 
@@ -209,9 +211,10 @@ bool specialize_policy(long long chains_x_actors) {
 
 bool same_effects(const SimParams& a, const SimParams& b) {
   if (a.k != b.k) return false;
-  // the specialised kernel compiles the rate-effect path in or out
-  // (SAOM_HAS_RATE): a launch is specialised only for groups that agree on it
-  if ((a.flags & F_RATE) != (b.flags & F_RATE)) return false;
+  // the specialised kernel compiles in the behaviour-effect count and the
+  // feature flags (rate effects and their count included): a launch is
+  // specialised only for groups that agree on them
+  if (a.kb != b.kb || a.flags != b.flags) return false;
   for (int e = 0; e < a.k; ++e)
     if (a.tid[e] != b.tid[e] || a.ety[e] != b.ety[e] || a.dyn[e] != b.dyn[e]) return false;
   return true;

@@ -98,6 +98,8 @@ struct PeriodLaunch {
   // generic shape
   LaunchShape exact{0, 0};
   size_t lds_exact = 0;
+  // every group of the launch has the actor count of Ph
+  bool same_n = true;
 };
 // Word bound and block shape of a specialised kernel for networks of W64
 // words: the generic shape rounds the word bound up to a power of two; the
@@ -116,14 +118,16 @@ int launch_period_ex(const PeriodLaunch& L, LaunchShape s, bool replay, bool sim
 
 // Runtime specialisation (rtc.hip): the compiled kernel for P's effect list
 // and this launch shape (null when unavailable), and its launch.
-void* rtc_period_kernel(const SimParams& P, LaunchShape s, bool replay, bool simple);
+// fixed_n: every group of the launch has P's actor count (compiled in).
+void* rtc_period_kernel(const SimParams& P, LaunchShape s, bool replay, bool simple,
+                        bool fixed_n);
 int launch_period_rtc(void* fn, const PeriodLaunch& L, int nw, void* stream);
 // Launch policy: CUSNA_SPECIALIZE=0 never, =1 always, unset: when the launch
 // has at least this many chains x actors.
 constexpr long long kSpecializeWork = 1ll << 18;
 bool specialize_policy(long long chains_x_actors);
-// Same effect list and the same rate-effect switch (what the specialised
-// kernel bakes in).
+// Same effect list, behaviour-effect count and feature flags (what the
+// specialised kernel bakes in).
 bool same_effects(const SimParams& a, const SimParams& b);
 
 // Multi-network period (saom_multi.hip): D networks over one actor set

@@ -64,7 +64,7 @@ int launch_period_ex(const PeriodLaunch& L, LaunchShape s, bool replay, bool sim
   if (L.specialize && L.Ph) {
     const bool ex = L.exact.nw > 0 && L.lds_exact <= 160 * 1024;
     const LaunchShape rs = ex ? L.exact : s;
-    void* fn = rtc_period_kernel(*L.Ph, rs, replay, simple);
+    void* fn = rtc_period_kernel(*L.Ph, rs, replay, simple, L.same_n);
     if (fn) {
       PeriodLaunch Lr = L;
       if (ex) Lr.lds = L.lds_exact;

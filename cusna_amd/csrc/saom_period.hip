@@ -75,6 +75,35 @@ __device__ __forceinline__ int eff_ety(const SimParams& P, int e) { return P.ety
 __device__ __forceinline__ int eff_dyn(const SimParams& P, int e) { return P.dyn[e]; }
 #endif
 
+// Shape values a specialised launch may fix at compile time when every
+// group of the launch shares them (the actor count, the behaviour-effect
+// count, the feature flags): LDS offsets and loop bounds then fold into
+// immediates instead of occupying scalar registers across the ministep loop.
+__device__ __forceinline__ int shape_n(const SimParams& P) {
+#ifdef SAOM_FIXED_N
+  (void)P;
+  return SAOM_FIXED_N;
+#else
+  return P.n;
+#endif
+}
+__device__ __forceinline__ int shape_kb(const SimParams& P) {
+#ifdef SAOM_FIXED_KB
+  (void)P;
+  return SAOM_FIXED_KB;
+#else
+  return P.kb;
+#endif
+}
+__device__ __forceinline__ int shape_flags(const SimParams& P) {
+#ifdef SAOM_FIXED_FLAGS
+  (void)P;
+  return SAOM_FIXED_FLAGS;
+#else
+  return P.flags;
+#endif
+}
+
 __device__ __forceinline__ int popc64(uint64_t x) { return __popcll(x); }
 
 // A wave-uniform 64-bit value moved to scalar registers (loops over its set
@@ -332,7 +361,7 @@ __device__ __forceinline__ void eff_base(const SimParams& P, int e, const Alt<R>
   const int* tpall = S.tpall;
   const int* soall = S.soall;
   const int t = eff_type(P, e);
-  const int n = P.n;
+  const int n = shape_n(P);
   const bool dyn = eff_dyn(P, e) != 0;
   const double* vrow = P.vmat + (size_t)e * n;
   const double* w = P.wtab + (size_t)e * (n + 1);
@@ -697,9 +726,19 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
   extern __shared__ __align__(16) unsigned char smem[];
   // word-major bit rows: word w of actor h at rows[w * RS + h], RS = n, so the
   // lanes of a wave reading word w of consecutive actors hit consecutive words
-  const int n = P.n, k = P.k, kb = P.kb, W64 = P.W64, RS = P.RS;
+  const int n = shape_n(P), kb = shape_kb(P), pflags = shape_flags(P);
+#ifdef SAOM_FIXED_K
+  constexpr int k = SAOM_FIXED_K;
+#else
+  const int k = P.k;
+#endif
+#ifdef SAOM_FIXED_N
+  constexpr int W64 = (SAOM_FIXED_N + 63) / 64, RS = SAOM_FIXED_N;
+#else
+  const int W64 = P.W64, RS = P.RS;
+#endif
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const LdsLayout L = lds_layout(n, k, kb, WMAX, RS, NW, P.flags);
+  const LdsLayout L = lds_layout(n, k, kb, WMAX, RS, NW, pflags);
   constexpr bool STREAM = WMAX > kLdsWordBound;
   // streamed: this chain's rows in device memory, same word-major layout
   // with W64 words (no padding)
@@ -733,7 +772,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
 #else
   constexpr int kRateMask = ~0;
 #endif
-  const int flags = (SIMPLE ? (P.flags & (F_TP | F_Z | F_RDEG | F_RATE)) : P.flags) & kRateMask;
+  const int flags = (SIMPLE ? (pflags & (F_TP | F_Z | F_RDEG | F_RATE)) : pflags) & kRateMask;
   const ChainLds S{rows, RS, W64, NW, wv, lane, indeg, outdeg, z, tpall, soall, tpm0, tpm1};
 
   // ---- chain set-up (libcusna_saom_cpu.cpp:146-160)
