@@ -824,14 +824,34 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
   long long cap = kMaxSteps;
   if (REPLAY && P.smax < cap) cap = P.smax;
   double t = 0.0;
+  // The draw of ministep s (Philox block (s, chain, 0, 0), log u0 and, with
+  // the basic rate, the waiting time) depends on s alone, not on the network:
+  // each ministep computes its successor's while its own LDS traffic is in
+  // flight, which keeps the logarithm off the ministep's serial path.
+  struct Draw {
+    uint32_t c1, c2, c3;
+    double lu0, tau;
+  };
+  const double rate_div = (double)nact * (lam + lamb);
+  auto draw = [&](long long s) -> Draw {
+    uint32_t c0 = (uint32_t)s, c1 = (uint32_t)chain, c2 = 0u, c3 = 0u;
+    philox10(c0, c1, c2, c3, key0, key1);
+    Draw d;
+    d.c1 = c1;
+    d.c2 = c2;
+    d.c3 = c3;
+    if (REPLAY) d.lu0 = s < cap ? P.lu0[(size_t)chain * P.smax + s] : 0.0;
+    else d.lu0 = log(unit_open(c0));
+    d.tau = -d.lu0 / rate_div;
+    return d;
+  };
+  Draw next = draw(0);
 
   for (;;) {
     // 1. ministep draw (:167, 194-200)
-    uint32_t c0 = (uint32_t)steps, c1 = (uint32_t)chain, c2 = 0u, c3 = 0u;
-    philox10(c0, c1, c2, c3, key0, key1);
-    double lu0;
-    if (REPLAY) lu0 = P.lu0[(size_t)chain * P.smax + steps];
-    else lu0 = log(unit_open(c0));
+    const Draw cur = next;
+    const uint32_t c1 = cur.c1, c2 = cur.c2, c3 = cur.c3;
+    const double lu0 = cur.lu0;
     int i;
     bool net;
     if (flags & F_RATE) {
@@ -929,7 +949,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
       i = ract[below < nact ? (int)below : nact - 1];
       net = true;
     } else {
-      t += -lu0 / ((double)nact * (lam + lamb));
+      t += cur.tau;
       if (!P.cond && t > 1.0) break;
       net = kb == 0 || unit_open(c1) < lam / (lam + lamb);
       long long ix = (long long)(unit_open(c2) * nact);
@@ -941,6 +961,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
     if (!net) {
       // ---- behaviour ministep (:202-227)
       const int zi = z[i];
+      next = draw(steps + 1);
       const double zic = (double)zi - P.zbar;
       long long nb = 0, nH = 0, nE = 0, nL = 0;
       double zs = 0.0;  // exact in any order: terms are multiples of ulp(float zbar)
@@ -1067,6 +1088,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
     }
     U.outdeg_i = outdeg[i];
     U.indeg_i = indeg[i];
+    next = draw(steps + 1);  // independent of the loads above: overlaps their latency
     U.zvi = (flags & F_Z) ? (double)z[i] - P.zbar : -P.zbar;
     U.rdeg_i = 0;
     if (flags & F_RDEG) {
@@ -1210,6 +1232,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
     double gb = -1e30, lm = -1e30;
     int jb = i;
     double shift = 0.0;  // softmax shift: lmax (replay) or the noisy maximum's bound
+    bool exact_choice = true;  // the choice needs the exact noise's arg-max
     if constexpr (REPLAY) {
 #pragma unroll
       for (int r = 0; r < R; ++r) {
@@ -1229,50 +1252,81 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
                              uj);
       double ga[R];
       double M = -INFINITY;
+      int jm = i;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         if (A.j[r] == i) f[r] = 0.0;
         ga[r] = -INFINITY;
         if (!((A.ok >> r) & 1)) continue;
         ga[r] = f[r] - (double)approx_loglog(uj[r]);
-        M = fmax(M, ga[r]);
+        arg_combine(M, jm, ga[r], A.j[r]);
       }
-      M = wave_max(M);
-      if (NW > 1) {
-        if (lane == 0) red[wv * 4 + 3] = M;
-        __syncthreads();
-        M = red[3];
-        for (int q = 1; q < NW; ++q) M = fmax(M, red[q * 4 + 3]);
-      }
-      const double cut = M - 2.0 * kGumbelSlack;
+      wave_argmax(M, jm);
+      // alternatives of this wave within 2e of its own largest bound
+      int near = 0;
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        if (ga[r] >= cut) {
-          const double ll = log(-log(unit_open(uj[r])));
-          arg_combine(gb, jb, f[r] - ll, A.j[r]);
+      for (int r = 0; r < R; ++r)
+        near += __popcll(__ballot(((A.ok >> r) & 1) && ga[r] >= M - 2.0 * kGumbelSlack));
+      // A single alternative within 2e of the largest bound is the exact
+      // maximiser (every maximiser of the exact g lies within 2e of it), so
+      // the exact noise is evaluated only when there are several (rarely:
+      // the top gap of a Gumbel maximum is below 2e in about 0.8 % of draws).
+      bool unique = near == 1;
+      if (NW > 1) {
+        if (lane == 0) {
+          red[wv * 4 + 3] = M;
+          cnt[wv * 8 + 6] = near;
+          cnt[wv * 8 + 7] = jm;
         }
+        __syncthreads();
+        int qm = 0;
+        M = red[3];
+        for (int q = 1; q < NW; ++q)
+          if (red[q * 4 + 3] > M) {
+            M = red[q * 4 + 3];
+            qm = q;
+          }
+        unique = cnt[qm * 8 + 6] == 1;
+        for (int q = 0; q < NW; ++q)
+          if (q != qm && red[q * 4 + 3] >= M - 2.0 * kGumbelSlack) unique = false;
+        jm = (int)cnt[qm * 8 + 7];
       }
       // M >= max_j f_j - 3.14 (the noise -log(-log u) is at least
       // -log(33 ln 2) for every u >= 2^-33 the stream yields), so
       // exp(f_j - M) <= e^3.14 and M serves as the softmax shift: the scores
       // are ratios of the weighted sums, invariant to the shift up to rounding
       shift = M;
-    }
-    wave_argmax(gb, jb);
-    if constexpr (REPLAY) lm = wave_max(lm);
-    if (NW > 1) {
-      if (lane == 0) {
-        red[wv * 4 + 0] = gb;
-        red[wv * 4 + 1] = lm;
-        red[wv * 4 + 2] = (double)jb;
+      if (unique) {
+        jb = jm;
+      } else {
+        const double cut = M - 2.0 * kGumbelSlack;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          if (ga[r] >= cut) {
+            const double ll = log(-log(unit_open(uj[r])));
+            arg_combine(gb, jb, f[r] - ll, A.j[r]);
+          }
+        }
       }
-      __syncthreads();
-      gb = red[0];
-      lm = red[1];
-      jb = (int)red[2];
-      for (int q = 1; q < NW; ++q) {
-        arg_combine(gb, jb, red[q * 4 + 0], (int)red[q * 4 + 2]);
-        if constexpr (REPLAY) lm = fmax(lm, red[q * 4 + 1]);
+      exact_choice = !unique;
+    }
+    if (exact_choice) {  // uniform over the workgroup
+      wave_argmax(gb, jb);
+      if constexpr (REPLAY) lm = wave_max(lm);
+      if (NW > 1) {
+        if (lane == 0) {
+          red[wv * 4 + 0] = gb;
+          red[wv * 4 + 1] = lm;
+          red[wv * 4 + 2] = (double)jb;
+        }
+        __syncthreads();
+        gb = red[0];
+        lm = red[1];
+        jb = (int)red[2];
+        for (int q = 1; q < NW; ++q) {
+          arg_combine(gb, jb, red[q * 4 + 0], (int)red[q * 4 + 2]);
+          if constexpr (REPLAY) lm = fmax(lm, red[q * 4 + 1]);
+        }
       }
     }
     if constexpr (REPLAY) shift = lm;
