@@ -747,21 +747,23 @@ cusna_status simulate_groups(const cusna_saom_problem* const* pbs, int G,
     std::stable_sort(b.groups.begin(), b.groups.end(),
                      [&](int a, int c) { return cost(a) > cost(c); });
     const size_t first = items.size();
-    size_t lds = 0, lds_exact = 0;
+    size_t lds = 0, lds_exact = 0, lds_alt = 0;
     bool uniform = true, free_shape = true;
     long long work = 0;
     for (int g : b.groups) {
       uniform = uniform && saom::same_effects(plans[g]->P, plans[b.groups[0]]->P);
       free_shape = free_shape && plans[g]->free_shape;
       work += (long long)plans[g]->M * plans[g]->n;
+      lds = std::max(lds, saom::sim_lds_bytes(plans[g]->P, b.nw, b.wmax));
     }
-    const saom::LaunchShape ex =
-        saom::exact_shape(b.W64, saom::LaunchShape{b.nw, b.wmax},
-                          uniform && free_shape ? plans[b.groups[0]]->P.k : 0);
+    const int kf = uniform && free_shape ? plans[b.groups[0]]->P.k : 0;
+    const saom::LaunchShape gen{b.nw, b.wmax};
+    const saom::LaunchShape ex = saom::exact_shape(b.W64, gen, kf, lds);
+    const saom::LaunchShape alt = saom::exact_shape(b.W64, gen, kf);
     for (int g : b.groups) {
       for (int c = 0; c < plans[g]->M; ++c) items.push_back(((uint32_t)g << 20) | (uint32_t)c);
-      lds = std::max(lds, saom::sim_lds_bytes(plans[g]->P, b.nw, b.wmax));
       lds_exact = std::max(lds_exact, saom::sim_lds_bytes(plans[g]->P, ex.nw, ex.wmax));
+      lds_alt = std::max(lds_alt, saom::sim_lds_bytes(plans[g]->P, alt.nw, alt.wmax));
     }
     saom::PeriodLaunch& L = launches[bi];
     L.Pd = D.at<saom::SimParams>(o_grp);
@@ -774,6 +776,8 @@ cusna_status simulate_groups(const cusna_saom_problem* const* pbs, int G,
     for (int g : b.groups) L.same_n = L.same_n && plans[g]->n == plans[b.groups[0]]->n;
     L.exact = ex;
     L.lds_exact = lds_exact;
+    L.exact_alt = alt;
+    L.lds_exact_alt = lds_alt;
   }
   HIP_TRY(hipMemcpyAsync(D.at<uint32_t>(o_items), items.data(), items.size() * 4,
                          hipMemcpyHostToDevice, s));

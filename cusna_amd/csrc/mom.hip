@@ -620,9 +620,12 @@ cusna_status engine_simulate(cusna_mom_engine* E, SimSet& X, int M) {
     L.key0 = L.key1 = 0u;
     L.Ph = &p0->P;
     L.specialize = X.specialize;
-    L.exact = saom::exact_shape(p0->W64, saom::LaunchShape{p0->nw, p0->wmax},
-                                p0->free_shape ? p0->P.k : 0);
+    const saom::LaunchShape gen{p0->nw, p0->wmax};
+    const int kf = p0->free_shape ? p0->P.k : 0;
+    L.exact = saom::exact_shape(p0->W64, gen, kf, L.lds);
     L.lds_exact = saom::sim_lds_bytes(p0->P, L.exact.nw, L.exact.wmax);
+    L.exact_alt = saom::exact_shape(p0->W64, gen, kf);
+    L.lds_exact_alt = saom::sim_lds_bytes(p0->P, L.exact_alt.nw, L.exact_alt.wmax);
     for (int pp = 1; pp < E->P; ++pp) L.same_n = L.same_n && X.plans[pp]->n == p0->n;
     const int le = saom::launch_period_ex(L, saom::LaunchShape{p0->nw, p0->wmax}, false,
                                           saom::simple_effects(p0->P.tid, p0->P.k), s);

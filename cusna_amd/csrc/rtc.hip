@@ -158,7 +158,8 @@ void raise_lds(void* fn) {
 
 }  // namespace
 
-void* rtc_period_kernel(const SimParams& P, LaunchShape s, bool replay, bool simple, bool fixed_n) {
+void* rtc_period_kernel(const SimParams& P, LaunchShape s, bool replay, bool simple, bool fixed_n,
+                        int waves_per_eu) {
   const int R = s.wmax / s.nw;
   std::vector<std::string> opts = {
       "--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
@@ -173,6 +174,7 @@ void* rtc_period_kernel(const SimParams& P, LaunchShape s, bool replay, bool sim
       std::string("-DSAOM_HAS_RATE=") + ((P.flags & F_RATE) ? "1" : "0"),
       "-DSAOM_FIXED_KB=" + std::to_string(P.kb), "-DSAOM_FIXED_FLAGS=" + std::to_string(P.flags)};
   if (fixed_n) opts.push_back("-DSAOM_FIXED_N=" + std::to_string(P.n));
+  if (waves_per_eu > 0) opts.push_back("-DSAOM_WAVES_PER_EU=" + std::to_string(waves_per_eu));
   return rtc_get(kRtcPeriodSource, "saom_period_rtc", opts);
 }
 
@@ -200,6 +202,25 @@ int launch_multi_rtc(void* fn, const MultiParams* Pd, int grid, size_t lds, int 
   void* args[] = {&pd};
   return (int)hipModuleLaunchKernel((hipFunction_t)fn, (unsigned)grid, 1, 1, 64u * (unsigned)nw,
                                     1, 1, (unsigned)lds, (hipStream_t)stream, args, nullptr);
+}
+
+bool uses_scratch(void* fn) {
+  int bytes = 0;
+  if (hipFuncGetAttribute(&bytes, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, (hipFunction_t)fn) !=
+      hipSuccess) {
+    (void)hipGetLastError();
+    return true;
+  }
+  return bytes > 0;
+}
+
+int kernel_vgprs(void* fn) {
+  int regs = 0;
+  if (hipFuncGetAttribute(&regs, HIP_FUNC_ATTRIBUTE_NUM_REGS, (hipFunction_t)fn) != hipSuccess) {
+    (void)hipGetLastError();
+    return 1 << 30;
+  }
+  return regs;
 }
 
 bool specialize_policy(long long chains_x_actors) {

@@ -60,6 +60,7 @@ struct SimParams {
 struct LaunchShape {
   int nw;    // waves per chain (workgroup = 64*nw threads)
   int wmax;  // compile-time bound on W64 (1 .. 16 LDS-resident; 32, 64, 128 streamed)
+  int wpe = 0;  // specialised kernel's register budget in waves per SIMD (0: two)
 };
 
 // LDS bytes one chain needs with the given shape.
@@ -95,9 +96,12 @@ struct PeriodLaunch {
   bool specialize = false;
   bool* used = nullptr;
   // shape and LDS bytes of the specialised kernel (exact_shape); nw = 0: the
-  // generic shape
+  // generic shape. exact_alt: the shape to use instead when the preferred
+  // one's register budget (exact.wpe) would make the compiler spill.
   LaunchShape exact{0, 0};
   size_t lds_exact = 0;
+  LaunchShape exact_alt{0, 0};
+  size_t lds_exact_alt = 0;
   // every group of the launch has the actor count of Ph
   bool same_n = true;
 };
@@ -111,16 +115,24 @@ struct PeriodLaunch {
 // the waves per chain up to 8 words when that brings k x R within the 32
 // change statistics per lane that the specialised kernel keeps in registers
 // between its choice and score passes (without them it computes every change
-// statistic twice).
-LaunchShape exact_shape(int W64, LaunchShape generic, int k_free = 0);
+// statistic twice), and also when a CU holds at most four chains of
+// lds_generic bytes (rows of 5..8 words, 385..512 actors): four waves per
+// SIMD then hide latency that two cannot, given a four-wave register budget
+// (wpe = 4; at 500 actors 118 against 124 ms per 4096-chain period).
+LaunchShape exact_shape(int W64, LaunchShape generic, int k_free = 0, size_t lds_generic = 0);
 int launch_period_ex(const PeriodLaunch& L, LaunchShape s, bool replay, bool simple,
                      void* stream);
 
 // Runtime specialisation (rtc.hip): the compiled kernel for P's effect list
 // and this launch shape (null when unavailable), and its launch.
-// fixed_n: every group of the launch has P's actor count (compiled in).
+// fixed_n: every group of the launch has P's actor count (compiled in);
+// waves_per_eu > 0: the register budget in waves per SIMD (0: two waves).
 void* rtc_period_kernel(const SimParams& P, LaunchShape s, bool replay, bool simple,
-                        bool fixed_n);
+                        bool fixed_n, int waves_per_eu = 0);
+// Whether a compiled kernel uses scratch memory (register spills), and its
+// vector registers per lane.
+bool uses_scratch(void* fn);
+int kernel_vgprs(void* fn);
 int launch_period_rtc(void* fn, const PeriodLaunch& L, int nw, void* stream);
 // Launch policy: CUSNA_SPECIALIZE=0 never, =1 always, unset: when the launch
 // has at least this many chains x actors.

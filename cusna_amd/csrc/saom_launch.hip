@@ -1,6 +1,8 @@
 // saom_launch.hip -- dispatch of the period kernel over its compiled shapes.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "saom_kernels.h"
 
 namespace saom {
@@ -34,13 +36,16 @@ bool simple_effects(const int* t, int k) {
   return true;
 }
 
-LaunchShape exact_shape(int W64, LaunchShape g, int k_free) {
-  int nw, R;
+LaunchShape exact_shape(int W64, LaunchShape g, int k_free, size_t lds_generic) {
+  int nw, R, wpe = 0;
   if (W64 <= 8) {
     nw = g.nw;
     R = (W64 + nw - 1) / nw;
     const int R2 = (W64 + 2 * nw - 1) / (2 * nw);
-    if (k_free > 0 && k_free * R > 32 && 2 * nw <= 8 && k_free * R2 <= 32) {
+    const int chains_cu = lds_generic > 0 ? (int)((160 * 1024) / lds_generic) : 64;
+    if (k_free > 0 && 2 * nw <= 8 && k_free * R2 <= 32 &&
+        (k_free * R > 32 || (W64 >= 5 && chains_cu <= 4))) {
+      if (k_free * R <= 32) wpe = 4;  // doubled for occupancy: the four-wave budget
       nw *= 2;
       R = R2;
     }
@@ -51,7 +56,7 @@ LaunchShape exact_shape(int W64, LaunchShape g, int k_free) {
     R = (W64 + 15) / 16;
     nw = (W64 + R - 1) / R;
   }
-  return LaunchShape{nw, nw * R};
+  return LaunchShape{nw, nw * R, wpe};
 }
 
 int launch_period_ex(const PeriodLaunch& L, LaunchShape s, bool replay, bool simple,
@@ -62,12 +67,30 @@ int launch_period_ex(const PeriodLaunch& L, LaunchShape s, bool replay, bool sim
   // gave one; a launch the runtime refuses (for instance for its dynamic LDS
   // size) falls back to the generic kernel
   if (L.specialize && L.Ph) {
-    const bool ex = L.exact.nw > 0 && L.lds_exact <= 160 * 1024;
-    const LaunchShape rs = ex ? L.exact : s;
-    void* fn = rtc_period_kernel(*L.Ph, rs, replay, simple, L.same_n);
+    bool ex = L.exact.nw > 0 && L.lds_exact <= 160 * 1024;
+    LaunchShape rs = ex ? L.exact : s;
+    size_t lds = ex ? L.lds_exact : L.lds;
+    void* fn = nullptr;
+    if (ex && rs.wpe > 2) {
+      // the preferred shape needs rs.wpe waves per SIMD: compiled with the
+      // default budget when it fits that occupancy's registers anyway (the
+      // default budget schedules better: 113 against 117 ms at 500 actors),
+      // else with the tighter budget, unless that spills; then the
+      // alternative shape in the two-wave budget
+      fn = rtc_period_kernel(*L.Ph, rs, replay, simple, L.same_n);
+      if (fn && (uses_scratch(fn) || kernel_vgprs(fn) > 512 / rs.wpe)) {
+        fn = rtc_period_kernel(*L.Ph, rs, replay, simple, L.same_n, rs.wpe);
+        if (fn && uses_scratch(fn)) fn = nullptr;
+      }
+      if (!fn && L.exact_alt.nw > 0 && L.lds_exact_alt <= 160 * 1024) {
+        rs = L.exact_alt;
+        lds = L.lds_exact_alt;
+      }
+    }
+    if (!fn) fn = rtc_period_kernel(*L.Ph, rs, replay, simple, L.same_n);
     if (fn) {
       PeriodLaunch Lr = L;
-      if (ex) Lr.lds = L.lds_exact;
+      if (ex) Lr.lds = lds;
       const int e = launch_period_rtc(fn, Lr, rs.nw, stream);
       if (e == 0) {
         if (L.used) *L.used = true;
@@ -130,8 +153,10 @@ int launch_period(const SimParams& P, const SimParams* Pd, LaunchShape s, bool r
   L.Ph = &P;
   L.specialize = specialize;
   L.used = used;
-  L.exact = exact_shape(P.W64, s, free_shape ? P.k : 0);
+  L.exact = exact_shape(P.W64, s, free_shape ? P.k : 0, L.lds);
   L.lds_exact = sim_lds_bytes(P, L.exact.nw, L.exact.wmax);
+  L.exact_alt = exact_shape(P.W64, s, free_shape ? P.k : 0);
+  L.lds_exact_alt = sim_lds_bytes(P, L.exact_alt.nw, L.exact_alt.wmax);
   return launch_period_ex(L, s, replay, simple_effects(P.tid, P.k), stream);
 }
 

@@ -130,6 +130,13 @@ __device__ __forceinline__ double dpp_f64(double v) {
   const int hi = dpp_i32<CTRL>((int)(b >> 32));
   return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
+// A wave-uniform double moved to scalar registers.
+__device__ __forceinline__ double uniform_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)b);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
 __device__ __forceinline__ double readlane_f64(double v, int l) {
   const long long b = __double_as_longlong(v);
   const int lo = __builtin_amdgcn_readlane((int)b, l);
@@ -312,6 +319,9 @@ struct Alt {
 // Ego-level state shared by all alternatives of the ministep. Row and column
 // of i sit in registers when the network is LDS-resident; a streamed network
 // (rows in device memory, W64 up to 128 words) keeps them in LDS instead.
+// (Reading them from LDS where they lie was measured slower also at four and
+// eight waves per chain: 121 against 118 ms at 500 actors, 99 against 90 ms
+// at 1000.)
 template <int WMAX, bool STREAM>
 struct Ego {
   static constexpr int WR = STREAM ? 1 : WMAX;
@@ -349,10 +359,9 @@ struct ChainLds {
 // alternatives (libcusna_saom_cpu.cpp:49-101), creation/endowment masking
 // applied (:258-259). Covariate values: vmat row e, or the current behaviour
 // minus zbar for behaviour-dynamic effects (:233, 244, 252-253).
-template <int R, int WMAX, bool STREAM, bool SIMPLE>
+template <int R, int WMAX, bool STREAM, bool SIMPLE, class EgoT>
 __device__ __forceinline__ void eff_base(const SimParams& P, int e, const Alt<R>& A,
-                                         const Ego<WMAX, STREAM>& U, const ChainLds& S,
-                                         double (&d)[R]) {
+                                         const EgoT& U, const ChainLds& S, double (&d)[R]) {
   const uint64_t* rows = S.rows;
   const int RS = S.RS;
   const int* indeg = S.indeg;
@@ -594,10 +603,9 @@ __device__ __forceinline__ void eff_base(const SimParams& P, int e, const Alt<R>
 // Full change statistic of effect e, including interactions (type 32):
 // d = (c2 < 0 ? delta : 1) * (d_c0 * d_c1) [* d_c2] (:246-251). Alternatives
 // j == i get 0 (:238).
-template <int R, int WMAX, bool STREAM, bool SIMPLE>
+template <int R, int WMAX, bool STREAM, bool SIMPLE, class EgoT>
 __device__ __forceinline__ void eff_full(const SimParams& P, int e, const Alt<R>& A,
-                                         const Ego<WMAX, STREAM>& U, const ChainLds& S,
-                                         double (&d)[R]) {
+                                         const EgoT& U, const ChainLds& S, double (&d)[R]) {
   if constexpr (SIMPLE) {
     eff_base<R, WMAX, STREAM, true>(P, e, A, U, S, d);
   } else {
@@ -934,7 +942,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
           rsc_s[q] -= dec;
         }
       }
-      t += tau;
+      t = uniform_f64(t + tau);
       if (!P.cond && t > 1.0) break;
       const double tgt = unit_open(c2) * S0;
       long long below = 0;  // running sums are non-decreasing: count those below tgt
@@ -949,7 +957,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
       i = ract[below < nact ? (int)below : nact - 1];
       net = true;
     } else {
-      t += cur.tau;
+      t = uniform_f64(t + cur.tau);  // the chain time lives in scalar registers
       if (!P.cond && t > 1.0) break;
       net = kb == 0 || unit_open(c1) < lam / (lam + lamb);
       long long ix = (long long)(unit_open(c2) * nact);
@@ -1564,8 +1572,14 @@ size_t SAOM_CAT(sim_lds_bytes_w, SAOM_WMAX)(const SimParams& P, int nw, int wmax
 #else   // SAOM_RTC: one specialised kernel, shape and effect list from macros
 }  // namespace
 
+// Register budget: the launch's occupancy in waves per SIMD (set by the
+// host from the chain's LDS size; two waves, as the generic kernel, when the
+// tighter budget would spill).
+#ifndef SAOM_WAVES_PER_EU
+#define SAOM_WAVES_PER_EU (SAOM_NW == 1 ? 1 : 2)
+#endif
 extern "C" __global__ void __launch_bounds__(64 * SAOM_NW)
-    __attribute__((amdgpu_waves_per_eu(SAOM_NW == 1 ? 1 : 2)))  // as the generic kernel
+    __attribute__((amdgpu_waves_per_eu(SAOM_WAVES_PER_EU)))
     saom_period_rtc(const SimParams* __restrict__ Pp, const uint32_t* __restrict__ items,
                     uint32_t key0_arg, uint32_t key1_arg) {
   period_body<SAOM_NW, SAOM_WMAX / SAOM_NW, SAOM_WMAX, SAOM_REPLAY != 0, SAOM_SIMPLE != 0>(

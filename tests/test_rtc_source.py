@@ -97,6 +97,11 @@ CASES["k12"] = dict(SAOM_WMAX=8, SAOM_NW=4, SAOM_REPLAY=0, SAOM_SIMPLE=0, SAOM_F
                     SAOM_FIXED_ETYPES=",".join(["0"] * 12), SAOM_FIXED_DYN=",".join(["0"] * 12),
                     SAOM_CACHE_D=1)
 
+# the 500-actor form the launcher prefers: four waves x 2 words, actor count,
+# behaviour-effect count and flags compiled in, four-wave register budget
+CASES["n500_w4"] = dict(CASES["n500"], SAOM_NW=4, SAOM_FIXED_N=500, SAOM_FIXED_KB=0,
+                        SAOM_FIXED_FLAGS=1, SAOM_HAS_RATE=0, SAOM_WAVES_PER_EU=4)
+
 MULTI_CASES = {
     # two networks, cross effects, cached statistics
     "multi_lds": dict(SAOM_WMAX=8, SAOM_NW=2, SAOM_FIXED_K=8,
@@ -132,7 +137,7 @@ def test_embedded_multi_source_compiles(name):
 
 
 @pytest.mark.parametrize("which,name", [("period", "n500"), ("period", "k12"),
-                                        ("multi", "multi_lds")])
+                                        ("period", "n500_w4"), ("multi", "multi_lds")])
 def test_specialised_kernels_keep_state_in_registers(which, name):
     """The specialised kernels walk a fixed effect list with a compile-time
     index (not a '#pragma unroll' loop, which the compiler refuses to unroll
