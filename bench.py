@@ -36,16 +36,35 @@ METRIC = ("SAOM chain-periods/sec at n=500 (GPU vs host CPU); MoM wall time to "
 VALU_PEAK_OPS = 256 * 4 * 16 * 2.4e9
 
 
-def algorithmic_ops_per_alternative(n: int, k: int) -> float:
-    """VALU operations one alternative evaluation needs, counted from the
+def algorithmic_ops_per_alternative(n: int, k: int) -> dict:
+    """Operations one alternative evaluation (i, j) needs, counted from the
     algorithm (not from the compiled code):
-      Philox4x32-10: 10 rounds x (2 mul-hi + 2 mul-lo + 4 xor + 2 key add) = 100
-      two u64 AND + popcount features over W64 words: 8 lane-ops per word
-      linear predictor: k multiplies + k adds; change statistics ~2 per effect
-      Gumbel noise and softmax weight: 2 log + 1 exp, counted as 1 op each,
-      plus 4 flops (subtract, compare, scale)."""
+      integer: Philox4x32-10, about 100 (ten rounds of two 32x32 -> 64-bit
+               products, xors and key additions), plus 6 per 64-bit word of a
+               row (AND + popcount for the two-path, shared-out-tie and
+               3-cycle counts): 6 W64 + 100, 148 at n = 500
+      float64: 4 per effect (multiply and add of the linear predictor and of
+               the softmax-weighted change-statistic sum): 28 at k = 7
+      transcendental: 3 (the two logarithms of the Gumbel noise and the
+               exponential of the softmax weight), counted as one each
+    179 at n = 500, k = 7."""
     w64 = (n + 63) // 64
-    return 100 + 8 * w64 + 4 * k + 3 + 4
+    ints, f64, trans = 6 * w64 + 100, 4 * k, 3
+    return {"int": ints, "f64": f64, "transcendental": trans, "total": ints + f64 + trans}
+
+
+def host_threads() -> dict:
+    """Host threads for the CPU legs: every core this process may run on
+    (its affinity mask), capped by OMP_NUM_THREADS where the environment
+    sets it."""
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    threads = min(aff, int(omp)) if omp.isdigit() and int(omp) > 0 else aff
+    return {"threads": threads, "affinity_cpus": aff, "cpu_count": os.cpu_count(),
+            "omp_num_threads": omp or None}
 
 
 def measured_traffic(n: int, chains: int):
@@ -61,7 +80,9 @@ def measured_traffic(n: int, chains: int):
     return None
 
 
-def cpu_baseline(prob_factory, threads: int, chains: int) -> dict:
+def cpu_baseline(prob_factories, threads: int, chains: int, host: dict) -> dict:
+    """The reference CPU engine (compiled from the reference's source, else
+    the oracle) on `chains` chains of each period, OpenMP over chains."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
@@ -69,21 +90,26 @@ def cpu_baseline(prob_factory, threads: int, chains: int) -> dict:
     kind = "reference"
     if lib is None:
         lib, kind = O.oracle(), "port"
-    prob = prob_factory(chains)
     setter = getattr(O.oracle(), "oracle_set_threads")
     setter(threads)
     os.environ["OMP_NUM_THREADS"] = str(threads)
-    t0 = time.perf_counter()
-    if kind == "reference":
-        r = O.simulate_reference(prob)
-    else:
-        r = O.simulate(prob)
-    wall = time.perf_counter() - t0
-    return {"value": round(chains / wall, 4), "unit": "chain-periods/s", "cores": threads,
-            "kind": kind,
-            "sample": (f"{chains} chains of period 1 of the same workload (n=500, rate 8, 7 "
-                       f"effects), OpenMP static schedule over chains, {threads} threads, "
-                       f"{wall:.1f} s wall, mean ministeps {float(r['steps'].mean()):.0f}")}
+    walls, steps = [], []
+    for factory in prob_factories:
+        prob = factory(chains)
+        t0 = time.perf_counter()
+        r = O.simulate_reference(prob) if kind == "reference" else O.simulate(prob)
+        walls.append(time.perf_counter() - t0)
+        steps.append(float(r["steps"].mean()))
+    wall = sum(walls)
+    P = len(prob_factories)
+    return {"value": round(P * chains / wall, 4), "unit": "chain-periods/s", "cores": threads,
+            "kind": kind, "affinity_cpus": host["affinity_cpus"], "cpu_count": host["cpu_count"],
+            "omp_num_threads": host["omp_num_threads"],
+            "sample": (f"{chains} chains of each of the {P} periods of the same workload "
+                       f"(n=500, rate 8, 7 effects), OpenMP static schedule over chains, "
+                       f"{threads} threads, {wall:.1f} s wall "
+                       f"({' + '.join(f'{w:.1f}' for w in walls)}), mean ministeps "
+                       f"{' / '.join(f'{x:.0f}' for x in steps)}")}
 
 
 def groups_measure(lib) -> dict:
@@ -374,6 +400,9 @@ GROUPS_MOM_RUNS = {
     # the surrogate straight from phase 1 (its plain fallback engages when
     # the corrections leave a large residual)
     "surrogate_from_phase1": dict(surrogate=True, surrogate_pairs_log2=12, surrogate_warmup=-1),
+    # larger training sets: 2^14 and 2^16 (theta, statistics) pairs
+    "surrogate_p14": dict(surrogate=True, surrogate_pairs_log2=14),
+    "surrogate_p16": dict(surrogate=True, surrogate_pairs_log2=16),
 }
 
 
@@ -418,6 +447,53 @@ def groups_mom_measure(lib, runs: list) -> dict:
                                                                  <= plain.se))
                 out[key]["wall_fraction_of_plain"] = round(out[key]["wall_s"]
                                                            / out["plain"]["wall_s"], 3)
+    return out
+
+
+def v1_call_measure(lib) -> dict:
+    """Per-call wall time of the one-shot entry points at the R glue's
+    phase-2 shape (R/mom.R:238-244): one period of the 50-actor fit's
+    problem, 256 chains, conditional, host inputs to host outputs, one call
+    per Robbins-Monro step. The ctypes structs are built once; each call is
+    cusna_saom_simulate_period (or its no-allocation variant) as the glue
+    makes it. Next to it the period kernel's own time on a device-resident
+    plan of the same problem."""
+    import ctypes as C
+
+    import numpy as np
+
+    from cusna_amd.abi import SaomResult
+    from cusna_amd.effects import Effect
+    from cusna_amd.workloads import effect_problem, poisson_capped_network
+
+    rng = np.random.default_rng(8)
+    x = poisson_capped_network(50, 2.3, 6, 0.4, rng)
+    prob = effect_problem(x, [Effect("density"), Effect("recip"), Effect("transTrip")],
+                          [-2.5, 2.0, 0.4], 1.0, 256, 44, conditional=True, target_dist=40)
+    L = lib.load()
+    p = prob.build()
+    res = SaomResult(prob.n, prob.k, prob.M, prob.kb, False)
+    out = {"workload": "one period call: 50 actors, density/recip/transTrip, conditional "
+                       "(target distance 40), 256 chains, host buffers in and out"}
+    for key, fn in (("simulate_period", L.cusna_saom_simulate_period),
+                    ("simulate_period_noalloc", L.cusna_saom_simulate_period_noalloc)):
+        for _ in range(10):
+            assert fn(C.byref(p), C.byref(res.c)) == 0
+        calls = 300
+        t0 = time.perf_counter()
+        for _ in range(calls):
+            fn(C.byref(p), C.byref(res.c))
+        out[key + "_ms_per_call"] = round(1e3 * (time.perf_counter() - t0) / calls, 4)
+    plan = lib.PeriodPlan(prob)
+    try:
+        ms = []
+        for s in range(20):
+            plan.run(100 + s)
+            plan.sync()
+            ms.append(plan.kernel_ms())
+        out["kernel_ms"] = round(float(np.median(ms[5:])), 4)
+    finally:
+        plan.close()
     return out
 
 
@@ -505,6 +581,8 @@ def main() -> int:
     ap.add_argument("--no-coevo", action="store_true")
     ap.add_argument("--mlp-only", action="store_true",
                     help="only the perceptron training section (for counter runs)")
+    ap.add_argument("--v1-only", action="store_true",
+                    help="only the per-call time of the one-shot period entry points")
     ap.add_argument("--mom-only", action="store_true",
                     help="only the 50-actor MoM fit, without its CPU leg (for profiler runs)")
     ap.add_argument("--groups-mom", default="", metavar="RUNS",
@@ -530,6 +608,9 @@ def main() -> int:
         return 0
     if args.mom_only:
         print(json.dumps({"mom": mom_measure(lib, False, args.threads)}), flush=True)
+        return 0
+    if args.v1_only:
+        print(json.dumps({"v1_call": v1_call_measure(lib)}), flush=True)
         return 0
     if args.groups_mom:
         runs = args.groups_mom.split(",")
@@ -570,9 +651,9 @@ def main() -> int:
     value = chain_periods / elapsed
     # algorithmic work of the last launches: alternatives = ministeps x n
     units = [float(o["steps"].astype(np.int64).sum()) * n for o in outs]
-    ops_unit = algorithmic_ops_per_alternative(n, probs[0].k)
+    ops = algorithmic_ops_per_alternative(n, probs[0].k)
     kms = float(np.mean(kern))
-    achieved = float(np.mean(units)) * ops_unit / (kms * 1e-3)
+    achieved = float(np.mean(units)) * ops["total"] / (kms * 1e-3)
     line = {
         "metric": METRIC, "value": round(value, 2), "unit": "chain-periods/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
@@ -589,30 +670,30 @@ def main() -> int:
                      "kernel": "saom_period_rtc" if plans[0].specialized
                                else "saom_period_kernel", "kernel_ms": round(kms, 3),
                      "alternatives_per_launch": float(np.mean(units)),
-                     "ops_per_alternative": ops_unit},
+                     "ops_per_alternative": ops["total"],
+                     "ops_split": {k: v for k, v in ops.items() if k != "total"}},
     }
+    host = host_threads()
+    threads = host["threads"]
     if not args.no_cpu:
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
         chains = args.cpu_chains or 8 * threads  # at least 8 chains per core
         line["cpu_baseline"] = cpu_baseline(
-            lambda m: n500_problem(M=m, n=n, x0=w1, period_seed=11), threads, chains)
+            [lambda m, w=w, q=q: n500_problem(M=m, n=n, x0=w, period_seed=11 + q)
+             for q, w in enumerate((w1, w2))], threads, chains, host)
     for pl in plans:
         pl.close()
     if gof is not None:
         line["gof"] = gof
     if not args.no_rate:
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
         line["rate"] = rate_measure(lib, w1, not args.no_cpu, threads)
     if not args.no_coevo:
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
         line["coevo"] = coevo_measure(lib, not args.no_cpu, threads)
     if not args.no_multi:
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
         line["multi"] = multi_measure(lib, (w1, w2), not args.no_cpu, threads)
     if not args.no_groups:
         line["groups"] = groups_measure(lib)
+    line["v1_call"] = v1_call_measure(lib)
     if not args.no_mom:
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
         line["mom"] = mom_measure(lib, not args.no_cpu, threads)
     if not args.no_mlp:
         line["mlp"] = mlp_measure(lib)

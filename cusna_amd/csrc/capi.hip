@@ -270,7 +270,8 @@ cusna_status plan_create(const cusna_saom_problem* pb, bool want_dense, const do
     if (P.dci[e] + 1 > nd) nd = P.dci[e] + 1;
   }
   for (int e = 0; e < kb; ++e) P.bty[e] = pb->bty[e];
-  pl->specialize = !replay && !arena && saom::specialize_policy((long long)M * n);
+  // a fixed arena (no allocation) rules out loading a specialised module
+  pl->specialize = !replay && !(arena && !arena->grow) && saom::specialize_policy((long long)M * n);
   pl->nw = pick_nw(wmax, pb->threads);
   if (pb->threads <= 0) {
     const int tuned = cached_nw(ShapeKey{wmax, W64, k, P.flags});
@@ -318,7 +319,9 @@ cusna_status plan_create(const cusna_saom_problem* pb, bool want_dense, const do
     if (pb->bkdist) bkd[q] = pb->bkdist[q];
   }
 
-  // device buffers
+  // device buffers: the inputs first, then the outputs the host reads, then
+  // the dense end networks and the working buffers, so that a staged call
+  // moves its inputs and its outputs with one copy each
   DeviceBlock& D = pl->D;
   const size_t rowsb = (size_t)n * W64 * 8;
   const size_t o_x0 = D.add(rowsb), o_xs = D.add(rowsb), o_kd = D.add(rowsb), o_pm = D.add(rowsb);
@@ -336,33 +339,53 @@ cusna_status plan_create(const cusna_saom_problem* pb, bool want_dense, const do
   pl->o_vmiss = D.add((size_t)k * n);
   pl->o_od0 = D.add((size_t)n * 4);
   pl->o_id0 = D.add((size_t)n * 4);
-  const size_t o_end = D.add((size_t)M * rowsb);
-  const size_t o_ntim = D.add((size_t)M * 8), o_sc = D.add((size_t)M * (k + 1) * 8);
-  const size_t o_bsc = D.add((size_t)M * (kb + 1) * 8);
-  const size_t o_dist = D.add((size_t)M * 4), o_dstat = D.add((size_t)M * 4);
-  const size_t o_steps = D.add((size_t)M * 4);
-  const size_t o_bdist = D.add((size_t)M * 4), o_zend = D.add((size_t)M * n * 4);
-  pl->o_stats = D.add((size_t)M * k * 8);
   const size_t o_ru = D.add(ru.size() * 8);
-  pl->o_rsc = D.add((size_t)M * P.nrate * 8);
-  pl->o_adist = D.add(P.nrate > 0 ? (size_t)M * n * 4 : 0);
   pl->o_params = D.add(sizeof(saom::SimParams));
-  const size_t o_work = D.add(stream_rows ? (size_t)M * rowsb : 0);
-  pl->o_sscr = D.add((size_t)M * stat_words * 8);
-  pl->has_sscr = stat_words > 0;
-  if (want_dense) pl->o_dense = D.add((size_t)M * n * n);
   size_t o_lu0 = 0, o_llu = 0;
   if (replay) {
     o_lu0 = D.add((size_t)M * smax * 8);
     o_llu = D.add((size_t)M * smax * n * 8);
   }
+  pl->in_bytes = D.bytes();
+  const size_t o_ntim = D.add((size_t)M * 8), o_sc = D.add((size_t)M * (k + 1) * 8);
+  const size_t o_bsc = D.add((size_t)M * (kb + 1) * 8);
+  const size_t o_dist = D.add((size_t)M * 4), o_dstat = D.add((size_t)M * 4);
+  const size_t o_steps = D.add((size_t)M * 4);
+  const size_t o_bdist = D.add((size_t)M * 4), o_zend = D.add(kb > 0 ? (size_t)M * n * 4 : 0);
+  pl->o_stats = D.add((size_t)M * k * 8);
+  pl->o_rsc = D.add((size_t)M * P.nrate * 8);
+  pl->o_adist = D.add(P.nrate > 0 ? (size_t)M * n * 4 : 0);
+  pl->out_begin = pl->in_bytes;
+  pl->out_end = D.bytes();
+  if (want_dense) pl->o_dense = D.add((size_t)M * n * n);
+  const size_t o_end = D.add((size_t)M * rowsb);
+  const size_t o_work = D.add(stream_rows ? (size_t)M * rowsb : 0);
+  pl->o_sscr = D.add((size_t)M * stat_words * 8);
+  pl->has_sscr = stat_words > 0;
+  // the staged one-shot call also fills the per-chain parameters from the
+  // problem (the conversions of plan_set_params)
+  const bool staged = arena && arena->staging;
   if (arena) {
-    if (!D.place(arena->base, arena->capacity)) {
+    void* base = arena->grow ? arena->grow(D.bytes())
+                             : (D.bytes() <= arena->capacity ? arena->base : nullptr);
+    if (!base || !D.place(base, arena->grow ? D.bytes() : arena->capacity)) {
+      if (arena->grow) {
+        set_error("device memory of " + std::to_string(D.bytes()) + " bytes not available");
+        return CUSNA_ERR_CUDA;
+      }
       set_error("the call needs " + std::to_string(D.bytes()) + " bytes of device memory, above the " +
                 std::to_string(arena->capacity) + " bytes it may use");
       return CUSNA_ERR_SHAPE;
     }
     pl->s = stream;
+    if (staged) {
+      pl->staging = static_cast<char*>(arena->staging(std::max(pl->in_bytes,
+                                                               pl->out_end - pl->out_begin)));
+      if (!pl->staging) {
+        set_error("pinned host memory not available");
+        return CUSNA_ERR_CUDA;
+      }
+    }
   } else {
     HIP_TRY(D.alloc());
     if (stream) {
@@ -375,8 +398,14 @@ cusna_status plan_create(const cusna_saom_problem* pb, bool want_dense, const do
     HIP_TRY(hipEventCreate(&pl->ev1));
   }
   hipStream_t s = pl->s;
+  // an upload: into the pinned staging buffer (copied with the others at the
+  // end) or one asynchronous copy of its own
   auto up = [&](size_t off, const void* src, size_t bytes) -> hipError_t {
     if (!bytes) return hipSuccess;
+    if (staged) {
+      std::memcpy(pl->staging + off, src, bytes);
+      return hipSuccess;
+    }
     return hipMemcpyAsync(D.at<char>(off), src, bytes, hipMemcpyHostToDevice, s);
   };
   HIP_TRY(up(o_x0, x0.data(), rowsb));
@@ -403,6 +432,20 @@ cusna_status plan_create(const cusna_saom_problem* pb, bool want_dense, const do
   if (replay) {
     HIP_TRY(up(o_lu0, log_u0, (size_t)M * smax * 8));
     HIP_TRY(up(o_llu, loglog_u, (size_t)M * smax * n * 8));
+  }
+  if (staged) {
+    double* th = reinterpret_cast<double*>(pl->staging + pl->o_th);
+    double* lm = reinterpret_cast<double*>(pl->staging + pl->o_lam);
+    double* thb = reinterpret_cast<double*>(pl->staging + pl->o_thb);
+    double* lmb = reinterpret_cast<double*>(pl->staging + pl->o_lamb);
+    for (size_t q = 0; q < (size_t)M * k; ++q) th[q] = pb->theta[q];
+    for (int c = 0; c < M; ++c) {
+      lm[c] = pb->lam ? (double)pb->lam[c] : 1.0;
+      lmb[c] = kb > 0 ? (double)pb->lamb[c] : 0.0;
+    }
+    for (size_t q = 0; q < (size_t)M * (kb > 0 ? kb : 1); ++q)
+      thb[q] = kb > 0 ? (double)pb->thetab[q] : 0.0;
+    pl->params_set = true;
   }
   P.x0 = D.at<uint64_t>(o_x0);
   P.xs = D.at<uint64_t>(o_xs);
@@ -439,8 +482,13 @@ cusna_status plan_create(const cusna_saom_problem* pb, bool want_dense, const do
   P.ru = ru.empty() ? nullptr : D.at<double>(o_ru);
   P.rscores = P.nrate > 0 ? D.at<double>(pl->o_rsc) : nullptr;
   P.work = stream_rows ? D.at<uint64_t>(o_work) : nullptr;
-  HIP_TRY(hipMemcpyAsync(D.at<char>(pl->o_params), &P, sizeof(P), hipMemcpyHostToDevice, s));
-  HIP_TRY(hipStreamSynchronize(s));
+  HIP_TRY(up(pl->o_params, &P, sizeof(P)));
+  if (staged) {
+    // every input in one copy (the staging buffer outlives the call)
+    HIP_TRY(hipMemcpyAsync(D.at<char>(0), pl->staging, pl->in_bytes, hipMemcpyHostToDevice, s));
+  } else {
+    HIP_TRY(hipStreamSynchronize(s));  // pageable sources: the copies have consumed them
+  }
   *out = pl.release();
   return CUSNA_OK;
 }
@@ -481,7 +529,8 @@ cusna_status plan_run(cusna_period_plan* pl, uint64_t seed, bool want_stats, int
   if (timed) HIP_TRY(hipEventRecord(pl->ev0, s));
   int le = saom::launch_period(P, pl->D.at<saom::SimParams>(pl->o_params),
                                saom::LaunchShape{pl->nw, pl->wmax}, pl->replay, P.key0, P.key1,
-                               s, pl->specialize, &pl->last_rtc, pl->free_shape);
+                               s, pl->specialize, &pl->last_rtc, pl->free_shape,
+                               &pl->last_threads);
   if (le != 0) {
     set_error(std::string("period kernel launch: ") + hipGetErrorString((hipError_t)le));
     return CUSNA_ERR_CUDA;
@@ -545,11 +594,46 @@ cusna_status plan_fetch(cusna_period_plan* pl, cusna_saom_result* r) {
     if (saom::launch_expand_dense(P.end_rows, n, pl->W64, M, pl->D.at<uint8_t>(pl->o_dense), s))
       return CUSNA_ERR_CUDA;
   }
-  std::vector<double> sc((size_t)M * (k + 1)), bsc((size_t)M * (kb + 1));
   auto down = [&](void* dst, const void* src, size_t bytes) -> hipError_t {
     if (!dst || !bytes) return hipSuccess;
     return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s);
   };
+  if (pl->staging) {
+    // a staged call: the outputs the host reads in one copy, then unpacked
+    const char* dev0 = pl->D.at<char>(pl->out_begin);
+    HIP_TRY(hipMemcpyAsync(pl->staging, dev0, pl->out_end - pl->out_begin,
+                           hipMemcpyDeviceToHost, s));
+    if (dense) HIP_TRY(down(r->a_ends, pl->D.at<char>(pl->o_dense), (size_t)M * n * n));
+    HIP_TRY(hipStreamSynchronize(s));
+    auto host = [&](const void* dev) -> const char* {
+      return pl->staging + (static_cast<const char*>(dev) - dev0);
+    };
+    auto take = [&](void* dst, const void* dev, size_t bytes) {
+      if (dst && bytes) std::memcpy(dst, host(dev), bytes);
+    };
+    auto take_f32 = [&](float* dst, const void* dev, size_t count) {
+      if (!dst) return;
+      const double* v = reinterpret_cast<const double*>(host(dev));
+      for (size_t q = 0; q < count; ++q) dst[q] = (float)v[q];
+    };
+    if (pl->stats_valid) take(r->stats, pl->D.at<char>(pl->o_stats), (size_t)M * k * 8);
+    take(r->ntim, P.ntim, (size_t)M * 8);
+    take_f32(r->scores, P.scores, (size_t)M * (k + 1));
+    take(r->dist, P.dist, (size_t)M * 4);
+    take(r->dist_stat, P.dist_stat, (size_t)M * 4);
+    take(r->steps, P.steps, (size_t)M * 4);
+    if (kb > 0) {
+      take(r->bdist, P.bdist, (size_t)M * 4);
+      take_f32(r->bscores, P.bscores, (size_t)M * (kb + 1));
+      take(r->z_end, P.z_end, (size_t)M * n * 4);
+    }
+    if (P.nrate > 0) {
+      take_f32(r->rscores, P.rscores, (size_t)M * P.nrate);
+      take(r->actor_dist, P.actor_dist, (size_t)M * n * 4);
+    }
+    return CUSNA_OK;
+  }
+  std::vector<double> sc((size_t)M * (k + 1)), bsc((size_t)M * (kb + 1));
   if (r->stats && pl->stats_valid)
     HIP_TRY(down(r->stats, pl->D.at<char>(pl->o_stats), (size_t)M * k * 8));
   HIP_TRY(down(r->ntim, P.ntim, (size_t)M * 8));
@@ -578,17 +662,85 @@ cusna_status plan_fetch(cusna_period_plan* pl, cusna_saom_result* r) {
   return CUSNA_OK;
 }
 
+// Device and pinned host memory of the one-shot period call, cached per host
+// thread (grown on demand, kept for the thread's later calls), so that a call
+// makes no allocation once the thread has made one of its size.
+struct CallBuffers {
+  int dev = -1;
+  void* d = nullptr;
+  size_t dcap = 0;
+  void* h = nullptr;
+  size_t hcap = 0;
+  ~CallBuffers() {
+    if (h) (void)hipHostFree(h);
+    if (d) (void)hipFree(d);
+  }
+};
+thread_local CallBuffers g_call;
+
+CallBuffers& call_buffers() {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  if (g_call.dev != dev) {  // another device: its buffers are of no use here
+    if (g_call.d) (void)hipFree(g_call.d);
+    g_call.d = nullptr;
+    g_call.dcap = 0;
+    g_call.dev = dev;
+  }
+  return g_call;
+}
+
+void* call_device(size_t bytes) {
+  CallBuffers& b = call_buffers();
+  if (bytes > b.dcap) {
+    if (b.d) (void)hipFree(b.d);
+    b.d = nullptr;
+    b.dcap = 0;
+    const size_t want = std::max(bytes, b.dcap + b.dcap / 2);
+    if (hipMalloc(&b.d, want) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    b.dcap = want;
+  }
+  return b.d;
+}
+
+void* call_host(size_t bytes) {
+  CallBuffers& b = g_call;
+  if (bytes > b.hcap) {
+    if (b.h) (void)hipHostFree(b.h);
+    b.h = nullptr;
+    b.hcap = 0;
+    const size_t want = std::max(bytes, b.hcap + b.hcap / 2);
+    if (hipHostMalloc(&b.h, want, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    b.hcap = want;
+  }
+  return b.h;
+}
+
 cusna_status simulate(const cusna_saom_problem* pb, cusna_saom_result* r, const double* log_u0,
                       const double* loglog_u, int32_t smax) {
   cusna_status st = validate(pb, r);
   if (st != CUSNA_OK) return st;
   cusna_period_plan* raw = nullptr;
+  // one-shot calls run on the thread's cached buffers, inputs and outputs
+  // staged through pinned memory (the replay variant's recorded noise is
+  // large and passes through its own copies)
+  DeviceArena cached;
+  cached.grow = call_device;
+  cached.staging = call_host;
   st = plan_create(pb, r->want_end_nets && r->a_ends, log_u0, loglog_u, smax, thread_stream(),
-                   &raw);
+                   &raw, log_u0 ? nullptr : &cached);
   if (st != CUSNA_OK) return st;
   std::unique_ptr<cusna_period_plan> pl(raw);
-  st = plan_set_params(pl.get(), pb->theta, pb->lam, pb->thetab, pb->lamb);
-  if (st != CUSNA_OK) return st;
+  if (!pl->params_set) {
+    st = plan_set_params(pl.get(), pb->theta, pb->lam, pb->thetab, pb->lamb);
+    if (st != CUSNA_OK) return st;
+  }
   st = plan_run(pl.get(), pb->seed, r->stats != nullptr, -1, nullptr, true);
   if (st != CUSNA_OK) return st;
   return plan_fetch(pl.get(), r);
@@ -855,6 +1007,7 @@ cusna_status autotune(cusna_period_plan* pl, int probe, int* best_threads) {
     }
   }
   pl->nw = best;
+  pl->last_threads = 0;  // the shape query reports the choice until the next run
   {
     std::lock_guard<std::mutex> lk(g_shape_mu);
     g_shape_cache[ShapeKey{pl->wmax, pl->W64, pl->k, pl->P.flags}] = best;
@@ -1271,7 +1424,8 @@ CUSNA_API double cusna_period_plan_kernel_ms(cusna_period_plan* plan) {
 }
 
 CUSNA_API int32_t cusna_period_plan_shape(const cusna_period_plan* plan) {
-  return plan ? 64 * plan->nw : 0;
+  if (!plan) return 0;
+  return plan->last_threads > 0 ? plan->last_threads : 64 * plan->nw;
 }
 
 CUSNA_API void cusna_period_plan_destroy(cusna_period_plan* plan) { delete plan; }

@@ -49,10 +49,16 @@ class DeviceBlock {
   bool owned_ = true;
 };
 
-// Device memory a call places its buffers in instead of allocating them.
+// Device memory a call places its buffers in instead of allocating them:
+// fixed memory of `capacity` bytes at `base`, or `grow(bytes)` returning at
+// least `bytes` (cached per thread, null when unavailable). `staging(bytes)`
+// (optional) returns pinned host memory of at least `bytes` through which
+// the call moves its inputs and its outputs with one copy each.
 struct DeviceArena {
-  void* base;
-  size_t capacity;
+  void* base = nullptr;
+  size_t capacity = 0;
+  void* (*grow)(size_t bytes) = nullptr;
+  void* (*staging)(size_t bytes) = nullptr;
 };
 
 void set_error(const std::string& s);
@@ -100,6 +106,12 @@ struct cusna_period_plan {
   bool specialize = false;  // launch the runtime-specialised period kernel
   bool free_shape = false;  // block shape left to the library (no threads, none tuned)
   bool last_rtc = false;    // the last run did
+  int last_threads = 0;     // threads per chain of the last run's launch (0: none yet)
+  // device layout: inputs in [0, in_bytes), the outputs the host reads in
+  // [out_begin, out_end); staging: pinned host memory of a staged call
+  size_t in_bytes = 0, out_begin = 0, out_end = 0;
+  char* staging = nullptr;
+  bool params_set = false;  // the per-chain parameters came with the inputs
   bool stats_valid = false;
   ~cusna_period_plan() {
     if (ev0) (void)hipEventDestroy(ev0);

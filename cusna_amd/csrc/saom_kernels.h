@@ -73,9 +73,10 @@ bool simple_effects(const int* type_ids, int k);
 
 // Launch the period kernel. P is the host copy (shape, LDS size); Pd the same
 // struct in device memory, read by the kernel. Returns 0 or a hipError_t.
+// *used_threads (optional): threads per chain of the launch that ran.
 int launch_period(const SimParams& P, const SimParams* Pd, LaunchShape s, bool replay,
                   uint32_t key0, uint32_t key1, void* stream, bool specialize = false,
-                  bool* used = nullptr, bool free_shape = false);
+                  bool* used = nullptr, bool free_shape = false, int* used_threads = nullptr);
 
 // General form. One group: Pd, grid = chains, key0/key1 the Philox key.
 // Several groups: Pd[g] are group g's device parameters (its Philox key in
@@ -95,6 +96,7 @@ struct PeriodLaunch {
   const SimParams* Ph = nullptr;
   bool specialize = false;
   bool* used = nullptr;
+  int* used_threads = nullptr;  // threads per chain of the launch that ran
   // shape and LDS bytes of the specialised kernel (exact_shape); nw = 0: the
   // generic shape. exact_alt: the shape to use instead when the preferred
   // one's register budget (exact.wpe) would make the compiler spill.

@@ -94,11 +94,13 @@ int launch_period_ex(const PeriodLaunch& L, LaunchShape s, bool replay, bool sim
       const int e = launch_period_rtc(fn, Lr, rs.nw, stream);
       if (e == 0) {
         if (L.used) *L.used = true;
+        if (L.used_threads) *L.used_threads = 64 * rs.nw;
         return 0;
       }
       (void)hipGetLastError();
     }
   }
+  if (L.used_threads) *L.used_threads = 64 * s.nw;
   switch (s.wmax) {
     case 1: return launch_period_w1(L, s.nw, replay, simple, stream);
     case 2: return launch_period_w2(L, s.nw, replay, simple, stream);
@@ -142,7 +144,7 @@ int launch_multi(const MultiParams& P, const MultiParams* Pd, int nw, int wmax, 
 
 int launch_period(const SimParams& P, const SimParams* Pd, LaunchShape s, bool replay,
                   uint32_t k0, uint32_t k1, void* stream, bool specialize, bool* used,
-                  bool free_shape) {
+                  bool free_shape, int* used_threads) {
   PeriodLaunch L;
   L.Pd = Pd;
   L.items = nullptr;
@@ -153,6 +155,7 @@ int launch_period(const SimParams& P, const SimParams* Pd, LaunchShape s, bool r
   L.Ph = &P;
   L.specialize = specialize;
   L.used = used;
+  L.used_threads = used_threads;
   L.exact = exact_shape(P.W64, s, free_shape ? P.k : 0, L.lds);
   L.lds_exact = sim_lds_bytes(P, L.exact.nw, L.exact.wmax);
   L.exact_alt = exact_shape(P.W64, s, free_shape ? P.k : 0);

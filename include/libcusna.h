@@ -118,10 +118,13 @@ typedef struct cusna_saom_problem {
   int32_t zmin, zmax;
   float zbar, brange, bsimmean;
 
-  int32_t nrate;        /* non-basic rate effects (not supported: must be 0)  */
-  const int32_t *rkind;
-  const float *ralpha;
-  const float *ru;
+  int32_t nrate;        /* non-basic rate effects, 0..CUSNA_MAXRATE: actor a's */
+                        /* rate is lam * exp(sum_q ralpha[q] u_q(a)), clamped */
+                        /* to exp(+-30); the result's rscores and actor_dist */
+                        /* are filled when nrate > 0                          */
+  const int32_t *rkind; /* [nrate] 0 covariate ru row, 1 outdegree, 2 indegree */
+  const float *ralpha;  /* [nrate] rate-effect parameters                     */
+  const float *ru;      /* [nrate*n] covariate rows (rows of kind 1, 2 unread) */
 
   /* float64 descriptors; when given they replace the float32 ones above. */
   const double *vmat64;
@@ -413,6 +416,9 @@ CUSNA_API cusna_status cusna_gumbel_bound_check(uint32_t x0, uint32_t stride, ui
  *   run           launch one period with the given Philox seed (asynchronous)
  *   sync / fetch  wait; copy outputs into a cusna_saom_result
  *   kernel_ms     duration of the last period kernel (HIP events)
+ *   shape         threads per chain of the last run's launch (the specialised
+ *                 kernel may run its own block shape); before the first run,
+ *                 and after autotune, the plan's block shape
  */
 typedef struct cusna_period_plan cusna_period_plan;
 CUSNA_API cusna_status cusna_period_plan_create(const cusna_saom_problem *problem,
