@@ -394,8 +394,11 @@ def mlp_measure(lib) -> dict:
 
 GROUPS_MOM_RUNS = {
     "plain": {},
-    # the surrogate (2^12 pairs) with the automatic warm start: one plain
-    # sub-phase first when phase 1 lies far from the targets
+    # the surrogate with the automatic warm start (one plain sub-phase first
+    # when phase 1 lies far from the targets), trained on 2^12 (theta,
+    # statistics) pairs: 2^14 and 2^16 pairs gave the same estimates to four
+    # digits at 1.4 and 3.2 x the wall time, and simulating 2^18 pairs alone
+    # (about 1.8 ms per pair at 16 x 1000 actors) would outlast the plain fit
     "surrogate": dict(surrogate=True, surrogate_pairs_log2=12),
     # the surrogate straight from phase 1 (its plain fallback engages when
     # the corrections leave a large residual)
@@ -579,6 +582,9 @@ def main() -> int:
     ap.add_argument("--no-rate", action="store_true")
     ap.add_argument("--no-mlp", action="store_true")
     ap.add_argument("--no-coevo", action="store_true")
+    ap.add_argument("--no-groups-mom", action="store_true",
+                    help="skip the 16 x 1000-actor plain vs surrogate MoM comparison "
+                         "(about five minutes)")
     ap.add_argument("--mlp-only", action="store_true",
                     help="only the perceptron training section (for counter runs)")
     ap.add_argument("--v1-only", action="store_true",
@@ -697,6 +703,8 @@ def main() -> int:
         line["mom"] = mom_measure(lib, not args.no_cpu, threads)
     if not args.no_mlp:
         line["mlp"] = mlp_measure(lib)
+    if not args.no_groups_mom:
+        line["groups_mom"] = groups_mom_measure(lib, ["plain", "surrogate"])
     print(json.dumps(line), flush=True)
     return 0
 

@@ -880,7 +880,10 @@ cusna_status simulate_groups(const cusna_saom_problem* const* pbs, int G,
     const double lam = pb->lam ? (double)pb->lam[0] : 1.0;
     const double nact = pb->act ? pb->nact : pb->n;
     const double steps = pb->conditional ? 2.0 * pb->target_dist + 1.0 : lam * nact + 1.0;
-    return steps * pb->n * (plans[g]->W64 + pb->k + 8.0);
+    // a chain whose rows live in device memory runs about 1.8 x an
+    // LDS-resident chain of the same size (measured on the 256-group batch)
+    const double streamed = plans[g]->wmax > 16 ? 1.8 : 1.0;
+    return streamed * steps * pb->n * (plans[g]->W64 + pb->k + 8.0);
   };
   size_t n_items = 0;
   for (int g = 0; g < G; ++g) n_items += (size_t)plans[g]->M;
@@ -933,25 +936,33 @@ cusna_status simulate_groups(const cusna_saom_problem* const* pbs, int G,
   }
   HIP_TRY(hipMemcpyAsync(D.at<uint32_t>(o_items), items.data(), items.size() * 4,
                          hipMemcpyHostToDevice, s));
-  // The buckets run concurrently on up to four streams (largest first,
-  // round-robin), so one bucket's tail and a CU's spare LDS and wave slots
-  // take work from another; chains are independent, results do not depend
-  // on the order.
+  // The buckets run concurrently on up to four streams (largest first, each
+  // onto the least-loaded stream), so one bucket's tail and a CU's spare LDS
+  // and wave slots take work from another; chains are independent, results
+  // do not depend on the order.
   GroupStreams gs;
   const int nst = (int)std::min<size_t>(kGroupStreams, buckets.size());
   HIP_TRY(gs.create(nst));
   HIP_TRY(hipEventRecord(gs.ready, s));
+  // longest bucket first onto the stream with the least work queued
+  // (estimated as the chains' expected ministeps x alternatives x words),
+  // so that no stream is left with a long tail of its own
+  std::vector<double> work_b(buckets.size(), 0.0);
+  for (size_t bi = 0; bi < buckets.size(); ++bi)
+    for (int g : buckets[bi].groups) work_b[bi] += cost(g) * plans[g]->M;
   std::vector<size_t> order(buckets.size());
   for (size_t bi = 0; bi < order.size(); ++bi) order[bi] = bi;
-  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t c) {
-    return (double)launches[a].grid * buckets[a].W64 > (double)launches[c].grid * buckets[c].W64;
-  });
+  std::stable_sort(order.begin(), order.end(),
+                   [&](size_t a, size_t c) { return work_b[a] > work_b[c]; });
+  std::vector<double> queued(nst, 0.0);
   for (int q = 0; q < nst; ++q) HIP_TRY(hipStreamWaitEvent(gs.st[q], gs.ready, 0));
   for (size_t o = 0; o < order.size(); ++o) {
     const size_t bi = order[o];
     const Bucket& b = buckets[bi];
+    const int q = (int)(std::min_element(queued.begin(), queued.end()) - queued.begin());
+    queued[q] += work_b[bi];
     const int le = saom::launch_period_ex(launches[bi], saom::LaunchShape{b.nw, b.wmax}, false,
-                                          b.simple, gs.st[o % nst]);
+                                          b.simple, gs.st[q]);
     if (le != 0) {
       set_error(std::string("period kernel launch: ") + hipGetErrorString((hipError_t)le));
       return CUSNA_ERR_CUDA;
