@@ -587,6 +587,8 @@ def main() -> int:
                          "(about five minutes)")
     ap.add_argument("--mlp-only", action="store_true",
                     help="only the perceptron training section (for counter runs)")
+    ap.add_argument("--no-v1", action="store_true",
+                    help="skip the per-call time of the one-shot entry points")
     ap.add_argument("--v1-only", action="store_true",
                     help="only the per-call time of the one-shot period entry points")
     ap.add_argument("--mom-only", action="store_true",
@@ -698,7 +700,8 @@ def main() -> int:
         line["multi"] = multi_measure(lib, (w1, w2), not args.no_cpu, threads)
     if not args.no_groups:
         line["groups"] = groups_measure(lib)
-    line["v1_call"] = v1_call_measure(lib)
+    if not args.no_v1:
+        line["v1_call"] = v1_call_measure(lib)
     if not args.no_mom:
         line["mom"] = mom_measure(lib, not args.no_cpu, threads)
     if not args.no_mlp:
