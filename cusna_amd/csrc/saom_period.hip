@@ -657,7 +657,7 @@ __device__ __forceinline__ double beh_d(int t, double dz, double zic, long long 
 
 struct LdsLayout {
   size_t rows, indeg, outdeg, z, tpall, soall, coli, rowi, tpm0, tpm1, red, tsl, dstar, sc, bsc,
-      cnt, crate, ract, ru, rate, total;
+      cnt, crate, ract, ru, rate, draw, dbatch, total;
 };
 
 // Word bounds above this stream the network from device memory.
@@ -700,6 +700,13 @@ __host__ __device__ inline LdsLayout lds_layout(int n, int k, int kb, int wmax, 
   L.ract = o; if (flags & F_RATE) o += (size_t)n * 4;
   o = al8(o);
   L.rate = o; if (flags & F_RATE) o += (size_t)(2 * kMaxRate + 3) * 8;
+  // the ring of ministep draws (period_body): a double and three words per
+  // entry, 64 entries, fewer where the full ring would cost a chain per CU
+  L.draw = o;
+  const size_t lds_cu = 160 * 1024, per_cu = lds_cu / (o + 20);
+  L.dbatch = 64;
+  while (L.dbatch > 1 && per_cu > 0 && (o + 20 * L.dbatch) * per_cu > lds_cu) L.dbatch >>= 1;
+  o += 20 * L.dbatch;
   L.total = o;
   return L;
 }
@@ -823,8 +830,10 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
   }
   __syncthreads();
 
-  const double lam = P.lam[(size_t)chain * P.lam_stride];
-  const double lamb = kb > 0 ? P.lamb[(size_t)chain * P.lam_stride] : 0.0;
+  // uniform over the workgroup: scalar registers (the compiler cannot prove
+  // a load from a kernel that stores to global memory uniform by itself)
+  const double lam = uniform_f64(P.lam[(size_t)chain * P.lam_stride]);
+  const double lamb = kb > 0 ? uniform_f64(P.lamb[(size_t)chain * P.lam_stride]) : 0.0;
   const double* th = P.theta + (size_t)chain * P.th_stride;
   const double* thb = kb > 0 ? P.thetab + (size_t)chain * P.thb_stride : nullptr;
   const int nact = P.nact;
@@ -832,34 +841,45 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
   long long cap = kMaxSteps;
   if (REPLAY && P.smax < cap) cap = P.smax;
   double t = 0.0;
-  // The draw of ministep s (Philox block (s, chain, 0, 0), log u0 and, with
-  // the basic rate, the waiting time) depends on s alone, not on the network:
-  // each ministep computes its successor's while its own LDS traffic is in
-  // flight, which keeps the logarithm off the ministep's serial path.
-  struct Draw {
-    uint32_t c1, c2, c3;
-    double lu0, tau;
-  };
+  // The draw of ministep s (Philox block (s, chain, 0, 0) and log u0, or,
+  // with the basic rate, the waiting time -log u0 / (nact (lam + lamb)))
+  // depends on s alone, not on the network. One wave computes a batch of
+  // them at once, one per lane, into an LDS ring that every wave reads
+  // where each value is used; the wave that fills a batch rotates with the
+  // batch number, so the SIMDs share the work. A fill starts after the
+  // previous ministep's closing barrier (every read of the old batch is
+  // done) and ends with a barrier of its own.
+  const int DB = (int)L.dbatch;  // a power of two
+  double* dv = (double*)(smem + L.draw);
+  uint32_t* dc1 = (uint32_t*)(dv + DB);
+  uint32_t* dc2 = dc1 + DB;
+  uint32_t* dc3 = dc2 + DB;
   const double rate_div = (double)nact * (lam + lamb);
-  auto draw = [&](long long s) -> Draw {
-    uint32_t c0 = (uint32_t)s, c1 = (uint32_t)chain, c2 = 0u, c3 = 0u;
-    philox10(c0, c1, c2, c3, key0, key1);
-    Draw d;
-    d.c1 = c1;
-    d.c2 = c2;
-    d.c3 = c3;
-    if (REPLAY) d.lu0 = s < cap ? P.lu0[(size_t)chain * P.smax + s] : 0.0;
-    else d.lu0 = log(unit_open(c0));
-    d.tau = -d.lu0 / rate_div;
-    return d;
+  auto fill = [&](long long s0) {
+    const int b = (int)(s0 / DB);
+    if (NW == 1 || __builtin_amdgcn_readfirstlane(wv) == b % NW) {  // uniform branch
+      const int x = lane;  // DB <= 64
+      if (x < DB) {
+        const long long s = s0 + x;
+        uint32_t c0 = (uint32_t)s, c1 = (uint32_t)chain, c2 = 0u, c3 = 0u;
+        philox10(c0, c1, c2, c3, key0, key1);
+        double lu0;
+        if (REPLAY) lu0 = s < cap ? P.lu0[(size_t)chain * P.smax + s] : 0.0;
+        else lu0 = log(unit_open(c0));
+        dv[x] = (flags & F_RATE) ? lu0 : -lu0 / rate_div;
+        dc1[x] = c1;
+        dc2[x] = c2;
+        dc3[x] = c3;
+      }
+    }
+    __syncthreads();
   };
-  Draw next = draw(0);
 
   for (;;) {
     // 1. ministep draw (:167, 194-200)
-    const Draw cur = next;
-    const uint32_t c1 = cur.c1, c2 = cur.c2, c3 = cur.c3;
-    const double lu0 = cur.lu0;
+    if ((steps & (DB - 1)) == 0) fill(steps);
+    const int di = (int)(steps & (DB - 1));
+    const uint32_t c1 = dc1[di], c2 = dc2[di];
     int i;
     bool net;
     if (flags & F_RATE) {
@@ -932,7 +952,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
       }
       __syncthreads();
       const double S0 = rsum_s[0];
-      const double tau = -lu0 / (lam * S0);
+      const double tau = -dv[di] / (lam * S0);
       double tau_eff = tau;
       if (!P.cond && t + tau > 1.0) tau_eff = 1.0 - t;
       if (tid == 0) {
@@ -957,7 +977,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
       i = ract[below < nact ? (int)below : nact - 1];
       net = true;
     } else {
-      t = uniform_f64(t + cur.tau);  // the chain time lives in scalar registers
+      t = uniform_f64(t + dv[di]);  // the chain time lives in scalar registers
       if (!P.cond && t > 1.0) break;
       net = kb == 0 || unit_open(c1) < lam / (lam + lamb);
       long long ix = (long long)(unit_open(c2) * nact);
@@ -969,7 +989,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
     if (!net) {
       // ---- behaviour ministep (:202-227)
       const int zi = z[i];
-      next = draw(steps + 1);
+      const uint32_t c3 = dc3[di];
       const double zic = (double)zi - P.zbar;
       long long nb = 0, nH = 0, nE = 0, nL = 0;
       double zs = 0.0;  // exact in any order: terms are multiples of ulp(float zbar)
@@ -1096,7 +1116,6 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
     }
     U.outdeg_i = outdeg[i];
     U.indeg_i = indeg[i];
-    next = draw(steps + 1);  // independent of the loads above: overlaps their latency
     U.zvi = (flags & F_Z) ? (double)z[i] - P.zbar : -P.zbar;
     U.rdeg_i = 0;
     if (flags & F_RDEG) {
