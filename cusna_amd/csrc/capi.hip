@@ -278,6 +278,14 @@ cusna_status plan_create(const cusna_saom_problem* pb, bool want_dense, const do
     if (tuned > 0) pl->nw = tuned;
     pl->free_shape = tuned <= 0;
   }
+  if (stream_rows) {
+    // partial residency: the leading words of every row in LDS, as many as
+    // fit beside the rest of the chain state (budgeted for the largest
+    // block, 16 waves, so that any launch shape fits)
+    const size_t base = saom::sim_lds_bytes(P, 16, wmax);
+    if (base < kLdsLimit)
+      P.lw = (int)std::min<size_t>((size_t)W64, (kLdsLimit - base) / ((size_t)n * 8));
+  }
   if (saom::sim_lds_bytes(P, pl->nw, wmax) > kLdsLimit) {
     set_error("chain state exceeds the 160 KB LDS of a CU");
     return CUSNA_ERR_UNSUPPORTED;

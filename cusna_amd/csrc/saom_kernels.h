@@ -27,6 +27,7 @@ struct SimParams {
   int nact, has_comp, symflag, maxdeg, cond, target;
   int zmin, zmax;
   int smax;  // replay: recorded ministeps per chain (0 = free running)
+  int lw;    // streamed rows: leading words of every row kept in LDS (0 .. W64)
   int flags;
   uint32_t key0, key1;
   double zbar, brange, bsim;
@@ -111,8 +112,10 @@ struct PeriodLaunch {
 // words: the generic shape rounds the word bound up to a power of two; the
 // specialised kernel is compiled for any bound, so it pads only to a multiple
 // of the waves per chain (R words per lane): R = ceil(W64 / nw) with the
-// generic waves up to 8 words, two words per lane from 9 words (more waves,
-// less LDS padding), and at most 16 waves for streamed networks. k_free > 0
+// generic waves up to 8 words, three words per lane on four waves at 9..12
+// words (one wave per SIMD per chain), two words per lane from 13 words
+// (more waves, less LDS padding), and at most 16 waves for streamed
+// networks. k_free > 0
 // (the caller left the block shape to the library; k_free effects) doubles
 // the waves per chain up to 8 words when that brings k x R within the 32
 // change statistics per lane that the specialised kernel keeps in registers

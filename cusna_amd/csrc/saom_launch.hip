@@ -49,6 +49,12 @@ LaunchShape exact_shape(int W64, LaunchShape g, int k_free, size_t lds_generic) 
       nw *= 2;
       R = R2;
     }
+  } else if (W64 <= 12) {
+    // three words per wave on four waves, one wave per SIMD per chain:
+    // five or six waves of two words left one SIMD with two waves of the
+    // chain, 1.7-1.8 x slower per chain on the 256-group batch
+    R = 3;
+    nw = 4;
   } else if (W64 <= 32) {
     R = 2;
     nw = (W64 + 1) / 2;

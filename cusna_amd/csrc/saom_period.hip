@@ -347,12 +347,24 @@ struct Ego {
   }
 };
 
-// LDS views of the chain state.
+// LDS views of the chain state. Word w of row x: rows[w * RS + x] (LDS, or
+// device memory for a streamed network without partial residency); a
+// streamed network with partial residency keeps words w < lw in LDS (rows)
+// and the rest in device memory (rows_d, same layout, every word).
 struct ChainLds {
   const uint64_t* rows;
   int RS, W64, NW, wv, lane;
   const int *indeg, *outdeg, *z, *tpall, *soall;
   const uint64_t *tpm0, *tpm1;
+  const uint64_t* rows_d = nullptr;
+  int lw = 1 << 30;
+  template <bool STREAM>
+  __device__ __forceinline__ uint64_t row(int w, int x) const {
+    if constexpr (STREAM) {
+      if (w >= lw) return rows_d[(size_t)w * RS + x];  // w is uniform: no divergence
+    }
+    return rows[(size_t)w * RS + x];
+  }
 };
 
 // Change statistics of one non-interaction effect e for the lane's R
@@ -362,8 +374,7 @@ struct ChainLds {
 template <int R, int WMAX, bool STREAM, bool SIMPLE, class EgoT>
 __device__ __forceinline__ void eff_base(const SimParams& P, int e, const Alt<R>& A,
                                          const EgoT& U, const ChainLds& S, double (&d)[R]) {
-  const uint64_t* rows = S.rows;
-  const int RS = S.RS;
+  auto ROW = [&](int w, int x) -> uint64_t { return S.template row<STREAM>(w, x); };
   const int* indeg = S.indeg;
   const int* outdeg = S.outdeg;
   const int* z = S.z;
@@ -493,7 +504,7 @@ __device__ __forceinline__ void eff_base(const SimParams& P, int e, const Alt<R>
           int c = 0;
 #pragma unroll
           for (int wd = 0; wd < U.words(); ++wd)
-            c += popc64(U.rowi(wd) & rows[(size_t)(wd) * RS + A.j[r]] & tm[wd]);
+            c += popc64(U.rowi(wd) & ROW(wd, A.j[r]) & tm[wd]);
           d[r] = A.dl(r) * (double)(c + (A.tp[r] > 0 ? 1 : 0));
         }
         break;
@@ -503,7 +514,7 @@ __device__ __forceinline__ void eff_base(const SimParams& P, int e, const Alt<R>
           int c = 0;
 #pragma unroll
           for (int wd = 0; wd < U.words(); ++wd)
-            c += popc64(U.rowi(wd) & U.coli(wd) & rows[(size_t)(wd) * RS + A.j[r]]);
+            c += popc64(U.rowi(wd) & U.coli(wd) & ROW(wd, A.j[r]));
           d[r] = A.dl(r) * (double)((A.bji(r) ? A.tp[r] : 0) + c);
         }
         break;
@@ -538,7 +549,7 @@ __device__ __forceinline__ void eff_base(const SimParams& P, int e, const Alt<R>
 #pragma unroll
             for (int r = 0; r < R; ++r) {
               const int word = S.NW * r + S.wv;
-              const uint64_t x = (!STREAM || word < S.W64) ? rows[(size_t)(word) * RS + h] : 0ull;
+              const uint64_t x = (!STREAM || word < S.W64) ? ROW(word, h) : 0ull;
               c[r] += (int)((x >> S.lane) & 1);
             }
           }
@@ -556,7 +567,7 @@ __device__ __forceinline__ void eff_base(const SimParams& P, int e, const Alt<R>
         for (int r = 0; r < R; ++r) {
           double s = 0.0;
           for (int wd = 0; wd < P.W64; ++wd) {
-            uint64_t m = rows[(size_t)(wd) * RS + A.j[r]];
+            uint64_t m = ROW(wd, A.j[r]);
             while (m) {
               const int h = wd * 64 + __ffsll((long long)m) - 1;
               m &= m - 1;
@@ -574,7 +585,7 @@ __device__ __forceinline__ void eff_base(const SimParams& P, int e, const Alt<R>
           double s = 0.0;
           const int jw = A.j[r] >> 6, jb = A.j[r] & 63;
           for (int h = 0; h < n; ++h) {
-            if (h == U.i || !((rows[(size_t)jw * RS + h] >> jb) & 1)) continue;
+            if (h == U.i || !((ROW(jw, h) >> jb) & 1)) continue;
             const int p = soall[h];
             s += A.bij(r) ? (w[p] - w[p - 1]) : (w[p + 1] - w[p]);
           }
@@ -669,12 +680,12 @@ __host__ __device__ inline size_t al8(size_t x) { return (x + 7) & ~(size_t)7; }
 // the ministep need no bounds tests. Streamed networks (wmax above
 // kLdsWordBound) keep their rows in device memory and row i in LDS.
 __host__ __device__ inline LdsLayout lds_layout(int n, int k, int kb, int wmax, int RS, int nw,
-                                                int flags) {
+                                                int flags, int lw) {
   const int W64 = wmax;
   const bool stream = wmax > kLdsWordBound;
   LdsLayout L;
   size_t o = 0;
-  L.rows = o; if (!stream) o += (size_t)W64 * RS * 8;
+  L.rows = o; o += (size_t)(stream ? lw : W64) * RS * 8;  // streamed: the leading lw words
   L.indeg = o; o += (size_t)n * 4;
   L.outdeg = o; o += (size_t)n * 4;
   L.z = o; if (flags & F_Z) o += (size_t)n * 4;
@@ -752,14 +763,32 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
 #else
   const int W64 = P.W64, RS = P.RS;
 #endif
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const LdsLayout L = lds_layout(n, k, kb, WMAX, RS, NW, pflags);
+  // the wave index is uniform: scalar registers and scalar branches on it
+  const int tid = threadIdx.x, lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   constexpr bool STREAM = WMAX > kLdsWordBound;
-  // streamed: this chain's rows in device memory, same word-major layout
-  // with W64 words (no padding)
-  uint64_t* rows;
-  if constexpr (STREAM) rows = P.work + (size_t)chain * W64 * RS;
-  else rows = (uint64_t*)(smem + L.rows);
+  // streamed: this chain's rows in device memory (rows_d), same word-major
+  // layout with W64 words (no padding), of which the leading lw words live
+  // in LDS instead (partial residency: read and written there only)
+  const int lw = STREAM ? P.lw : 0;
+  const LdsLayout L = lds_layout(n, k, kb, WMAX, RS, NW, pflags, lw);
+  uint64_t* rows = (uint64_t*)(smem + L.rows);
+  uint64_t* rows_d = STREAM ? P.work + (size_t)chain * W64 * RS : nullptr;
+  // word w of row x; w is uniform at every call, so the branch does not diverge
+  auto RGET = [&](int w, int x) -> uint64_t {
+    if constexpr (STREAM) {
+      if (w >= lw) return rows_d[(size_t)w * RS + x];
+    }
+    return rows[(size_t)w * RS + x];
+  };
+  auto RPUT = [&](int w, int x, uint64_t v) {
+    if constexpr (STREAM) {
+      if (w >= lw) {
+        rows_d[(size_t)w * RS + x] = v;
+        return;
+      }
+    }
+    rows[(size_t)w * RS + x] = v;
+  };
   uint64_t* rowi_s = (uint64_t*)(smem + L.rowi);
   int* indeg = (int*)(smem + L.indeg);
   int* outdeg = (int*)(smem + L.outdeg);
@@ -788,14 +817,15 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
   constexpr int kRateMask = ~0;
 #endif
   const int flags = (SIMPLE ? (pflags & (F_TP | F_Z | F_RDEG | F_RATE)) : pflags) & kRateMask;
-  const ChainLds S{rows, RS, W64, NW, wv, lane, indeg, outdeg, z, tpall, soall, tpm0, tpm1};
+  const ChainLds S{rows, RS, W64, NW, wv, lane, indeg, outdeg, z, tpall, soall, tpm0, tpm1,
+                   rows_d, lw};
 
   // ---- chain set-up (libcusna_saom_cpu.cpp:146-160)
   constexpr int WI = STREAM ? 1 : WMAX;  // words per row stored (x W64 when streamed)
   const int WS = STREAM ? W64 : WI;
   for (int q = tid; q < n * WS; q += NT) {
     const int h = q / WS, wd = q - h * WS;
-    rows[(size_t)wd * RS + h] = wd < W64 ? P.xs[(size_t)h * W64 + wd] : 0ull;
+    RPUT(wd, h, wd < W64 ? P.xs[(size_t)h * W64 + wd] : 0ull);
   }
   for (int q = tid; q < WMAX; q += NT) coli_s[q] = 0ull;
   for (int q = tid; q < n; q += NT) {
@@ -818,7 +848,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
   for (int h = tid; h < n; h += NT) {
     int od = 0;
     for (int wd = 0; wd < W64; ++wd) {
-      uint64_t m = rows[(size_t)wd * RS + h];
+      uint64_t m = RGET(wd, h);
       od += popc64(m);
       while (m) {
         const int j = wd * 64 + __ffsll((long long)m) - 1;
@@ -984,7 +1014,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
       if (ix >= nact) ix = nact - 1;
       i = P.act ? P.act[ix] : (int)ix;
     }
-    auto ROWI = [&](int w) -> uint64_t { return rows[(size_t)w * RS + i]; };
+    auto ROWI = [&](int w) -> uint64_t { return RGET(w, i); };
 
     if (!net) {
       // ---- behaviour ministep (:202-227)
@@ -1100,7 +1130,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
       const int j = word * 64 + lane;
       const bool in = j < n;
       A.j[r] = in ? j : i;
-      const int xji = in ? (int)((rows[(size_t)iw * RS + A.j[r]] >> ib) & 1) : 0;
+      const int xji = in ? (int)((RGET(iw, A.j[r]) >> ib) & 1) : 0;
       const uint64_t bal = __ballot(xji);
       if (lane == 0) coli_s[word] = bal;
       const uint64_t rw = (!STREAM || word < W64) ? ROWI(word) : 0ull;
@@ -1139,7 +1169,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
       A.cy[r] = 0;
     }
     auto bit_of_row_j = [&](int r, int h) -> int {  // x_{j h}
-      const uint64_t x = rows[(size_t)(h >> 6) * RS + A.j[r]];
+      const uint64_t x = RGET(h >> 6, A.j[r]);
       const uint32_t half = (h & 32) ? (uint32_t)(x >> 32) : (uint32_t)x;
       return (int)((half >> (h & 31)) & 1u);
     };
@@ -1155,7 +1185,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
             for (int r = 0; r < R; ++r) {
               const int word = NW * r + wv;
               const uint64_t x =
-                  (!STREAM || word < W64) ? rows[(size_t)(word) * RS + h] : 0ull;
+                  (!STREAM || word < W64) ? RGET(word, h) : 0ull;
               A.tp[r] += (int)((x >> lane) & 1);
             }
           }
@@ -1183,7 +1213,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
         int so = 0, cy = 0;
 #pragma unroll
         for (int wd = 0; wd < U.words(); ++wd) {
-          const uint64_t x = rows[(size_t)(wd) * RS + A.j[r]];
+          const uint64_t x = RGET(wd, A.j[r]);
           so += popc64(U.rowi(wd) & x);
           cy += popc64(x & U.coli(wd));
         }
@@ -1432,11 +1462,11 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
     if (tid == 0) {
       if (jstar != i) {
         const int dd = 1 - 2 * aij;
-        rows[(size_t)(jstar >> 6) * RS + i] ^= (1ull << (jstar & 63));
+        RPUT(jstar >> 6, i, RGET(jstar >> 6, i) ^ (1ull << (jstar & 63)));
         indeg[jstar] += dd;
         outdeg[i] += dd;
         if (P.symflag) {
-          rows[(size_t)(i >> 6) * RS + jstar] ^= (1ull << (i & 63));
+          RPUT(i >> 6, jstar, RGET(i >> 6, jstar) ^ (1ull << (i & 63)));
           indeg[i] += dd;
           outdeg[jstar] += dd;
         }
@@ -1464,8 +1494,8 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
       for (int wd = 0; wd < W64; ++wd) {
         const uint64_t x0 = P.x0[(size_t)h * W64 + wd];
         const uint64_t im = P.inactw[wd];
-        const uint64_t v = rows[(size_t)wd * RS + h];
-        rows[(size_t)wd * RS + h] = hin ? x0 : ((v & ~im) | (x0 & im));
+        const uint64_t v = RGET(wd, h);
+        RPUT(wd, h, hin ? x0 : ((v & ~im) | (x0 & im)));
       }
     }
     __syncthreads();
@@ -1475,7 +1505,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
   for (int h = tid; h < n; h += NT) {
     int c = 0;
     for (int wd = 0; wd < W64; ++wd) {
-      const uint64_t v = rows[(size_t)wd * RS + h];
+      const uint64_t v = RGET(wd, h);
       out_rows[(size_t)h * W64 + wd] = v;
       c += popc64((v ^ P.x0[(size_t)h * W64 + wd]) & P.kd[(size_t)h * W64 + wd]);
     }
@@ -1584,7 +1614,7 @@ int SAOM_CAT(launch_period_w, SAOM_WMAX)(const PeriodLaunch& L, int nw, bool rep
 }
 
 size_t SAOM_CAT(sim_lds_bytes_w, SAOM_WMAX)(const SimParams& P, int nw, int wmax) {
-  return lds_layout(P.n, P.k, P.kb, wmax, P.RS, nw, P.flags).total;
+  return lds_layout(P.n, P.k, P.kb, wmax, P.RS, nw, P.flags, P.lw).total;
 }
 
 }  // namespace saom
