@@ -822,15 +822,23 @@ constexpr int kGroupStreams = 4;
 struct GroupStreams {
   hipStream_t st[kGroupStreams] = {};
   hipEvent_t done[kGroupStreams] = {};
-  hipEvent_t ready = nullptr;
+  std::vector<hipEvent_t> ready;  // one per bucket: its inputs are on the device
   int n = 0;
   hipError_t create(int count) {
-    hipError_t e = hipEventCreateWithFlags(&ready, hipEventDisableTiming);
+    hipError_t e = hipSuccess;
     for (; e == hipSuccess && n < count; ++n) {
       e = hipStreamCreateWithFlags(&st[n], hipStreamNonBlocking);
       if (e == hipSuccess) e = hipEventCreateWithFlags(&done[n], hipEventDisableTiming);
     }
     return e;
+  }
+  hipError_t record_ready(hipStream_t s, hipEvent_t* out) {
+    hipEvent_t e = nullptr;
+    hipError_t r = hipEventCreateWithFlags(&e, hipEventDisableTiming);
+    if (r != hipSuccess) return r;
+    ready.push_back(e);
+    *out = e;
+    return hipEventRecord(e, s);
   }
   ~GroupStreams() {
     for (int q = 0; q < kGroupStreams; ++q) {
@@ -838,14 +846,32 @@ struct GroupStreams {
       if (done[q]) (void)hipEventDestroy(done[q]);
       if (st[q]) (void)hipStreamDestroy(st[q]);
     }
-    if (ready) (void)hipEventDestroy(ready);
+    for (hipEvent_t e : ready) (void)hipEventDestroy(e);
   }
 };
+
+// The launch shape plan_create gives pb's plan (waves per chain, word
+// bound, words), computed without building the plan.
+struct PlanShape {
+  int nw, wmax, W64;
+};
+PlanShape plan_shape(const cusna_saom_problem* pb) {
+  PlanShape sh;
+  sh.W64 = (pb->n + 63) / 64;
+  sh.wmax = wmax_for(sh.W64);
+  sh.nw = pick_nw(sh.wmax, pb->threads);
+  if (pb->threads <= 0) {
+    int flags = effect_flags(pb->type_ids, pb->k, pb->kb, pb->dyn_flags);
+    if (pb->nrate > 0) flags |= saom::F_RATE | (pb->nrate << 16);
+    const int tuned = cached_nw(ShapeKey{sh.wmax, sh.W64, pb->k, flags});
+    if (tuned > 0) sh.nw = tuned;
+  }
+  return sh;
+}
 
 cusna_status simulate_groups(const cusna_saom_problem* const* pbs, int G,
                              cusna_saom_result* const* rs) {
   hipStream_t s = thread_stream();
-  std::vector<std::unique_ptr<cusna_period_plan>> plans(G);
   for (int g = 0; g < G; ++g) {
     if (!pbs[g] || !rs[g]) return CUSNA_ERR_NULL_ARG;
     cusna_status st = validate(pbs[g], rs[g]);
@@ -854,12 +880,6 @@ cusna_status simulate_groups(const cusna_saom_problem* const* pbs, int G,
       set_error("cusna_saom_simulate_groups: at most 2^20 chains per group");
       return CUSNA_ERR_SHAPE;
     }
-    cusna_period_plan* raw = nullptr;
-    st = plan_create(pbs[g], rs[g]->want_end_nets && rs[g]->a_ends, nullptr, nullptr, 0, s, &raw);
-    if (st != CUSNA_OK) return st;
-    plans[g].reset(raw);
-    st = plan_set_params(raw, pbs[g]->theta, pbs[g]->lam, pbs[g]->thetab, pbs[g]->lamb);
-    if (st != CUSNA_OK) return st;
   }
   // buckets of one launch shape
   // (one word count per bucket, so a specialised launch can use the exact
@@ -870,14 +890,16 @@ cusna_status simulate_groups(const cusna_saom_problem* const* pbs, int G,
     std::vector<int> groups;
   };
   std::vector<Bucket> buckets;
+  std::vector<PlanShape> shape(G);
   for (int g = 0; g < G; ++g) {
-    const cusna_period_plan* pl = plans[g].get();
-    const bool simple = saom::simple_effects(pl->P.tid, pl->P.k);
+    shape[g] = plan_shape(pbs[g]);
+    const PlanShape& sh = shape[g];
+    const bool simple = saom::simple_effects(pbs[g]->type_ids, pbs[g]->k);
     Bucket* b = nullptr;
     for (auto& x : buckets)
-      if (x.nw == pl->nw && x.wmax == pl->wmax && x.W64 == pl->W64 && x.simple == simple) b = &x;
+      if (x.nw == sh.nw && x.wmax == sh.wmax && x.W64 == sh.W64 && x.simple == simple) b = &x;
     if (!b) {
-      buckets.push_back(Bucket{pl->nw, pl->wmax, pl->W64, simple, {}});
+      buckets.push_back(Bucket{sh.nw, sh.wmax, sh.W64, simple, {}});
       b = &buckets.back();
     }
     b->groups.push_back(g);
@@ -888,103 +910,131 @@ cusna_status simulate_groups(const cusna_saom_problem* const* pbs, int G,
     const double lam = pb->lam ? (double)pb->lam[0] : 1.0;
     const double nact = pb->act ? pb->nact : pb->n;
     const double steps = pb->conditional ? 2.0 * pb->target_dist + 1.0 : lam * nact + 1.0;
-    // a chain whose rows live in device memory runs about 1.8 x an
+    // a chain whose rows live (partly) in device memory runs about 1.3 x an
     // LDS-resident chain of the same size (measured on the 256-group batch)
-    const double streamed = plans[g]->wmax > 16 ? 1.8 : 1.0;
-    return streamed * steps * pb->n * (plans[g]->W64 + pb->k + 8.0);
+    const double streamed = shape[g].wmax > 16 ? 1.3 : 1.0;
+    return streamed * steps * pb->n * (shape[g].W64 + pb->k + 8.0);
   };
-  size_t n_items = 0;
-  for (int g = 0; g < G; ++g) n_items += (size_t)plans[g]->M;
-  DeviceBlock D;
-  const size_t o_grp = D.add((size_t)G * sizeof(saom::SimParams)), o_items = D.add(n_items * 4);
-  HIP_TRY(D.alloc());
-  std::vector<saom::SimParams> grp(G);
-  for (int g = 0; g < G; ++g) grp[g] = plans[g]->P;
-  HIP_TRY(hipMemcpyAsync(D.at<char>(o_grp), grp.data(), G * sizeof(saom::SimParams),
-                         hipMemcpyHostToDevice, s));
-  std::vector<uint32_t> items;
-  items.reserve(n_items);
-  std::vector<saom::PeriodLaunch> launches(buckets.size());
+  std::vector<double> work_b(buckets.size(), 0.0);
   for (size_t bi = 0; bi < buckets.size(); ++bi) {
     Bucket& b = buckets[bi];
     std::stable_sort(b.groups.begin(), b.groups.end(),
                      [&](int a, int c) { return cost(a) > cost(c); });
+    for (int g : b.groups) work_b[bi] += cost(g) * pbs[g]->M;
+  }
+  std::vector<size_t> order(buckets.size());
+  for (size_t bi = 0; bi < order.size(); ++bi) order[bi] = bi;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](size_t a, size_t c) { return work_b[a] > work_b[c]; });
+
+  // One parameter slot per group in launch order, so that a bucket's
+  // parameters and work items are contiguous; items are (slot << 20 | chain).
+  size_t n_items = 0;
+  for (int g = 0; g < G; ++g) n_items += (size_t)pbs[g]->M;
+  DeviceBlock D;
+  const size_t o_grp = D.add((size_t)G * sizeof(saom::SimParams)), o_items = D.add(n_items * 4);
+  HIP_TRY(D.alloc());
+  std::vector<saom::SimParams> grp(G);
+  std::vector<uint32_t> items;
+  items.reserve(n_items);
+  std::vector<std::unique_ptr<cusna_period_plan>> plans(G);
+
+  // The buckets run concurrently on up to four streams (largest first, each
+  // onto the least-loaded stream), so one bucket's tail and a CU's spare LDS
+  // and wave slots take work from another; chains are independent, results
+  // do not depend on the order. A bucket's plans are built (and their inputs
+  // uploaded on the calling thread's stream) just before its launch, so that
+  // the set-up of the later buckets overlaps the earlier buckets' periods.
+  GroupStreams gs;
+  const int nst = (int)std::min<size_t>(kGroupStreams, buckets.size());
+  HIP_TRY(gs.create(nst));
+  std::vector<double> queued(nst, 0.0);
+  int slot = 0;
+  for (size_t o = 0; o < order.size(); ++o) {
+    const size_t bi = order[o];
+    Bucket& b = buckets[bi];
+    const int slot0 = slot;
     const size_t first = items.size();
+    for (int g : b.groups) {
+      cusna_period_plan* raw = nullptr;
+      cusna_status st =
+          plan_create(pbs[g], rs[g]->want_end_nets && rs[g]->a_ends, nullptr, nullptr, 0, s, &raw);
+      if (st != CUSNA_OK) return st;
+      plans[g].reset(raw);
+      st = plan_set_params(raw, pbs[g]->theta, pbs[g]->lam, pbs[g]->thetab, pbs[g]->lamb);
+      if (st != CUSNA_OK) return st;
+      if (raw->nw != b.nw || raw->wmax != b.wmax || raw->W64 != b.W64) {
+        set_error("cusna_saom_simulate_groups: plan shape differs from its bucket");
+        return CUSNA_ERR_SHAPE;
+      }
+      grp[slot] = raw->P;
+      for (int c = 0; c < raw->M; ++c) items.push_back(((uint32_t)slot << 20) | (uint32_t)c);
+      ++slot;
+    }
+    const int ng = slot - slot0;
+    HIP_TRY(hipMemcpyAsync(D.at<saom::SimParams>(o_grp) + slot0, grp.data() + slot0,
+                           (size_t)ng * sizeof(saom::SimParams), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(D.at<uint32_t>(o_items) + first, items.data() + first,
+                           (items.size() - first) * 4, hipMemcpyHostToDevice, s));
+    hipEvent_t ready = nullptr;
+    HIP_TRY(gs.record_ready(s, &ready));
+
     size_t lds = 0, lds_exact = 0, lds_alt = 0;
     bool uniform = true, free_shape = true;
     long long work = 0;
+    const cusna_period_plan* lead = plans[b.groups[0]].get();
     for (int g : b.groups) {
-      uniform = uniform && saom::same_effects(plans[g]->P, plans[b.groups[0]]->P);
+      uniform = uniform && saom::same_effects(plans[g]->P, lead->P);
       free_shape = free_shape && plans[g]->free_shape;
       work += (long long)plans[g]->M * plans[g]->n;
       lds = std::max(lds, saom::sim_lds_bytes(plans[g]->P, b.nw, b.wmax));
     }
-    const int kf = uniform && free_shape ? plans[b.groups[0]]->P.k : 0;
+    const int kf = uniform && free_shape ? lead->P.k : 0;
     const saom::LaunchShape gen{b.nw, b.wmax};
     const saom::LaunchShape ex = saom::exact_shape(b.W64, gen, kf, lds);
     const saom::LaunchShape alt = saom::exact_shape(b.W64, gen, kf);
     for (int g : b.groups) {
-      for (int c = 0; c < plans[g]->M; ++c) items.push_back(((uint32_t)g << 20) | (uint32_t)c);
       lds_exact = std::max(lds_exact, saom::sim_lds_bytes(plans[g]->P, ex.nw, ex.wmax));
       lds_alt = std::max(lds_alt, saom::sim_lds_bytes(plans[g]->P, alt.nw, alt.wmax));
     }
-    saom::PeriodLaunch& L = launches[bi];
+    saom::PeriodLaunch L;
     L.Pd = D.at<saom::SimParams>(o_grp);
     L.items = D.at<uint32_t>(o_items) + first;
     L.grid = (int)(items.size() - first);
     L.lds = lds;
     L.key0 = L.key1 = 0u;
-    L.Ph = &plans[b.groups[0]]->P;
+    L.Ph = &lead->P;
     L.specialize = uniform && saom::specialize_policy(work);
-    for (int g : b.groups) L.same_n = L.same_n && plans[g]->n == plans[b.groups[0]]->n;
+    for (int g : b.groups) L.same_n = L.same_n && plans[g]->n == lead->n;
     L.exact = ex;
     L.lds_exact = lds_exact;
     L.exact_alt = alt;
     L.lds_exact_alt = lds_alt;
-  }
-  HIP_TRY(hipMemcpyAsync(D.at<uint32_t>(o_items), items.data(), items.size() * 4,
-                         hipMemcpyHostToDevice, s));
-  // The buckets run concurrently on up to four streams (largest first, each
-  // onto the least-loaded stream), so one bucket's tail and a CU's spare LDS
-  // and wave slots take work from another; chains are independent, results
-  // do not depend on the order.
-  GroupStreams gs;
-  const int nst = (int)std::min<size_t>(kGroupStreams, buckets.size());
-  HIP_TRY(gs.create(nst));
-  HIP_TRY(hipEventRecord(gs.ready, s));
-  // longest bucket first onto the stream with the least work queued
-  // (estimated as the chains' expected ministeps x alternatives x words),
-  // so that no stream is left with a long tail of its own
-  std::vector<double> work_b(buckets.size(), 0.0);
-  for (size_t bi = 0; bi < buckets.size(); ++bi)
-    for (int g : buckets[bi].groups) work_b[bi] += cost(g) * plans[g]->M;
-  std::vector<size_t> order(buckets.size());
-  for (size_t bi = 0; bi < order.size(); ++bi) order[bi] = bi;
-  std::stable_sort(order.begin(), order.end(),
-                   [&](size_t a, size_t c) { return work_b[a] > work_b[c]; });
-  std::vector<double> queued(nst, 0.0);
-  for (int q = 0; q < nst; ++q) HIP_TRY(hipStreamWaitEvent(gs.st[q], gs.ready, 0));
-  for (size_t o = 0; o < order.size(); ++o) {
-    const size_t bi = order[o];
-    const Bucket& b = buckets[bi];
+
+    // longest bucket first onto the stream with the least work queued
+    // (estimated as the chains' expected ministeps x alternatives x words),
+    // so that no stream is left with a long tail of its own
     const int q = (int)(std::min_element(queued.begin(), queued.end()) - queued.begin());
     queued[q] += work_b[bi];
-    const int le = saom::launch_period_ex(launches[bi], saom::LaunchShape{b.nw, b.wmax}, false,
-                                          b.simple, gs.st[q]);
+    HIP_TRY(hipStreamWaitEvent(gs.st[q], ready, 0));
+    const int le = saom::launch_period_ex(L, saom::LaunchShape{b.nw, b.wmax}, false, b.simple,
+                                          gs.st[q]);
     if (le != 0) {
       set_error(std::string("period kernel launch: ") + hipGetErrorString((hipError_t)le));
       return CUSNA_ERR_CUDA;
+    }
+    // the bucket's statistics follow it on its stream, so that they overlap
+    // the other buckets' periods
+    for (int g : b.groups) {
+      if (rs[g]->stats) {
+        cusna_status st = plan_stats(plans[g].get(), plans[g]->M, gs.st[q]);
+        if (st != CUSNA_OK) return st;
+      }
     }
   }
   for (int q = 0; q < nst; ++q) {
     HIP_TRY(hipEventRecord(gs.done[q], gs.st[q]));
     HIP_TRY(hipStreamWaitEvent(s, gs.done[q], 0));
-  }
-  for (int g = 0; g < G; ++g) {
-    if (rs[g]->stats) {
-      cusna_status st = plan_stats(plans[g].get(), plans[g]->M, s);
-      if (st != CUSNA_OK) return st;
-    }
   }
   for (int g = 0; g < G; ++g) {
     cusna_status st = plan_fetch(plans[g].get(), rs[g]);

@@ -174,6 +174,7 @@ void* rtc_period_kernel(const SimParams& P, LaunchShape s, bool replay, bool sim
       std::string("-DSAOM_HAS_RATE=") + ((P.flags & F_RATE) ? "1" : "0"),
       "-DSAOM_FIXED_KB=" + std::to_string(P.kb), "-DSAOM_FIXED_FLAGS=" + std::to_string(P.flags)};
   if (fixed_n) opts.push_back("-DSAOM_FIXED_N=" + std::to_string(P.n));
+  else opts.push_back("-DSAOM_FIXED_W64=" + std::to_string(P.W64));
   if (waves_per_eu > 0) opts.push_back("-DSAOM_WAVES_PER_EU=" + std::to_string(waves_per_eu));
   return rtc_get(kRtcPeriodSource, "saom_period_rtc", opts);
 }

@@ -130,7 +130,10 @@ int launch_period_ex(const PeriodLaunch& L, LaunchShape s, bool replay, bool sim
 
 // Runtime specialisation (rtc.hip): the compiled kernel for P's effect list
 // and this launch shape (null when unavailable), and its launch.
-// fixed_n: every group of the launch has P's actor count (compiled in);
+// fixed_n: every group of the launch has P's actor count (compiled in); the
+// word count W64 is compiled in either way (every launch holds networks of
+// one word count: a plan's chains, one bucket of a groups call, the periods
+// of a MoM fit);
 // waves_per_eu > 0: the register budget in waves per SIMD (0: two waves).
 void* rtc_period_kernel(const SimParams& P, LaunchShape s, bool replay, bool simple,
                         bool fixed_n, int waves_per_eu = 0);

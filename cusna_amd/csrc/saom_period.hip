@@ -760,6 +760,9 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
 #endif
 #ifdef SAOM_FIXED_N
   constexpr int W64 = (SAOM_FIXED_N + 63) / 64, RS = SAOM_FIXED_N;
+#elif defined(SAOM_FIXED_W64)
+  constexpr int W64 = SAOM_FIXED_W64;  // every chain of the launch has W64 words
+  const int RS = P.RS;
 #else
   const int W64 = P.W64, RS = P.RS;
 #endif
