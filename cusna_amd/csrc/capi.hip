@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -1175,14 +1176,16 @@ cusna_status gof_device(const uint64_t* rows_d, int n, int M, const uint64_t* ke
       kmax = std::max(kmax, levls[L]);
     }
     kmax = std::min(kmax, n);  // no shortest path is longer than n - 1
-    if (saom::gof_geodesic_lds(n, W64, kmax) == 0) {
-      set_error("geodesic distribution above 1024 actors is not supported on the device");
-      return CUSNA_ERR_UNSUPPORTED;
-    }
   }
+  // geodesic rows from device memory where the network exceeds the LDS
+  // (CUSNA_GOF_STREAM=1 takes that form at every size)
+  const char* force_stream = std::getenv("CUSNA_GOF_STREAM");
+  const bool stream = kind == 2 && (saom::gof_geodesic_lds(n, W64, kmax) == 0 ||
+                                    (force_stream && force_stream[0] == '1'));
   const int cols = nlev + (kind == 2 ? 1 : 0);
   DeviceBlock D;
   const size_t o_lv = D.add((size_t)nlev * 4), o_out = D.add((size_t)M * cols * 8);
+  const size_t o_cnt = stream ? D.add((size_t)M * (kmax + 1) * 8) : 0;
   HIP_TRY(D.alloc());
   HIP_TRY(hipMemcpyAsync(D.at<char>(o_lv), levls, (size_t)nlev * 4, hipMemcpyHostToDevice, s));
   saom::GofParams G;
@@ -1197,6 +1200,8 @@ cusna_status gof_device(const uint64_t* rows_d, int n, int M, const uint64_t* ke
   G.force = force_d;
   G.levls = D.at<int32_t>(o_lv);
   G.out = D.at<double>(o_out);
+  G.counters = stream ? D.at<unsigned long long>(o_cnt) : nullptr;
+  G.stream = stream ? 1 : 0;
   const int le = saom::launch_gof(G, s);
   if (le != 0) {
     set_error(std::string("gof kernel launch: ") + hipGetErrorString((hipError_t)le));

@@ -22,7 +22,7 @@ constexpr int kMlpLayers = 4;
 
 class Mlp {
  public:
-  // p_in, p_out <= 64; width a multiple of 64; batch a multiple of 64.
+  // p_in, p_out <= 64; width a multiple of 64 up to 256; batch a multiple of 64.
   cusna_status init(int p_in, int p_out, int width, int batch, uint64_t seed, hipStream_t s);
   // Fresh Glorot-uniform weights, zero biases and optimizer state.
   cusna_status reset(uint64_t seed);
@@ -47,8 +47,6 @@ class Mlp {
   cusna_status forward_backward(const float* x_dev, const float* y_dev);
   // Forward only (outputs).
   cusna_status forward(const float* x_dev);
-  // Adam step over all parameters with the current gradients.
-  cusna_status adam_step(float lr);
   // Train on n samples (device, float32, x n x p_in, y n x p_out): `steps`
   // Adam steps on minibatches drawn with replacement by a counter hash; the
   // learning rate decays linearly from lr to lr / 10.
@@ -62,9 +60,14 @@ class Mlp {
   hipStream_t stream() const { return s_; }
 
  private:
-  cusna_status gemm(const __bf16* A, const __bf16* Bt, int M, int N, int K, const float* bias,
-                    const float* dz, int act, float* Cf, __bf16* Cb, __bf16* CbT);
-  cusna_status loss_and_grad(const float* y_dev);
+  // One launch of the slice kernel over the batch: mode 0 forward, 1 forward
+  // and backward of x, y (B rows), 2 the same on rows drawn by hash from the
+  // n samples x, y (step `step` of the stream `seed`).
+  cusna_status slice(int mode, const float* x, const float* y, int64_t n, uint64_t seed,
+                     long long step);
+  // Weight and bias gradients from the slice kernel's transposed activations
+  // and gradients; with adam, one Adam step at learning rate lr.
+  cusna_status wgrad(bool adam, float lr);
   cusna_status cast_weights();
 
   int p_in_ = 0, p_out_ = 0, width_ = 0, B_ = 0;
@@ -75,21 +78,13 @@ class Mlp {
   hipStream_t s_ = nullptr;
   DeviceBlock D_;
   float *params_ = nullptr, *m_ = nullptr, *v_ = nullptr, *grads_ = nullptr;
-  float* part_ = nullptr;  // partial blocks of split products
-  size_t part_floats_ = 0;
-  __bf16* Wb_[kMlpLayers] = {};   // W_l [K][N] (products with W^T in the backward pass)
-  __bf16* Wtb_[kMlpLayers] = {};  // W_l^T [N][K] (forward products)
-  __bf16 *X_ = nullptr, *XT_ = nullptr;
-  float* Z_[kMlpLayers - 1] = {};  // pre-activations of the hidden layers
-  __bf16* H_[kMlpLayers - 1] = {};
-  __bf16* HT_[kMlpLayers - 1] = {};
-  float* O_ = nullptr;
-  __bf16 *dO_ = nullptr, *dOT_ = nullptr;
-  __bf16* dZ_[kMlpLayers - 1] = {};
-  __bf16* dZT_[kMlpLayers - 1] = {};
+  __bf16* Wb_[kMlpLayers] = {};   // W_l [K][N] (backward operands)
+  __bf16* Wtb_[kMlpLayers] = {};  // W_l^T [N][K] (forward operands)
+  __bf16* XT_ = nullptr;          // inputs, transposed (K_0 x B)
+  __bf16* HT_[kMlpLayers - 1] = {};  // hidden activations, transposed (N_l x B)
+  __bf16* GT_[kMlpLayers] = {};      // pre-activation (output) gradients, transposed
+  float* O_ = nullptr;     // outputs (B x N_3)
   float* loss_ = nullptr;  // per sample
-  float* Yb_ = nullptr;    // minibatch targets (training)
-  float* Xb_ = nullptr;    // minibatch inputs (training)
 };
 
 }  // namespace cusna_impl

@@ -1,24 +1,32 @@
 // mlp.hip -- the surrogate perceptron on gfx950 BF16 matrix cores.
 //
-// Every product is one kernel: a 256-thread workgroup computes a 64 x 64
-// block of C = A B (A row-major M x K, B given as B^T row-major N x K, both
-// bfloat16), each wave a 16 x 64 strip as four 16 x 16 MFMA tiles; per
-// 32-wide K step a lane loads 8 contiguous bfloat16 of its A row and of each
-// of its four B columns (16-byte loads) and issues four
-// v_mfma_f32_16x16x32_bf16. Operand layout of that instruction (checked on
-// the device): lane l holds A[l % 16][8 (l / 16) + e] and B[8 (l / 16) + e][l % 16],
-// e = 0..7, and receives D[4 (l / 16) + v][l % 16], v = 0..3. The epilogue
-// adds the bias, keeps the float32 pre-activation, applies SiLU (forward) or
-// multiplies by SiLU'(z) (backward), and writes bfloat16 row-major and/or
-// transposed copies, so that every later product reads contiguous K.
+// A training step is two launches.
 //
-// Latency, not throughput, bounds these products (a 1024 x 256 x 256 product
-// is ~0.13 GFLOP): the K loop is unrolled so that a wave's loads for U K
-// steps are all in flight before its MFMAs, and products with a long K and a
-// plain float32 result (the weight gradients, K = batch) are split over K
-// into partial blocks that a second kernel sums in split order
-// (deterministic). The bfloat16 outputs pass through LDS so that the
-// row-major and transposed copies are written as 16-byte stores.
+// mlp_slice_kernel: one workgroup per 32 batch rows runs the whole network
+// for its rows -- the four forward products, the loss and its output
+// gradient, and the three backward products of the activation gradients --
+// with every activation, SiLU'(z) and gradient of the slice in LDS. A
+// product is A (LDS, rows of the slice) times W^T or W (device memory, read
+// once per workgroup from L2): the four waves take interleaved 16-column
+// tiles of the output, each lane loading 16 bytes of A and of B per 32-wide
+// K step for v_mfma_f32_16x16x32_bf16 (float32 accumulation). Operand layout
+// of that instruction (checked on the device): lane l holds
+// A[l % 16][8 (l / 16) + e] and B[8 (l / 16) + e][l % 16], e = 0..7, and
+// receives D[4 (l / 16) + v][l % 16], v = 0..3. The slice's inputs and the
+// transposed bfloat16 activations and gradients (X^T, H_l^T, G_l^T) go to
+// device memory for the weight gradients.
+//
+// mlp_wgrad_adam_kernel: one workgroup per 64 x 64 tile of a weight matrix
+// forms dW = H^T G over the whole batch (the four waves take interleaved
+// 32-row K steps and their partial tiles are added in wave order: the
+// result does not depend on timing), the bias gradients of its columns, and
+// applies Adam to the tile, writing the float32 master weights and their
+// row-major and transposed bfloat16 copies through LDS as 16-byte stores.
+//
+// Rounding points: inputs, weights, each hidden activation and each
+// back-propagated gradient are rounded to bfloat16 as they enter a product;
+// pre-activations, outputs, SiLU'(z), the loss and the weight gradients are
+// float32.
 #include "mlp.h"
 
 #include <algorithm>
@@ -33,6 +41,9 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr int kMlpThreads = 256;
+constexpr int kSlice = 32;              // batch rows per workgroup of the slice kernel
+constexpr int kSliceTM = kSlice / 16;   // 16-row tiles per slice
+constexpr int kMaxWidth = 256;          // hidden width bound of the fused step (LDS)
 
 __device__ __forceinline__ float silu(float z) { return z / (1.0f + expf(-z)); }
 __device__ __forceinline__ float dsilu(float z) {
@@ -40,202 +51,395 @@ __device__ __forceinline__ float dsilu(float z) {
   return s * (1.0f + z * (1.0f - s));
 }
 
-struct GemmArgs {
-  const __bf16* A;   // M x K
-  const __bf16* Bt;  // N x K
-  int M, N, K;
-  const float* bias;  // N or null
-  const float* dz;    // M x N pre-activations: multiply by SiLU'(dz), or null
-  int act;            // 1: SiLU (forward)
-  float* Cf;          // M x N float32 result before the activation, or null
-  __bf16* Cb;         // M x N bfloat16 result, or null
-  __bf16* CbT;        // N x M bfloat16 result transposed, or null
-  int Ks;             // K per split (blockIdx.z); Ks = K without a split
-  float* part;        // split products: partial blocks [gridDim.z][M][N]
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+  x ^= x >> 33;
+  x *= 0xFF51AFD7ED558CCDull;
+  x ^= x >> 33;
+  x *= 0xC4CEB9FE1A85EC53ull;
+  return x ^ (x >> 33);
+}
+
+struct SliceArgs {
+  const __bf16* Wt[kMlpLayers];  // W_l^T [N_l][K_l]: forward operands
+  const __bf16* W[kMlpLayers];   // W_l [K_l][N_l]: backward operands
+  const float* bias[kMlpLayers];
+  int K[kMlpLayers], N[kMlpLayers];
+  int p_in, p_out, B;
+  int mode;                      // 0 forward, 1 forward + backward, 2 training step
+  const float* x;                // modes 0, 1: B x p_in; mode 2: n x p_in
+  const float* y;                // mode 1: B x p_out; mode 2: n x p_out
+  long long n;                   // mode 2: samples to draw from
+  uint64_t seed;                 // mode 2: row b of step s is sample
+  long long step;                //   mix64(seed + s B + b) mod n
+  float* O;                      // B x N_3 outputs
+  float* loss;                   // B per-sample losses (modes 1, 2)
+  __bf16* XT;                    // K_0 x B
+  __bf16* HT[kMlpLayers - 1];    // N_l x B
+  __bf16* GT[kMlpLayers];        // N_l x B: gradient of layer l's pre-activation (output)
 };
 
-// U: K steps of 32 whose loads are issued together (Ks a multiple of 32 U).
-template <int U>
-__global__ void __launch_bounds__(kMlpThreads) gemm_bf16_kernel(const GemmArgs g) {
-  const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int m0 = blockIdx.x * 64 + 16 * w, n0 = blockIdx.y * 64;
-  const int r = l & 15, q = l >> 4;
-  floatx4 acc[4];
+// LDS of the slice kernel: byte offsets and row strides (elements).
+struct SliceLds {
+  int xs, ha, hb, d, os, dos, idx, total;
+  int ldx, ldh, ldd, ldo;
+};
+
+__host__ __device__ inline SliceLds slice_lds(int K0, int width, int N3) {
+  SliceLds L;
+  L.ldx = K0 + 8;      // bfloat16 rows padded by 16 bytes (banks)
+  L.ldh = width + 8;
+  L.ldd = width + 1;   // float32 rows padded by one word
+  L.ldo = N3 + 8;
+  int o = 0;
+  L.xs = o;
+  o += kSlice * L.ldx * 2;
+  L.ha = o;  // activations / gradients, ping-pong
+  o += kSlice * L.ldh * 2;
+  L.hb = o;
+  o += kSlice * L.ldh * 2;
+  L.d = o;   // SiLU'(z) of the three hidden layers
+  o += 3 * kSlice * L.ldd * 4;
+  L.os = o;  // float32 outputs
+  o += kSlice * N3 * 4;
+  L.dos = o; // output gradient, bfloat16
+  o += kSlice * L.ldo * 2;
+  L.idx = o; // sample index of each row
+  o += kSlice * 8;
+  L.total = o;
+  return L;
+}
+
+// acc[i][j] = rows 16 i.. of A (LDS, row stride lda) times columns
+// 16 (w + 4 j).. of B, given as Bt [N][K] in device memory (row stride K),
+// over k = 0..K-1 (K a multiple of 64: two K steps' loads issued together).
+template <int TNW>
+__device__ __forceinline__ void slice_product(const __bf16* A, int lda, const __bf16* Bt, int K,
+                                              floatx4 (&acc)[kSliceTM][TNW]) {
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6, r = l & 15, q = l >> 4;
 #pragma unroll
-  for (int t = 0; t < 4; ++t) acc[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-  const __bf16* arow = g.A + (size_t)(m0 + r) * g.K + 8 * q;
-  const __bf16* brow = g.Bt + (size_t)(n0 + r) * g.K + 8 * q;
-  const int kbeg = blockIdx.z * g.Ks, kend = kbeg + g.Ks;
-  for (int k0 = kbeg; k0 < kend; k0 += 32 * U) {
-    bf16x8 a[U], b[U][4];
+  for (int i = 0; i < kSliceTM; ++i)
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      a[u] = *reinterpret_cast<const bf16x8*>(arow + k0 + 32 * u);
+    for (int j = 0; j < TNW; ++j) acc[i][j] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+  const __bf16* arow = A + r * lda + 8 * q;
+  const __bf16* brow = Bt + (size_t)(16 * w + r) * K + 8 * q;
+  for (int k0 = 0; k0 < K; k0 += 64) {
+    bf16x8 a[2][kSliceTM], b[2][TNW];
 #pragma unroll
-      for (int t = 0; t < 4; ++t)
-        b[u][t] = *reinterpret_cast<const bf16x8*>(brow + (size_t)16 * t * g.K + k0 + 32 * u);
+    for (int u = 0; u < 2; ++u) {
+#pragma unroll
+      for (int i = 0; i < kSliceTM; ++i)
+        a[u][i] = *reinterpret_cast<const bf16x8*>(arow + 16 * i * lda + k0 + 32 * u);
+#pragma unroll
+      for (int j = 0; j < TNW; ++j)
+        b[u][j] = *reinterpret_cast<const bf16x8*>(brow + (size_t)64 * j * K + k0 + 32 * u);
     }
-    // every load of the U steps is issued before the first MFMA (the default
-    // schedule keeps only ~7 loads in flight to save registers)
-    __builtin_amdgcn_sched_barrier(0);
+    __builtin_amdgcn_sched_barrier(0);  // every load of the two steps before the MFMAs
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
+    for (int u = 0; u < 2; ++u)
 #pragma unroll
-      for (int t = 0; t < 4; ++t)
-        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u], b[u][t], acc[t], 0, 0, 0);
-    }
+      for (int i = 0; i < kSliceTM; ++i)
+#pragma unroll
+        for (int j = 0; j < TNW; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u][i], b[u][j], acc[i][j], 0, 0, 0);
   }
-  const int mt = blockIdx.x * 64;  // the workgroup's 64 x 64 tile: rows mt.., columns n0..
-  if (gridDim.z > 1) {  // partial block of split blockIdx.z
-    const size_t MN = (size_t)g.M * g.N;
-    float* part = g.part + (size_t)blockIdx.z * MN;
+}
+
+// dst [ncols][B], columns b0..b0+kSlice-1 = src (LDS [kSlice][ld]) transposed,
+// 16-byte stores.
+__device__ __forceinline__ void store_transposed(const __bf16* src, int ld, int ncols,
+                                                 __bf16* dst, int B, int b0) {
+  constexpr int kChunks = kSlice / 8;
+  for (int it = threadIdx.x; it < ncols * kChunks; it += kMlpThreads) {
+    const int c = it / kChunks, r8 = 8 * (it - c * kChunks);
+    bf16x8 v;
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-#pragma unroll
-      for (int v = 0; v < 4; ++v)
-        part[(size_t)(m0 + 4 * q + v) * g.N + n0 + 16 * t + r] = acc[t][v];
-    }
-    return;
+    for (int e = 0; e < 8; ++e) v[e] = src[(r8 + e) * ld + c];
+    *reinterpret_cast<bf16x8*>(dst + (size_t)c * B + b0 + r8) = v;
   }
-  __shared__ __align__(16) __bf16 tile[64][72];  // bfloat16 outputs, rows padded
-#pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    const int col = n0 + 16 * t + r;
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int row = m0 + 4 * q + v;
-      const size_t ix = (size_t)row * g.N + col;
-      float val = acc[t][v];
-      if (g.bias) val += g.bias[col];
-      if (g.Cf) g.Cf[ix] = val;
-      if (g.dz) val *= dsilu(g.dz[ix]);
-      else if (g.act) val = silu(val);
-      tile[row - mt][col - n0] = (__bf16)val;
-    }
+}
+
+// NWW = width / 64, N3W = N_3 / 64.
+template <int NWW, int N3W>
+__global__ void __launch_bounds__(kMlpThreads) mlp_slice_kernel(const SliceArgs g) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int K0 = g.K[0], width = 64 * NWW, N3 = 64 * N3W, B = g.B, p = g.p_out;
+  const SliceLds L = slice_lds(K0, width, N3);
+  __bf16* Xs = reinterpret_cast<__bf16*>(smem + L.xs);
+  __bf16* Ha = reinterpret_cast<__bf16*>(smem + L.ha);
+  __bf16* Hb = reinterpret_cast<__bf16*>(smem + L.hb);
+  float* D = reinterpret_cast<float*>(smem + L.d);
+  float* Os = reinterpret_cast<float*>(smem + L.os);
+  __bf16* dOs = reinterpret_cast<__bf16*>(smem + L.dos);
+  long long* idx = reinterpret_cast<long long*>(smem + L.idx);
+  const int tid = threadIdx.x, l = tid & 63, w = tid >> 6, r = l & 15, q = l >> 4;
+  const int b0 = blockIdx.x * kSlice;
+  const bool bwd = g.mode != 0;
+  if (tid < kSlice) {
+    long long src = b0 + tid;
+    if (g.mode == 2)
+      src = (long long)(mix64(g.seed + (uint64_t)g.step * (uint64_t)B + (uint64_t)(b0 + tid)) %
+                        (uint64_t)g.n);
+    idx[tid] = src;
   }
-  if (!g.Cb && !g.CbT) return;
   __syncthreads();
-  const int a = threadIdx.x >> 2, c16 = 16 * (threadIdx.x & 3);
-  if (g.Cb) {  // row a of the tile, columns c16..c16+15
-    const bf16x8* src = reinterpret_cast<const bf16x8*>(&tile[a][c16]);
-    bf16x8* dst = reinterpret_cast<bf16x8*>(g.Cb + (size_t)(mt + a) * g.N + n0 + c16);
-    dst[0] = src[0];
-    dst[1] = src[1];
+  for (int e = tid; e < kSlice * K0; e += kMlpThreads) {
+    const int row = e / K0, k = e - row * K0;
+    Xs[row * L.ldx + k] = (__bf16)(k < g.p_in ? g.x[(size_t)idx[row] * g.p_in + k] : 0.0f);
   }
-  if (g.CbT) {  // column a of the tile, rows c16..c16+15, as a row of the transpose
-    bf16x8 lo, hi;
+  __syncthreads();
+  if (bwd) store_transposed(Xs, L.ldx, K0, g.XT, B, b0);
+  // hidden layers: H_l = bf16(SiLU(A W_l + b_l)), SiLU'(z) kept for the backward pass
+  const __bf16* in = Xs;
+  int ldin = L.ldx;
+#pragma unroll 1
+  for (int layer = 0; layer < kMlpLayers - 1; ++layer) {
+    __bf16* out = (layer & 1) ? Hb : Ha;
+    float* Dl = D + layer * kSlice * L.ldd;
+    const float* bias = g.bias[layer];
+    floatx4 acc[kSliceTM][NWW];
+    slice_product<NWW>(in, ldin, g.Wt[layer], g.K[layer], acc);
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      lo[e] = tile[c16 + e][a];
-      hi[e] = tile[c16 + 8 + e][a];
-    }
-    bf16x8* dst = reinterpret_cast<bf16x8*>(g.CbT + (size_t)(n0 + a) * g.M + mt + c16);
-    dst[0] = lo;
-    dst[1] = hi;
+    for (int i = 0; i < kSliceTM; ++i)
+#pragma unroll
+      for (int j = 0; j < NWW; ++j) {
+        const int col = 16 * w + 64 * j + r;
+        const float bc = bias[col];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = 16 * i + 4 * q + v;
+          const float z = acc[i][j][v] + bc;
+          if (bwd) Dl[row * L.ldd + col] = dsilu(z);
+          out[row * L.ldh + col] = (__bf16)silu(z);
+        }
+      }
+    __syncthreads();
+    if (bwd) store_transposed(out, L.ldh, width, g.HT[layer], B, b0);
+    in = out;
+    ldin = L.ldh;
   }
-}
-
-// out[i] = sum over s = 0..S-1 of part[s][i], in that order.
-__global__ void sum_parts_kernel(const float* part, int S, int MN, float* out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= MN) return;
-  float v = part[i];
-  for (int s = 1; s < S; ++s) v += part[(size_t)s * MN + i];
-  out[i] = v;
-}
-
-// x (B x p_in, float32) -> X (B x Kin) and X^T (Kin x B), bfloat16, zero padded.
-__global__ void prep_input_kernel(const float* x, int B, int p_in, int Kin, __bf16* X,
-                                  __bf16* XT) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= B * Kin) return;
-  const int b = i / Kin, k = i - b * Kin;
-  const __bf16 v = (__bf16)(k < p_in ? x[(size_t)b * p_in + k] : 0.0f);
-  X[i] = v;
-  XT[(size_t)k * B + b] = v;
-}
-
-// Gaussian negative log-likelihood per sample and its gradient with respect
-// to the outputs (mu in columns 0..p-1, log-variance in p..2p-1, clamped to
-// [-12, 12] with zero gradient outside), scaled by 1/B.
-__global__ void nll_kernel(const float* O, const float* Y, int B, int p, int Nout, float* loss,
-                           __bf16* dO, __bf16* dOT) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= B) return;
+  {  // outputs O = H_2 W_3 + b_3
+    const float* bias = g.bias[kMlpLayers - 1];
+    floatx4 acc[kSliceTM][N3W];
+    slice_product<N3W>(in, ldin, g.Wt[kMlpLayers - 1], g.K[kMlpLayers - 1], acc);
+#pragma unroll
+    for (int i = 0; i < kSliceTM; ++i)
+#pragma unroll
+      for (int j = 0; j < N3W; ++j) {
+        const int col = 16 * w + 64 * j + r;
+        const float bc = bias[col];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = 16 * i + 4 * q + v;
+          const float o = acc[i][j][v] + bc;
+          Os[row * N3 + col] = o;
+          g.O[(size_t)(b0 + row) * N3 + col] = o;
+        }
+      }
+  }
+  if (!bwd) return;
+  __syncthreads();
+  // Gaussian negative log-likelihood (mu in columns 0..p-1, log-variance in
+  // p..2p-1, clamped to [-12, 12] with zero gradient outside) and its
+  // gradient with respect to the outputs, scaled by 1/B; one (row, column)
+  // per thread, the row's loss terms added in column order below
+  // the loss terms take the two activation buffers (H_1, H_2 are stored and
+  // consumed; kSlice p floats fit in them as p <= width + 8)
+  float* terms = reinterpret_cast<float*>(Ha);
   const float invb = 1.0f / (float)B;
-  float L = 0.0f;
-  for (int a = 0; a < Nout; ++a) {
+  for (int it = tid; it < kSlice * N3; it += kMlpThreads) {
+    const int row = it / N3, a = it - row * N3;
+    const float* yr = g.y + (size_t)idx[row] * p;
     float gv = 0.0f;
     if (a < p) {
-      const float mu = O[(size_t)b * Nout + a];
-      const float lvr = O[(size_t)b * Nout + p + a];
+      const float mu = Os[row * N3 + a];
+      const float lvr = Os[row * N3 + p + a];
       const float lv = fminf(fmaxf(lvr, -12.0f), 12.0f);
       const float e = expf(-lv);
-      const float res = Y[(size_t)b * p + a] - mu;
-      L += 0.5f * (res * res * e + lv);
+      const float res = yr[a] - mu;
+      terms[row * p + a] = 0.5f * (res * res * e + lv);
       gv = -res * e * invb;
     } else if (a < 2 * p) {
       const int c = a - p;
-      const float mu = O[(size_t)b * Nout + c];
-      const float lvr = O[(size_t)b * Nout + a];
+      const float mu = Os[row * N3 + c];
+      const float lvr = Os[row * N3 + a];
       const float lv = fminf(fmaxf(lvr, -12.0f), 12.0f);
-      const float res = Y[(size_t)b * p + c] - mu;
+      const float res = yr[c] - mu;
       gv = (lvr > -12.0f && lvr < 12.0f) ? 0.5f * (1.0f - res * res * expf(-lv)) * invb : 0.0f;
     }
-    const __bf16 hv = (__bf16)gv;
-    dO[(size_t)b * Nout + a] = hv;
-    dOT[(size_t)a * B + b] = hv;
+    dOs[row * L.ldo + a] = (__bf16)gv;
   }
-  loss[b] = L;
-}
-
-// out[n] = sum_b XT[n][b] (bias gradients from the transposed bfloat16
-// gradients), one workgroup per n, fixed tree order.
-__global__ void __launch_bounds__(kMlpThreads) rowsum_kernel(const __bf16* XT, int B, float* out) {
-  __shared__ float sh[kMlpThreads];
-  const int n = blockIdx.x;
-  float v = 0.0f;
-  for (int b = threadIdx.x; b < B; b += kMlpThreads) v += (float)XT[(size_t)n * B + b];
-  sh[threadIdx.x] = v;
   __syncthreads();
-  for (int o = kMlpThreads / 2; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-    __syncthreads();
+  if (tid < kSlice) {
+    float s = 0.0f;
+    for (int a = 0; a < p; ++a) s += terms[tid * p + a];
+    g.loss[b0 + tid] = s;
   }
-  if (threadIdx.x == 0) out[n] = sh[0];
+  store_transposed(dOs, L.ldo, N3, g.GT[kMlpLayers - 1], B, b0);
+  __syncthreads();
+  // G_{l-1} = bf16((G_l W_l^T) * SiLU'(z_{l-1})), l = 3, 2, 1; the same lane
+  // wrote SiLU'(z) at each position in the forward pass
+  const __bf16* gin = dOs;
+  int ldg = L.ldo;
+#pragma unroll 1
+  for (int layer = kMlpLayers - 1; layer >= 1; --layer) {
+    __bf16* out = (layer & 1) ? Hb : Ha;
+    const float* Dl = D + (layer - 1) * kSlice * L.ldd;
+    floatx4 acc[kSliceTM][NWW];
+    slice_product<NWW>(gin, ldg, g.W[layer], g.N[layer], acc);
+#pragma unroll
+    for (int i = 0; i < kSliceTM; ++i)
+#pragma unroll
+      for (int j = 0; j < NWW; ++j) {
+        const int col = 16 * w + 64 * j + r;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = 16 * i + 4 * q + v;
+          out[row * L.ldh + col] = (__bf16)(acc[i][j][v] * Dl[row * L.ldd + col]);
+        }
+      }
+    __syncthreads();
+    store_transposed(out, L.ldh, width, g.GT[layer - 1], B, b0);
+    gin = out;
+    ldg = L.ldh;
+  }
 }
 
-// Where the weights of each layer sit in the flat parameter vector and their
-// bfloat16 copies W [K][N] and W^T [N][K].
-struct CastMap {
-  size_t off[kMlpLayers];
-  int K[kMlpLayers], N[kMlpLayers];
-  __bf16* Wb[kMlpLayers];
-  __bf16* Wtb[kMlpLayers];
+struct WgradArgs {
+  const __bf16* AT[kMlpLayers];  // layer inputs transposed: X^T, H_0^T, H_1^T, H_2^T (K_l x B)
+  const __bf16* GT[kMlpLayers];  // N_l x B
+  int K[kMlpLayers], N[kMlpLayers], B;
+  int first[kMlpLayers + 1];     // first workgroup of each layer's 64 x 64 tiles
+  float *params, *m, *v, *grads;
+  size_t off_w[kMlpLayers], off_b[kMlpLayers];
+  __bf16* Wb[kMlpLayers];        // W_l [K][N]
+  __bf16* Wtb[kMlpLayers];       // W_l^T [N][K]
+  int adam;                      // 0: gradients only
+  float lr, b1, b2, eps, c1, c2; // Adam: bias corrections c1 = 1/(1-b1^t), c2 = 1/(1-b2^t)
 };
 
-// Adam (bias-corrected): w -= lr * m_hat / (sqrt(v_hat) + eps); a weight's
-// new value is also written to its two bfloat16 copies (as cast_kernel).
-__global__ void adam_kernel(float* w, float* m, float* v, const float* g, size_t n, float lr,
-                            float b1, float b2, float eps, float c1, float c2, const CastMap cm) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float gi = g[i];
-  const float mi = b1 * m[i] + (1.0f - b1) * gi;
-  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
-  m[i] = mi;
-  v[i] = vi;
-  const float wi = w[i] - lr * (mi * c1) / (sqrtf(vi * c2) + eps);
-  w[i] = wi;
+constexpr int kRedLd = 65;  // row stride of the partial tiles (floats)
+constexpr size_t kWgradLds = (size_t)4 * 64 * kRedLd * 4 + 64 * 72 * 2 + 4 * 64 * 4;
+
+__device__ __forceinline__ void adam_update(const WgradArgs& g, size_t ix, float gi, float& wi) {
+  const float mi = g.b1 * g.m[ix] + (1.0f - g.b1) * gi;
+  const float vi = g.b2 * g.v[ix] + (1.0f - g.b2) * gi * gi;
+  g.m[ix] = mi;
+  g.v[ix] = vi;
+  wi = g.params[ix] - g.lr * (mi * g.c1) / (sqrtf(vi * g.c2) + g.eps);
+  g.params[ix] = wi;
+}
+
+__global__ void __launch_bounds__(kMlpThreads) mlp_wgrad_adam_kernel(const WgradArgs g) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  float* red = reinterpret_cast<float*>(smem);                           // [4][64][kRedLd]
+  __bf16* tile = reinterpret_cast<__bf16*>(smem + (size_t)4 * 64 * kRedLd * 4);  // [64][72]
+  float* bpart = reinterpret_cast<float*>(smem + (size_t)4 * 64 * kRedLd * 4 + 64 * 72 * 2);
+  const int bid = blockIdx.x;
+  int layer = 0;
+  while (layer < kMlpLayers - 1 && bid >= g.first[layer + 1]) ++layer;
+  const int N = g.N[layer], K = g.K[layer], B = g.B;
+  const int t = bid - g.first[layer], ntn = N / 64;
+  const int k0 = 64 * (t / ntn), n0 = 64 * (t - (t / ntn) * ntn);
+  const int tid = threadIdx.x, l = tid & 63, w = tid >> 6, r = l & 15, q = l >> 4;
+  const __bf16* A = g.AT[layer];
+  const __bf16* G = g.GT[layer];
+  floatx4 acc[4][4];
 #pragma unroll
-  for (int l = 0; l < kMlpLayers; ++l) {
-    const size_t e = i - cm.off[l];
-    if (i >= cm.off[l] && e < (size_t)cm.K[l] * cm.N[l]) {
-      const int k = (int)(e / cm.N[l]), c = (int)(e - (size_t)k * cm.N[l]);
-      const __bf16 hv = (__bf16)wi;
-      cm.Wb[l][e] = hv;
-      cm.Wtb[l][(size_t)c * cm.K[l] + k] = hv;
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+  const int S = B / 32;  // K steps; wave w takes steps w, w + 4, ...
+  for (int s = w; s < S; s += 8) {
+    const bool two = s + 4 < S;
+    bf16x8 a[2][4], b[2][4];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int su = (u == 0 || two) ? s + 4 * u : s;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        a[u][i] = *reinterpret_cast<const bf16x8*>(A + (size_t)(k0 + 16 * i + r) * B + 32 * su +
+                                                   8 * q);
+        b[u][i] = *reinterpret_cast<const bf16x8*>(G + (size_t)(n0 + 16 * i + r) * B + 32 * su +
+                                                   8 * q);
+      }
     }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[0][j], acc[i][j], 0, 0, 0);
+    if (two) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          acc[i][j] =
+              __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][i], b[1][j], acc[i][j], 0, 0, 0);
+    }
+  }
+  float* mine = red + (size_t)w * 64 * kRedLd;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) mine[(16 * i + 4 * q + v) * kRedLd + 16 * j + r] = acc[i][j][v];
+  // bias gradients of the tile's columns (first row of tiles): column c =
+  // tid % 64, quarter w of the batch, added in quarter order below
+  const bool has_bias = k0 == 0;
+  if (has_bias) {
+    const __bf16* gr = G + (size_t)(n0 + l) * B + (size_t)w * (B / 4);
+    float sb = 0.0f;
+    for (int b8 = 0; b8 < B / 4; b8 += 8) {
+      const bf16x8 v8 = *reinterpret_cast<const bf16x8*>(gr + b8);
+#pragma unroll
+      for (int e = 0; e < 8; ++e) sb += (float)v8[e];
+    }
+    bpart[w * 64 + l] = sb;
+  }
+  __syncthreads();
+  for (int e = tid; e < 64 * 64; e += kMlpThreads) {
+    const int row = e >> 6, col = e & 63;
+    const float gs = ((red[row * kRedLd + col] + red[64 * kRedLd + row * kRedLd + col]) +
+                      red[2 * 64 * kRedLd + row * kRedLd + col]) +
+                     red[3 * 64 * kRedLd + row * kRedLd + col];
+    const size_t ix = g.off_w[layer] + (size_t)(k0 + row) * N + n0 + col;
+    g.grads[ix] = gs;
+    if (g.adam) {
+      float wi;
+      adam_update(g, ix, gs, wi);
+      tile[row * 72 + col] = (__bf16)wi;
+    }
+  }
+  if (has_bias && tid < 64) {
+    const float gb = ((bpart[tid] + bpart[64 + tid]) + bpart[128 + tid]) + bpart[192 + tid];
+    const size_t ix = g.off_b[layer] + n0 + tid;
+    g.grads[ix] = gb;
+    if (g.adam) {
+      float wi;
+      adam_update(g, ix, gb, wi);
+    }
+  }
+  if (!g.adam) return;
+  __syncthreads();
+  const int a = tid >> 2, c16 = 16 * (tid & 3);
+  {  // row a of the tile into W [K][N]
+    const bf16x8* src = reinterpret_cast<const bf16x8*>(&tile[a * 72 + c16]);
+    bf16x8* dst = reinterpret_cast<bf16x8*>(g.Wb[layer] + (size_t)(k0 + a) * N + n0 + c16);
+    dst[0] = src[0];
+    dst[1] = src[1];
+  }
+  {  // column a of the tile as a row of W^T [N][K]
+    bf16x8 lo, hi;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      lo[e] = tile[(c16 + e) * 72 + a];
+      hi[e] = tile[(c16 + 8 + e) * 72 + a];
+    }
+    bf16x8* dst = reinterpret_cast<bf16x8*>(g.Wtb[layer] + (size_t)(n0 + a) * K + k0 + c16);
+    dst[0] = lo;
+    dst[1] = hi;
   }
 }
 
@@ -249,35 +453,41 @@ __global__ void cast_kernel(const float* W, int K, int N, __bf16* Wb, __bf16* Wt
   Wtb[(size_t)n * K + k] = v;
 }
 
-__device__ __forceinline__ uint64_t mix64(uint64_t x) {
-  x ^= x >> 33;
-  x *= 0xFF51AFD7ED558CCDull;
-  x ^= x >> 33;
-  x *= 0xC4CEB9FE1A85EC53ull;
-  return x ^ (x >> 33);
-}
-
-// Minibatch by sampling with replacement: row b of step s is sample
-// mix64(seed + s * B + b) mod n.
-__global__ void gather_kernel(const float* x_all, const float* y_all, long long n, int p_in,
-                              int p_out, int B, uint64_t seed, long long step, float* xb,
-                              float* yb) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= B) return;
-  const uint64_t h = mix64(seed + (uint64_t)step * (uint64_t)B + (uint64_t)b);
-  const long long idx = (long long)(h % (uint64_t)n);
-  for (int a = 0; a < p_in; ++a) xb[(size_t)b * p_in + a] = x_all[(size_t)idx * p_in + a];
-  for (int a = 0; a < p_out; ++a) yb[(size_t)b * p_out + a] = y_all[(size_t)idx * p_out + a];
-}
-
 int pad64(int x) { return (x + 63) / 64 * 64; }
+
+typedef void (*SliceKernel)(const SliceArgs);
+
+template <int NWW, int N3W>
+SliceKernel slice_instance() {
+  return mlp_slice_kernel<NWW, N3W>;
+}
+
+// The slice kernel for width / 64 and N_3 / 64, its LDS limit raised once.
+SliceKernel slice_kernel(int nww, int n3w, size_t lds) {
+  static const SliceKernel table[4][2] = {
+      {slice_instance<1, 1>(), slice_instance<1, 2>()},
+      {slice_instance<2, 1>(), slice_instance<2, 2>()},
+      {slice_instance<3, 1>(), slice_instance<3, 2>()},
+      {slice_instance<4, 1>(), slice_instance<4, 2>()}};
+  if (nww < 1 || nww > 4 || n3w < 1 || n3w > 2) return nullptr;
+  SliceKernel k = table[nww - 1][n3w - 1];
+  static bool raised[4][2] = {};
+  if (!raised[nww - 1][n3w - 1]) {
+    if (hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds) != hipSuccess)
+      return nullptr;
+    raised[nww - 1][n3w - 1] = true;
+  }
+  return k;
+}
 
 }  // namespace
 
 cusna_status Mlp::init(int p_in, int p_out, int width, int batch, uint64_t seed, hipStream_t s) {
   if (p_in < 1 || p_in > 64 || p_out < 1 || p_out > 64 || width < 64 || width % 64 ||
-      width > 1024 || batch < 64 || batch % 64) {
-    set_error("mlp: p_in, p_out in 1..64, width and batch multiples of 64");
+      width > kMaxWidth || batch < 64 || batch % 64) {
+    set_error("mlp: p_in, p_out in 1..64, width a multiple of 64 up to 256, batch a multiple "
+              "of 64");
     return CUSNA_ERR_SHAPE;
   }
   p_in_ = p_in;
@@ -300,32 +510,16 @@ cusna_status Mlp::init(int p_in, int p_out, int width, int batch, uint64_t seed,
   n_params_ = o;
   const size_t P4 = n_params_ * 4;
   const size_t o_p = D_.add(P4), o_m = D_.add(P4), o_v = D_.add(P4), o_g = D_.add(P4);
-  size_t o_wb[kMlpLayers], o_wtb[kMlpLayers];
+  size_t o_wb[kMlpLayers], o_wtb[kMlpLayers], o_gt[kMlpLayers], o_ht[kMlpLayers - 1];
   for (int l = 0; l < kMlpLayers; ++l) {
     o_wb[l] = D_.add((size_t)K_[l] * N_[l] * 2);
     o_wtb[l] = D_.add((size_t)K_[l] * N_[l] * 2);
+    o_gt[l] = D_.add((size_t)N_[l] * B_ * 2);
   }
-  const int B = B_;
-  const size_t o_x = D_.add((size_t)B * K_[0] * 2), o_xt = D_.add((size_t)B * K_[0] * 2);
-  size_t o_z[3], o_h[3], o_ht[3], o_dz[3], o_dzt[3];
-  for (int l = 0; l < 3; ++l) {
-    o_z[l] = D_.add((size_t)B * N_[l] * 4);
-    o_h[l] = D_.add((size_t)B * N_[l] * 2);
-    o_ht[l] = D_.add((size_t)B * N_[l] * 2);
-    o_dz[l] = D_.add((size_t)B * N_[l] * 2);
-    o_dzt[l] = D_.add((size_t)B * N_[l] * 2);
-  }
-  const size_t o_o = D_.add((size_t)B * N_[3] * 4), o_do = D_.add((size_t)B * N_[3] * 2);
-  const size_t o_dot = D_.add((size_t)B * N_[3] * 2), o_loss = D_.add((size_t)B * 4);
-  const size_t o_yb = D_.add((size_t)B * p_out * 4), o_xb = D_.add((size_t)B * p_in * 4);
-  // partial blocks of the split weight-gradient products (K = B)
-  part_floats_ = 0;
-  if (B >= 512 && B % 128 == 0)
-    for (int l = 0; l < kMlpLayers; ++l)
-      part_floats_ = std::max(part_floats_, (size_t)(B / 128) * K_[l] * N_[l]);
-  const size_t o_part = D_.add(part_floats_ * 4);
+  for (int l = 0; l < kMlpLayers - 1; ++l) o_ht[l] = D_.add((size_t)N_[l] * B_ * 2);
+  const size_t o_xt = D_.add((size_t)K_[0] * B_ * 2);
+  const size_t o_o = D_.add((size_t)B_ * N_[3] * 4), o_loss = D_.add((size_t)B_ * 4);
   HIP_TRY(D_.alloc());
-  part_ = D_.at<float>(o_part);
   params_ = D_.at<float>(o_p);
   m_ = D_.at<float>(o_m);
   v_ = D_.at<float>(o_v);
@@ -333,23 +527,12 @@ cusna_status Mlp::init(int p_in, int p_out, int width, int batch, uint64_t seed,
   for (int l = 0; l < kMlpLayers; ++l) {
     Wb_[l] = D_.at<__bf16>(o_wb[l]);
     Wtb_[l] = D_.at<__bf16>(o_wtb[l]);
+    GT_[l] = D_.at<__bf16>(o_gt[l]);
   }
-  X_ = D_.at<__bf16>(o_x);
+  for (int l = 0; l < kMlpLayers - 1; ++l) HT_[l] = D_.at<__bf16>(o_ht[l]);
   XT_ = D_.at<__bf16>(o_xt);
-  for (int l = 0; l < 3; ++l) {
-    Z_[l] = D_.at<float>(o_z[l]);
-    H_[l] = D_.at<__bf16>(o_h[l]);
-    HT_[l] = D_.at<__bf16>(o_ht[l]);
-    dZ_[l] = D_.at<__bf16>(o_dz[l]);
-    dZT_[l] = D_.at<__bf16>(o_dzt[l]);
-  }
   O_ = D_.at<float>(o_o);
-  dO_ = D_.at<__bf16>(o_do);
-  dOT_ = D_.at<__bf16>(o_dot);
   loss_ = D_.at<float>(o_loss);
-  Yb_ = D_.at<float>(o_yb);
-  Xb_ = D_.at<float>(o_xb);
-
   return reset(seed);
 }
 
@@ -406,116 +589,102 @@ cusna_status Mlp::cast_weights() {
   return CUSNA_OK;
 }
 
-cusna_status Mlp::gemm(const __bf16* A, const __bf16* Bt, int M, int N, int K, const float* bias,
-                       const float* dz, int act, float* Cf, __bf16* Cb, __bf16* CbT) {
-  if (M % 64 || N % 64 || K % 32) {
-    set_error("mlp: product shape not a multiple of the 64 x 64 x 32 tile");
+cusna_status Mlp::slice(int mode, const float* x, const float* y, int64_t n, uint64_t seed,
+                        long long step) {
+  const SliceLds L = slice_lds(K_[0], width_, N_[3]);
+  SliceKernel k = slice_kernel(width_ / 64, N_[3] / 64, (size_t)L.total);
+  if (!k || (size_t)L.total > 160 * 1024) {
+    set_error("mlp: no fused kernel for this shape");
     return CUSNA_ERR_INTERNAL;
   }
-  // split K (128 per split) for long plain float32 products
-  const bool plain = Cf && !bias && !dz && !act && !Cb && !CbT;
-  const int S = (plain && K >= 512 && K % 128 == 0) ? K / 128 : 1;
-  if (S > 1 && (size_t)S * M * N > part_floats_) {
-    set_error("mlp: split-product scratch too small");
-    return CUSNA_ERR_INTERNAL;
+  SliceArgs a;
+  for (int l = 0; l < kMlpLayers; ++l) {
+    a.Wt[l] = Wtb_[l];
+    a.W[l] = Wb_[l];
+    a.bias[l] = params_ + off_b_[l];
+    a.K[l] = K_[l];
+    a.N[l] = N_[l];
+    a.GT[l] = GT_[l];
   }
-  GemmArgs g{A, Bt, M, N, K, bias, dz, act, Cf, Cb, CbT, K / S, part_};
-  const dim3 grid(M / 64, N / 64, S);
-  if (g.Ks % 256 == 0)
-    hipLaunchKernelGGL(gemm_bf16_kernel<8>, grid, dim3(kMlpThreads), 0, s_, g);
-  else if (g.Ks % 128 == 0)
-    hipLaunchKernelGGL(gemm_bf16_kernel<4>, grid, dim3(kMlpThreads), 0, s_, g);
-  else if (g.Ks % 64 == 0)
-    hipLaunchKernelGGL(gemm_bf16_kernel<2>, grid, dim3(kMlpThreads), 0, s_, g);
-  else
-    hipLaunchKernelGGL(gemm_bf16_kernel<1>, grid, dim3(kMlpThreads), 0, s_, g);
-  if (S > 1)
-    hipLaunchKernelGGL(sum_parts_kernel, dim3((M * N + 255) / 256), dim3(256), 0, s_, part_, S,
-                       M * N, Cf);
+  for (int l = 0; l < kMlpLayers - 1; ++l) a.HT[l] = HT_[l];
+  a.p_in = p_in_;
+  a.p_out = p_out_;
+  a.B = B_;
+  a.mode = mode;
+  a.x = x;
+  a.y = y;
+  a.n = (long long)n;
+  a.seed = seed;
+  a.step = step;
+  a.O = O_;
+  a.loss = loss_;
+  a.XT = XT_;
+  hipLaunchKernelGGL(k, dim3(B_ / kSlice), dim3(kMlpThreads), (size_t)L.total, s_, a);
+  HIP_TRY(hipGetLastError());
+  return CUSNA_OK;
+}
+
+cusna_status Mlp::wgrad(bool adam, float lr) {
+  static bool raised = false;
+  if (!raised) {
+    HIP_TRY(hipFuncSetAttribute((const void*)mlp_wgrad_adam_kernel,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kWgradLds));
+    raised = true;
+  }
+  WgradArgs a;
+  int tiles = 0;
+  for (int l = 0; l < kMlpLayers; ++l) {
+    a.AT[l] = l == 0 ? XT_ : HT_[l - 1];
+    a.GT[l] = GT_[l];
+    a.K[l] = K_[l];
+    a.N[l] = N_[l];
+    a.first[l] = tiles;
+    tiles += (K_[l] / 64) * (N_[l] / 64);
+    a.off_w[l] = off_w_[l];
+    a.off_b[l] = off_b_[l];
+    a.Wb[l] = Wb_[l];
+    a.Wtb[l] = Wtb_[l];
+  }
+  a.first[kMlpLayers] = tiles;
+  a.B = B_;
+  a.params = params_;
+  a.m = m_;
+  a.v = v_;
+  a.grads = grads_;
+  a.adam = adam ? 1 : 0;
+  a.b1 = 0.9f;
+  a.b2 = 0.999f;
+  a.eps = 1e-8f;
+  a.lr = lr;
+  a.c1 = a.c2 = 1.0f;
+  if (adam) {
+    ++t_;
+    a.c1 = (float)(1.0 / (1.0 - std::pow(0.9, (double)t_)));
+    a.c2 = (float)(1.0 / (1.0 - std::pow(0.999, (double)t_)));
+  }
+  hipLaunchKernelGGL(mlp_wgrad_adam_kernel, dim3(tiles), dim3(kMlpThreads), kWgradLds, s_, a);
   HIP_TRY(hipGetLastError());
   return CUSNA_OK;
 }
 
 cusna_status Mlp::forward(const float* x_dev) {
-  const int B = B_;
-  hipLaunchKernelGGL(prep_input_kernel, dim3((B * K_[0] + 255) / 256), dim3(256), 0, s_, x_dev,
-                     B, p_in_, K_[0], X_, XT_);
-  cusna_status st;
-  const __bf16* in = X_;
-  for (int l = 0; l < kMlpLayers - 1; ++l) {
-    st = gemm(in, Wtb_[l], B, N_[l], K_[l], params_ + off_b_[l], nullptr, 1, Z_[l], H_[l],
-              HT_[l]);
-    if (st != CUSNA_OK) return st;
-    in = H_[l];
-  }
-  return gemm(in, Wtb_[3], B, N_[3], K_[3], params_ + off_b_[3], nullptr, 0, O_, nullptr,
-              nullptr);
-}
-
-cusna_status Mlp::loss_and_grad(const float* y_dev) {
-  hipLaunchKernelGGL(nll_kernel, dim3((B_ + 63) / 64), dim3(64), 0, s_, O_, y_dev, B_, p_out_,
-                     N_[3], loss_, dO_, dOT_);
-  HIP_TRY(hipGetLastError());
-  return CUSNA_OK;
+  return slice(0, x_dev, nullptr, B_, 0, 0);
 }
 
 cusna_status Mlp::forward_backward(const float* x_dev, const float* y_dev) {
-  cusna_status st = forward(x_dev);
+  cusna_status st = slice(1, x_dev, y_dev, B_, 0, 0);
   if (st != CUSNA_OK) return st;
-  if ((st = loss_and_grad(y_dev)) != CUSNA_OK) return st;
-  const int B = B_;
-  // layer l gradient G_l (B x N_l): dO for the output layer, dZ below
-  const __bf16* G = dO_;
-  const __bf16* GT = dOT_;
-  for (int l = kMlpLayers - 1; l >= 0; --l) {
-    const __bf16* inT = l == 0 ? XT_ : HT_[l - 1];
-    // dW_l = H_{l-1}^T G_l  (K_l x N_l), reduction over the batch
-    st = gemm(inT, GT, K_[l], N_[l], B, nullptr, nullptr, 0, grads_ + off_w_[l], nullptr,
-              nullptr);
-    if (st != CUSNA_OK) return st;
-    hipLaunchKernelGGL(rowsum_kernel, dim3(N_[l]), dim3(kMlpThreads), 0, s_, GT, B,
-                       grads_ + off_b_[l]);
-    if (l == 0) break;
-    // G_{l-1} = (G_l W_l^T) * SiLU'(Z_{l-1})
-    st = gemm(G, Wb_[l], B, K_[l], N_[l], nullptr, Z_[l - 1], 0, nullptr, dZ_[l - 1],
-              dZT_[l - 1]);
-    if (st != CUSNA_OK) return st;
-    G = dZ_[l - 1];
-    GT = dZT_[l - 1];
-  }
-  HIP_TRY(hipGetLastError());
-  return CUSNA_OK;
-}
-
-cusna_status Mlp::adam_step(float lr) {
-  ++t_;
-  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-  const float c1 = (float)(1.0 / (1.0 - std::pow(0.9, (double)t_)));
-  const float c2 = (float)(1.0 / (1.0 - std::pow(0.999, (double)t_)));
-  CastMap cm;
-  for (int l = 0; l < kMlpLayers; ++l) {
-    cm.off[l] = off_w_[l];
-    cm.K[l] = K_[l];
-    cm.N[l] = N_[l];
-    cm.Wb[l] = Wb_[l];
-    cm.Wtb[l] = Wtb_[l];
-  }
-  hipLaunchKernelGGL(adam_kernel, dim3((unsigned)((n_params_ + 255) / 256)), dim3(256), 0, s_,
-                     params_, m_, v_, grads_, n_params_, lr, b1, b2, eps, c1, c2, cm);
-  HIP_TRY(hipGetLastError());
-  return CUSNA_OK;
+  return wgrad(false, 0.0f);
 }
 
 cusna_status Mlp::train(const float* x_all, const float* y_all, int64_t n, int steps, float lr,
                         uint64_t seed) {
   if (n < 1) return CUSNA_ERR_SHAPE;
   for (int sidx = 0; sidx < steps; ++sidx) {
-    hipLaunchKernelGGL(gather_kernel, dim3((B_ + 63) / 64), dim3(64), 0, s_, x_all, y_all,
-                       (long long)n, p_in_, p_out_, B_, seed, (long long)sidx, Xb_, Yb_);
-    cusna_status st = forward_backward(Xb_, Yb_);
+    cusna_status st = slice(2, x_all, y_all, n, seed, sidx);
     if (st != CUSNA_OK) return st;
     const float lr_s = lr * (1.0f - 0.9f * (float)sidx / (float)(steps > 1 ? steps - 1 : 1));
-    if ((st = adam_step(lr_s)) != CUSNA_OK) return st;
+    if ((st = wgrad(true, lr_s)) != CUSNA_OK) return st;
   }
   return CUSNA_OK;
 }

@@ -227,9 +227,14 @@ struct GofParams {
   const uint64_t *keep, *force;  // n*W64 or null
   const int32_t* levls;          // nlev (device)
   double* out;                   // M*(nlev + (kind == 2))
+  // geodesic on rows read from device memory (gof_geodesic_lds() == 0 or
+  // stream set): per-network reach counters, M*(kmax + 1), cleared by the launch
+  unsigned long long* counters;
+  int stream;                    // 1: the device-memory form even where the LDS one fits
 };
 int launch_gof(const GofParams& P, void* stream);
-// LDS bytes of the geodesic kernel (0: the network is too large for it).
+// LDS bytes of the geodesic kernel (0: the network is too large for it and
+// the rows are read from device memory).
 size_t gof_geodesic_lds(int n, int W64, int kmax);
 
 // Cross-network statistic of M packed networks against a predictor network

@@ -739,6 +739,114 @@ __global__ void __launch_bounds__(kStatThreads) gof_geodesic_kernel(const GofPar
   }
 }
 
+// Geodesic distribution of networks whose rows exceed the LDS: one wave per
+// (network, source actor). The reached set and the frontier are bit rows
+// spread over the wave's lanes (word w in lane w % 64, slot w / 64, WPL slots:
+// W64 <= 64 WPL); the frontier's actors are walked in order from scalar
+// copies of its words, and the rows of four of them at a time are read from
+// device memory (masked as gof_word), each lane ORing its words. The same
+// reached/frontier recurrence as gof_geodesic_kernel; newly reached counts go
+// to the network's counters (integer atomics: order-free).
+template <int WPL>
+__global__ void __launch_bounds__(256) gof_geodesic_stream_kernel(const GofParams P) {
+  const int lane = threadIdx.x & 63;
+  const long long gw = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int n = P.n, W64 = P.W64, kmax = P.kmax;
+  if (gw >= (long long)P.M * n) return;  // whole waves
+  const int net = (int)(gw / n), i = (int)(gw - (long long)net * n);
+  const uint64_t* src = P.rows + (size_t)net * n * W64;
+  unsigned long long* delta = P.counters + (size_t)net * (kmax + 1);
+  const int iw = i >> 6;
+  const uint64_t ib = 1ull << (i & 63);
+  uint64_t R[WPL], F[WPL];
+  int cnt = 0;
+#pragma unroll
+  for (int q = 0; q < WPL; ++q) {
+    const int w = lane + 64 * q;
+    R[q] = w < W64 ? gof_word(P, src, i, w) : 0ull;
+    F[q] = R[q];
+    cnt += __popcll(w == iw ? (R[q] & ~ib) : R[q]);
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
+  if (kmax >= 1 && cnt && lane == 0) atomicAdd(&delta[1], (unsigned long long)cnt);
+  for (int k = 2; k <= kmax; ++k) {
+    uint64_t N[WPL];
+#pragma unroll
+    for (int q = 0; q < WPL; ++q) N[q] = 0ull;
+#pragma unroll
+    for (int q = 0; q < WPL; ++q) {
+      uint64_t lanes = __ballot(F[q] != 0ull);  // lanes holding frontier words in slot q
+      while (lanes) {
+        const int ln = __ffsll((long long)lanes) - 1;
+        lanes &= lanes - 1;
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)F[q], ln);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(F[q] >> 32), ln);
+        uint64_t word = ((uint64_t)hi << 32) | lo;
+        const int base = (ln + 64 * q) * 64;
+        while (word) {
+          int h[4];
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            h[t] = word ? base + __ffsll((long long)word) - 1 : -1;
+            word &= word ? word - 1 : 0ull;
+          }
+          uint64_t got[4][WPL];
+#pragma unroll
+          for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int q2 = 0; q2 < WPL; ++q2) {
+              const int w2 = lane + 64 * q2;
+              got[t][q2] = (h[t] >= 0 && w2 < W64) ? gof_word(P, src, h[t], w2) : 0ull;
+            }
+#pragma unroll
+          for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int q2 = 0; q2 < WPL; ++q2) N[q2] |= got[t][q2];
+        }
+      }
+    }
+    int grow = 0, any = 0;
+#pragma unroll
+    for (int q = 0; q < WPL; ++q) {
+      const int w = lane + 64 * q;
+      uint64_t rn = R[q] | N[q];
+      if (w == iw) rn &= ~ib;
+      const uint64_t old = w == iw ? (R[q] & ~ib) : R[q];
+      grow += __popcll(rn) - __popcll(old);
+      F[q] = rn & ~R[q];
+      any |= F[q] != 0ull;
+      R[q] = rn;
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) grow += __shfl_xor(grow, o, 64);
+    if (grow && lane == 0) atomicAdd(&delta[k], (unsigned long long)grow);
+    if (__ballot(any) == 0ull) break;
+  }
+}
+
+// Clears the reach counters of the device-memory geodesic form.
+__global__ void gof_counters_clear_kernel(unsigned long long* c, size_t len) {
+  const size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x < len) c[x] = 0ull;
+}
+
+// counts[k] = newly reached pairs up to distance k; outputs as the LDS form.
+__global__ void gof_geodesic_finish_kernel(const GofParams P) {
+  const int net = blockIdx.x * blockDim.x + threadIdx.x;
+  if (net >= P.M) return;
+  const int kmax = P.kmax;
+  unsigned long long* delta = P.counters + (size_t)net * (kmax + 1);
+  for (int k = 2; k <= kmax; ++k) delta[k] += delta[k - 1];
+  double* out = P.out + (size_t)net * (P.nlev + 1);
+  for (int L = 0; L < P.nlev; ++L) {
+    const int lev = P.levls[L] < kmax ? P.levls[L] : kmax;
+    out[L] = lev >= 1 ? (double)delta[lev] : 0.0;
+  }
+  const unsigned long long reach = kmax >= 1 ? delta[kmax] : 0ull;
+  out[P.nlev] = (double)((long long)P.n * (P.n - 1) - (long long)reach);
+}
+
 size_t gof_geodesic_lds(int n, int W64, int kmax) {
   if (W64 > 16) return 0;
   const size_t b = (size_t)n * W64 * 8 + (size_t)(kmax + 1) * 8;
@@ -753,7 +861,20 @@ int launch_gof(const GofParams& P, void* stream) {
     return (int)hipGetLastError();
   }
   const size_t lds = gof_geodesic_lds(P.n, P.W64, P.kmax);
-  if (lds == 0) return (int)hipErrorInvalidValue;
+  if (lds == 0 || P.stream) {
+    if (!P.counters || P.W64 > 128) return (int)hipErrorInvalidValue;
+    const size_t len = (size_t)P.M * (P.kmax + 1);
+    hipLaunchKernelGGL(gof_counters_clear_kernel, dim3((unsigned)((len + 255) / 256)), dim3(256),
+                       0, st, P.counters, len);
+    const long long waves = (long long)P.M * P.n;
+    const dim3 grid((unsigned)((waves + 3) / 4));
+    if (P.W64 <= 64)
+      hipLaunchKernelGGL(gof_geodesic_stream_kernel<1>, grid, dim3(256), 0, st, P);
+    else
+      hipLaunchKernelGGL(gof_geodesic_stream_kernel<2>, grid, dim3(256), 0, st, P);
+    hipLaunchKernelGGL(gof_geodesic_finish_kernel, dim3((P.M + 63) / 64), dim3(64), 0, st, P);
+    return (int)hipGetLastError();
+  }
   int wmax = 1;
   while (wmax < P.W64) wmax <<= 1;
   auto go = [&](const void* k, void (*kern)(const GofParams)) -> int {

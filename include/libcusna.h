@@ -343,8 +343,9 @@ CUSNA_API cusna_status cusna_behavior_stats(const int32_t *z_batch, int32_t n, i
  * libcusna.h:334-343, libcusna_saom_cpu.cpp:716-765). a_batch is M*n*n dense;
  * kind 0 indegree / 1 outdegree: out[c][L] = actors with degree <= levls[L]
  * (M*nlev); kind 2 geodesic: ordered pairs i != j at distance <= levls[L],
- * then the unreachable pairs (M*(nlev+1)); geodesic levels must be >= 0 and
- * n <= 1024 on the device. Status codes as the reference's. */
+ * then the unreachable pairs (M*(nlev+1)); geodesic levels must be >= 0.
+ * Any n <= CUSNA_MAX_N: networks whose rows exceed the LDS (n > ~1000) are
+ * walked from device memory. Status codes as the reference's. */
 CUSNA_API cusna_status cusna_gof_distribution(const uint8_t *a_batch, int32_t n, int32_t M,
                                               int32_t kind, const int32_t *levls, int32_t nlev,
                                               double *out);
@@ -581,7 +582,10 @@ typedef struct cusna_mom_result {
  * cores with float32 accumulation (weights and activations rounded to
  * bfloat16 as they enter a product), float32 master weights.
  *
- *   create        p_in, p_out in 1..64; width, batch multiples of 64
+ *   create        p_in, p_out in 1..64; width a multiple of 64 up to 256,
+ *                 batch a multiple of 64 (a training step is two launches:
+ *                 the batch in 32-row slices through the whole network, then
+ *                 the weight gradients with Adam per 64 x 64 weight tile)
  *   param_count   floats in the flat parameter vector: for each layer l,
  *                 W_l [K_l][N_l] row-major then b_l [N_l] (layer_shape);
  *                 K_0 = p_in and N_3 = 2 p_out padded to multiples of 64

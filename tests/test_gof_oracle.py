@@ -71,3 +71,14 @@ def test_gof_argument_checks(oracle_lib):
         oracle_lib.gof_distribution(a, 3, [1])
     with pytest.raises(RuntimeError, match="status 1"):
         oracle_lib.gof_distribution(a, 0, [])
+
+
+def test_shortest_path_checker_matches_oracle(oracle_lib):
+    """The breadth-first checker that the device tests use above the oracle's
+    sizes gives the oracle's counts (self-loops, unreached pairs, level 0)."""
+    from geodesic_check import geodesic_counts
+
+    a = _nets(3, 150, 1.3, 21, self_loops=True)
+    for lv in ([1, 2, 3, 5, 8], [0, 2], [4, 1, 4]):
+        np.testing.assert_array_equal(geodesic_counts(a, lv),
+                                      oracle_lib.gof_distribution(a, 2, lv))

@@ -5,7 +5,12 @@ The oracle follows libcusna_saom_cpu.cpp:716-765 and is itself checked
 against the compiled reference and against breadth-first search in
 test_gof_oracle.py. Cases cover the degree kinds and the geodesic kind at
 word bounds 1..16 (n up to 1024), ragged n, self-loops, unsorted and repeated
-levels, levels beyond n, level 0, and the reference's argument checks.
+levels, levels beyond n, level 0, and the reference's argument checks. The
+geodesic form that reads rows from device memory (networks above the LDS) is
+checked against the oracle where the oracle's boolean powers are affordable
+(forced at small n, and n = 1100 at two levels) and against breadth-first
+distances (tests/geodesic_check.py, itself checked against the oracle) up to
+n = 4200 (two words per lane).
 """
 import numpy as np
 import pytest
@@ -79,8 +84,8 @@ def test_empty_and_complete_networks(gpu_lib, oracle_lib):
 
 
 def test_argument_checks(gpu_lib):
-    """Status codes as libcusna_saom_cpu.cpp:719,764; the device geodesic
-    kernel holds the network in LDS (n <= 1024)."""
+    """Status codes as libcusna_saom_cpu.cpp:719,764; a geodesic call above
+    the LDS (n = 1100, empty network) succeeds: every pair unreached."""
     from cusna_amd.abi import CusnaError
 
     a = _nets(1, 6, 1.0, 1)
@@ -91,9 +96,44 @@ def test_argument_checks(gpu_lib):
     with pytest.raises(CusnaError, match="SHAPE|status 2"):
         gpu_lib.gof_distribution(a, 2, [1, -1])
     big = np.zeros((1, 1100, 1100), np.uint8)
-    with pytest.raises(CusnaError, match="UNSUPPORTED|status 5"):
-        gpu_lib.gof_distribution(big, 2, [1])
+    np.testing.assert_array_equal(gpu_lib.gof_distribution(big, 2, [1]), [[0, 1100 * 1099]])
     np.testing.assert_array_equal(gpu_lib.gof_distribution(big, 0, [0]), [[1100]])
+
+
+@pytest.mark.parametrize("n,deg", [(1, 0.0), (5, 1.0), (37, 1.5), (65, 2.0), (300, 1.5),
+                                   (600, 1.1)])
+def test_geodesic_device_memory_form_matches_oracle(n, deg, gpu_lib, oracle_lib, monkeypatch):
+    """The form that walks rows in device memory, forced where the LDS form
+    would run (CUSNA_GOF_STREAM=1), against the oracle bit for bit."""
+    monkeypatch.setenv("CUSNA_GOF_STREAM", "1")
+    M = 3 if n <= 300 else 1
+    a = _nets(M, n, deg, 11 * n + 1)
+    lv = GEO_LEVELS if n < 300 else [1, 2, 4, 8, 16]
+    print(f"case geodesic device-memory form n={n}", flush=True)
+    np.testing.assert_array_equal(gpu_lib.gof_distribution(a, 2, lv),
+                                  oracle_lib.gof_distribution(a, 2, lv))
+
+
+def test_geodesic_above_lds_matches_oracle(gpu_lib, oracle_lib):
+    """n = 1100 (18 words, rows beyond the LDS): against the oracle's boolean
+    powers at levels 1 and 2."""
+    a = _nets(1, 1100, 1.2, 1100)
+    print("case geodesic n=1100 vs oracle", flush=True)
+    np.testing.assert_array_equal(gpu_lib.gof_distribution(a, 2, [2, 1]),
+                                  oracle_lib.gof_distribution(a, 2, [2, 1]))
+
+
+@pytest.mark.parametrize("n,deg,M", [(1100, 1.3, 2), (2100, 1.6, 1), (4200, 2.0, 1)])
+def test_geodesic_above_lds_matches_bfs(n, deg, M, gpu_lib):
+    """Networks above the LDS (one and two words per lane) against
+    breadth-first distances at levels up to the diameter of the giant
+    component and beyond."""
+    from geodesic_check import geodesic_counts
+
+    a = _nets(M, n, deg, 3 * n + 7)
+    lv = [1, 2, 3, 5, 8, 13, 21, 40]
+    print(f"case geodesic n={n} vs breadth-first distances", flush=True)
+    np.testing.assert_array_equal(gpu_lib.gof_distribution(a, 2, lv), geodesic_counts(a, lv))
 
 
 def _plan_problem(n, M, seed, masks):
@@ -124,9 +164,13 @@ def test_plan_gof_matches_dense(n, masks, gpu_lib, oracle_lib):
     stat = ends
     if masks:
         stat = (ends & pr.keep[None]) | pr.force[None]
-    kinds = ((0, DEG_LEVELS), (1, DEG_LEVELS)) + (((2, [1, 2, 3, 6]),) if n <= 1024 else ())
-    for kind, lv in kinds:
+    from geodesic_check import geodesic_counts
+
+    for kind, lv in ((0, DEG_LEVELS), (1, DEG_LEVELS), (2, [1, 2, 3, 6])):
+        # above the LDS the geodesic counts come from breadth-first distances
+        check = (lambda a: geodesic_counts(a, lv)) if (kind == 2 and n > 1024) else \
+            (lambda a: oracle_lib.gof_distribution(a, kind, lv))
         raw = plan.gof(kind, lv, masked=False)
-        np.testing.assert_array_equal(raw, oracle_lib.gof_distribution(ends, kind, lv))
+        np.testing.assert_array_equal(raw, check(ends))
         got = plan.gof(kind, lv, masked=True)
-        np.testing.assert_array_equal(got, oracle_lib.gof_distribution(stat, kind, lv))
+        np.testing.assert_array_equal(got, check(stat))

@@ -19,8 +19,9 @@ pytestmark = pytest.mark.gpu
 @pytest.mark.parametrize("p_in,p_out,B", [(3, 3, 128), (13, 13, 128), (40, 7, 128),
                                           (13, 13, 1024)])
 def test_forward_backward_matches_oracle(gpu_lib, p_in, p_out, B):
-    """Batch 1024: the weight gradients (K = batch) run as split products
-    summed by a second kernel."""
+    """Batch 128 and 1024: the slice kernel's forward and backward passes
+    over 4 and 32 slices of 32 rows, and the weight-gradient kernel's
+    interleaved K steps (1 and 8 per wave) added in wave order."""
     rng = np.random.default_rng(p_in + B)
     width = 256
     m = gpu_lib.Mlp(p_in, p_out, width=width, batch=B, seed=3)
@@ -108,3 +109,36 @@ def test_training_fits_a_smooth_map(gpu_lib):
     assert np.sqrt(np.mean(err ** 2)) < 0.05, np.sqrt(np.mean(err ** 2))
     lv = pred[:, p_out:]
     assert np.corrcoef(lv[:, 0], np.log(sd(xt)[:, 0] ** 2))[0, 1] > 0.8
+
+
+@pytest.mark.parametrize("width,p_in,p_out", [(64, 2, 1), (192, 7, 40)])
+def test_narrow_and_wide_output_shapes(gpu_lib, width, p_in, p_out):
+    """Other compiled shapes of the slice kernel: width 64 (one column tile
+    per wave) with one output, width 192 with 2 p_out = 80 outputs (N_3 = 128,
+    two column tiles per wave in the output product), batch 64 (two
+    slices; the weight-gradient kernel's waves 2 and 3 take no K step)."""
+    rng = np.random.default_rng(width + p_out)
+    B = 64
+    m = gpu_lib.Mlp(p_in, p_out, width=width, batch=B, seed=3)
+    try:
+        flat = MO.init_flat(p_in, p_out, width, rng).astype(np.float32)
+        m.set_params(flat)
+        x = rng.uniform(-1, 1, (B, p_in)).astype(np.float32)
+        y = rng.normal(size=(B, p_out)).astype(np.float32)
+        got = m.forward_backward(x, y)
+    finally:
+        m.close()
+    ref = MO.forward_backward(flat.astype(np.float64), x.astype(np.float64),
+                              y.astype(np.float64), p_in, p_out, width, bf16=True)
+    err, scale = np.abs(got["out"] - ref["out"]), np.max(np.abs(ref["out"]))
+    assert err.max() <= 5e-3 * scale, err.max()
+    assert abs(got["loss"] - ref["loss"]) <= 1e-3 * abs(ref["loss"])
+    g, rg = got["grads"].astype(np.float64), ref["grads"]
+    assert np.linalg.norm(g - rg) <= 5e-3 * np.linalg.norm(rg)
+
+
+def test_width_above_the_fused_bound_is_refused(gpu_lib):
+    from cusna_amd.abi import CusnaError
+
+    with pytest.raises(CusnaError, match="SHAPE|status 2"):
+        gpu_lib.Mlp(3, 3, width=320, batch=128, seed=1)
