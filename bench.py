@@ -378,7 +378,9 @@ def mlp_measure(lib) -> dict:
     """Training of the surrogate perceptron at the shape the surrogate phase
     uses (3 parameters in, 3 moments out, width 256, batch 1024, 2^13
     samples, 200 Adam steps): wall time of the train call after one warm-up
-    call. Per step: 11 BF16 MFMA products (4 forward, 7 backward)."""
+    call. Per step two launches: the batch in 16-row slices through the
+    forward and backward passes, then the weight gradients and Adam per
+    64 x 64 weight tile (BF16 MFMA products)."""
     import numpy as np
 
     rng = np.random.default_rng(3)

@@ -2,7 +2,7 @@
 //
 // A training step is two launches.
 //
-// mlp_slice_kernel: one workgroup per 32 batch rows runs the whole network
+// mlp_slice_kernel: one workgroup per 16 batch rows runs the whole network
 // for its rows -- the four forward products, the loss and its output
 // gradient, and the three backward products of the activation gradients --
 // with every activation, SiLU'(z) and gradient of the slice in LDS. A
@@ -18,9 +18,10 @@
 //
 // mlp_wgrad_adam_kernel: one workgroup per 64 x 64 tile of a weight matrix
 // forms dW = H^T G over the whole batch (the four waves take interleaved
-// 32-row K steps and their partial tiles are added in wave order: the
-// result does not depend on timing), the bias gradients of its columns, and
-// applies Adam to the tile, writing the float32 master weights and their
+// pairs of 32-row K steps, eight steps' loads in flight at once, and their partial
+// tiles are added in wave order: the result does not depend on timing), the
+// bias gradients of its columns, and applies Adam to the tile (its state
+// read before the product), writing the float32 master weights and their
 // row-major and transposed bfloat16 copies through LDS as 16-byte stores.
 //
 // Rounding points: inputs, weights, each hidden activation and each
@@ -41,14 +42,16 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr int kMlpThreads = 256;
-constexpr int kSlice = 32;              // batch rows per workgroup of the slice kernel
+constexpr int kSlice = 16;              // batch rows per workgroup of the slice kernel
 constexpr int kSliceTM = kSlice / 16;   // 16-row tiles per slice
 constexpr int kMaxWidth = 256;          // hidden width bound of the fused step (LDS)
 
-__device__ __forceinline__ float silu(float z) { return z / (1.0f + expf(-z)); }
-__device__ __forceinline__ float dsilu(float z) {
-  const float s = 1.0f / (1.0f + expf(-z));
-  return s * (1.0f + z * (1.0f - s));
+// sigmoid(z) from the hardware exp2 and reciprocal (about 1 ulp each; the
+// epilogues run at one wave per SIMD, where IEEE division and expf's range
+// reduction were most of a layer's time); SiLU(z) = z s, SiLU'(z) = s (1 + z (1 - s)).
+__device__ __forceinline__ float sigmoid_fast(float z) {
+  const float e = __builtin_amdgcn_exp2f(-1.44269504088896341f * z);
+  return __builtin_amdgcn_rcpf(1.0f + e);
 }
 
 __device__ __forceinline__ uint64_t mix64(uint64_t x) {
@@ -109,38 +112,49 @@ __host__ __device__ inline SliceLds slice_lds(int K0, int width, int N3) {
   return L;
 }
 
-// acc[i][j] = rows 16 i.. of A (LDS, row stride lda) times columns
-// 16 (w + 4 j).. of B, given as Bt [N][K] in device memory (row stride K),
-// over k = 0..K-1 (K a multiple of 64: two K steps' loads issued together).
-template <int TNW>
-__device__ __forceinline__ void slice_product(const __bf16* A, int lda, const __bf16* Bt, int K,
-                                              floatx4 (&acc)[kSliceTM][TNW]) {
+// B operand fragments of one product for the wave: columns 16 (w + 4 j)..
+// (j < TNW) of B, given as Bt [N][K] in device memory with K = 32 KS, every
+// K step. Loaded before they are needed: weights do not depend on the
+// activations, so a product's fragments are in flight while the previous
+// product's epilogue, barrier and stores run.
+template <int TNW, int KS>
+struct BFrag {
+  bf16x8 v[KS][TNW];
+};
+
+template <int TNW, int KS>
+__device__ __forceinline__ void load_b(const __bf16* Bt, BFrag<TNW, KS>& f) {
   const int l = threadIdx.x & 63, w = threadIdx.x >> 6, r = l & 15, q = l >> 4;
+  constexpr int K = 32 * KS;
+  const __bf16* brow = Bt + (size_t)(16 * w + r) * K + 8 * q;
+#pragma unroll
+  for (int s = 0; s < KS; ++s)
+#pragma unroll
+    for (int j = 0; j < TNW; ++j)
+      f.v[s][j] = *reinterpret_cast<const bf16x8*>(brow + (size_t)64 * j * K + 32 * s);
+}
+
+// acc[i][j] = rows 16 i.. of A (LDS, row stride lda) times the fragments' columns.
+template <int TNW, int KS>
+__device__ __forceinline__ void slice_mma(const __bf16* A, int lda, const BFrag<TNW, KS>& f,
+                                          floatx4 (&acc)[kSliceTM][TNW]) {
+  const int l = threadIdx.x & 63, r = l & 15, q = l >> 4;
 #pragma unroll
   for (int i = 0; i < kSliceTM; ++i)
 #pragma unroll
     for (int j = 0; j < TNW; ++j) acc[i][j] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
   const __bf16* arow = A + r * lda + 8 * q;
-  const __bf16* brow = Bt + (size_t)(16 * w + r) * K + 8 * q;
-  for (int k0 = 0; k0 < K; k0 += 64) {
-    bf16x8 a[2][kSliceTM], b[2][TNW];
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
+  for (int s = 0; s < KS; ++s) {
+    bf16x8 a[kSliceTM];
 #pragma unroll
-      for (int i = 0; i < kSliceTM; ++i)
-        a[u][i] = *reinterpret_cast<const bf16x8*>(arow + 16 * i * lda + k0 + 32 * u);
+    for (int i = 0; i < kSliceTM; ++i)
+      a[i] = *reinterpret_cast<const bf16x8*>(arow + 16 * i * lda + 32 * s);
+#pragma unroll
+    for (int i = 0; i < kSliceTM; ++i)
 #pragma unroll
       for (int j = 0; j < TNW; ++j)
-        b[u][j] = *reinterpret_cast<const bf16x8*>(brow + (size_t)64 * j * K + k0 + 32 * u);
-    }
-    __builtin_amdgcn_sched_barrier(0);  // every load of the two steps before the MFMAs
-#pragma unroll
-    for (int u = 0; u < 2; ++u)
-#pragma unroll
-      for (int i = 0; i < kSliceTM; ++i)
-#pragma unroll
-        for (int j = 0; j < TNW; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u][i], b[u][j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], f.v[s][j], acc[i][j], 0, 0, 0);
   }
 }
 
@@ -158,11 +172,61 @@ __device__ __forceinline__ void store_transposed(const __bf16* src, int ld, int 
   }
 }
 
-// NWW = width / 64, N3W = N_3 / 64.
+// Hidden-layer epilogue: H = bf16(SiLU(z)), z = acc + bias; SiLU'(z) kept
+// (by the lane that will need it in the backward pass) when bwd.
+template <int TNW>
+__device__ __forceinline__ void load_bias(const float* bias, float (&bc)[TNW]) {
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6, r = l & 15;
+#pragma unroll
+  for (int j = 0; j < TNW; ++j) bc[j] = bias[16 * w + 64 * j + r];
+}
+
+template <int NWW>
+__device__ __forceinline__ void hidden_epilogue(const floatx4 (&acc)[kSliceTM][NWW],
+                                                const float (&bias)[NWW], __bf16* out, int ldh,
+                                                float* Dl, int ldd, bool bwd) {
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6, r = l & 15, q = l >> 4;
+#pragma unroll
+  for (int i = 0; i < kSliceTM; ++i)
+#pragma unroll
+    for (int j = 0; j < NWW; ++j) {
+      const int col = 16 * w + 64 * j + r;
+      const float bc = bias[j];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = 16 * i + 4 * q + v;
+        const float z = acc[i][j][v] + bc;
+        const float sg = sigmoid_fast(z);
+        if (bwd) Dl[row * ldd + col] = sg * (1.0f + z * (1.0f - sg));
+        out[row * ldh + col] = (__bf16)(z * sg);
+      }
+    }
+}
+
+// Backward epilogue: G = bf16(acc * SiLU'(z)) at the same positions.
+template <int NWW>
+__device__ __forceinline__ void grad_epilogue(const floatx4 (&acc)[kSliceTM][NWW], __bf16* out,
+                                              int ldh, const float* Dl, int ldd) {
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6, r = l & 15, q = l >> 4;
+#pragma unroll
+  for (int i = 0; i < kSliceTM; ++i)
+#pragma unroll
+    for (int j = 0; j < NWW; ++j) {
+      const int col = 16 * w + 64 * j + r;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = 16 * i + 4 * q + v;
+        out[row * ldh + col] = (__bf16)(acc[i][j][v] * Dl[row * ldd + col]);
+      }
+    }
+}
+
+// NWW = width / 64, N3W = N_3 / 64 (K_0 = 64: p_in <= 64).
 template <int NWW, int N3W>
 __global__ void __launch_bounds__(kMlpThreads) mlp_slice_kernel(const SliceArgs g) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int K0 = g.K[0], width = 64 * NWW, N3 = 64 * N3W, B = g.B, p = g.p_out;
+  constexpr int K0 = 64, width = 64 * NWW, N3 = 64 * N3W;
+  const int B = g.B, p = g.p_out;
   const SliceLds L = slice_lds(K0, width, N3);
   __bf16* Xs = reinterpret_cast<__bf16*>(smem + L.xs);
   __bf16* Ha = reinterpret_cast<__bf16*>(smem + L.ha);
@@ -174,6 +238,14 @@ __global__ void __launch_bounds__(kMlpThreads) mlp_slice_kernel(const SliceArgs 
   const int tid = threadIdx.x, l = tid & 63, w = tid >> 6, r = l & 15, q = l >> 4;
   const int b0 = blockIdx.x * kSlice;
   const bool bwd = g.mode != 0;
+  const int dstride = kSlice * L.ldd;
+  BFrag<NWW, 2> f0;
+  load_b(g.Wt[0], f0);
+  float bc0[NWW], bc1[NWW], bc2[NWW], bc3[N3W];  // biases, read with the first weights
+  load_bias(g.bias[0], bc0);
+  load_bias(g.bias[1], bc1);
+  load_bias(g.bias[2], bc2);
+  load_bias(g.bias[kMlpLayers - 1], bc3);
   if (tid < kSlice) {
     long long src = b0 + tid;
     if (g.mode == 2)
@@ -186,51 +258,51 @@ __global__ void __launch_bounds__(kMlpThreads) mlp_slice_kernel(const SliceArgs 
     const int row = e / K0, k = e - row * K0;
     Xs[row * L.ldx + k] = (__bf16)(k < g.p_in ? g.x[(size_t)idx[row] * g.p_in + k] : 0.0f);
   }
+  // the loss phase's targets, read now: thread tid takes output column
+  // tid % N_3 of rows tid / N_3 + (256 / N_3) c
+  constexpr int kNll = kSlice * N3 / kMlpThreads, kRowStep = kMlpThreads / N3;
+  const int ac = tid % N3, rbase = tid / N3;
+  float yv[kNll];
+#pragma unroll
+  for (int c = 0; c < kNll; ++c)
+    yv[c] = (bwd && ac < 2 * p) ? g.y[(size_t)idx[rbase + kRowStep * c] * p + (ac < p ? ac : ac - p)]
+                                : 0.0f;
   __syncthreads();
   if (bwd) store_transposed(Xs, L.ldx, K0, g.XT, B, b0);
   // hidden layers: H_l = bf16(SiLU(A W_l + b_l)), SiLU'(z) kept for the backward pass
-  const __bf16* in = Xs;
-  int ldin = L.ldx;
-#pragma unroll 1
-  for (int layer = 0; layer < kMlpLayers - 1; ++layer) {
-    __bf16* out = (layer & 1) ? Hb : Ha;
-    float* Dl = D + layer * kSlice * L.ldd;
-    const float* bias = g.bias[layer];
-    floatx4 acc[kSliceTM][NWW];
-    slice_product<NWW>(in, ldin, g.Wt[layer], g.K[layer], acc);
-#pragma unroll
-    for (int i = 0; i < kSliceTM; ++i)
-#pragma unroll
-      for (int j = 0; j < NWW; ++j) {
-        const int col = 16 * w + 64 * j + r;
-        const float bc = bias[col];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int row = 16 * i + 4 * q + v;
-          const float z = acc[i][j][v] + bc;
-          if (bwd) Dl[row * L.ldd + col] = dsilu(z);
-          out[row * L.ldh + col] = (__bf16)silu(z);
-        }
-      }
-    __syncthreads();
-    if (bwd) store_transposed(out, L.ldh, width, g.HT[layer], B, b0);
-    in = out;
-    ldin = L.ldh;
-  }
+  floatx4 acc[kSliceTM][NWW];
+  BFrag<NWW, 2 * NWW> fh;
+  slice_mma(Xs, L.ldx, f0, acc);
+  load_b(g.Wt[1], fh);
+  hidden_epilogue(acc, bc0, Ha, L.ldh, D, L.ldd, bwd);
+  __syncthreads();
+  if (bwd) store_transposed(Ha, L.ldh, width, g.HT[0], B, b0);
+  slice_mma(Ha, L.ldh, fh, acc);
+  load_b(g.Wt[2], fh);
+  hidden_epilogue(acc, bc1, Hb, L.ldh, D + dstride, L.ldd, bwd);
+  __syncthreads();
+  if (bwd) store_transposed(Hb, L.ldh, width, g.HT[1], B, b0);
+  BFrag<N3W, 2 * NWW> fo;
+  slice_mma(Hb, L.ldh, fh, acc);
+  load_b(g.Wt[3], fo);
+  hidden_epilogue(acc, bc2, Ha, L.ldh, D + 2 * dstride, L.ldd, bwd);
+  __syncthreads();
+  if (bwd) store_transposed(Ha, L.ldh, width, g.HT[2], B, b0);
+  BFrag<NWW, 2 * N3W> f3;
   {  // outputs O = H_2 W_3 + b_3
-    const float* bias = g.bias[kMlpLayers - 1];
-    floatx4 acc[kSliceTM][N3W];
-    slice_product<N3W>(in, ldin, g.Wt[kMlpLayers - 1], g.K[kMlpLayers - 1], acc);
+    floatx4 acco[kSliceTM][N3W];
+    slice_mma(Ha, L.ldh, fo, acco);
+    if (bwd) load_b(g.W[3], f3);
 #pragma unroll
     for (int i = 0; i < kSliceTM; ++i)
 #pragma unroll
       for (int j = 0; j < N3W; ++j) {
         const int col = 16 * w + 64 * j + r;
-        const float bc = bias[col];
+        const float bc = bc3[j];
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int row = 16 * i + 4 * q + v;
-          const float o = acc[i][j][v] + bc;
+          const float o = acco[i][j][v] + bc;
           Os[row * N3 + col] = o;
           g.O[(size_t)(b0 + row) * N3 + col] = o;
         }
@@ -240,30 +312,30 @@ __global__ void __launch_bounds__(kMlpThreads) mlp_slice_kernel(const SliceArgs 
   __syncthreads();
   // Gaussian negative log-likelihood (mu in columns 0..p-1, log-variance in
   // p..2p-1, clamped to [-12, 12] with zero gradient outside) and its
-  // gradient with respect to the outputs, scaled by 1/B; one (row, column)
-  // per thread, the row's loss terms added in column order below
-  // the loss terms take the two activation buffers (H_1, H_2 are stored and
-  // consumed; kSlice p floats fit in them as p <= width + 8)
+  // gradient with respect to the outputs, scaled by 1/B; (row, column) items
+  // as the targets were read, the row's loss terms added in column order below. The terms
+  // take the two activation buffers (H_1, H_2 are stored and consumed;
+  // kSlice p floats fit in them as p <= width + 8).
   float* terms = reinterpret_cast<float*>(Ha);
   const float invb = 1.0f / (float)B;
-  for (int it = tid; it < kSlice * N3; it += kMlpThreads) {
-    const int row = it / N3, a = it - row * N3;
-    const float* yr = g.y + (size_t)idx[row] * p;
+#pragma unroll
+  for (int c = 0; c < kNll; ++c) {
+    const int row = rbase + kRowStep * c, a = ac;
     float gv = 0.0f;
     if (a < p) {
       const float mu = Os[row * N3 + a];
       const float lvr = Os[row * N3 + p + a];
       const float lv = fminf(fmaxf(lvr, -12.0f), 12.0f);
       const float e = expf(-lv);
-      const float res = yr[a] - mu;
+      const float res = yv[c] - mu;
       terms[row * p + a] = 0.5f * (res * res * e + lv);
       gv = -res * e * invb;
     } else if (a < 2 * p) {
-      const int c = a - p;
-      const float mu = Os[row * N3 + c];
+      const int cc = a - p;
+      const float mu = Os[row * N3 + cc];
       const float lvr = Os[row * N3 + a];
       const float lv = fminf(fmaxf(lvr, -12.0f), 12.0f);
-      const float res = yr[c] - mu;
+      const float res = yv[c] - mu;
       gv = (lvr > -12.0f && lvr < 12.0f) ? 0.5f * (1.0f - res * res * expf(-lv)) * invb : 0.0f;
     }
     dOs[row * L.ldo + a] = (__bf16)gv;
@@ -276,32 +348,21 @@ __global__ void __launch_bounds__(kMlpThreads) mlp_slice_kernel(const SliceArgs 
   }
   store_transposed(dOs, L.ldo, N3, g.GT[kMlpLayers - 1], B, b0);
   __syncthreads();
-  // G_{l-1} = bf16((G_l W_l^T) * SiLU'(z_{l-1})), l = 3, 2, 1; the same lane
-  // wrote SiLU'(z) at each position in the forward pass
-  const __bf16* gin = dOs;
-  int ldg = L.ldo;
-#pragma unroll 1
-  for (int layer = kMlpLayers - 1; layer >= 1; --layer) {
-    __bf16* out = (layer & 1) ? Hb : Ha;
-    const float* Dl = D + (layer - 1) * kSlice * L.ldd;
-    floatx4 acc[kSliceTM][NWW];
-    slice_product<NWW>(gin, ldg, g.W[layer], g.N[layer], acc);
-#pragma unroll
-    for (int i = 0; i < kSliceTM; ++i)
-#pragma unroll
-      for (int j = 0; j < NWW; ++j) {
-        const int col = 16 * w + 64 * j + r;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int row = 16 * i + 4 * q + v;
-          out[row * L.ldh + col] = (__bf16)(acc[i][j][v] * Dl[row * L.ldd + col]);
-        }
-      }
-    __syncthreads();
-    store_transposed(out, L.ldh, width, g.GT[layer - 1], B, b0);
-    gin = out;
-    ldg = L.ldh;
-  }
+  // G_{l-1} = bf16((G_l W_l^T) * SiLU'(z_{l-1})), l = 3, 2, 1
+  slice_mma(dOs, L.ldo, f3, acc);
+  load_b(g.W[2], fh);
+  grad_epilogue(acc, Hb, L.ldh, D + 2 * dstride, L.ldd);
+  __syncthreads();
+  store_transposed(Hb, L.ldh, width, g.GT[2], B, b0);
+  slice_mma(Hb, L.ldh, fh, acc);
+  load_b(g.W[1], fh);
+  grad_epilogue(acc, Ha, L.ldh, D + dstride, L.ldd);
+  __syncthreads();
+  store_transposed(Ha, L.ldh, width, g.GT[1], B, b0);
+  slice_mma(Ha, L.ldh, fh, acc);
+  grad_epilogue(acc, Hb, L.ldh, D, L.ldd);
+  __syncthreads();
+  store_transposed(Hb, L.ldh, width, g.GT[0], B, b0);
 }
 
 struct WgradArgs {
@@ -317,23 +378,20 @@ struct WgradArgs {
   float lr, b1, b2, eps, c1, c2; // Adam: bias corrections c1 = 1/(1-b1^t), c2 = 1/(1-b2^t)
 };
 
+constexpr int kWgradThreads = 256;            // four waves per weight tile (512 registers each)
+constexpr int kWgradWaves = kWgradThreads / 64;
+constexpr int kWgradPer = 64 * 64 / kWgradThreads;  // tile elements per thread
+constexpr int kWgradU = 8;  // K steps per wave whose loads are in flight together
 constexpr int kRedLd = 65;  // row stride of the partial tiles (floats)
-constexpr size_t kWgradLds = (size_t)4 * 64 * kRedLd * 4 + 64 * 72 * 2 + 4 * 64 * 4;
+constexpr size_t kWgradLds =
+    (size_t)kWgradWaves * 64 * kRedLd * 4 + 64 * 72 * 2 + kWgradWaves * 64 * 4;
 
-__device__ __forceinline__ void adam_update(const WgradArgs& g, size_t ix, float gi, float& wi) {
-  const float mi = g.b1 * g.m[ix] + (1.0f - g.b1) * gi;
-  const float vi = g.b2 * g.v[ix] + (1.0f - g.b2) * gi * gi;
-  g.m[ix] = mi;
-  g.v[ix] = vi;
-  wi = g.params[ix] - g.lr * (mi * g.c1) / (sqrtf(vi * g.c2) + g.eps);
-  g.params[ix] = wi;
-}
-
-__global__ void __launch_bounds__(kMlpThreads) mlp_wgrad_adam_kernel(const WgradArgs g) {
+__global__ void __launch_bounds__(kWgradThreads) mlp_wgrad_adam_kernel(const WgradArgs g) {
   extern __shared__ __align__(16) unsigned char smem[];
-  float* red = reinterpret_cast<float*>(smem);                           // [4][64][kRedLd]
-  __bf16* tile = reinterpret_cast<__bf16*>(smem + (size_t)4 * 64 * kRedLd * 4);  // [64][72]
-  float* bpart = reinterpret_cast<float*>(smem + (size_t)4 * 64 * kRedLd * 4 + 64 * 72 * 2);
+  float* red = reinterpret_cast<float*>(smem);  // [waves][64][kRedLd]
+  __bf16* tile = reinterpret_cast<__bf16*>(smem + (size_t)kWgradWaves * 64 * kRedLd * 4);
+  float* bpart = reinterpret_cast<float*>(smem + (size_t)kWgradWaves * 64 * kRedLd * 4 +
+                                          64 * 72 * 2);  // [waves][64]
   const int bid = blockIdx.x;
   int layer = 0;
   while (layer < kMlpLayers - 1 && bid >= g.first[layer + 1]) ++layer;
@@ -341,6 +399,26 @@ __global__ void __launch_bounds__(kMlpThreads) mlp_wgrad_adam_kernel(const Wgrad
   const int t = bid - g.first[layer], ntn = N / 64;
   const int k0 = 64 * (t / ntn), n0 = 64 * (t - (t / ntn) * ntn);
   const int tid = threadIdx.x, l = tid & 63, w = tid >> 6, r = l & 15, q = l >> 4;
+  const bool has_bias = k0 == 0;  // the first row of tiles also takes the bias columns
+  // Adam state of the thread's tile elements (tid + 256 c), read ahead of the product
+  float pw[kWgradPer], pm[kWgradPer], pv[kWgradPer];
+  float bw = 0.0f, bm = 0.0f, bv = 0.0f;
+  if (g.adam) {
+#pragma unroll
+    for (int c = 0; c < kWgradPer; ++c) {
+      const int e = tid + kWgradThreads * c;
+      const size_t ix = g.off_w[layer] + (size_t)(k0 + (e >> 6)) * N + n0 + (e & 63);
+      pw[c] = g.params[ix];
+      pm[c] = g.m[ix];
+      pv[c] = g.v[ix];
+    }
+    if (has_bias && tid < 64) {
+      const size_t ix = g.off_b[layer] + n0 + tid;
+      bw = g.params[ix];
+      bm = g.m[ix];
+      bv = g.v[ix];
+    }
+  }
   const __bf16* A = g.AT[layer];
   const __bf16* G = g.GT[layer];
   floatx4 acc[4][4];
@@ -348,13 +426,16 @@ __global__ void __launch_bounds__(kMlpThreads) mlp_wgrad_adam_kernel(const Wgrad
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-  const int S = B / 32;  // K steps; wave w takes steps w, w + 4, ...
-  for (int s = w; s < S; s += 8) {
-    const bool two = s + 4 < S;
-    bf16x8 a[2][4], b[2][4];
+  float bsum[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // column n0 + 16 j + r, the lane's batch rows
+  // K steps of 32 batch rows; wave w takes the pairs of steps 2 w, 2 w + 1,
+  // 2 w + 8, 2 w + 9, ... (a pair reads whole 128-byte lines), eight at a time
+  const int S = B / 32;
+  for (int s0 = 2 * w; s0 < S; s0 += kWgradU * kWgradWaves) {
+    bf16x8 a[kWgradU][4], b[kWgradU][4];
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      const int su = (u == 0 || two) ? s + 4 * u : s;
+    for (int u = 0; u < kWgradU; ++u) {
+      const int sx = s0 + 2 * kWgradWaves * (u >> 1) + (u & 1);
+      const int su = sx < S ? sx : s0;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         a[u][i] = *reinterpret_cast<const bf16x8*>(A + (size_t)(k0 + 16 * i + r) * B + 32 * su +
@@ -363,19 +444,21 @@ __global__ void __launch_bounds__(kMlpThreads) mlp_wgrad_adam_kernel(const Wgrad
                                                    8 * q);
       }
     }
-    __builtin_amdgcn_sched_barrier(0);
+    __builtin_amdgcn_sched_barrier(0);  // every step's loads before the MFMAs
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[0][j], acc[i][j], 0, 0, 0);
-    if (two) {
+    for (int u = 0; u < kWgradU; ++u) {
+      if (s0 + 2 * kWgradWaves * (u >> 1) + (u & 1) >= S) break;
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j)
           acc[i][j] =
-              __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][i], b[1][j], acc[i][j], 0, 0, 0);
+              __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u][i], b[u][j], acc[i][j], 0, 0, 0);
+      if (has_bias)  // the bias gradient is the sum of the same gradient fragments
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+          for (int e = 0; e < 8; ++e) bsum[j] += (float)b[u][j][e];
     }
   }
   float* mine = red + (size_t)w * 64 * kRedLd;
@@ -385,40 +468,49 @@ __global__ void __launch_bounds__(kMlpThreads) mlp_wgrad_adam_kernel(const Wgrad
     for (int j = 0; j < 4; ++j)
 #pragma unroll
       for (int v = 0; v < 4; ++v) mine[(16 * i + 4 * q + v) * kRedLd + 16 * j + r] = acc[i][j][v];
-  // bias gradients of the tile's columns (first row of tiles): column c =
-  // tid % 64, quarter w of the batch, added in quarter order below
-  const bool has_bias = k0 == 0;
+  // bias gradients of the tile's columns: the four lanes of a column (q)
+  // added as a fixed tree, the waves in wave order below
   if (has_bias) {
-    const __bf16* gr = G + (size_t)(n0 + l) * B + (size_t)w * (B / 4);
-    float sb = 0.0f;
-    for (int b8 = 0; b8 < B / 4; b8 += 8) {
-      const bf16x8 v8 = *reinterpret_cast<const bf16x8*>(gr + b8);
 #pragma unroll
-      for (int e = 0; e < 8; ++e) sb += (float)v8[e];
+    for (int j = 0; j < 4; ++j) {
+      float v = bsum[j];
+      v += __shfl_xor(v, 16, 64);
+      v += __shfl_xor(v, 32, 64);
+      if (q == 0) bpart[w * 64 + 16 * j + r] = v;
     }
-    bpart[w * 64 + l] = sb;
   }
   __syncthreads();
-  for (int e = tid; e < 64 * 64; e += kMlpThreads) {
-    const int row = e >> 6, col = e & 63;
-    const float gs = ((red[row * kRedLd + col] + red[64 * kRedLd + row * kRedLd + col]) +
-                      red[2 * 64 * kRedLd + row * kRedLd + col]) +
-                     red[3 * 64 * kRedLd + row * kRedLd + col];
+  const float b1 = g.b1, b2 = g.b2;
+#pragma unroll
+  for (int c = 0; c < kWgradPer; ++c) {
+    const int e = tid + kWgradThreads * c, row = e >> 6, col = e & 63;
+    float gs = red[row * kRedLd + col];
+#pragma unroll
+    for (int u = 1; u < kWgradWaves; ++u) gs += red[(size_t)u * 64 * kRedLd + row * kRedLd + col];
     const size_t ix = g.off_w[layer] + (size_t)(k0 + row) * N + n0 + col;
     g.grads[ix] = gs;
     if (g.adam) {
-      float wi;
-      adam_update(g, ix, gs, wi);
+      const float mi = b1 * pm[c] + (1.0f - b1) * gs;
+      const float vi = b2 * pv[c] + (1.0f - b2) * gs * gs;
+      const float wi = pw[c] - g.lr * (mi * g.c1) / (sqrtf(vi * g.c2) + g.eps);
+      g.m[ix] = mi;
+      g.v[ix] = vi;
+      g.params[ix] = wi;
       tile[row * 72 + col] = (__bf16)wi;
     }
   }
   if (has_bias && tid < 64) {
-    const float gb = ((bpart[tid] + bpart[64 + tid]) + bpart[128 + tid]) + bpart[192 + tid];
+    float gb = bpart[tid];
+#pragma unroll
+    for (int u = 1; u < kWgradWaves; ++u) gb += bpart[u * 64 + tid];
     const size_t ix = g.off_b[layer] + n0 + tid;
     g.grads[ix] = gb;
     if (g.adam) {
-      float wi;
-      adam_update(g, ix, gb, wi);
+      const float mi = b1 * bm + (1.0f - b1) * gb;
+      const float vi = b2 * bv + (1.0f - b2) * gb * gb;
+      g.m[ix] = mi;
+      g.v[ix] = vi;
+      g.params[ix] = bw - g.lr * (mi * g.c1) / (sqrtf(vi * g.c2) + g.eps);
     }
   }
   if (!g.adam) return;
@@ -662,7 +754,8 @@ cusna_status Mlp::wgrad(bool adam, float lr) {
     a.c1 = (float)(1.0 / (1.0 - std::pow(0.9, (double)t_)));
     a.c2 = (float)(1.0 / (1.0 - std::pow(0.999, (double)t_)));
   }
-  hipLaunchKernelGGL(mlp_wgrad_adam_kernel, dim3(tiles), dim3(kMlpThreads), kWgradLds, s_, a);
+  hipLaunchKernelGGL(mlp_wgrad_adam_kernel, dim3(tiles), dim3(kWgradThreads), kWgradLds, s_,
+                     a);
   HIP_TRY(hipGetLastError());
   return CUSNA_OK;
 }

@@ -584,7 +584,7 @@ typedef struct cusna_mom_result {
  *
  *   create        p_in, p_out in 1..64; width a multiple of 64 up to 256,
  *                 batch a multiple of 64 (a training step is two launches:
- *                 the batch in 32-row slices through the whole network, then
+ *                 the batch in 16-row slices through the whole network, then
  *                 the weight gradients with Adam per 64 x 64 weight tile)
  *   param_count   floats in the flat parameter vector: for each layer l,
  *                 W_l [K_l][N_l] row-major then b_l [N_l] (layer_shape);

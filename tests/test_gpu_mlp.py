@@ -20,8 +20,9 @@ pytestmark = pytest.mark.gpu
                                           (13, 13, 1024)])
 def test_forward_backward_matches_oracle(gpu_lib, p_in, p_out, B):
     """Batch 128 and 1024: the slice kernel's forward and backward passes
-    over 4 and 32 slices of 32 rows, and the weight-gradient kernel's
-    interleaved K steps (1 and 8 per wave) added in wave order."""
+    over 8 and 64 slices of 16 rows, and the weight-gradient kernel's
+    interleaved pairs of K steps (one pair for two waves, four pairs per
+    wave) added in wave order."""
     rng = np.random.default_rng(p_in + B)
     width = 256
     m = gpu_lib.Mlp(p_in, p_out, width=width, batch=B, seed=3)
@@ -115,8 +116,8 @@ def test_training_fits_a_smooth_map(gpu_lib):
 def test_narrow_and_wide_output_shapes(gpu_lib, width, p_in, p_out):
     """Other compiled shapes of the slice kernel: width 64 (one column tile
     per wave) with one output, width 192 with 2 p_out = 80 outputs (N_3 = 128,
-    two column tiles per wave in the output product), batch 64 (two
-    slices; the weight-gradient kernel's waves 2 and 3 take no K step)."""
+    two column tiles per wave in the output product), batch 64 (four
+    slices; the weight-gradient kernel's waves 1-3 take no K step)."""
     rng = np.random.default_rng(width + p_out)
     B = 64
     m = gpu_lib.Mlp(p_in, p_out, width=width, batch=B, seed=3)
