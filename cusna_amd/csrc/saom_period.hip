@@ -172,6 +172,26 @@ __device__ __forceinline__ double wave_max(double v) {
               fmax(readlane_f64(v, 32), readlane_f64(v, 48)));
 }
 
+__device__ __forceinline__ float wave_max_f32(float v) {
+  v = fmaxf(v, __int_as_float(dpp_i32<0xB1>(__float_as_int(v))));
+  v = fmaxf(v, __int_as_float(dpp_i32<0x4E>(__float_as_int(v))));
+  v = fmaxf(v, __int_as_float(dpp_i32<0x141>(__float_as_int(v))));
+  v = fmaxf(v, __int_as_float(dpp_i32<0x140>(__float_as_int(v))));
+  const float a = fmaxf(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0)),
+                        __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16)));
+  const float b = fmaxf(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32)),
+                        __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48)));
+  return fmaxf(a, b);
+}
+
+// Threshold of the choice's candidates for a float32 maximum Mw of the
+// float64 bounds: Mw = fl32(Ma) of the largest bound Ma, |Mw - Ma| <=
+// 2^-24 |Ma| < (|Mw| + 1) 2^-23, and every maximiser of the exact g lies
+// within 2 eps of Ma, so every one of them reaches Mw - 2 eps - (|Mw| + 1) 2^-23.
+__device__ __forceinline__ double choice_threshold(double Mw) {
+  return Mw - 2.0 * kGumbelSlack - (fabs(Mw) + 1.0) * 0x1p-23;
+}
+
 // Lane exchange with the partner that differs in lane bit B: row_mirror
 // (15 - l) for bit 3, row_half_mirror (7 - l) for bit 2, quad_perm xor 2 /
 // xor 1 for bits 1 / 0, ds_bpermute xor 16 / xor 32 for bits 4 / 5.
@@ -1305,32 +1325,39 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
     } else {
       // Bound every g_j = f_j - log(-log u_j) in float, then evaluate the
       // exact float64 noise only where g_j can reach the maximum: every j
-      // with the largest exact g (ties included) satisfies approx >= M - 2e.
+      // with the largest exact g (ties included) has a bound within 2 eps of
+      // the largest bound. The largest bound is taken in float32 (a four-step
+      // DPP maximum) with the threshold widened by its rounding
+      // (choice_threshold); the candidates are found by ballot.
       uint32_t uj[R];
       philox_alternatives<R>(__builtin_amdgcn_readfirstlane((uint32_t)steps),
                              __builtin_amdgcn_readfirstlane((uint32_t)chain), A.j, key0, key1,
                              uj);
       double ga[R];
-      double M = -INFINITY;
-      int jm = i;
+      float m32 = -INFINITY;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         if (A.j[r] == i) f[r] = 0.0;
         ga[r] = -INFINITY;
         if (!((A.ok >> r) & 1)) continue;
         ga[r] = f[r] - (double)approx_loglog(uj[r]);
-        arg_combine(M, jm, ga[r], A.j[r]);
+        m32 = fmaxf(m32, (float)ga[r]);
       }
-      wave_argmax(M, jm);
-      // alternatives of this wave within 2e of its own largest bound
-      int near = 0;
+      double M = (double)wave_max_f32(m32);
+      double thr = choice_threshold(M);
+      // candidates of this wave and, when there is one, its index
+      int near = 0, jm = i;
 #pragma unroll
-      for (int r = 0; r < R; ++r)
-        near += __popcll(__ballot(((A.ok >> r) & 1) && ga[r] >= M - 2.0 * kGumbelSlack));
-      // A single alternative within 2e of the largest bound is the exact
-      // maximiser (every maximiser of the exact g lies within 2e of it), so
-      // the exact noise is evaluated only when there are several (rarely:
-      // the top gap of a Gumbel maximum is below 2e in about 0.8 % of draws).
+      for (int r = 0; r < R; ++r) {
+        const uint64_t b = __ballot(((A.ok >> r) & 1) && ga[r] >= thr);
+        if (b) {
+          near += __popcll(b);
+          jm = 64 * (NW * r + wv) + __ffsll((long long)b) - 1;
+        }
+      }
+      // A single candidate is the exact maximiser, so the exact noise is
+      // evaluated only when there are several (rarely: the top gap of a
+      // Gumbel maximum is below 2 eps in about 0.8 % of draws).
       bool unique = near == 1;
       if (NW > 1) {
         if (lane == 0) {
@@ -1346,23 +1373,27 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
             M = red[q * 4 + 3];
             qm = q;
           }
+        thr = choice_threshold(M);  // the leading wave's own threshold
         unique = cnt[qm * 8 + 6] == 1;
-        for (int q = 0; q < NW; ++q)
-          if (q != qm && red[q * 4 + 3] >= M - 2.0 * kGumbelSlack) unique = false;
+        // another wave's bounds lie below its float32 maximum plus its rounding
+        for (int q = 0; q < NW; ++q) {
+          const double Mq = red[q * 4 + 3];
+          if (q != qm && Mq + (fabs(Mq) + 1.0) * 0x1p-23 >= thr) unique = false;
+        }
         jm = (int)cnt[qm * 8 + 7];
       }
-      // M >= max_j f_j - 3.14 (the noise -log(-log u) is at least
-      // -log(33 ln 2) for every u >= 2^-33 the stream yields), so
-      // exp(f_j - M) <= e^3.14 and M serves as the softmax shift: the scores
-      // are ratios of the weighted sums, invariant to the shift up to rounding
+      // M >= max_j f_j - 3.14 - 2^-23 (|M| + 1) (the noise -log(-log u) is at
+      // least -log(33 ln 2) for every u >= 2^-33 the stream yields), so
+      // exp(f_j - M) stays below e^3.15 and M serves as the softmax shift: the
+      // scores are ratios of the weighted sums, invariant to the shift up to
+      // rounding
       shift = M;
       if (unique) {
         jb = jm;
       } else {
-        const double cut = M - 2.0 * kGumbelSlack;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          if (ga[r] >= cut) {
+          if (ga[r] >= thr) {
             const double ll = log(-log(unit_open(uj[r])));
             arg_combine(gb, jb, f[r] - ll, A.j[r]);
           }
