@@ -1364,6 +1364,21 @@ CUSNA_API cusna_status cusna_gumbel_bound_check(uint32_t x0, uint32_t stride, ui
   return CUSNA_OK;
 }
 
+CUSNA_API cusna_status cusna_exp_weight_eval(const double* x, int32_t count, double* out) {
+  if (!x || !out) return CUSNA_ERR_NULL_ARG;
+  if (count < 1) return CUSNA_ERR_SHAPE;
+  if (device_count() < 1) return CUSNA_ERR_NO_CUDA;
+  DeviceBlock D;
+  const size_t ox = D.add((size_t)count * 8), oo = D.add((size_t)count * 8);
+  HIP_TRY(D.alloc());
+  hipStream_t s = thread_stream();
+  HIP_TRY(hipMemcpyAsync(D.at<char>(ox), x, (size_t)count * 8, hipMemcpyHostToDevice, s));
+  if (saom::launch_exp_weight(D.at<double>(ox), count, D.at<double>(oo), s)) return CUSNA_ERR_CUDA;
+  HIP_TRY(hipMemcpyAsync(out, D.at<char>(oo), (size_t)count * 8, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return CUSNA_OK;
+}
+
 CUSNA_API cusna_status cusna_network_stats(const uint8_t* a, int32_t n, int32_t k,
                                            const int32_t* type_ids, const double* p1,
                                            const double* p2, const double* vmat,

@@ -28,4 +28,28 @@ __device__ __forceinline__ float approx_loglog(uint32_t x) {
   return __builtin_amdgcn_logf(a) * kLn2;
 }
 
+// exp(x) for the softmax weights (x = f - shift <= 4, finite): x = (32 m + j)
+// ln2/32 + r with |r| <= ln2/64, exp(x) = 2^m 2^(j/32) p(r), p the degree-6
+// Taylor polynomial (truncation below 4e-18 relative), 2^(j/32) from a
+// 32-entry table (in LDS in the period kernel). Within a few ulps of the
+// correctly rounded value, like the library's exp (the weights enter only the
+// scores, through ratios of their sums; checked against the host's exp by a
+// test through cusna_exp_weight_eval); about half the library's instructions,
+// with no range checks and without a branch per alternative.
+constexpr int kExpTab = 32;
+__device__ __forceinline__ double exp_table(double x, const double* tab) {
+  const double kd = rint(x * 46.166241308446828);  // 32 / ln 2
+  const int k = (int)kd;
+  // ln2/32 = hi + lo, hi with 29 significant bits: k hi is exact for |k| < 2^24
+  double r = fma(-kd, 0x1.62e42ff000000p-6, x);
+  r = fma(-kd, -0x1.718432a1b0e26p-40, r);
+  double q = fma(r, 1.0 / 720.0, 1.0 / 120.0);
+  q = fma(r, q, 1.0 / 24.0);
+  q = fma(r, q, 1.0 / 6.0);
+  q = fma(r, q, 0.5);
+  q = fma(r, q, 1.0);
+  q = fma(r, q, 1.0);
+  return ldexp(tab[k & (kExpTab - 1)] * q, k >> 5);
+}
+
 }  // namespace saom

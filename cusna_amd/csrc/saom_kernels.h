@@ -256,6 +256,8 @@ int launch_expand_dense(const uint64_t* rows, int n, int W64, int M, uint8_t* de
 
 int launch_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out, void* stream);
 
+// The period kernel's softmax-weight exponential on count values (device).
+int launch_exp_weight(const double* x, int count, double* out, void* stream);
 int launch_gumbel_check(uint32_t x0, uint32_t stride, uint32_t count,
                         unsigned long long* max_bits, void* stream);
 

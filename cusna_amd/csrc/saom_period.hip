@@ -106,6 +106,15 @@ __device__ __forceinline__ int shape_flags(const SimParams& P) {
 
 __device__ __forceinline__ int popc64(uint64_t x) { return __popcll(x); }
 
+// A load at a wave-uniform address of memory that no kernel writes while the
+// period runs (problem inputs): through the constant address space the
+// compiler issues a scalar load, which it cannot prove safe for a generic
+// pointer.
+template <class T>
+__device__ __forceinline__ T load_uniform(const T* p) {
+  return *(const __attribute__((address_space(4))) T*)p;
+}
+
 // A wave-uniform 64-bit value moved to scalar registers (loops over its set
 // bits then run on the scalar unit).
 __device__ __forceinline__ uint64_t uniform64(uint64_t v) {
@@ -409,7 +418,7 @@ __device__ __forceinline__ void eff_base(const SimParams& P, int e, const Alt<R>
   auto VJ = [&](int r) -> double {
     return dyn ? ((double)z[A.j[r]] - P.zbar) : vrow[A.j[r]];
   };
-  const double vi = dyn ? U.zvi : vrow[U.i];
+  const double vi = dyn ? U.zvi : load_uniform(vrow + U.i);
   switch (t) {
     case 0:
 #pragma unroll
@@ -688,7 +697,7 @@ __device__ __forceinline__ double beh_d(int t, double dz, double zic, long long 
 
 struct LdsLayout {
   size_t rows, indeg, outdeg, z, tpall, soall, coli, rowi, tpm0, tpm1, red, tsl, dstar, sc, bsc,
-      cnt, crate, ract, ru, rate, draw, dbatch, total;
+      cnt, crate, ract, ru, rate, exptab, draw, dbatch, total;
 };
 
 // Word bounds above this stream the network from device memory.
@@ -731,6 +740,7 @@ __host__ __device__ inline LdsLayout lds_layout(int n, int k, int kb, int wmax, 
   L.ract = o; if (flags & F_RATE) o += (size_t)n * 4;
   o = al8(o);
   L.rate = o; if (flags & F_RATE) o += (size_t)(2 * kMaxRate + 3) * 8;
+  L.exptab = o; o += (size_t)kExpTab * 8;  // 2^(j / kExpTab), j < kExpTab (exp_table)
   // the ring of ministep draws (period_body): a double and three words per
   // entry, 64 entries, fewer where the full ring would cost a chain per CU
   L.draw = o;
@@ -795,6 +805,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
   const int lw = STREAM ? P.lw : 0;
   const LdsLayout L = lds_layout(n, k, kb, WMAX, RS, NW, pflags, lw);
   uint64_t* rows = (uint64_t*)(smem + L.rows);
+  const uint32_t* rows32 = (const uint32_t*)(smem + L.rows);  // halves of the LDS words
   uint64_t* rows_d = STREAM ? P.work + (size_t)chain * W64 * RS : nullptr;
   // word w of row x; w is uniform at every call, so the branch does not diverge
   auto RGET = [&](int w, int x) -> uint64_t {
@@ -827,6 +838,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
   double* sc = (double*)(smem + L.sc);
   double* bsc = (double*)(smem + L.bsc);
   long long* cnt = (long long*)(smem + L.cnt);
+  double* exptab = (double*)(smem + L.exptab);
   double* crate = (double*)(smem + L.crate);    // rates, then their prefix sums
   double* ru_s = (double*)(smem + L.ru);         // rate covariates, q * n + actor
   int* ract = (int*)(smem + L.ract);             // active actors
@@ -857,6 +869,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
   }
   for (int q = tid; q < k; q += NT) sc[q] = 0.0;
   for (int q = tid; q < kb; q += NT) bsc[q] = 0.0;
+  for (int q = tid; q < kExpTab; q += NT) exptab[q] = exp2((double)q / (double)kExpTab);
   if (flags & F_RATE) {
     if (tid == 0) {
       for (int q = 0; q < kMaxRate; ++q) rsc_s[q] = 0.0;
@@ -1037,6 +1050,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
       if (ix >= nact) ix = nact - 1;
       i = P.act ? P.act[ix] : (int)ix;
     }
+    i = __builtin_amdgcn_readfirstlane(i);  // uniform: scalar registers and loads
     auto ROWI = [&](int w) -> uint64_t { return RGET(w, i); };
 
     if (!net) {
@@ -1207,9 +1221,13 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
 #pragma unroll
             for (int r = 0; r < R; ++r) {
               const int word = NW * r + wv;
-              const uint64_t x =
-                  (!STREAM || word < W64) ? RGET(word, h) : 0ull;
-              A.tp[r] += (int)((x >> lane) & 1);
+              if constexpr (STREAM) {
+                const uint64_t x = word < W64 ? RGET(word, h) : 0ull;
+                A.tp[r] += (int)((x >> lane) & 1);
+              } else {  // the 32-bit half of the word that holds the lane's bit
+                const uint32_t x = rows32[2 * ((size_t)word * RS + h) + (lane >> 5)];
+                A.tp[r] += (int)((x >> (lane & 31)) & 1u);
+              }
             }
           }
           if (walk) {
@@ -1271,7 +1289,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
       if (!((A.in >> r) & 1)) ok = false;
       else if (A.j[r] == i) ok = true;
       else {
-        ok = (pmrow[word] >> lane) & 1;
+        ok = (load_uniform(pmrow + word) >> lane) & 1;
         if (ok && P.maxdeg > 0 && A.bij(r) == 0 && U.outdeg_i >= P.maxdeg) ok = false;
       }
       A.ok |= (uint32_t)ok << r;
@@ -1288,7 +1306,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
       constexpr int e = EC;
       double d[R];
       eff_full<R, WMAX, STREAM, SIMPLE>(P, e, A, U, S, d);
-      const double te = th[e];
+      const double te = load_uniform(th + e);
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const double prod = te * d[r];
@@ -1300,7 +1318,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
     for (int e = 0; e < k; ++e) {
       double d[R];
       eff_full<R, WMAX, STREAM, SIMPLE>(P, e, A, U, S, d);
-      const double te = th[e];
+      const double te = load_uniform(th + e);
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const double prod = te * d[r];
@@ -1421,14 +1439,17 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
       }
     }
     if constexpr (REPLAY) shift = lm;
-    const int jstar = jb;
+    // uniform over the workgroup: scalar registers (and scalar loads below)
+    const int jstar = __builtin_amdgcn_readfirstlane(jb);
 
     // 6. softmax-weighted expected change statistics (:271-274)
     double wgt[R];
     double Sw = 0.0;
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      wgt[r] = ((A.ok >> r) & 1) ? exp(f[r] - shift) : 0.0;
+    for (int r = 0; r < R; ++r) {  // every alternative evaluated, no branch
+      const bool okr = (A.ok >> r) & 1;
+      const double e = exp_table(okr ? f[r] - shift : -1000.0, exptab);
+      wgt[r] = okr ? e : 0.0;
       Sw += wgt[r];
     }
 #if defined(SAOM_FIXED_K) && SAOM_CACHE_D
@@ -1443,12 +1464,16 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
       for (int e = 0; e < SAOM_FIXED_K; ++e) {
         double T = 0.0;
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-          T += wgt[r] * dc[e][r];
-          if (((A.in >> r) & 1) && A.j[r] == jstar) dstar[e] = dc[e][r];
-        }
+        for (int r = 0; r < R; ++r) T += wgt[r] * dc[e][r];
         x[e] = T;
       }
+      // the chosen alternative's change statistics, from the lane that holds it
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        if (((A.in >> r) & 1) && A.j[r] == jstar) {
+#pragma unroll
+          for (int e = 0; e < SAOM_FIXED_K; ++e) dstar[e] = dc[e][r];
+        }
       x[SAOM_FIXED_K] = Sw;
       const double tot = transpose_sum<KP>(x, lane);
       const int v = transpose_index<KP>(lane);
@@ -1481,8 +1506,8 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
     if (jstar != i) {
       const int jw = jstar >> 6, jbit = jstar & 63;
       aij = (int)((ROWI(jw) >> jbit) & 1);
-      x0ij = (int)((P.x0[(size_t)i * W64 + jw] >> jbit) & 1);
-      kdij = (int)((P.kd[(size_t)i * W64 + jw] >> jbit) & 1);
+      x0ij = (int)((load_uniform(P.x0 + (size_t)i * W64 + jw) >> jbit) & 1);
+      kdij = (int)((load_uniform(P.kd + (size_t)i * W64 + jw) >> jbit) & 1);
     }
     __syncthreads();
     if (tid < k) {  // one lane per effect: score_e += d_e(j*) - T_e / S

@@ -542,7 +542,22 @@ __global__ void gumbel_check_kernel(uint32_t x0, uint32_t stride, uint32_t count
   atomicMax(max_bits, (unsigned long long)__double_as_longlong(m));
 }
 
+// out[q] = exp_table(x[q]) with the period kernel's table.
+__global__ void exp_weight_kernel(const double* x, int count, double* out) {
+  __shared__ double tab[kExpTab];
+  if (threadIdx.x < kExpTab) tab[threadIdx.x] = exp2((double)threadIdx.x / (double)kExpTab);
+  __syncthreads();
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < count; q += gridDim.x * blockDim.x)
+    out[q] = exp_table(x[q], tab);
+}
+
 }  // namespace
+
+int launch_exp_weight(const double* x, int count, double* out, void* stream) {
+  hipLaunchKernelGGL(exp_weight_kernel, dim3(256), dim3(256), 0, (hipStream_t)stream, x, count,
+                     out);
+  return (int)hipGetLastError();
+}
 
 int launch_gumbel_check(uint32_t x0, uint32_t stride, uint32_t count,
                         unsigned long long* max_bits, void* stream) {

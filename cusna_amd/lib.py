@@ -25,7 +25,7 @@ EXPORTS = (
     "cusna_last_error", "cusna_period_plan_create", "cusna_period_plan_set_params",
     "cusna_period_plan_run", "cusna_period_plan_sync", "cusna_period_plan_fetch",
     "cusna_period_plan_kernel_ms", "cusna_period_plan_shape", "cusna_period_plan_destroy",
-    "cusna_gumbel_bound_check", "cusna_mom_estimate", "cusna_period_plan_autotune",
+    "cusna_gumbel_bound_check", "cusna_exp_weight_eval", "cusna_mom_estimate", "cusna_period_plan_autotune",
     "cusna_saom_simulate_groups", "cusna_mlp_create", "cusna_mlp_param_count",
     "cusna_mlp_layer_shape", "cusna_mlp_get_params", "cusna_mlp_set_params",
     "cusna_mlp_forward_backward", "cusna_mlp_train", "cusna_mlp_predict", "cusna_mlp_destroy",
@@ -86,6 +86,8 @@ def load(path: str | None = None) -> C.CDLL:
     L.cusna_period_plan_kernel_ms.argtypes = [_vp]
     L.cusna_period_plan_shape.restype = C.c_int32
     L.cusna_period_plan_shape.argtypes = [_vp]
+    L.cusna_exp_weight_eval.restype = C.c_int
+    L.cusna_exp_weight_eval.argtypes = [_vp, C.c_int32, _vp]
     L.cusna_gumbel_bound_check.restype = C.c_int
     L.cusna_gumbel_bound_check.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32,
                                            C.POINTER(C.c_double)]
@@ -561,6 +563,15 @@ def gumbel_bound_error(x0: int, stride: int, count: int) -> float:
            load().cusna_gumbel_bound_check(x0 & 0xFFFFFFFF, stride & 0xFFFFFFFF, count,
                                            C.byref(out)))
     return out.value
+
+
+def exp_weight(x) -> np.ndarray:
+    """The period kernel's softmax-weight exponential on the device
+    (cusna_exp_weight_eval)."""
+    xs = np.ascontiguousarray(x, np.float64)
+    out = np.zeros_like(xs)
+    _check("cusna_exp_weight_eval", load().cusna_exp_weight_eval(_p(xs), len(xs), _p(out)))
+    return out
 
 
 def philox4x32(counter, key) -> np.ndarray:

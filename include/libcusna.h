@@ -403,6 +403,14 @@ CUSNA_API cusna_status cusna_gumbel_bound_check(uint32_t x0, uint32_t stride, ui
                                                double *max_abs_err);
 
 /*
+ * Verification hook, no counterpart in the reference. The period kernel
+ * weighs the alternatives of the score sums by exp(f_j - shift) with an
+ * exponential of its own (a 32-entry table of 2^(j/32) and a degree-6
+ * polynomial); this evaluates it on count device-side copies of x.
+ */
+CUSNA_API cusna_status cusna_exp_weight_eval(const double *x, int32_t count, double *out);
+
+/*
  * Device-resident period plans. No counterpart in the reference, whose period
  * call re-marshals everything every time (cusna_saom_glue.cpp:57-189); the
  * reserved Layer-2 estimator types (libcusna.h:541-542) are where the

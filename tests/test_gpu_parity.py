@@ -277,3 +277,24 @@ def test_gumbel_float_bound(gpu_lib):
         worst = max(worst, gpu_lib.gumbel_bound_error(x0, 1, 4096))
     print("max |approx - exact| =", worst)
     assert worst < 2.0 ** -12
+
+
+def test_softmax_weight_exponential(gpu_lib):
+    """The period kernel's own exp (table of 2^(j/32), degree-6 polynomial)
+    against the host's exp: within 4 ulps (relative 1e-15) where the weights
+    are normal numbers, on random arguments, on the boundaries of its range
+    reduction (multiples of ln2/32 and half-steps) and at the ends of its range;
+    below the normal range it may differ by a denormal rounding only."""
+    rng = np.random.default_rng(17)
+    grid = np.arange(-32 * 40, 32 * 4 + 1) * (np.log(2.0) / 32)
+    xs = np.concatenate([rng.uniform(-700.0, 4.0, 200_000), rng.uniform(-1.0, 1.0, 100_000),
+                         grid, grid + np.log(2.0) / 64, grid - np.log(2.0) / 64,
+                         np.nextafter(grid, 1.0), np.nextafter(grid, -1.0),
+                         [0.0, -0.0, 4.0, 3.15, -1e-300, 1e-300, -708.0, -700.0]])
+    got = gpu_lib.exp_weight(xs)
+    ref = np.exp(xs)
+    rel = np.abs(got - ref) / ref
+    print("max relative error", rel.max())
+    assert rel.max() <= 1e-15, (xs[np.argmax(rel)], rel.max())
+    low = np.array([-720.0, -740.0, -745.0, -800.0, -1000.0])
+    assert np.all(np.abs(gpu_lib.exp_weight(low) - np.exp(low)) <= 1e-310)
