@@ -806,6 +806,9 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
   const LdsLayout L = lds_layout(n, k, kb, WMAX, RS, NW, pflags, lw);
   uint64_t* rows = (uint64_t*)(smem + L.rows);
   const uint32_t* rows32 = (const uint32_t*)(smem + L.rows);  // halves of the LDS words
+  // the lane's half of word wv of row 0 (two-path walk: word NW r + wv of row h
+  // at a fixed offset from it)
+  const uint32_t* tp_base = rows32 + 2 * (size_t)wv * RS + (lane >> 5);
   uint64_t* rows_d = STREAM ? P.work + (size_t)chain * W64 * RS : nullptr;
   // word w of row x; w is uniform at every call, so the branch does not diverge
   auto RGET = [&](int w, int x) -> uint64_t {
@@ -1220,12 +1223,12 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
           if (flags & F_TP) {
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-              const int word = NW * r + wv;
               if constexpr (STREAM) {
+                const int word = NW * r + wv;
                 const uint64_t x = word < W64 ? RGET(word, h) : 0ull;
                 A.tp[r] += (int)((x >> lane) & 1);
               } else {  // the 32-bit half of the word that holds the lane's bit
-                const uint32_t x = rows32[2 * ((size_t)word * RS + h) + (lane >> 5)];
+                const uint32_t x = tp_base[2 * ((size_t)NW * r * RS + h)];
                 A.tp[r] += (int)((x >> (lane & 31)) & 1u);
               }
             }
@@ -1464,7 +1467,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
       for (int e = 0; e < SAOM_FIXED_K; ++e) {
         double T = 0.0;
 #pragma unroll
-        for (int r = 0; r < R; ++r) T += wgt[r] * dc[e][r];
+        for (int r = 0; r < R; ++r) T = fma(wgt[r], dc[e][r], T);  // enters only the scores
         x[e] = T;
       }
       // the chosen alternative's change statistics, from the lane that holds it
