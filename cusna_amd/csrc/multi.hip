@@ -87,7 +87,8 @@ int multi_nw(int wmax, int threads) {
   // default: two words per lane, at most four waves (D networks' rows leave
   // room for few chains per CU, so the waves of one chain hide the latency;
   // two networks of 500 actors: 368 / 262 / 218 ms per 4096-chain period at
-  // one / two / four waves per chain)
+  // one / two / four waves per chain; eight waves of one word: 216 against
+  // 178 ms with the float32-bound choice)
   return std::max(1, std::min(4, wmax / 2));
 }
 

@@ -59,7 +59,7 @@ __device__ __forceinline__ int eff_netref(const MultiParams& Q, int e) { return 
 
 struct MultiLds {
   size_t rows, indeg, outdeg, tpall, soall, coli, rowi, tpm0, tpm1, red, tsl, dstar, sc, sd,
-      dc, total;
+      dc, exptab, total;
 };
 
 __host__ __device__ inline MultiLds multi_layout(int n, int k, int D, int wmax, int RS, int nw,
@@ -78,12 +78,14 @@ __host__ __device__ inline MultiLds multi_layout(int n, int k, int D, int wmax, 
   L.rowi = o; if (stream) o += (size_t)wmax * 8;
   L.tpm0 = o; if (all) o += (size_t)wmax * 8;
   L.tpm1 = o; if (all) o += (size_t)wmax * 8;
-  L.red = o; o += (size_t)nw * 4 * 8;
+  L.red = o; o += (size_t)nw * 6 * 8;  // per wave: exact-noise arg-max, bound maximum, candidates
   L.tsl = o; o += (size_t)nw * (k + 1) * 8;
   L.dstar = o; o += (size_t)k * 8;
   L.sc = o; o += (size_t)k * 8;
   L.sd = o; o += (size_t)kMaxNet * 8;
   L.dc = o; o += (size_t)kMaxNet * 4;
+  o = al8(o);
+  L.exptab = o; o += (size_t)kExpTab * 8;  // 2^(j / kExpTab) (exp_table)
   L.total = al8(o);
   return L;
 }
@@ -196,6 +198,7 @@ __device__ __forceinline__ void multi_body(const MultiParams* __restrict__ Qp) {
   double* dstar = (double*)(smem + L.dstar);
   double* sc = (double*)(smem + L.sc);
   long long* sd = (long long*)(smem + L.sd);
+  double* exptab = (double*)(smem + L.exptab);
   int* dc = (int*)(smem + L.dc);
   const int flags = P.flags;
 
@@ -210,6 +213,7 @@ __device__ __forceinline__ void multi_body(const MultiParams* __restrict__ Qp) {
   for (int q = tid; q < WMAX; q += NT) coli_s[q] = 0ull;
   for (int q = tid; q < D * n; q += NT) indeg_all[q] = 0;
   for (int q = tid; q < k; q += NT) sc[q] = 0.0;
+  for (int q = tid; q < kExpTab; q += NT) exptab[q] = exp2((double)q / (double)kExpTab);
   for (int q = tid; q < kMaxNet; q += NT) {
     sd[q] = 0;
     dc[q] = 0;
@@ -257,7 +261,8 @@ __device__ __forceinline__ void multi_body(const MultiParams* __restrict__ Qp) {
     }
     long long ix = (long long)(unit_open(c2) * nact);
     if (ix >= nact) ix = nact - 1;
-    const int i = P.act ? P.act[ix] : (int)ix;
+    // uniform: scalar registers and loads
+    const int i = __builtin_amdgcn_readfirstlane(P.act ? P.act[ix] : (int)ix);
     uint64_t* rows = rows_all + (size_t)d * netw;
     int* indeg = indeg_all + d * n;
     int* outdeg = outdeg_all + d * n;
@@ -413,7 +418,7 @@ __device__ __forceinline__ void multi_body(const MultiParams* __restrict__ Qp) {
       if (eff_netof(Q, e) != d) return;
       double dv[R];
       multi_eff<R, WMAX, STREAM>(Q, e, A, U, S, rows_all, netw, dv);
-      const double te = th[e];
+      const double te = load_uniform(th + e);
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const double prod = te * dv[r];
@@ -426,7 +431,7 @@ __device__ __forceinline__ void multi_body(const MultiParams* __restrict__ Qp) {
       if (eff_netof(Q, e) != d) continue;
       double dv[R];
       multi_eff<R, WMAX, STREAM>(Q, e, A, U, S, rows_all, netw, dv);
-      const double te = th[e];
+      const double te = load_uniform(th + e);
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const double prod = te * dv[r];
@@ -434,62 +439,101 @@ __device__ __forceinline__ void multi_body(const MultiParams* __restrict__ Qp) {
       }
     }
 #endif
-    // 4. Gumbel-max choice with the float32 bound (as the single-network kernel)
+    // 4. Gumbel-max choice with the float32 bound, as the single-network
+    // kernel: the largest bound as a float32 DPP maximum, the candidates
+    // (bounds within the slack of it, widened by its rounding) by ballot, and
+    // the exact float64 noise only when there are several
     double gb = -1e30, shift = 0.0;
     int jb = i;
+    bool exact_choice;
     {
       uint32_t uj[R];
       philox_alternatives<R>(__builtin_amdgcn_readfirstlane((uint32_t)steps),
                              __builtin_amdgcn_readfirstlane((uint32_t)chain), A.j, key0, key1,
                              uj);
       double ga[R];
-      double Mx = -INFINITY;
+      float m32 = -INFINITY;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         if (A.j[r] == i) f[r] = 0.0;
         ga[r] = -INFINITY;
         if (!((A.ok >> r) & 1)) continue;
         ga[r] = f[r] - (double)approx_loglog(uj[r]);
-        Mx = fmax(Mx, ga[r]);
+        m32 = fmaxf(m32, (float)ga[r]);
       }
-      Mx = wave_max(Mx);
-      if (NW > 1) {
-        if (lane == 0) red[wv * 4 + 3] = Mx;
-        __syncthreads();
-        Mx = red[3];
-        for (int q = 1; q < NW; ++q) Mx = fmax(Mx, red[q * 4 + 3]);
-      }
-      const double cut = Mx - 2.0 * kGumbelSlack;
+      double M = (double)wave_max_f32(m32);
+      double thr = choice_threshold(M);
+      int near = 0, jm = i;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        if (ga[r] >= cut) {
-          const double ll = log(-log(unit_open(uj[r])));
-          arg_combine(gb, jb, f[r] - ll, A.j[r]);
+        const uint64_t b = __ballot(((A.ok >> r) & 1) && ga[r] >= thr);
+        if (b) {
+          near += __popcll(b);
+          jm = 64 * (NW * r + wv) + __ffsll((long long)b) - 1;
         }
       }
-      // the noise bound's maximum is within 3.14 of max_j f_j: the softmax
-      // shift (as in the single-network kernel)
-      shift = Mx;
-    }
-    wave_argmax(gb, jb);
-    if (NW > 1) {
-      if (lane == 0) {
-        red[wv * 4 + 0] = gb;
-        red[wv * 4 + 2] = (double)jb;
+      bool unique = near == 1;
+      if (NW > 1) {
+        if (lane == 0) {
+          red[wv * 6 + 3] = M;
+          red[wv * 6 + 4] = (double)near;
+          red[wv * 6 + 5] = (double)jm;
+        }
+        __syncthreads();
+        int qm = 0;
+        M = red[3];
+        for (int q = 1; q < NW; ++q)
+          if (red[q * 6 + 3] > M) {
+            M = red[q * 6 + 3];
+            qm = q;
+          }
+        thr = choice_threshold(M);
+        unique = red[qm * 6 + 4] == 1.0;
+        for (int q = 0; q < NW; ++q) {
+          const double Mq = red[q * 6 + 3];
+          if (q != qm && Mq + (fabs(Mq) + 1.0) * 0x1p-23 >= thr) unique = false;
+        }
+        jm = (int)red[qm * 6 + 5];
       }
-      __syncthreads();
-      gb = red[0];
-      jb = (int)red[2];
-      for (int q = 1; q < NW; ++q) arg_combine(gb, jb, red[q * 4 + 0], (int)red[q * 4 + 2]);
+      // the bound's maximum is within 3.15 of max_j f_j: the softmax shift
+      shift = M;
+      if (unique) {
+        jb = jm;
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          if (ga[r] >= thr) {
+            const double ll = log(-log(unit_open(uj[r])));
+            arg_combine(gb, jb, f[r] - ll, A.j[r]);
+          }
+        }
+      }
+      exact_choice = !unique;
     }
-    const int jstar = jb;
+    if (exact_choice) {  // uniform over the workgroup
+      wave_argmax(gb, jb);
+      if (NW > 1) {
+        if (lane == 0) {
+          red[wv * 6 + 0] = gb;
+          red[wv * 6 + 2] = (double)jb;
+        }
+        __syncthreads();
+        gb = red[0];
+        jb = (int)red[2];
+        for (int q = 1; q < NW; ++q) arg_combine(gb, jb, red[q * 6 + 0], (int)red[q * 6 + 2]);
+      }
+    }
+    // uniform over the workgroup: scalar registers
+    const int jstar = __builtin_amdgcn_readfirstlane(jb);
 
     // softmax-weighted expected change statistics of d's effects (:499-502)
     double wgt[R];
     double Sw = 0.0;
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      wgt[r] = ((A.ok >> r) & 1) ? exp(f[r] - shift) : 0.0;
+    for (int r = 0; r < R; ++r) {  // every alternative evaluated, no branch
+      const bool okr = (A.ok >> r) & 1;
+      const double e = exp_table(okr ? f[r] - shift : -1000.0, exptab);
+      wgt[r] = okr ? e : 0.0;
       Sw += wgt[r];
     }
 #if defined(SAOM_FIXED_K) && SAOM_CACHE_D
@@ -505,12 +549,16 @@ __device__ __forceinline__ void multi_body(const MultiParams* __restrict__ Qp) {
       for (int e = 0; e < SAOM_FIXED_K; ++e) {
         double T = 0.0;
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-          T += wgt[r] * dc[e][r];
-          if (((A.in >> r) & 1) && A.j[r] == jstar) dstar[e] = dc[e][r];
-        }
+        for (int r = 0; r < R; ++r) T = fma(wgt[r], dc[e][r], T);  // enters only the scores
         x[e] = T;
       }
+      // the chosen alternative's change statistics, from the lane that holds it
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        if (((A.in >> r) & 1) && A.j[r] == jstar) {
+#pragma unroll
+          for (int e = 0; e < SAOM_FIXED_K; ++e) dstar[e] = dc[e][r];
+        }
       x[SAOM_FIXED_K] = Sw;
       const double tot = transpose_sum<KP>(x, lane);
       const int v = transpose_index<KP>(lane);
