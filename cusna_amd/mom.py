@@ -7,7 +7,7 @@ per-period problems, default start (R/mom.R:199-212) -- and the three phases of
 """
 from __future__ import annotations
 
-from dataclasses import dataclass, field
+from dataclasses import dataclass, field, replace
 
 import numpy as np
 
@@ -122,6 +122,48 @@ class MomEstimator:
             lamb0 = 1.5 * self.targets[self.P:2 * self.P] / self.n + 1.0
             return np.concatenate([lam0, lamb0, ev0, np.zeros(self.kb)])
         return np.concatenate([lam0, ev0])
+
+    def simulate(self, theta, M: int, rng: np.random.Generator | None = None) -> dict:
+        """All periods for M chains at theta (``.mom_simulate``, R/mom.R:133-183),
+        through the device period simulator: per period one
+        cusna_saom_simulate_period call, its seed drawn from ``rng`` (numpy's
+        global stream when None, as R's ``sample.int`` draws from R's stream),
+        and the moment rows of the estimator's layout -- ``sp``, the period's
+        statistics with its rate statistic (distance plus the rate constant) in
+        the period's rate column when unconditional, and ``scp``, the scores.
+        Returns stats (M x p, summed over periods), pieces [{sp, scp}] per
+        period and ntim (M x P)."""
+        if self.coevol:
+            raise NotImplementedError("simulate: network-only estimators (the co-evolution "
+                                      "layout runs inside cusna_mom_estimate)")
+        theta = np.asarray(theta, dtype=float)
+        if theta.size != self.p:
+            raise ValueError(f"theta has {theta.size} values, the estimator {self.p}")
+        P, p, k = self.P, self.p, self.k
+        if self.conditional:
+            lam, th, off = np.ones(P), theta, 0
+        else:
+            lam, th, off = theta[:P], theta[P:], P
+        stats = np.zeros((M, p))
+        ntim = np.zeros((M, P))
+        pieces = []
+        for q, pr in enumerate(self.problems):
+            # sample.int(.Machine$integer.max, 1): uniform on 1 .. 2^31 - 1
+            seed = int(rng.integers(1, 2**31)) if rng is not None else \
+                int(np.random.randint(1, 2**31))
+            r = lib.simulate_period(replace(pr, theta=th.copy(), lam=float(lam[q]), M=int(M),
+                                            seed=seed))
+            sp = np.zeros((M, p))
+            scp = np.zeros((M, p))
+            if not self.conditional:
+                sp[:, q] = r["dist_stat"] + self.rate_consts[q]
+                scp[:, q] = r["scores"][:, 0]
+            sp[:, off:off + k] = r["stats"]
+            scp[:, off:off + k] = r["scores"][:, 1:k + 1]
+            ntim[:, q] = r["ntim"]
+            stats += sp
+            pieces.append({"sp": sp, "scp": scp, "seed": seed})
+        return {"stats": stats, "pieces": pieces, "ntim": ntim}
 
     def run(self, theta0=None, use_graph: bool = True, max_repeats: int = 0,
             conv_tol: float = 0.1, surrogate: bool = False, surrogate_pairs_log2: int = 0,
