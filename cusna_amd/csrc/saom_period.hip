@@ -130,7 +130,9 @@ __device__ __forceinline__ uint64_t uniform64(uint64_t v) {
 // same kind of partial, and a + b == b + a exactly).
 template <int CTRL>
 __device__ __forceinline__ int dpp_i32(int v) {
-  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+  // every lane's source lies in the wave for these patterns, so the value
+  // kept for a lane without a source is never used (no initialising move)
+  return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, false);
 }
 template <int CTRL>
 __device__ __forceinline__ double dpp_f64(double v) {
@@ -145,6 +147,11 @@ __device__ __forceinline__ double uniform_f64(double v) {
   const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)b);
   const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(b >> 32));
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, l);
+  const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), l);
+  return ((uint64_t)hi << 32) | lo;
 }
 __device__ __forceinline__ double readlane_f64(double v, int l) {
   const long long b = __double_as_longlong(v);
@@ -213,22 +220,55 @@ __device__ __forceinline__ double lane_exchange(double v) {
   else return __shfl_xor(v, 1 << B, 64);
 }
 
+// Pairwise sums across lane bit 5 (B = 5) or 4 (B = 4) of two values a, b:
+// lanes with the bit clear get a(lane) + a(partner), lanes with it set
+// b(partner) + b(lane). One v_permlane32_swap / v_permlane16_swap per 32-bit
+// half exchanges the upper half-wave (odd rows of 16 lanes) of a with the
+// lower half-wave (even rows) of b, which leaves both addends of every lane in
+// the two registers without a select.
+template <int B>
+__device__ __forceinline__ double half_swap_sum(double a, double b) {
+  const long long ab = __double_as_longlong(a), bb = __double_as_longlong(b);
+  uint32_t a0 = (uint32_t)ab, a1 = (uint32_t)(ab >> 32);
+  uint32_t b0 = (uint32_t)bb, b1 = (uint32_t)(bb >> 32);
+  if constexpr (B == 5) {
+    const auto s0 = __builtin_amdgcn_permlane32_swap(a0, b0, false, false);
+    const auto s1 = __builtin_amdgcn_permlane32_swap(a1, b1, false, false);
+    a0 = s0[0], b0 = s0[1], a1 = s1[0], b1 = s1[1];
+  } else {
+    static_assert(B == 4, "half_swap_sum: lane bit 4 or 5");
+    const auto s0 = __builtin_amdgcn_permlane16_swap(a0, b0, false, false);
+    const auto s1 = __builtin_amdgcn_permlane16_swap(a1, b1, false, false);
+    a0 = s0[0], b0 = s0[1], a1 = s1[0], b1 = s1[1];
+  }
+  const double x = __longlong_as_double((long long)(((unsigned long long)a1 << 32) | a0));
+  const double y = __longlong_as_double((long long)(((unsigned long long)b1 << 32) | b0));
+  return x + y;
+}
+
 // Transposing all-reduction of KP (a power of two, <= 32) values per lane:
 // level t exchanges half of the values still held with the partner across
-// lane bit B[t] (B = 3, 2, 1, 0, 4, 5) and keeps the other half, so after
+// lane bit B[t] (B = 5, 4, 3, 2, 1, 0) and keeps the other half, so after
 // log2(KP) levels each lane holds one value index; the remaining levels add
 // the partner's copy. Returns the wave total of value index
 // v(lane) = sum_t bit(lane, B[t]) << (log2(KP) - 1 - t); every lane whose
-// remaining bits are zero holds a distinct index. Costs about 7 VALU per value
-// on the first level instead of a full reduction per value.
+// remaining bits are zero holds a distinct index. The half-wave levels (bits
+// 5 and 4) come first: their exchanges need no select (half_swap_sum), and
+// they carry the most values.
 template <int KP, int T = 0>
 __device__ __forceinline__ double transpose_sum(double (&x)[KP], int lane) {
-  constexpr int kBits[6] = {3, 2, 1, 0, 4, 5};
+  constexpr int kBits[6] = {5, 4, 3, 2, 1, 0};
   if constexpr (T == 6) {
     return x[0];
   } else {
     constexpr int B = kBits[T];
-    if constexpr (KP > 1) {
+    if constexpr (B >= 4) {
+      constexpr int H = KP > 1 ? KP / 2 : 1;
+      double y[H];
+#pragma unroll
+      for (int i = 0; i < H; ++i) y[i] = half_swap_sum<B>(x[i], x[KP > 1 ? i + H : i]);
+      return transpose_sum<H, T + 1>(y, lane);
+    } else if constexpr (KP > 1) {
       constexpr int H = KP / 2;
       const bool hi = (lane >> B) & 1;
       double y[H];
@@ -249,7 +289,7 @@ __device__ __forceinline__ double transpose_sum(double (&x)[KP], int lane) {
 // Value index held by `lane` after transpose_sum<KP> (-1: a duplicate).
 template <int KP>
 __device__ __forceinline__ int transpose_index(int lane) {
-  constexpr int kBits[6] = {3, 2, 1, 0, 4, 5};
+  constexpr int kBits[6] = {5, 4, 3, 2, 1, 0};
   int v = 0, L = 0;
   while ((1 << L) < KP) ++L;
   for (int t = 0; t < 6; ++t) {
@@ -748,6 +788,13 @@ __host__ __device__ inline LdsLayout lds_layout(int n, int k, int kb, int wmax, 
   L.dbatch = 64;
   while (L.dbatch > 1 && per_cu > 0 && (o + 20 * L.dbatch) * per_cu > lds_cu) L.dbatch >>= 1;
   o += 20 * L.dbatch;
+  if (!stream) {
+    // the tie walk of the ministep reads word h >> 6 (h < n) of the rows of
+    // every lane's actor 64 (nw r + wave) + lane, which reaches 64 wmax - 1
+    // past n (lanes that are masked out): keep those reads inside the block
+    const size_t walk_end = ((size_t)((n + 63) / 64 - 1) * RS + 64 * (size_t)wmax) * 8;
+    if (o < walk_end) o = walk_end;
+  }
   L.total = o;
   return L;
 }
@@ -809,6 +856,9 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
   // the lane's half of word wv of row 0 (two-path walk: word NW r + wv of row h
   // at a fixed offset from it)
   const uint32_t* tp_base = rows32 + 2 * (size_t)wv * RS + (lane >> 5);
+  // word 0 of the row of the lane's actor 64 wv + lane, unclamped (the tie walk
+  // reads bit h of row j as the half of word h >> 6 at a uniform offset from it)
+  const uint32_t* so_base = rows32 + 2 * (size_t)(64 * wv + lane);
   uint64_t* rows_d = STREAM ? P.work + (size_t)chain * W64 * RS : nullptr;
   // word w of row x; w is uniform at every call, so the branch does not diverge
   auto RGET = [&](int w, int x) -> uint64_t {
@@ -1170,11 +1220,19 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
       const int j = word * 64 + lane;
       const bool in = j < n;
       A.j[r] = in ? j : i;
-      const int xji = in ? (int)((RGET(iw, A.j[r]) >> ib) & 1) : 0;
+      int xji, xij;
+      if constexpr (STREAM) {
+        xji = in ? (int)((RGET(iw, A.j[r]) >> ib) & 1) : 0;
+        const uint64_t rw = word < W64 ? ROWI(word) : 0ull;
+        xij = (int)((rw >> lane) & 1);
+      } else {  // 32-bit halves: bit i of row j (unclamped) and bit j of row i
+        const uint32_t cj = so_base[2 * ((size_t)iw * RS + 64 * NW * r) + (ib >> 5)];
+        xji = in ? (int)((cj >> (ib & 31)) & 1u) : 0;
+        const uint32_t ri = rows32[2 * ((size_t)word * RS + i) + (lane >> 5)];
+        xij = (int)((ri >> (lane & 31)) & 1u);
+      }
       const uint64_t bal = __ballot(xji);
       if (lane == 0) coli_s[word] = bal;
-      const uint64_t rw = (!STREAM || word < W64) ? ROWI(word) : 0ull;
-      const int xij = (int)((rw >> lane) & 1);
       A.in |= (uint32_t)in << r;
       A.xji |= (uint32_t)xji << r;
       A.xij |= (uint32_t)xij << r;
@@ -1209,49 +1267,75 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
       A.cy[r] = 0;
     }
     auto bit_of_row_j = [&](int r, int h) -> int {  // x_{j h}
-      const uint64_t x = RGET(h >> 6, A.j[r]);
-      const uint32_t half = (h & 32) ? (uint32_t)(x >> 32) : (uint32_t)x;
-      return (int)((half >> (h & 31)) & 1u);
+      if constexpr (!STREAM) {
+        // the 32-bit half that holds bit h, at the actor's unclamped row:
+        // lanes past n read another word of the block and are masked out
+        const uint32_t x = so_base[2 * ((size_t)(h >> 6) * RS + 64 * NW * r) + ((h >> 5) & 1)];
+        return (int)((x >> (h & 31)) & 1u);
+      } else {
+        const uint64_t x = RGET(h >> 6, A.j[r]);
+        const uint32_t half = (h & 32) ? (uint32_t)(x >> 32) : (uint32_t)x;
+        return (int)((half >> (h & 31)) & 1u);
+      }
     };
-    if ((flags & F_TP) || walk) {
+    auto out_tie = [&](int h) {  // an out-tie i -> h of the ego
+      if (flags & F_TP) {
 #pragma unroll
-      for (int wd = 0; wd < U.words(); ++wd) {
-        uint64_t m = uniform64(U.rowi(wd));  // scalar loop over i's out-ties
-        while (m) {
-          const int h = wd * 64 + __ffsll((long long)m) - 1;
-          m &= m - 1;
-          if (flags & F_TP) {
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-              if constexpr (STREAM) {
-                const int word = NW * r + wv;
-                const uint64_t x = word < W64 ? RGET(word, h) : 0ull;
-                A.tp[r] += (int)((x >> lane) & 1);
-              } else {  // the 32-bit half of the word that holds the lane's bit
-                const uint32_t x = tp_base[2 * ((size_t)NW * r * RS + h)];
-                A.tp[r] += (int)((x >> (lane & 31)) & 1u);
-              }
-            }
-          }
-          if (walk) {
-#pragma unroll
-            for (int r = 0; r < R; ++r) A.so[r] += bit_of_row_j(r, h);
+        for (int r = 0; r < R; ++r) {
+          if constexpr (STREAM) {
+            const int word = NW * r + wv;
+            const uint64_t x = word < W64 ? RGET(word, h) : 0ull;
+            A.tp[r] += (int)((x >> lane) & 1);
+          } else {  // the 32-bit half of the word that holds the lane's bit
+            const uint32_t x = tp_base[2 * ((size_t)NW * r * RS + h)];
+            A.tp[r] += (int)((x >> (lane & 31)) & 1u);
           }
         }
       }
-    }
-    if (walk) {
+      if (walk) {
 #pragma unroll
-      for (int wd = 0; wd < U.words(); ++wd) {
-        uint64_t m = uniform64(U.coli(wd));  // scalar loop over i's in-ties
-        while (m) {
-          const int h = wd * 64 + __ffsll((long long)m) - 1;
-          m &= m - 1;
+        for (int r = 0; r < R; ++r) A.so[r] += bit_of_row_j(r, h);
+      }
+    };
+    auto in_tie = [&](int h) {  // an in-tie h -> i of the ego
 #pragma unroll
-          for (int r = 0; r < R; ++r) A.cy[r] += bit_of_row_j(r, h);
+      for (int r = 0; r < R; ++r) A.cy[r] += bit_of_row_j(r, h);
+    };
+    if constexpr (!STREAM) {
+      // scalar loops over the set bits of the ego's nonzero row and column
+      // words: lane w holds word w, a ballot names the nonzero words, and only
+      // those are read into scalar registers
+      const uint64_t rl = lane < WMAX ? ROWI(lane) : 0ull;
+      if ((flags & F_TP) || walk) {
+        for (uint64_t nz = __ballot(rl != 0); nz; nz &= nz - 1) {
+          const int wd = __ffsll((long long)nz) - 1;
+          for (uint64_t m = readlane64(rl, wd); m; m &= m - 1)
+            out_tie(wd * 64 + __ffsll((long long)m) - 1);
+        }
+      }
+      if (walk) {
+        const uint64_t cl = lane < WMAX ? coli_s[lane] : 0ull;
+        for (uint64_t nz = __ballot(cl != 0); nz; nz &= nz - 1) {
+          const int wd = __ffsll((long long)nz) - 1;
+          for (uint64_t m = readlane64(cl, wd); m; m &= m - 1)
+            in_tie(wd * 64 + __ffsll((long long)m) - 1);
         }
       }
     } else {
+      if ((flags & F_TP) || walk) {
+        for (int wd = 0; wd < U.words(); ++wd) {
+          for (uint64_t m = uniform64(U.rowi(wd)); m; m &= m - 1)  // i's out-ties
+            out_tie(wd * 64 + __ffsll((long long)m) - 1);
+        }
+      }
+      if (walk) {
+        for (int wd = 0; wd < U.words(); ++wd) {
+          for (uint64_t m = uniform64(U.coli(wd)); m; m &= m - 1)  // i's in-ties
+            in_tie(wd * 64 + __ffsll((long long)m) - 1);
+        }
+      }
+    }
+    if (!walk) {
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         int so = 0, cy = 0;
