@@ -685,7 +685,8 @@ __device__ __forceinline__ void eff_base(const SimParams& P, int e, const Alt<R>
 // j == i get 0 (:238).
 template <int R, int WMAX, bool STREAM, bool SIMPLE, class EgoT>
 __device__ __forceinline__ void eff_full(const SimParams& P, int e, const Alt<R>& A,
-                                         const EgoT& U, const ChainLds& S, double (&d)[R]) {
+                                         const EgoT& U, const ChainLds& S, double (&d)[R],
+                                         bool mask = true) {
   if constexpr (SIMPLE) {
     eff_base<R, WMAX, STREAM, true>(P, e, A, U, S, d);
   } else {
@@ -708,9 +709,13 @@ __device__ __forceinline__ void eff_full(const SimParams& P, int e, const Alt<R>
       }
     }
   }
+  // the ego's own slot (no change) and slots past n: no change statistics
+  // (callers that weight d themselves skip this)
+  if (mask) {
 #pragma unroll
-  for (int r = 0; r < R; ++r)
-    if (A.j[r] == U.i || !((A.in >> r) & 1)) d[r] = 0.0;
+    for (int r = 0; r < R; ++r)
+      if (A.j[r] == U.i || !((A.in >> r) & 1)) d[r] = 0.0;
+  }
 }
 
 // Behaviour change statistic (libcusna_saom_cpu.cpp:103-113).
@@ -1278,6 +1283,9 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
         return (int)((half >> (h & 31)) & 1u);
       }
     };
+    // row i, word w in lane w (LDS-resident rows): the walks below and the
+    // chosen dyad's state, read here before this ministep's toggle
+    const uint64_t rl = (!STREAM && lane < WMAX) ? ROWI(lane) : 0ull;
     auto out_tie = [&](int h) {  // an out-tie i -> h of the ego
       if (flags & F_TP) {
 #pragma unroll
@@ -1303,9 +1311,8 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
     };
     if constexpr (!STREAM) {
       // scalar loops over the set bits of the ego's nonzero row and column
-      // words: lane w holds word w, a ballot names the nonzero words, and only
-      // those are read into scalar registers
-      const uint64_t rl = lane < WMAX ? ROWI(lane) : 0ull;
+      // words: lane w holds word w (rl, cl), a ballot names the nonzero
+      // words, and only those are read into scalar registers
       if ((flags & F_TP) || walk) {
         for (uint64_t nz = __ballot(rl != 0); nz; nz &= nz - 1) {
           const int wd = __ffsll((long long)nz) - 1;
@@ -1389,10 +1396,13 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
 #ifdef SAOM_FIXED_K
     constexpr int KC = SAOM_CACHE_D ? SAOM_FIXED_K : 1;
     double dc[KC][R];
+    // unmasked: f of the ego's own slot is set to 0 below, slots past n are
+    // not choosable, and the score sums weight the cached values (the own
+    // slot with weight 0)
     static_for<0, SAOM_FIXED_K>([&](auto EC) {
       constexpr int e = EC;
       double d[R];
-      eff_full<R, WMAX, STREAM, SIMPLE>(P, e, A, U, S, d);
+      eff_full<R, WMAX, STREAM, SIMPLE>(P, e, A, U, S, d, !SAOM_CACHE_D);
       const double te = load_uniform(th + e);
 #pragma unroll
       for (int r = 0; r < R; ++r) {
@@ -1547,14 +1557,20 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
       double x[KP];
 #pragma unroll
       for (int e = 0; e < KP; ++e) x[e] = 0.0;
+      // the cached change statistics are unmasked: the ego's own slot enters
+      // the sums with weight 0 (slots past n have weight 0 already)
+      double wt[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) wt[r] = A.j[r] == i ? 0.0 : wgt[r];
 #pragma unroll
       for (int e = 0; e < SAOM_FIXED_K; ++e) {
         double T = 0.0;
 #pragma unroll
-        for (int r = 0; r < R; ++r) T = fma(wgt[r], dc[e][r], T);  // enters only the scores
+        for (int r = 0; r < R; ++r) T = fma(wt[r], dc[e][r], T);  // enters only the scores
         x[e] = T;
       }
-      // the chosen alternative's change statistics, from the lane that holds it
+      // the chosen alternative's change statistics, from the lane that holds
+      // it (unmasked: the score update takes none when the ego keeps its ties)
 #pragma unroll
       for (int r = 0; r < R; ++r)
         if (((A.in >> r) & 1) && A.j[r] == jstar) {
@@ -1588,24 +1604,19 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
     for (int e = 0; e < k; ++e) expected(e);
 #endif
 #endif
-    // 7. state of the chosen dyad before the toggle (:277-281)
+    // 7. state of the chosen dyad before the toggle (:277-281), from row i as
+    // read in this ministep (registers, or the LDS copy of a streamed row)
     int aij = 0, x0ij = 0, kdij = 0;
     if (jstar != i) {
       const int jw = jstar >> 6, jbit = jstar & 63;
-      aij = (int)((ROWI(jw) >> jbit) & 1);
+      uint64_t rw;
+      if constexpr (STREAM) rw = rowi_s[jw];
+      else rw = readlane64(rl, jw);
+      aij = (int)((rw >> jbit) & 1);
       x0ij = (int)((load_uniform(P.x0 + (size_t)i * W64 + jw) >> jbit) & 1);
       kdij = (int)((load_uniform(P.kd + (size_t)i * W64 + jw) >> jbit) & 1);
     }
-    __syncthreads();
-    if (tid < k) {  // one lane per effect: score_e += d_e(j*) - T_e / S
-      double St = 0.0, Tt = 0.0;
-      for (int q = 0; q < NW; ++q) {
-        St += tsl[q * (k + 1) + k];
-        Tt += tsl[q * (k + 1) + tid];
-      }
-      sc[tid] += dstar[tid] - Tt / St;
-    }
-    if (tid == 0) {
+    auto toggle = [&]() {  // thread 0: the step's tie change and the rate scores
       if (jstar != i) {
         const int dd = 1 - 2 * aij;
         RPUT(jstar >> 6, i, RGET(jstar >> 6, i) ^ (1ull << (jstar & 63)));
@@ -1621,13 +1632,37 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
       if (flags & F_RATE)
         for (int q = 0; q < P.nrate; ++q)
           rsc_s[q] += rate_cov(P.rkind[q], ru_s + (size_t)q * n, i, outdeg, indeg);
+    };
+    // With the change statistics cached in registers nothing after the choice
+    // reads the network, its degrees or the rate state, so the toggle goes
+    // before the barrier that publishes the score sums, and the score update
+    // (wave 0) needs no closing barrier: the next writes of the sums and of
+    // dstar come after the next ministep's first two barriers, and the next
+    // reads of the network after this barrier.
+#if defined(SAOM_FIXED_K) && SAOM_CACHE_D
+    constexpr bool kEarlyToggle = true;
+#else
+    constexpr bool kEarlyToggle = false;
+#endif
+    if (kEarlyToggle && tid == 0) toggle();
+    __syncthreads();
+    if (tid < k) {  // one lane per effect: score_e += d_e(j*) - T_e / S
+      double St = 0.0, Tt = 0.0;
+      for (int q = 0; q < NW; ++q) {
+        St += tsl[q * (k + 1) + k];
+        Tt += tsl[q * (k + 1) + tid];
+      }
+      // no change statistics when the ego keeps its ties (dstar holds the
+      // unmasked values of its own slot in the cached form)
+      sc[tid] += (jstar == i ? 0.0 : dstar[tid]) - Tt / St;
     }
+    if (!kEarlyToggle && tid == 0) toggle();
     steps++;
     if (jstar != i && kdij) {
       const long long chg = P.symflag ? 2 : 1;
       dist += (aij == x0ij) ? chg : -chg;
     }
-    __syncthreads();
+    if (!kEarlyToggle) __syncthreads();
     if (P.cond && dist >= P.target) break;
     if (steps >= cap) break;
   }
