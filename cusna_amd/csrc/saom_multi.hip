@@ -113,15 +113,15 @@ __device__ __forceinline__ void multi_eff(const MultiParams& Q, int e, const Alt
     switch (t) {
       case 33:
 #pragma unroll
-        for (int r = 0; r < R; ++r) d[r] = A.dl(r) * (double)wij(r);
+        for (int r = 0; r < R; ++r) d[r] = A.sdl(r, (double)wij(r));
         break;
       case 34:
 #pragma unroll
-        for (int r = 0; r < R; ++r) d[r] = A.dl(r) * (double)wji(r);
+        for (int r = 0; r < R; ++r) d[r] = A.sdl(r, (double)wji(r));
         break;
       case 35:
 #pragma unroll
-        for (int r = 0; r < R; ++r) d[r] = A.dl(r) * (double)(wij(r) & wji(r));
+        for (int r = 0; r < R; ++r) d[r] = A.sdl(r, (double)(wij(r) & wji(r)));
         break;
       case 36: {
         const bool root = P.p1[e] >= 2.0;
@@ -130,7 +130,7 @@ __device__ __forceinline__ void multi_eff(const MultiParams& Q, int e, const Alt
           int c = 0;
           for (int wd = 0; wd < U.words(); ++wd)
             c += popc64(rw[(size_t)wd * RS + i] & rw[(size_t)wd * RS + A.j[r]]);
-          d[r] = A.dl(r) * (root ? sqrt((double)c) : (double)c);
+          d[r] = A.sdl(r, (root ? sqrt((double)c) : (double)c));
         }
         break;
       }
@@ -153,7 +153,7 @@ __device__ __forceinline__ void multi_eff(const MultiParams& Q, int e, const Alt
           }
         }
 #pragma unroll
-        for (int r = 0; r < R; ++r) d[r] = A.dl(r) * (double)c[r];
+        for (int r = 0; r < R; ++r) d[r] = A.sdl(r, (double)c[r]);
         break;
       }
     }

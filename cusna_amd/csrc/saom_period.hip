@@ -383,6 +383,11 @@ struct Alt {
   __device__ __forceinline__ int bij(int r) const { return (xij >> r) & 1; }
   __device__ __forceinline__ int bji(int r) const { return (xji >> r) & 1; }
   __device__ __forceinline__ double dl(int r) const { return 1.0 - 2.0 * (double)bij(r); }
+  // dl(r) * y with the same bits (dl is +-1, the product a sign flip)
+  __device__ __forceinline__ double sdl(int r, double y) const {
+    return __longlong_as_double(
+        (long long)((unsigned long long)__double_as_longlong(y) ^ ((unsigned long long)bij(r) << 63)));
+  }
 };
 
 // Ego-level state shared by all alternatives of the ministep. Row and column
@@ -466,61 +471,61 @@ __device__ __forceinline__ void eff_base(const SimParams& P, int e, const Alt<R>
       break;
     case 1:
 #pragma unroll
-      for (int r = 0; r < R; ++r) d[r] = A.dl(r) * (double)A.bji(r);
+      for (int r = 0; r < R; ++r) d[r] = A.sdl(r, (double)A.bji(r));
       break;
     case 2:
 #pragma unroll
-      for (int r = 0; r < R; ++r) d[r] = A.dl(r) * (double)(A.tp[r] + A.so[r]);
+      for (int r = 0; r < R; ++r) d[r] = A.sdl(r, (double)(A.tp[r] + A.so[r]));
       break;
     case 3:
     case 23:
 #pragma unroll
-      for (int r = 0; r < R; ++r) d[r] = A.dl(r) * (double)A.cy[r];
+      for (int r = 0; r < R; ++r) d[r] = A.sdl(r, (double)A.cy[r]);
       break;
     case 4:
 #pragma unroll
-      for (int r = 0; r < R; ++r) d[r] = A.dl(r) * (double)(indeg[A.j[r]] + (1 - A.bij(r)));
+      for (int r = 0; r < R; ++r) d[r] = A.sdl(r, (double)(indeg[A.j[r]] + (1 - A.bij(r))));
       break;
     case 7:
 #pragma unroll
       for (int r = 0; r < R; ++r)
-        d[r] = A.dl(r) * sqrt((double)(indeg[A.j[r]] + (1 - A.bij(r))));
+        d[r] = A.sdl(r, sqrt((double)(indeg[A.j[r]] + (1 - A.bij(r)))));
       break;
     case 8:
 #pragma unroll
-      for (int r = 0; r < R; ++r) d[r] = A.dl(r) * (double)outdeg[A.j[r]];
+      for (int r = 0; r < R; ++r) d[r] = A.sdl(r, (double)outdeg[A.j[r]]);
       break;
     case 9:
 #pragma unroll
-      for (int r = 0; r < R; ++r) d[r] = A.dl(r) * sqrt((double)outdeg[A.j[r]]);
+      for (int r = 0; r < R; ++r) d[r] = A.sdl(r, sqrt((double)outdeg[A.j[r]]));
       break;
     case 10:
 #pragma unroll
-      for (int r = 0; r < R; ++r) d[r] = A.dl(r) * vi;
+      for (int r = 0; r < R; ++r) d[r] = A.sdl(r, vi);
       break;
     case 11:
 #pragma unroll
-      for (int r = 0; r < R; ++r) d[r] = A.dl(r) * VJ(r);
+      for (int r = 0; r < R; ++r) d[r] = A.sdl(r, VJ(r));
       break;
     case 12:
 #pragma unroll
-      for (int r = 0; r < R; ++r) d[r] = A.dl(r) * (1.0 - fabs(vi - VJ(r)) * p1 - p2);
+      for (int r = 0; r < R; ++r) d[r] = A.sdl(r, (1.0 - fabs(vi - VJ(r)) * p1 - p2));
       break;
     case 13:
 #pragma unroll
-      for (int r = 0; r < R; ++r) d[r] = A.dl(r) * (fabs(vi - VJ(r)) < 1e-9 ? 1.0 : 0.0);
+      for (int r = 0; r < R; ++r) d[r] = A.sdl(r, (fabs(vi - VJ(r)) < 1e-9 ? 1.0 : 0.0));
       break;
     case 14:
 #pragma unroll
-      for (int r = 0; r < R; ++r) d[r] = A.dl(r) * (double)U.indeg_i;
+      for (int r = 0; r < R; ++r) d[r] = A.sdl(r, (double)U.indeg_i);
       break;
     case 15:
 #pragma unroll
-      for (int r = 0; r < R; ++r) d[r] = A.dl(r) * sqrt((double)U.indeg_i);
+      for (int r = 0; r < R; ++r) d[r] = A.sdl(r, sqrt((double)U.indeg_i));
       break;
     case 16:
 #pragma unroll
-      for (int r = 0; r < R; ++r) d[r] = A.dl(r) * (2.0 * (double)U.outdeg_i + A.dl(r));
+      for (int r = 0; r < R; ++r) d[r] = A.sdl(r, (2.0 * (double)U.outdeg_i + A.dl(r)));
       break;
     case 17: {
       const double d0 = (double)U.outdeg_i;
@@ -541,8 +546,7 @@ __device__ __forceinline__ void eff_base(const SimParams& P, int e, const Alt<R>
     case 19:
 #pragma unroll
       for (int r = 0; r < R; ++r)
-        d[r] = A.dl(r) *
-               ((double)U.rdeg_i + (double)A.bji(r) * ((double)U.outdeg_i + A.dl(r)));
+        d[r] = A.sdl(r, ((double)U.rdeg_i + (double)A.bji(r) * ((double)U.outdeg_i + A.dl(r))));
       break;
     case 20:
 #pragma unroll
@@ -553,15 +557,15 @@ __device__ __forceinline__ void eff_base(const SimParams& P, int e, const Alt<R>
       break;
     case 21:
 #pragma unroll
-      for (int r = 0; r < R; ++r) d[r] = A.dl(r) * vi * VJ(r);
+      for (int r = 0; r < R; ++r) d[r] = A.sdl(r, vi * VJ(r));
       break;
     case 25:
 #pragma unroll
-      for (int r = 0; r < R; ++r) d[r] = A.dl(r) * (vi + VJ(r));
+      for (int r = 0; r < R; ++r) d[r] = A.sdl(r, (vi + VJ(r)));
       break;
     case 31:
 #pragma unroll
-      for (int r = 0; r < R; ++r) d[r] = A.dl(r) * (VJ(r) >= p1 - 1e-12 ? 1.0 : 0.0);
+      for (int r = 0; r < R; ++r) d[r] = A.sdl(r, (VJ(r) >= p1 - 1e-12 ? 1.0 : 0.0));
       break;
     default:
       if constexpr (!SIMPLE) {
@@ -574,7 +578,7 @@ __device__ __forceinline__ void eff_base(const SimParams& P, int e, const Alt<R>
 #pragma unroll
           for (int wd = 0; wd < U.words(); ++wd)
             c += popc64(U.rowi(wd) & ROW(wd, A.j[r]) & tm[wd]);
-          d[r] = A.dl(r) * (double)(c + (A.tp[r] > 0 ? 1 : 0));
+          d[r] = A.sdl(r, (double)(c + (A.tp[r] > 0 ? 1 : 0)));
         }
         break;
       case 6:  // transRecTrip: x_ji tp_j + |row_i & col_i & row_j|
@@ -584,7 +588,7 @@ __device__ __forceinline__ void eff_base(const SimParams& P, int e, const Alt<R>
 #pragma unroll
           for (int wd = 0; wd < U.words(); ++wd)
             c += popc64(U.rowi(wd) & U.coli(wd) & ROW(wd, A.j[r]));
-          d[r] = A.dl(r) * (double)((A.bji(r) ? A.tp[r] : 0) + c);
+          d[r] = A.sdl(r, (double)((A.bji(r) ? A.tp[r] : 0) + c));
         }
         break;
       case 22: {
@@ -593,17 +597,17 @@ __device__ __forceinline__ void eff_base(const SimParams& P, int e, const Alt<R>
         for (int r = 0; r < R; ++r) {
           const double dyw =
               (dc >= 0 && P.dyc) ? P.dyc[((size_t)dc * n + U.i) * n + A.j[r]] : 0.0;
-          d[r] = A.dl(r) * dyw;
+          d[r] = A.sdl(r, dyw);
         }
         break;
       }
       case 24:
 #pragma unroll
-        for (int r = 0; r < R; ++r) d[r] = A.dl(r) * w[A.tp[r]];
+        for (int r = 0; r < R; ++r) d[r] = A.sdl(r, w[A.tp[r]]);
         break;
       case 26:
 #pragma unroll
-        for (int r = 0; r < R; ++r) d[r] = A.dl(r) * w[A.so[r]];
+        for (int r = 0; r < R; ++r) d[r] = A.sdl(r, w[A.so[r]]);
         break;
       case 27: {  // gwespBF: common in-neighbours #{h in in(i): h -> j}
         int c[R];
@@ -624,12 +628,12 @@ __device__ __forceinline__ void eff_base(const SimParams& P, int e, const Alt<R>
           }
         }
 #pragma unroll
-        for (int r = 0; r < R; ++r) d[r] = A.dl(r) * w[c[r]];
+        for (int r = 0; r < R; ++r) d[r] = A.sdl(r, w[c[r]]);
         break;
       }
       case 28:
 #pragma unroll
-        for (int r = 0; r < R; ++r) d[r] = A.dl(r) * w[A.cy[r]];
+        for (int r = 0; r < R; ++r) d[r] = A.sdl(r, w[A.cy[r]]);
         break;
       case 29:  // gwdspFF: out-neighbours h of j (h != i), in ascending order
 #pragma unroll
@@ -645,7 +649,7 @@ __device__ __forceinline__ void eff_base(const SimParams& P, int e, const Alt<R>
               s += A.bij(r) ? (w[p] - w[p - 1]) : (w[p + 1] - w[p]);
             }
           }
-          d[r] = A.dl(r) * s;
+          d[r] = A.sdl(r, s);
         }
         break;
       case 30:  // gwdspFB: in-neighbours h of j (h != i), in ascending order
@@ -658,7 +662,7 @@ __device__ __forceinline__ void eff_base(const SimParams& P, int e, const Alt<R>
             const int p = soall[h];
             s += A.bij(r) ? (w[p] - w[p - 1]) : (w[p + 1] - w[p]);
           }
-          d[r] = A.dl(r) * s;
+          d[r] = A.sdl(r, s);
         }
         break;
           default:
@@ -1404,10 +1408,20 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
       double d[R];
       eff_full<R, WMAX, STREAM, SIMPLE>(P, e, A, U, S, d, !SAOM_CACHE_D);
       const double te = load_uniform(th + e);
+      // density and reciprocity change by -1, 0 or 1, so te * d is exact and
+      // one fused operation rounds f + te * d as the separate ones do; the
+      // first term starts the sum (0 + te * d differs only in a zero's sign)
+      constexpr bool kUnitD = kFixedT[e] == 0 || kFixedT[e] == 1;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        const double prod = te * d[r];
-        f[r] = f[r] + prod;
+        if constexpr (e == 0) {
+          f[r] = te * d[r];
+        } else if constexpr (kUnitD) {
+          f[r] = fma(te, d[r], f[r]);
+        } else {
+          const double prod = te * d[r];
+          f[r] = f[r] + prod;
+        }
         if (SAOM_CACHE_D) dc[SAOM_CACHE_D ? e : 0][r] = d[r];
       }
     });
