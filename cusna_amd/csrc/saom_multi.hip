@@ -96,7 +96,7 @@ template <int R, int WMAX, bool STREAM>
 __device__ __forceinline__ void multi_eff(const MultiParams& Q, int e, const Alt<R>& A,
                                           const Ego<WMAX, STREAM>& U, const ChainLds& S,
                                           const uint64_t* rows_all, size_t netw,
-                                          double (&d)[R]) {
+                                          double (&d)[R], bool mask = true) {
   const SimParams& P = Q.base;
   const int t = eff_type(P, e);
   if (t < 33 || t > 37) {
@@ -164,9 +164,13 @@ __device__ __forceinline__ void multi_eff(const MultiParams& Q, int e, const Alt
         if ((et == 1 && A.bij(r) == 1) || (et == 2 && A.bij(r) == 0)) d[r] = 0.0;
     }
   }
+  // the ego's own slot and slots past n: none (callers that weight d
+  // themselves skip this)
+  if (mask) {
 #pragma unroll
-  for (int r = 0; r < R; ++r)
-    if (A.j[r] == U.i || !((A.in >> r) & 1)) d[r] = 0.0;
+    for (int r = 0; r < R; ++r)
+      if (A.j[r] == U.i || !((A.in >> r) & 1)) d[r] = 0.0;
+  }
 }
 
 template <int NW, int R, int WMAX>
@@ -416,8 +420,11 @@ __device__ __forceinline__ void multi_body(const MultiParams* __restrict__ Qp) {
         for (int r = 0; r < R; ++r) dc[SAOM_CACHE_D ? e : 0][r] = 0.0;
       }
       if (eff_netof(Q, e) != d) return;
+      // unmasked: f of the ego's own slot is set to 0 below, slots past n are
+      // not choosable, and the score sums weight the cached values (the own
+      // slot with weight 0)
       double dv[R];
-      multi_eff<R, WMAX, STREAM>(Q, e, A, U, S, rows_all, netw, dv);
+      multi_eff<R, WMAX, STREAM>(Q, e, A, U, S, rows_all, netw, dv, !SAOM_CACHE_D);
       const double te = load_uniform(th + e);
 #pragma unroll
       for (int r = 0; r < R; ++r) {
@@ -545,14 +552,20 @@ __device__ __forceinline__ void multi_body(const MultiParams* __restrict__ Qp) {
       double x[KP];
 #pragma unroll
       for (int e = 0; e < KP; ++e) x[e] = 0.0;
+      // the ego's own slot enters the sums with weight 0 (cached values are
+      // unmasked; slots past n have weight 0 already)
+      double wt[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) wt[r] = A.j[r] == i ? 0.0 : wgt[r];
 #pragma unroll
       for (int e = 0; e < SAOM_FIXED_K; ++e) {
         double T = 0.0;
 #pragma unroll
-        for (int r = 0; r < R; ++r) T = fma(wgt[r], dc[e][r], T);  // enters only the scores
+        for (int r = 0; r < R; ++r) T = fma(wt[r], dc[e][r], T);  // enters only the scores
         x[e] = T;
       }
-      // the chosen alternative's change statistics, from the lane that holds it
+      // the chosen alternative's change statistics, from the lane that holds
+      // it (unmasked: the score update takes none when the ego keeps its ties)
 #pragma unroll
       for (int r = 0; r < R; ++r)
         if (((A.in >> r) & 1) && A.j[r] == jstar) {
@@ -587,20 +600,10 @@ __device__ __forceinline__ void multi_body(const MultiParams* __restrict__ Qp) {
     for (int e = 0; e < k; ++e) expected(e);
 #endif
 #endif
-    int aij = 0;
-    if (jstar != i) aij = (int)((ROWI(jstar >> 6) >> (jstar & 63)) & 1);
-    __syncthreads();
-    if (tid < k && Q.netof[tid] == d) {  // (netof of a lane's own effect)
-      double St = 0.0, Tt = 0.0;
-      for (int q = 0; q < NW; ++q) {
-        St += tsl[q * (k + 1) + k];
-        Tt += tsl[q * (k + 1) + tid];
-      }
-      sc[tid] += dstar[tid] - Tt / St;
-    }
-    if (tid == 0) {
+    auto toggle = [&]() {  // thread 0: the step's tie change in network d
       sd[d] += 1;
       if (jstar != i) {
+        const int aij = (int)((ROWI(jstar >> 6) >> (jstar & 63)) & 1);
         const int dd = 1 - 2 * aij;
         rows[(size_t)(jstar >> 6) * RS + i] ^= (1ull << (jstar & 63));
         indeg[jstar] += dd;
@@ -611,9 +614,32 @@ __device__ __forceinline__ void multi_body(const MultiParams* __restrict__ Qp) {
           outdeg[jstar] += dd;
         }
       }
-    }
-    steps++;
+    };
+    // With the change statistics cached in registers nothing after the choice
+    // reads the networks, so the toggle goes before the barrier that
+    // publishes the score sums and the ministep needs no closing barrier (the
+    // next writes of the sums and of dstar follow the next ministep's first
+    // barriers; its reads of the networks follow this one).
+#if defined(SAOM_FIXED_K) && SAOM_CACHE_D
+    constexpr bool kEarlyToggle = true;
+#else
+    constexpr bool kEarlyToggle = false;
+#endif
+    if (kEarlyToggle && tid == 0) toggle();
     __syncthreads();
+    if (tid < k && Q.netof[tid] == d) {  // (netof of a lane's own effect)
+      double St = 0.0, Tt = 0.0;
+      for (int q = 0; q < NW; ++q) {
+        St += tsl[q * (k + 1) + k];
+        Tt += tsl[q * (k + 1) + tid];
+      }
+      // no change statistics when the ego keeps its ties (dstar holds the
+      // unmasked values of its own slot in the cached form)
+      sc[tid] += (jstar == i ? 0.0 : dstar[tid]) - Tt / St;
+    }
+    if (!kEarlyToggle && tid == 0) toggle();
+    steps++;
+    if (!kEarlyToggle) __syncthreads();
     if (steps >= kMaxSteps) break;
   }
 
