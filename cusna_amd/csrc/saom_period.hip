@@ -1223,28 +1223,46 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
     Alt<R> A;
     A.in = A.ok = A.xij = A.xji = 0u;
     const int iw = i >> 6, ib = i & 63;
+    int xji[R], xij[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int word = NW * r + wv;
       const int j = word * 64 + lane;
       const bool in = j < n;
       A.j[r] = in ? j : i;
-      int xji, xij;
+      A.in |= (uint32_t)in << r;
       if constexpr (STREAM) {
-        xji = in ? (int)((RGET(iw, A.j[r]) >> ib) & 1) : 0;
+        xji[r] = in ? (int)((RGET(iw, A.j[r]) >> ib) & 1) : 0;
         const uint64_t rw = word < W64 ? ROWI(word) : 0ull;
-        xij = (int)((rw >> lane) & 1);
+        xij[r] = (int)((rw >> lane) & 1);
       } else {  // 32-bit halves: bit i of row j (unclamped) and bit j of row i
         const uint32_t cj = so_base[2 * ((size_t)iw * RS + 64 * NW * r) + (ib >> 5)];
-        xji = in ? (int)((cj >> (ib & 31)) & 1u) : 0;
+        xji[r] = in ? (int)((cj >> (ib & 31)) & 1u) : 0;
         const uint32_t ri = rows32[2 * ((size_t)word * RS + i) + (lane >> 5)];
-        xij = (int)((ri >> (lane & 31)) & 1u);
+        xij[r] = (int)((ri >> (lane & 31)) & 1u);
       }
-      const uint64_t bal = __ballot(xji);
-      if (lane == 0) coli_s[word] = bal;
-      A.in |= (uint32_t)in << r;
-      A.xji |= (uint32_t)xji << r;
-      A.xij |= (uint32_t)xij << r;
+    }
+    // Philox words of the lane's alternatives (counter steps, chain, 1, j): they
+    // depend on the step and j only, so they are computed here, between the
+    // loads above and their first use
+    uint32_t uj[R];
+    float llb[R];  // float32 values of log(-log u_j) (approx_loglog) for the bounds
+    if constexpr (!REPLAY) {
+      philox_alternatives<R>(__builtin_amdgcn_readfirstlane((uint32_t)steps),
+                             __builtin_amdgcn_readfirstlane((uint32_t)chain), A.j, key0, key1,
+                             uj);
+#pragma unroll
+      for (int r = 0; r < R; ++r) llb[r] = approx_loglog(uj[r]);
+      // keep them here (the compiler would otherwise sink them to their use)
+#pragma unroll
+      for (int r = 0; r < R; ++r) asm volatile("" : "+v"(uj[r]), "+v"(llb[r]));
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const uint64_t bal = __ballot(xji[r]);
+      if (lane == 0) coli_s[NW * r + wv] = bal;
+      A.xji |= (uint32_t)xji[r] << r;
+      A.xij |= (uint32_t)xij[r] << r;
     }
     __syncthreads();
     if constexpr (!STREAM) {
@@ -1458,10 +1476,6 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
       // the largest bound. The largest bound is taken in float32 (a four-step
       // DPP maximum) with the threshold widened by its rounding
       // (choice_threshold); the candidates are found by ballot.
-      uint32_t uj[R];
-      philox_alternatives<R>(__builtin_amdgcn_readfirstlane((uint32_t)steps),
-                             __builtin_amdgcn_readfirstlane((uint32_t)chain), A.j, key0, key1,
-                             uj);
       double ga[R];
       float m32 = -INFINITY;
 #pragma unroll
@@ -1469,7 +1483,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
         if (A.j[r] == i) f[r] = 0.0;
         ga[r] = -INFINITY;
         if (!((A.ok >> r) & 1)) continue;
-        ga[r] = f[r] - (double)approx_loglog(uj[r]);
+        ga[r] = f[r] - (double)llb[r];
         m32 = fmaxf(m32, (float)ga[r]);
       }
       double M = (double)wave_max_f32(m32);
