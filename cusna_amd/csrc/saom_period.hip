@@ -48,6 +48,14 @@ namespace {
 constexpr int kFixedT[] = {SAOM_FIXED_TYPES};
 constexpr int kFixedE[] = {SAOM_FIXED_ETYPES};
 constexpr int kFixedD[] = {SAOM_FIXED_DYN};
+// effect e's change statistic reads a static covariate at the alternative
+// (altX, simX, sameX, higher, egoXaltX, egoPlusAltX, altRThresholdX): the
+// values at the lane's actors are loaded once per chain
+constexpr bool alt_cov(int e) {
+  const int t = kFixedT[e];
+  return (t == 11 || t == 12 || t == 13 || t == 20 || t == 21 || t == 25 || t == 31) &&
+         kFixedD[e] == 0;
+}
 __device__ __forceinline__ int eff_type(const SimParams&, int e) { return kFixedT[e]; }
 __device__ __forceinline__ int eff_ety(const SimParams&, int e) { return kFixedE[e]; }
 __device__ __forceinline__ int eff_dyn(const SimParams&, int e) { return kFixedD[e]; }
@@ -447,7 +455,8 @@ struct ChainLds {
 // minus zbar for behaviour-dynamic effects (:233, 244, 252-253).
 template <int R, int WMAX, bool STREAM, bool SIMPLE, class EgoT>
 __device__ __forceinline__ void eff_base(const SimParams& P, int e, const Alt<R>& A,
-                                         const EgoT& U, const ChainLds& S, double (&d)[R]) {
+                                         const EgoT& U, const ChainLds& S, double (&d)[R],
+                                         const double* vjr = nullptr) {
   auto ROW = [&](int w, int x) -> uint64_t { return S.template row<STREAM>(w, x); };
   const int* indeg = S.indeg;
   const int* outdeg = S.outdeg;
@@ -460,7 +469,10 @@ __device__ __forceinline__ void eff_base(const SimParams& P, int e, const Alt<R>
   const double* vrow = P.vmat + (size_t)e * n;
   const double* w = P.wtab + (size_t)e * (n + 1);
   const double p1 = P.p1[e], p2 = P.p2[e];
+  // vjr: the alternatives' covariate values held by the caller (a static
+  // covariate's values at the lane's actors do not change within a period)
   auto VJ = [&](int r) -> double {
+    if (vjr) return vjr[r];
     return dyn ? ((double)z[A.j[r]] - P.zbar) : vrow[A.j[r]];
   };
   const double vi = dyn ? U.zvi : load_uniform(vrow + U.i);
@@ -690,9 +702,9 @@ __device__ __forceinline__ void eff_base(const SimParams& P, int e, const Alt<R>
 template <int R, int WMAX, bool STREAM, bool SIMPLE, class EgoT>
 __device__ __forceinline__ void eff_full(const SimParams& P, int e, const Alt<R>& A,
                                          const EgoT& U, const ChainLds& S, double (&d)[R],
-                                         bool mask = true) {
+                                         bool mask = true, const double* vjr = nullptr) {
   if constexpr (SIMPLE) {
-    eff_base<R, WMAX, STREAM, true>(P, e, A, U, S, d);
+    eff_base<R, WMAX, STREAM, true>(P, e, A, U, S, d, vjr);
   } else {
     // One eff_base call site: a plain effect runs the loop once; an
     // interaction runs it over its two or three components.
@@ -703,7 +715,7 @@ __device__ __forceinline__ void eff_full(const SimParams& P, int e, const Alt<R>
       const int ce = !inter ? (q == 0 ? e : -1) : (q == 0 ? P.c0[e] : (q == 1 ? P.c1[e] : c2));
       if (ce < 0) break;
       double tmp[R];
-      eff_base<R, WMAX, STREAM, false>(P, ce, A, U, S, tmp);
+      eff_base<R, WMAX, STREAM, false>(P, ce, A, U, S, tmp, inter ? nullptr : vjr);
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         if (!inter) d[r] = tmp[r];
@@ -916,6 +928,22 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
   const int flags = (SIMPLE ? (pflags & (F_TP | F_Z | F_RDEG | F_RATE)) : pflags) & kRateMask;
   const ChainLds S{rows, RS, W64, NW, wv, lane, indeg, outdeg, z, tpall, soall, tpm0, tpm1,
                    rows_d, lw};
+
+#ifdef SAOM_FIXED_K
+  // static covariate values at the lane's alternatives, per effect that reads
+  // them (slots past n: 0, never chosen)
+  double vjpre[SAOM_FIXED_K][R];
+  static_for<0, SAOM_FIXED_K>([&](auto EC) {
+    constexpr int e = EC;
+    if constexpr (alt_cov(e)) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int j = (NW * r + wv) * 64 + lane;
+        vjpre[e][r] = j < n ? P.vmat[(size_t)e * n + j] : 0.0;
+      }
+    }
+  });
+#endif
 
   // ---- chain set-up (libcusna_saom_cpu.cpp:146-160)
   constexpr int WI = STREAM ? 1 : WMAX;  // words per row stored (x W64 when streamed)
@@ -1424,7 +1452,8 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
     static_for<0, SAOM_FIXED_K>([&](auto EC) {
       constexpr int e = EC;
       double d[R];
-      eff_full<R, WMAX, STREAM, SIMPLE>(P, e, A, U, S, d, !SAOM_CACHE_D);
+      eff_full<R, WMAX, STREAM, SIMPLE>(P, e, A, U, S, d, !SAOM_CACHE_D,
+                                        alt_cov(e) ? vjpre[e] : nullptr);
       const double te = load_uniform(th + e);
       // density and reciprocity change by -1, 0 or 1, so te * d is exact and
       // one fused operation rounds f + te * d as the separate ones do; the
