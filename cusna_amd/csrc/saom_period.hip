@@ -929,6 +929,9 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
   const ChainLds S{rows, RS, W64, NW, wv, lane, indeg, outdeg, z, tpall, soall, tpm0, tpm1,
                    rows_d, lw};
 
+  // start network and distance mask (problem inputs, read every ministep)
+  const uint64_t* const x0p = P.x0;
+  const uint64_t* const kdp = P.kd;
 #ifdef SAOM_FIXED_K
   // static covariate values at the lane's alternatives, per effect that reads
   // them (slots past n: 0, never chosen)
@@ -1285,6 +1288,12 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
 #pragma unroll
       for (int r = 0; r < R; ++r) asm volatile("" : "+v"(uj[r]), "+v"(llb[r]));
     }
+    // the ego's degrees and behaviour and row i with word w in lane w: read
+    // before the barrier (the previous toggle precedes it), used after it
+    U.outdeg_i = outdeg[i];
+    U.indeg_i = indeg[i];
+    U.zvi = (flags & F_Z) ? (double)z[i] - P.zbar : -P.zbar;
+    const uint64_t rl = (!STREAM && lane < WMAX) ? ROWI(lane) : 0ull;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const uint64_t bal = __ballot(xji[r]);
@@ -1297,9 +1306,6 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
 #pragma unroll
       for (int wd = 0; wd < WMAX; ++wd) U.coli_r[wd] = coli_s[wd];
     }
-    U.outdeg_i = outdeg[i];
-    U.indeg_i = indeg[i];
-    U.zvi = (flags & F_Z) ? (double)z[i] - P.zbar : -P.zbar;
     U.rdeg_i = 0;
     if (flags & F_RDEG) {
 #pragma unroll
@@ -1333,9 +1339,6 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
         return (int)((half >> (h & 31)) & 1u);
       }
     };
-    // row i, word w in lane w (LDS-resident rows): the walks below and the
-    // chosen dyad's state, read here before this ministep's toggle
-    const uint64_t rl = (!STREAM && lane < WMAX) ? ROWI(lane) : 0ull;
     auto out_tie = [&](int h) {  // an out-tie i -> h of the ego
       if (flags & F_TP) {
 #pragma unroll
@@ -1669,9 +1672,11 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
       uint64_t rw;
       if constexpr (STREAM) rw = rowi_s[jw];
       else rw = readlane64(rl, jw);
+      const uint64_t x0w = load_uniform(x0p + (size_t)i * W64 + jw);
+      const uint64_t kdw = load_uniform(kdp + (size_t)i * W64 + jw);
       aij = (int)((rw >> jbit) & 1);
-      x0ij = (int)((load_uniform(P.x0 + (size_t)i * W64 + jw) >> jbit) & 1);
-      kdij = (int)((load_uniform(P.kd + (size_t)i * W64 + jw) >> jbit) & 1);
+      x0ij = (int)((x0w >> jbit) & 1);
+      kdij = (int)((kdw >> jbit) & 1);
     }
     auto toggle = [&]() {  // thread 0: the step's tie change and the rate scores
       if (jstar != i) {
