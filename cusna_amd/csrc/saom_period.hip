@@ -1667,9 +1667,9 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
     // 7. state of the chosen dyad before the toggle (:277-281), from row i as
     // read in this ministep (registers, or the LDS copy of a streamed row)
     int aij = 0, x0ij = 0, kdij = 0;
+    uint64_t rw = 0;  // word jstar >> 6 of row i before the toggle
     if (jstar != i) {
       const int jw = jstar >> 6, jbit = jstar & 63;
-      uint64_t rw;
       if constexpr (STREAM) rw = rowi_s[jw];
       else rw = readlane64(rl, jw);
       const uint64_t x0w = load_uniform(x0p + (size_t)i * W64 + jw);
@@ -1679,14 +1679,14 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
       kdij = (int)((kdw >> jbit) & 1);
     }
     auto toggle = [&]() {  // thread 0: the step's tie change and the rate scores
-      if (jstar != i) {
+      if (jstar != i) {  // row i's word and i's degrees as read in this ministep
         const int dd = 1 - 2 * aij;
-        RPUT(jstar >> 6, i, RGET(jstar >> 6, i) ^ (1ull << (jstar & 63)));
+        RPUT(jstar >> 6, i, rw ^ (1ull << (jstar & 63)));
         indeg[jstar] += dd;
-        outdeg[i] += dd;
+        outdeg[i] = U.outdeg_i + dd;
         if (P.symflag) {
           RPUT(i >> 6, jstar, RGET(i >> 6, jstar) ^ (1ull << (i & 63)));
-          indeg[i] += dd;
+          indeg[i] = U.indeg_i + dd;
           outdeg[jstar] += dd;
         }
       }
