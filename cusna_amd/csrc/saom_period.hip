@@ -1716,7 +1716,11 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
       }
       // no change statistics when the ego keeps its ties (dstar holds the
       // unmasked values of its own slot in the cached form)
-      sc[tid] += (jstar == i ? 0.0 : dstar[tid]) - Tt / St;
+      // T / S from one Newton step on the hardware reciprocal (within 2 ulps
+      // of the division; the scores are float32 outputs)
+      double rs = __builtin_amdgcn_rcp(St);
+      rs = fma(fma(-St, rs, 1.0), rs, rs);
+      sc[tid] += (jstar == i ? 0.0 : dstar[tid]) - Tt * rs;
     }
     if (!kEarlyToggle && tid == 0) toggle();
     steps++;
