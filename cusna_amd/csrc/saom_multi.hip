@@ -304,6 +304,18 @@ __device__ __forceinline__ void multi_body(const MultiParams* __restrict__ Qp) {
       A.xji |= (uint32_t)xji << r;
       A.xij |= (uint32_t)xij << r;
     }
+    // Philox words of the lane's alternatives and their float32 Gumbel-bound
+    // logarithms: they depend on the step and j only, so they fill the waits
+    // of the loads above and of the barrier (kept here by an empty asm operand)
+    uint32_t uj[R];
+    float llb[R];
+    philox_alternatives<R>(__builtin_amdgcn_readfirstlane((uint32_t)steps),
+                           __builtin_amdgcn_readfirstlane((uint32_t)chain), A.j, key0, key1, uj);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      llb[r] = approx_loglog(uj[r]);
+      asm volatile("" : "+v"(uj[r]), "+v"(llb[r]));
+    }
     __syncthreads();
     if constexpr (!STREAM) {
 #pragma unroll
@@ -454,10 +466,6 @@ __device__ __forceinline__ void multi_body(const MultiParams* __restrict__ Qp) {
     int jb = i;
     bool exact_choice;
     {
-      uint32_t uj[R];
-      philox_alternatives<R>(__builtin_amdgcn_readfirstlane((uint32_t)steps),
-                             __builtin_amdgcn_readfirstlane((uint32_t)chain), A.j, key0, key1,
-                             uj);
       double ga[R];
       float m32 = -INFINITY;
 #pragma unroll
@@ -465,7 +473,7 @@ __device__ __forceinline__ void multi_body(const MultiParams* __restrict__ Qp) {
         if (A.j[r] == i) f[r] = 0.0;
         ga[r] = -INFINITY;
         if (!((A.ok >> r) & 1)) continue;
-        ga[r] = f[r] - (double)approx_loglog(uj[r]);
+        ga[r] = f[r] - (double)llb[r];
         m32 = fmaxf(m32, (float)ga[r]);
       }
       double M = (double)wave_max_f32(m32);
