@@ -283,8 +283,7 @@ __device__ __forceinline__ void multi_body(const MultiParams* __restrict__ Qp) {
       U.coli_s = coli_s;
       U.W = W64;
     } else {
-#pragma unroll
-      for (int wd = 0; wd < WMAX; ++wd) U.rowi_r[wd] = ROWI(wd);
+      U.rl = lane < WMAX ? ROWI(lane) : 0ull;  // word `lane` of row i
     }
     Alt<R> A;
     A.in = A.ok = A.xij = A.xji = 0u;
@@ -317,10 +316,7 @@ __device__ __forceinline__ void multi_body(const MultiParams* __restrict__ Qp) {
       asm volatile("" : "+v"(uj[r]), "+v"(llb[r]));
     }
     __syncthreads();
-    if constexpr (!STREAM) {
-#pragma unroll
-      for (int wd = 0; wd < WMAX; ++wd) U.coli_r[wd] = coli_s[wd];
-    }
+    if constexpr (!STREAM) U.cl = lane < WMAX ? coli_s[lane] : 0ull;  // word `lane` of column i
     U.outdeg_i = outdeg[i];
     U.indeg_i = indeg[i];
     U.zvi = 0.0;

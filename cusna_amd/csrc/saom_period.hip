@@ -399,29 +399,29 @@ struct Alt {
 };
 
 // Ego-level state shared by all alternatives of the ministep. Row and column
-// of i sit in registers when the network is LDS-resident; a streamed network
+// of i sit in registers when the network is LDS-resident, word w in lane w
+// (a word is read into scalar registers where it is used); a streamed network
 // (rows in device memory, W64 up to 128 words) keeps them in LDS instead.
 // (Reading them from LDS where they lie was measured slower also at four and
 // eight waves per chain: 121 against 118 ms at 500 actors, 99 against 90 ms
-// at 1000.)
+// at 1000; every wave holding every word in registers, 82.0 against 81.5 ms at
+// 500 actors.)
 template <int WMAX, bool STREAM>
 struct Ego {
-  static constexpr int WR = STREAM ? 1 : WMAX;
   int i;
   int outdeg_i, indeg_i, rdeg_i;
   double zvi;
-  uint64_t rowi_r[WR];  // out-ties of i
-  uint64_t coli_r[WR];  // in-ties of i
+  uint64_t rl = 0, cl = 0;  // word `lane` of row i (out-ties) and column i (in-ties)
   const uint64_t* rowi_s;
   const uint64_t* coli_s;
   int W;  // words looped over: WMAX (zero-padded rows) or W64 (streamed)
   __device__ __forceinline__ uint64_t rowi(int w) const {
     if constexpr (STREAM) return rowi_s[w];
-    else return rowi_r[w];
+    else return readlane64(rl, w);
   }
   __device__ __forceinline__ uint64_t coli(int w) const {
     if constexpr (STREAM) return coli_s[w];
-    else return coli_r[w];
+    else return readlane64(cl, w);
   }
   __device__ __forceinline__ int words() const {
     if constexpr (STREAM) return W;
@@ -1247,9 +1247,6 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
       U.rowi_s = rowi_s;
       U.coli_s = coli_s;
       U.W = W64;
-    } else {
-#pragma unroll
-      for (int wd = 0; wd < WMAX; ++wd) U.rowi_r[wd] = ROWI(wd);
     }
     Alt<R> A;
     A.in = A.ok = A.xij = A.xji = 0u;
@@ -1294,6 +1291,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
     U.indeg_i = indeg[i];
     U.zvi = (flags & F_Z) ? (double)z[i] - P.zbar : -P.zbar;
     const uint64_t rl = (!STREAM && lane < WMAX) ? ROWI(lane) : 0ull;
+    U.rl = rl;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const uint64_t bal = __ballot(xji[r]);
@@ -1302,10 +1300,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
       A.xij |= (uint32_t)xij[r] << r;
     }
     __syncthreads();
-    if constexpr (!STREAM) {
-#pragma unroll
-      for (int wd = 0; wd < WMAX; ++wd) U.coli_r[wd] = coli_s[wd];
-    }
+    if constexpr (!STREAM) U.cl = lane < WMAX ? coli_s[lane] : 0ull;
     U.rdeg_i = 0;
     if (flags & F_RDEG) {
 #pragma unroll
@@ -1374,10 +1369,9 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
         }
       }
       if (walk) {
-        const uint64_t cl = lane < WMAX ? coli_s[lane] : 0ull;
-        for (uint64_t nz = __ballot(cl != 0); nz; nz &= nz - 1) {
+        for (uint64_t nz = __ballot(U.cl != 0); nz; nz &= nz - 1) {
           const int wd = __ffsll((long long)nz) - 1;
-          for (uint64_t m = readlane64(cl, wd); m; m &= m - 1)
+          for (uint64_t m = readlane64(U.cl, wd); m; m &= m - 1)
             in_tie(wd * 64 + __ffsll((long long)m) - 1);
         }
       }
