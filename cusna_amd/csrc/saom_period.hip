@@ -1239,7 +1239,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
     }
 
     // ---- network ministep (:229-288)
-    // 2. row i (broadcast) and col i (ballot of bit i over the rows)
+    // 2. row i (word w in lane w) and col i (ballot of bit i over the rows)
     Ego<WMAX, STREAM> U;
     U.i = i;
     if constexpr (STREAM) {
@@ -1270,9 +1270,10 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
         xij[r] = (int)((ri >> (lane & 31)) & 1u);
       }
     }
-    // Philox words of the lane's alternatives (counter steps, chain, 1, j): they
-    // depend on the step and j only, so they are computed here, between the
-    // loads above and their first use
+    // Philox words of the lane's alternatives (counter steps, chain, 1, j) and
+    // their Gumbel-bound logarithms: they depend on the step and j only, so they
+    // are computed here, where they fill the waits of the loads above and of
+    // the barrier, instead of lengthening the choice (not used in replay)
     uint32_t uj[R];
     float llb[R];  // float32 values of log(-log u_j) (approx_loglog) for the bounds
     if constexpr (!REPLAY) {
