@@ -159,7 +159,7 @@ void raise_lds(void* fn) {
 }  // namespace
 
 void* rtc_period_kernel(const SimParams& P, LaunchShape s, bool replay, bool simple, bool fixed_n,
-                        int waves_per_eu) {
+                        int waves_per_eu, bool lean) {
   const int R = s.wmax / s.nw;
   std::vector<std::string> opts = {
       "--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
@@ -176,6 +176,10 @@ void* rtc_period_kernel(const SimParams& P, LaunchShape s, bool replay, bool sim
   if (fixed_n) opts.push_back("-DSAOM_FIXED_N=" + std::to_string(P.n));
   else opts.push_back("-DSAOM_FIXED_W64=" + std::to_string(P.W64));
   if (waves_per_eu > 0) opts.push_back("-DSAOM_WAVES_PER_EU=" + std::to_string(waves_per_eu));
+  // CUSNA_RTC_LEAN=1: every specialisation lean (a check that the lean build
+  // gives the same results)
+  const char* le = std::getenv("CUSNA_RTC_LEAN");
+  if (lean || (le && le[0] == '1')) opts.push_back("-DSAOM_LEAN=1");
   return rtc_get(kRtcPeriodSource, "saom_period_rtc", opts);
 }
 

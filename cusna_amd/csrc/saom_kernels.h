@@ -134,9 +134,11 @@ int launch_period_ex(const PeriodLaunch& L, LaunchShape s, bool replay, bool sim
 // word count W64 is compiled in either way (every launch holds networks of
 // one word count: a plan's chains, one bucket of a groups call, the periods
 // of a MoM fit);
-// waves_per_eu > 0: the register budget in waves per SIMD (0: two waves).
+// waves_per_eu > 0: the register budget in waves per SIMD (0: two waves);
+// lean: without the registers that hold the alternatives' static covariate
+// values for the whole period (they are loaded per ministep instead).
 void* rtc_period_kernel(const SimParams& P, LaunchShape s, bool replay, bool simple,
-                        bool fixed_n, int waves_per_eu = 0);
+                        bool fixed_n, int waves_per_eu = 0, bool lean = false);
 // Whether a compiled kernel uses scratch memory (register spills), and its
 // vector registers per lane.
 bool uses_scratch(void* fn);

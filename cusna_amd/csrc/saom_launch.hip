@@ -81,11 +81,15 @@ int launch_period_ex(const PeriodLaunch& L, LaunchShape s, bool replay, bool sim
       // the preferred shape needs rs.wpe waves per SIMD: compiled with the
       // default budget when it fits that occupancy's registers anyway (the
       // default budget schedules better: 113 against 117 ms at 500 actors),
-      // else with the tighter budget, unless that spills; then the
-      // alternative shape in the two-wave budget
+      // else without the per-chain covariate registers (the rate-effect path
+      // at 500 actors: 141 -> 127 VGPRs), else lean with the tighter budget,
+      // unless that spills; then the alternative shape in the two-wave budget
+      const int budget = 512 / rs.wpe;
+      auto fits = [&](void* f) { return f && !uses_scratch(f) && kernel_vgprs(f) <= budget; };
       fn = rtc_period_kernel(*L.Ph, rs, replay, simple, L.same_n);
-      if (fn && (uses_scratch(fn) || kernel_vgprs(fn) > 512 / rs.wpe)) {
-        fn = rtc_period_kernel(*L.Ph, rs, replay, simple, L.same_n, rs.wpe);
+      if (!fits(fn)) fn = rtc_period_kernel(*L.Ph, rs, replay, simple, L.same_n, 0, true);
+      if (!fits(fn)) {
+        fn = rtc_period_kernel(*L.Ph, rs, replay, simple, L.same_n, rs.wpe, true);
         if (fn && uses_scratch(fn)) fn = nullptr;
       }
       if (!fn && L.exact_alt.nw > 0 && L.lds_exact_alt <= 160 * 1024) {

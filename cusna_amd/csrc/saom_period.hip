@@ -52,9 +52,13 @@ constexpr int kFixedD[] = {SAOM_FIXED_DYN};
 // (altX, simX, sameX, higher, egoXaltX, egoPlusAltX, altRThresholdX): the
 // values at the lane's actors are loaded once per chain
 constexpr bool alt_cov(int e) {
+#if defined(SAOM_LEAN) && SAOM_LEAN
+  return e < 0;  // lean build: none (the values are read per ministep)
+#else
   const int t = kFixedT[e];
   return (t == 11 || t == 12 || t == 13 || t == 20 || t == 21 || t == 25 || t == 31) &&
          kFixedD[e] == 0;
+#endif
 }
 __device__ __forceinline__ int eff_type(const SimParams&, int e) { return kFixedT[e]; }
 __device__ __forceinline__ int eff_ety(const SimParams&, int e) { return kFixedE[e]; }

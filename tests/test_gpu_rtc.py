@@ -72,3 +72,15 @@ def test_specialised_large_chain_state(gpu_lib):
         plan.close()
     for key in a:
         np.testing.assert_array_equal(np.asarray(b[key]), np.asarray(a[key]), err_msg=key)
+
+
+def test_lean_specialisation_matches_generic(gpu_lib, monkeypatch):
+    """The lean specialisation (static covariates at the alternatives read
+    per ministep rather than held in registers for the period, the build the
+    launch falls back to when the other exceeds the four-wave register
+    budget, as the rate-effect path at 500 actors does) gives the generic
+    kernel's results, on the 500-actor problem with altX and simX."""
+    from cusna_amd.workloads import n500_problem
+
+    monkeypatch.setenv("CUSNA_RTC_LEAN", "1")
+    _both(gpu_lib, n500_problem(M=64, n=500), seed=13)
