@@ -1000,6 +1000,9 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
   const double* th = P.theta + (size_t)chain * P.th_stride;
   const double* thb = kb > 0 ? P.thetab + (size_t)chain * P.thb_stride : nullptr;
   const int nact = P.nact;
+  // rate effects: first position in act order whose running rate sum is to be
+  // recomputed at the next draw (all at the first)
+  int rp = 0;
   long long steps = 0, steps_beh = 0, dist = 0, bdist = 0;
   long long cap = kMaxSteps;
   if (REPLAY && P.smax < cap) cap = P.smax;
@@ -1055,6 +1058,11 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
       // rate scores, are summed in parallel (per thread, then a fixed-order
       // lane and wave reduction): their rounding differs from the
       // reference's sequential sum by float64 ulps.
+      //
+      // Only the running sums from position rp on are recomputed: the rates
+      // before it are those of the previous draw (rp is the first position in
+      // act order whose actor's rate can have changed since, nact when none),
+      // so the sums there, and S0 when rp == nact, are the ones already held.
       const int nr = P.nrate;
       double part[kMaxRate];
 #pragma unroll
@@ -1074,7 +1082,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
         if (sx > 30.0) sx = 30.0;
         else if (sx < -30.0) sx = -30.0;
         const double c = exp(sx);
-        crate[x] = c;
+        if (x >= rp) crate[x] = c;
 #pragma unroll
         for (int q = 0; q < kMaxRate; ++q) part[q] += c * uq[q];
       }
@@ -1089,8 +1097,8 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
       __syncthreads();
       if (tid == 0) {
         // the loads of kB rates are in flight together before their sums
-        double acc = 0.0;
-        int x = 0;
+        int x = rp;
+        double acc = x > 0 ? crate[x - 1] : 0.0;
         constexpr int kB = 16;
         for (; x + kB <= nact; x += kB) {
           double cx[kB];
@@ -1106,7 +1114,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
           acc += crate[x];
           crate[x] = acc;
         }
-        rsum_s[0] = acc;
+        if (rp < nact) rsum_s[0] = acc;
         for (int q = 0; q < nr; ++q) {
           double t = 0.0;
           for (int w = 0; w < NW; ++w) t += wsum[w * 8 + q];
@@ -1114,6 +1122,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
         }
       }
       __syncthreads();
+      rp = nact;  // every running sum current
       const double S0 = rsum_s[0];
       const double tau = -dv[di] / (lam * S0);
       double tau_eff = tau;
@@ -1726,6 +1735,16 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
     if (jstar != i && kdij) {
       const long long chg = P.symflag ? 2 : 1;
       dist += (aij == x0ij) ? chg : -chg;
+    }
+    if ((flags & F_RATE) && jstar != i) {
+      // degree-based rates of the actors whose degrees changed: their running
+      // sums and all after them are recomputed at the next draw (all of them
+      // when the active actors are a list, whose positions are not kept)
+      for (int q = 0; q < P.nrate; ++q) {
+        const int kd = P.rkind[q];
+        if (kd == 1 || (kd == 2 && P.symflag)) rp = min(rp, P.act ? 0 : i);
+        if (kd == 2 || (kd == 1 && P.symflag)) rp = min(rp, P.act ? 0 : jstar);
+      }
     }
     if (!kEarlyToggle) __syncthreads();
     if (P.cond && dist >= P.target) break;
