@@ -1041,6 +1041,15 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
     __syncthreads();
   };
 
+  // problem scalars read every ministep, held in scalar registers for the
+  // period (the compiler would otherwise reload them from the parameter block
+  // at each use, an exposed scalar-memory wait for a lone wave)
+  int p_sym = P.symflag, p_cond = P.cond, p_target = P.target, p_maxdeg = P.maxdeg;
+  const uint64_t* p_pm = P.pm;
+  const int* p_act = P.act;
+  double p_zbar = P.zbar;
+  asm volatile("" : "+s"(p_sym), "+s"(p_cond), "+s"(p_target), "+s"(p_maxdeg));
+  asm volatile("" : "+s"(p_pm), "+s"(p_act), "+s"(p_zbar));
   for (;;) {
     // 1. ministep draw (:167, 194-200)
     if ((steps & (DB - 1)) == 0) fill(steps);
@@ -1126,7 +1135,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
       const double S0 = rsum_s[0];
       const double tau = -dv[di] / (lam * S0);
       double tau_eff = tau;
-      if (!P.cond && t + tau > 1.0) tau_eff = 1.0 - t;
+      if (!p_cond && t + tau > 1.0) tau_eff = 1.0 - t;
       if (tid == 0) {
         *rho_s += tau_eff * S0;
         for (int q = 0; q < nr; ++q) {
@@ -1135,7 +1144,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
         }
       }
       t = uniform_f64(t + tau);
-      if (!P.cond && t > 1.0) break;
+      if (!p_cond && t > 1.0) break;
       const double tgt = unit_open(c2) * S0;
       long long below = 0;  // running sums are non-decreasing: count those below tgt
       for (int x = tid; x < nact; x += NT) below += crate[x] < tgt ? 1 : 0;
@@ -1150,11 +1159,11 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
       net = true;
     } else {
       t = uniform_f64(t + dv[di]);  // the chain time lives in scalar registers
-      if (!P.cond && t > 1.0) break;
+      if (!p_cond && t > 1.0) break;
       net = kb == 0 || unit_open(c1) < lam / (lam + lamb);
       long long ix = (long long)(unit_open(c2) * nact);
       if (ix >= nact) ix = nact - 1;
-      i = P.act ? P.act[ix] : (int)ix;
+      i = p_act ? p_act[ix] : (int)ix;
     }
     i = __builtin_amdgcn_readfirstlane(i);  // uniform: scalar registers and loads
     auto ROWI = [&](int w) -> uint64_t { return RGET(w, i); };
@@ -1163,14 +1172,14 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
       // ---- behaviour ministep (:202-227)
       const int zi = z[i];
       const uint32_t c3 = dc3[di];
-      const double zic = (double)zi - P.zbar;
+      const double zic = (double)zi - p_zbar;
       long long nb = 0, nH = 0, nE = 0, nL = 0;
       double zs = 0.0;  // exact in any order: terms are multiples of ulp(float zbar)
       for (int q = tid; q < n; q += NT) {
         if ((ROWI(q >> 6) >> (q & 63)) & 1) {
           nb++;
           const int zj = z[q];
-          zs += (double)zj - P.zbar;
+          zs += (double)zj - p_zbar;
           if (zj > zi) nH++;
           else if (zj == zi) nE++;
           else nL++;
@@ -1246,7 +1255,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
       __syncthreads();
       steps_beh++;
       steps++;
-      if (P.cond && dist >= P.target) break;
+      if (p_cond && dist >= p_target) break;
       if (steps >= cap) break;
       continue;
     }
@@ -1303,7 +1312,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
     // before the barrier (the previous toggle precedes it), used after it
     U.outdeg_i = outdeg[i];
     U.indeg_i = indeg[i];
-    U.zvi = (flags & F_Z) ? (double)z[i] - P.zbar : -P.zbar;
+    U.zvi = (flags & F_Z) ? (double)z[i] - p_zbar : -p_zbar;
     const uint64_t rl = (!STREAM && lane < WMAX) ? ROWI(lane) : 0ull;
     U.rl = rl;
 #pragma unroll
@@ -1436,7 +1445,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
       __syncthreads();
     }
     // choosable alternatives (:238-241)
-    const uint64_t* pmrow = P.pm + (size_t)i * W64;
+    const uint64_t* pmrow = p_pm + (size_t)i * W64;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int word = NW * r + wv;
@@ -1445,7 +1454,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
       else if (A.j[r] == i) ok = true;
       else {
         ok = (load_uniform(pmrow + word) >> lane) & 1;
-        if (ok && P.maxdeg > 0 && A.bij(r) == 0 && U.outdeg_i >= P.maxdeg) ok = false;
+        if (ok && p_maxdeg > 0 && A.bij(r) == 0 && U.outdeg_i >= p_maxdeg) ok = false;
       }
       A.ok |= (uint32_t)ok << r;
     }
@@ -1692,7 +1701,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
         RPUT(jstar >> 6, i, rw ^ (1ull << (jstar & 63)));
         indeg[jstar] += dd;
         outdeg[i] = U.outdeg_i + dd;
-        if (P.symflag) {
+        if (p_sym) {
           RPUT(i >> 6, jstar, RGET(i >> 6, jstar) ^ (1ull << (i & 63)));
           indeg[i] = U.indeg_i + dd;
           outdeg[jstar] += dd;
@@ -1733,7 +1742,7 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
     if (!kEarlyToggle && tid == 0) toggle();
     steps++;
     if (jstar != i && kdij) {
-      const long long chg = P.symflag ? 2 : 1;
+      const long long chg = p_sym ? 2 : 1;
       dist += (aij == x0ij) ? chg : -chg;
     }
     if ((flags & F_RATE) && jstar != i) {
@@ -1742,12 +1751,12 @@ __device__ __forceinline__ void period_body(const SimParams* __restrict__ Pp,
       // when the active actors are a list, whose positions are not kept)
       for (int q = 0; q < P.nrate; ++q) {
         const int kd = P.rkind[q];
-        if (kd == 1 || (kd == 2 && P.symflag)) rp = min(rp, P.act ? 0 : i);
-        if (kd == 2 || (kd == 1 && P.symflag)) rp = min(rp, P.act ? 0 : jstar);
+        if (kd == 1 || (kd == 2 && p_sym)) rp = min(rp, p_act ? 0 : i);
+        if (kd == 2 || (kd == 1 && p_sym)) rp = min(rp, p_act ? 0 : jstar);
       }
     }
     if (!kEarlyToggle) __syncthreads();
-    if (P.cond && dist >= P.target) break;
+    if (p_cond && dist >= p_target) break;
     if (steps >= cap) break;
   }
 
